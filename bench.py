@@ -1,0 +1,157 @@
+"""Benchmark of the MDOODZ Newton-iteration grid path on one MI355X.
+
+One step = one Newton-iteration unit over the whole grid:
+StrainRateComponents -> NonNewtonianViscosityGrid -> EvaluateRHS ->
+RheologicalOperators(0) -> Picard CSR values -> RheologicalOperators(1) ->
+Jacobian CSR values -> RheologicalOperators(0) -> residual + norms,
+everything resident in HBM (the CSR patterns are built once per boundary-
+condition set, outside the timed region, as they are once per time step).
+
+Workload: the three-layer lithosphere (wet quartzite / plagioclase /
+olivine + Peierls, Maxwell elasticity, Drucker-Prager) at 2049 x 1025 cells,
+seeded synthetic fields.  The metric is cell updates per second:
+(Nx-1)(Nz-1) / time per unit.
+
+Prints ONE JSON line.
+"""
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+# algorithmic HBM bytes per cell update of each stage (each listed array
+# element read or written once; FP64 = 8 B, tags 1 B, equation numbers 4 B)
+STAGES = ["strain_rates", "rheology", "rhs", "dtensor_picard", "picard_values", "dtensor_jacobian",
+          "jacobian_values", "dtensor_picard_2", "residual"]
+BYTES_PER_CELL = {"strain_rates": 76, "rheology": 810, "rhs": 160, "dtensor_picard": 114, "picard_values": 340,
+                  "dtensor_jacobian": 330, "jacobian_values": 668, "dtensor_picard_2": 114, "residual": 204}
+HBM_PEAK_GBS = 8000.0
+
+
+def _pkg():
+    spec = importlib.util.spec_from_file_location("mdoodz_amd", os.path.join(ROOT, "mdoodz6.0_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["mdoodz_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(mdz, ncx, ncz, seed, units):
+    """The oracle (C/OpenMP restatement) on the host cores: full-size units,
+    one warm-up then the median of `units`."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind
+
+    orc = oracle_bind.load(mdz)
+    m = mdz.Model.lithosphere(ncx, ncz, seed)
+    S = mdz.SparseMat()
+    orc.orc_EvalNumberOfEquations(m.m.mesh, C.byref(S))
+    blk = [mdz.SparseMat() for _ in range(8)]
+    for i, b in enumerate(blk):
+        orc.orc_salloc(C.byref(b), S.neq_mom if i % 4 < 2 else S.neq_cont)
+    times = []
+    for r in range(units + 1):
+        t0 = time.perf_counter()
+        orc.orc_newton_unit(m.m.mesh, C.byref(m.m.materials), C.byref(m.m.model), C.byref(m.m.Nmodel),
+                            C.byref(m.m.scaling), C.byref(S), *[C.byref(b) for b in blk])
+        dt = time.perf_counter() - t0
+        for b in blk:
+            orc.orc_free_block(C.byref(b))
+        if r > 0:
+            times.append(dt)
+    m.free()
+    times.sort()
+    t = times[len(times) // 2]
+    return ncx * ncz / t, orc.orc_num_threads(), t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ncx", type=int, default=2049)
+    ap.add_argument("--ncz", type=int, default=1025)
+    ap.add_argument("--mode", default="fp64", choices=["fp64", "mixed", "learned"])
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-units", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the path is built and measured on a single MI355X; "
+              "multi-GPU runs are not part of this build (--gpus %d refused)" % args.gpus)
+        sys.exit(2)
+
+    mdz = _pkg()
+    hot = mdz.lib_hot()
+    m = mdz.Model.lithosphere(args.ncx, args.ncz, args.seed)
+    ctx = C.c_void_p()
+    mdz.check(hot.mdz_ctx_create(C.byref(ctx), m.m.mesh, C.byref(m.m.materials), C.byref(m.m.model),
+                                 C.byref(m.m.scaling)))
+    mode = {"fp64": 0, "mixed": 1, "learned": 2}[args.mode]
+    mdz.check(hot.mdz_ctx_set_rheology_mode(ctx, mode))
+    cells = args.ncx * args.ncz
+
+    for _ in range(args.warmup):
+        mdz.check(hot.mdz_ctx_newton_unit(ctx, None))
+    res = (C.c_double * 3)()
+    mdz.check(hot.mdz_ctx_newton_unit(ctx, res))
+    ms = C.c_double()
+    t0 = time.perf_counter()
+    mdz.check(hot.mdz_ctx_time_units(ctx, args.steps, C.byref(ms)))
+    wall = (time.perf_counter() - t0) / args.steps * 1e3
+    ms_per_step = ms.value
+    value = cells / (ms_per_step * 1e-3)
+
+    st = (C.c_double * 10)()
+    mdz.check(hot.mdz_ctx_stage_times(ctx, 3, st))
+    stage_ms = {s: st[i] for i, s in enumerate(STAGES)}
+    dom = max(STAGES, key=lambda s: stage_ms[s])
+    dom_bytes = BYTES_PER_CELL[dom] * cells
+    achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_cell": BYTES_PER_CELL[dom], "stage_ms": round(stage_ms[dom], 4)}
+    whole = 2816.0 * cells / (ms_per_step * 1e-3) / 1e9
+    hot.mdz_ctx_destroy(ctx)
+    m.free()
+
+    cpu = None
+    if not args.no_cpu:
+        v, cores, t = cpu_baseline(mdz, args.ncx, args.ncz, args.seed, args.cpu_units)
+        cpu = {"value": round(v, 1), "unit": "cell-updates/s", "cores": cores, "kind": "port",
+               "sample": "%d full Newton units of the same %dx%d model (median of %d after 1 warm-up), %.2f s each"
+                         % (args.cpu_units, args.ncx, args.ncz, args.cpu_units, t)}
+
+    line = {
+        "metric": "Newton-iter cell-updates/s (rheol+resid+assembly)",
+        "value": round(value, 1),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if mode == 0 else "f32+f64",
+        "data": "synthetic (seeded lithosphere model)",
+        "config": {"workload": "lithosphere %dx%d cells, 3 phases VEP, one Newton-iteration unit" % (args.ncx, args.ncz),
+                   "rheology": args.mode, "cells": cells, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "whole_unit_algorithmic_GBs": round(whole, 1),
+        "host_wall_ms_per_step": round(wall, 4),
+        "residual_norms": list(res),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,350 @@
+/*
+ * mdoodz_hot.h -- C ABI of the MI355X (gfx950) implementation of the MDOODZ 6.0
+ * per-Newton-iteration grid path: rheology, Stokes residual, decoupled
+ * Picard/Jacobian CSR assembly.
+ *
+ * Two layers are declared here:
+ *
+ *  1. Drop-in entry points with the reference's own names and prototypes
+ *     (SOURCE/header_MDOODZ.h).  A host driver that links libmdz_hot.so in
+ *     place of RheologyDensity.o / StokesAssemblyDecoupled.o / the hot part
+ *     of StokesRoutines.o gets GPU execution with host-resident structs:
+ *     every call uploads the fields it reads and downloads the fields it
+ *     writes.  CSR blocks are returned in malloc() buffers that the caller
+ *     releases with free(), exactly as SOURCE/MemoryAllocFree.c:113 FreeMat.
+ *
+ *  2. A device-resident interface (mdz_*): the grid lives in HBM between
+ *     calls, the CSR sparsity pattern is built once per boundary-condition
+ *     set and only values are rewritten per Newton iteration.
+ *
+ * The struct layouts below are the reference's (field order and types are
+ * the ABI and must not change); they follow SOURCE/header_MDOODZ.h:25-219.
+ */
+#ifndef MDOODZ_HOT_H
+#define MDOODZ_HOT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DoodzFP double
+
+/* ---------------------------------------------------------------------- */
+/* Reference data model (ABI).  SOURCE/header_MDOODZ.h:25-219              */
+/* ---------------------------------------------------------------------- */
+
+/* header_MDOODZ.h:25 */
+typedef struct _scale scale;
+struct _scale {
+    double eta, L, V, T, t, a, E, S, m, rho, F, J, W, Cv, rhoE, k;
+};
+
+/* header_MDOODZ.h:31 -- only passed through by UpdateNonLinearity */
+typedef struct _surface surface;
+/* header_MDOODZ.h:66 -- only passed through by UpdateNonLinearity */
+typedef struct _p_markers markers;
+
+/* header_MDOODZ.h:46 */
+typedef struct _mat_prop mat_prop;
+struct _mat_prop {
+    int     Nb_phases;
+    DoodzFP R;
+    DoodzFP eta0[20], rho[20], mu[20], Cv[20], k[20], Qr[20], C[20], phi[20], psi[20], Slim[20],
+            n[20], A[20], Ea[20], Va[20], alp[20], bet[20], Qm[20], T0[20], P0[20], drho[20], k_eff[20];
+    DoodzFP tpwl[20], Qpwl[20], Vpwl[20], npwl[20], mpwl[20], Apwl[20], apwl[20], fpwl[20], rpwl[20],
+            Fpwl[20], pref_pwl[20];
+    DoodzFP texp[20], Qexp[20], Vexp[20], Sexp[20], Eexp[20], Gexp[20], aexp[20], fexp[20], rexp[20],
+            qexp[20], nexp[20];
+    DoodzFP tlin[20], Qlin[20], Vlin[20], nlin[20], mlin[20], Alin[20], alin[20], flin[20], rlin[20],
+            Flin[20];
+    DoodzFP tgbs[20], Qgbs[20], Vgbs[20], ngbs[20], mgbs[20], Agbs[20], agbs[20], fgbs[20], rgbs[20],
+            Fgbs[20];
+    DoodzFP ppzm[20], Kpzm[20], Qpzm[20], Vpzm[20], Gpzm[20], cpzm[20], Lpzm[20], gs_ref[20];
+    int     gs[20], cstv[20], pwlv[20], linv[20], expv[20], gbsv[20], phase_diagram[20], density_model[20];
+    DoodzFP C_end[20], phi_end[20], psi_end[20], pls_start[20], pls_end[20], eta_vp[20], n_vp[20];
+    int     phi_soft[20], psi_soft[20], coh_soft[20], is_tensile[20];
+    DoodzFP Pr[20], tau_kin[20], dPr[20], k_chem[20];
+    int     reac_soft[20], reac_phase[20];
+    int     phase_mix[20], phase_two[20];
+    double  aniso_factor[20], aniso_angle[20];
+};
+
+/* header_MDOODZ.h:81 -- mechanical boundary tags and values */
+typedef struct _BC BC;
+struct _BC {
+    char   *type;
+    double *val;
+};
+
+/* header_MDOODZ.h:88 -- thermal-style boundary record (BCg is one) */
+typedef struct _BCT BCT;
+struct _BCT {
+    char   *type, *typW, *typE, *typS, *typN;
+    double *val, *valW, *valE, *valS, *valN;
+};
+
+/* header_MDOODZ.h:95 -- one CSR block plus its vectors */
+typedef struct _SparseMat SparseMat;
+struct _SparseMat {
+    double *A, *x, *b, *F, *d, *bbc;
+    int    *Ic, *J, neq;
+    int    *eqn_u, *eqn_v, *eqn_p;
+    int     nnz, neq_mom, neq_cont;
+};
+
+/* header_MDOODZ.h:104 */
+typedef struct _params params;
+struct _params {
+    double xmin, zmin, xmax, zmax, time, dx, dz, dt, dt0, dt_start, dt_max, L0, dt_min;
+    double xmin0, zmin0, xmax0, zmax0;
+    double gx, gz;
+    int    Nx, Nz, Nt, step, nit, Newton, noisy;
+    int    eta_avg, itp_stencil;
+    double nexp_radial_basis;
+    int    ismechanical, isperiodic_x, isinertial, iselastic, isnonnewtonian, isthermal, ispureshear_ale,
+           free_surf, eqn_state, write_markers, write_debug, write_energies, no_markers;
+    double free_surf_stab;
+    int    dt_constant, RK, line_search, thermal_eq, subgrid_diff, adiab_heat, shear_heat, advection,
+           fstrain, ConservInterp;
+    int    surf_processes, cpc, surf_remesh, loc_iter, therm_pert, surf_ised1, surf_ised2, MantleID,
+           topografix, Reseed, SmoothSoftening;
+    double EpsBG, user0, user1, user2, user3, user4, user5, user6, user7, user8;
+    char  *input_file;
+    int    Nb_phases;
+    int    ncont;
+    double Courant, mineta, maxeta;
+    int    initial_noise, initial_part;
+    int    decoupled_solve, lsolver, diag_scaling, pc_type;
+    double penalty, abs_tol_div, rel_tol_div, auto_penalty, compressible, rel_tol_KSP;
+    double line_search_min, safe_dt_div;
+    int    safe_mode, nstagmax;
+    int    nT, nE, nd, def_maps;
+    double Pn, Tmin, Tmax, Emin, Emax, dmin, dmax, PrBG;
+    double surf_diff, surf_sedirate, surf_baselev, surf_Winc, surf_Vinc;
+    double therm_pert_x0, therm_pert_z0, therm_pert_dT, therm_pert_rad, cooling_time;
+    int    force_act_vol_ast;
+    double act_vol_dis_ast, act_vol_dif_ast;
+    int    isPD, num_PD, *PDMnT, *PDMnP;
+    double **PDMrho, *PDMTmin, *PDMTmax, *PDMPmin, *PDMPmax;
+    int    rec_T_P_x_z, delete_breakpoints, GNUplot_residuals;
+    int    BC_setup_type, shear_style, polar;
+    int    StressRotation, StressUpdate, DirectNeighbour;
+    int    diffuse_X, diffuse_avg;
+    double diffusion_length;
+    int    ProgReac, NoReturn, VolChangeReac, Plith_trick, UnsplitDiffReac;
+    int    aniso, aniso_fstrain;
+};
+
+/* header_MDOODZ.h:157 */
+typedef struct _n_params Nparams;
+struct _n_params {
+    int    nit, nit_max, stagnated;
+    double abs_tol_u, rel_tol_u, abs_tol_p, rel_tol_p;
+    double resx, resz, resp, rest;
+    double resx0, resz0, resp0;
+    double resx_f, resz_f, resp_f;
+    double vrlx, prlx, trlx;
+    int    Picard2Newton, let_res_grow, nit_Pic_max;
+    double Pic2NewtCond;
+};
+
+/* header_MDOODZ.h:170 -- every array is a separate host allocation; index
+ * k + l*row_length, row-major in x.  Centroids (Nx-1)(Nz-1), vertices Nx*Nz,
+ * Vx nodes Nx(Nz+1), Vz nodes (Nx+1)Nz. */
+typedef struct _grid grid;
+struct _grid {
+    int    Nx, Nz, NN, NC;
+    double dx, dz;
+    double *roger_x, *roger_z, *div_u, *div_u_s, *div_u_el, *div_u_pl, *div_u_r, *u_in, *v_in, *p_in,
+           *p_corr, *sxxd, *szzd, *sxz, *exxd, *ezzd, *exz, *VE_s, *VE_n, *sxxd0, *szzd0, *sxz0, *mu_s,
+           *mu_n, *u_adv, *v_adv, *eta_phys_n, *kx, *kz, *Cv, *Qr, *eta_phys_s, *u_start, *v_start,
+           *p_start, *divth0_n, *T0_n;
+    int    *iter_smooth;
+    int    *nb_part_cell, *nb_part_vert;
+    BC      BCu, BCv, BCp, BCp_exp;
+    BCT     BCt, BCg, BCc;
+    double *xg_coord, *zg_coord, *xc_coord, *zc_coord, *xvz_coord, *zvx_coord, *xg_coord0, *zg_coord0,
+           *xg_coord_ext, *zg_coord_ext;
+    double *eta_s, *eta_n, *rho_s, *rho_n;
+    double *X_s, *X_n, *X0_s, *X0_n, *p0_n, *p0_s;
+    double *OverS_n, *OverS_s;
+    double *strain_n, *strain_s;
+    double *u, *v, *p;
+    double *ru, *rv, *rp;
+    double *rhs_u, *rhs_v, *rhs_p, *rhs_t, *gx, *gz;
+    double  p_scale;
+    double *alp, *bet_n, *bet_s, *p_lith, *p_lith0, *dp, *Qrho;
+    double *VxVz, *VzVx;
+    int    *P2N, *P2C;
+    int    *kvx, *lvx, *kvz, *lvz, *kp, *lp, *kn, *ln;
+    double **phase_perc_n, **phase_perc_s;
+    double *sxxd0_s, *szzd0_s, *sxz0_n, *exxd_s, *ezzd_s, *exz_n, *sxz_n;
+    double *rho0_n;
+    double  Ut, Ue, W, *Work, *Uelastic, *Uthermal, *Time, *Short;
+    double *T, *dT, *d_n, *d0_n, *phi_n, *phi0_n;
+    double *eII_el, *eII_pl, *eII_pl_s, *eII_pwl, *eII_exp, *eII_lin, *eII_gbs, *eII_cst;
+    double *eII_pwl_s;
+    double *exx_el, *ezz_el, *exz_el, *exx_diss, *ezz_diss, *exz_diss;
+    int    *comp_cells;
+    double *D11_n, *D12_n, *D13_n, *D14_n;
+    double *D21_n, *D22_n, *D23_n, *D24_n;
+    double *D31_s, *D32_s, *D33_s, *D34_s;
+    double *detadexx_n, *detadezz_n, *detadgxz_n, *detadp_n;
+    double *ddivpdexx_n, *ddivpdezz_n, *ddivpdgxz_n, *ddivpdp_n;
+    double *detadexx_s, *detadezz_s, *detadgxz_s, *detadp_s;
+    double *drhodp_n;
+    double *phi0_s, *d0_s, *T_s, *P_s;
+    double *nx_n, *nz_n, *nx_s, *nz_s, *FS_AR_n, *FS_AR_s, *aniso_factor_n, *aniso_factor_s;
+    double *nx0_n, *nz0_n, *nx0_s, *nz0_s;
+    double *cell_min_z, *cell_max_z, *vert_min_z, *vert_max_z;
+    double *dil_n, *dil_s, *fric_n, *fric_s, *C_n, *C_s;
+    double *exz_n_el, *exz_n_diss, *exz_n_pl, *Wdiss, *Wel, *Wtot;
+    double *kc_x, *kc_z;
+    double *FreeSurfW_s, *FreeSurfW_n;
+};
+
+/* ---------------------------------------------------------------------- */
+/* 1. Drop-in entry points (reference names, reference prototypes).        */
+/*    All run on the GPU; host structs in, host structs out.               */
+/* ---------------------------------------------------------------------- */
+
+/* RheologyDensity.c:44 Viscosity (no prototype in the reference header).
+ * Pointwise local constitutive update of one node and one phase. */
+double Viscosity(int phase, double G, double T, double P, double d, double phi, double X0,
+                 double Exx, double Ezz, double Exz, double Gxx, double Gzz, double Gxz,
+                 double Txx0, double Tzz0, double Txz0, mat_prop *materials, params *model,
+                 scale *scaling, double *txxn, double *tzzn, double *txzn, double *etaVE,
+                 double *VEcoeff, double *Eii_el, double *Eii_pl, double *Eii_pwl, double *Eii_exp,
+                 double *Eii_lin, double *Eii_gbs, double *Eii_cst, double *Exx_el, double *Ezz_el,
+                 double *Exz_el, double *Exx_diss, double *Ezz_diss, double *Exz_diss, double *d1,
+                 double strain_acc, double dil, double fric, double C, double *detadexx,
+                 double *detadezz, double *detadexz, double *detadp, double P0, double *X1,
+                 double *OverS, double *ddivpdexx, double *ddivpdezz, double *ddivpdexz,
+                 double *ddivpdp, double *Pcorr, double *drhodp, double *rho, double beta,
+                 double div, double *div_el, double *div_pl, double *div_r);
+
+/* header_MDOODZ.h:416 -> RheologyDensity.c:961 */
+void NonNewtonianViscosityGrid(grid *mesh, mat_prop *materials, params *model, Nparams Nmodel,
+                               scale *scaling);
+/* header_MDOODZ.h:417 -> RheologyDensity.c:1902 */
+void StrainRateComponents(grid *mesh, scale scaling, params *model);
+/* header_MDOODZ.h:580 -> RheologyDensity.c:713 */
+void RheologicalOperators(grid *mesh, params *model, scale *scaling, int Jacobian);
+/* header_MDOODZ.h:351 -> StokesRoutines.c:922 */
+void EvaluateRHS(grid *mesh, params model, scale scaling, double RHO_REF);
+/* header_MDOODZ.h:414 -> StokesRoutines.c:137 */
+void UpdateNonLinearity(grid *mesh, markers *particles, markers *topo_chain, surface *topo,
+                        mat_prop materials, params *model, Nparams *Nmodel, scale scaling, int mode,
+                        double h_contin);
+/* header_MDOODZ.h:316 -> GridRoutines.c:106 */
+void InterpCentroidsToVerticesDouble(double *CentroidArray, double *VertexArray, grid *mesh,
+                                     params *model);
+/* header_MDOODZ.h:393 -> StokesRoutines.c:1134 */
+void EvalNumberOfEquations(grid *mesh, SparseMat *Stokes);
+/* header_MDOODZ.h:474 -> StokesAssemblyDecoupled.c:1573 */
+void BuildStokesOperatorDecoupled(grid *mesh, params model, int lev, double *p_corr, double *p,
+                                  double *u, double *v, SparseMat *Stokes, SparseMat *StokesA,
+                                  SparseMat *StokesB, SparseMat *StokesC, SparseMat *StokesD,
+                                  int Assemble);
+/* header_MDOODZ.h:491 -> StokesAssemblyDecoupled.c:3834 */
+void BuildJacobianOperatorDecoupled(grid *mesh, params model, int lev, double *p_corr, double *p,
+                                    double *u, double *v, SparseMat *Stokes, SparseMat *StokesA,
+                                    SparseMat *StokesB, SparseMat *StokesC, SparseMat *StokesD,
+                                    int Assemble);
+/* header_MDOODZ.h:473 -> StokesRoutines.c:839 */
+void EvaluateStokesResidualDecoupled(SparseMat *Stokes, SparseMat *StokesA, SparseMat *StokesB,
+                                     SparseMat *StokesC, SparseMat *StokesD, Nparams *Nmodel,
+                                     grid *mesh, params model, scale scaling, int quiet);
+/* header_MDOODZ.h:477 -> StokesAssemblyCoupled.c:47.  Host bookkeeping
+ * helper of the assembly (append a coefficient or fold a Dirichlet
+ * neighbour into the right-hand side); exported because the reference
+ * exports it. */
+void AddCoeff2(int *J, double *A, int eqn, int jeq, int *nnzc, double coeff, int NODE_TYPE,
+               double NODE_VAL, double *RHS);
+
+/* ---------------------------------------------------------------------- */
+/* 2. Device-resident interface                                             */
+/* ---------------------------------------------------------------------- */
+
+/* Error handling: every mdz_* call returns 0 on success or a negative
+ * MDZ_E* code; mdz_last_error() gives the message.  The drop-in entry
+ * points above mirror the reference's behaviour instead (message on stdout
+ * then exit(code)), see mdz_set_exit_on_error(). */
+enum {
+    MDZ_OK = 0,
+    MDZ_EINVAL = -1,      /* bad argument / unsupported option */
+    MDZ_EHIP = -2,        /* HIP runtime failure */
+    MDZ_ENOMEM = -3,      /* device or host allocation failed */
+    MDZ_ENAN = -4,        /* NaN residual (reference exit code 122) */
+    MDZ_ED34 = -5,        /* NaN/Inf in D34 (reference exit code 1) */
+    MDZ_EUNSUPPORTED = -6 /* reference feature outside this implementation */
+};
+const char *mdz_last_error(void);
+/* 1 (default): the drop-in entry points print the reference's message and
+ * call exit() with the reference's code.  0: they only record the error,
+ * readable with mdz_last_error()/mdz_last_status(). */
+void mdz_set_exit_on_error(int on);
+int  mdz_last_status(void);
+
+/* Rheology arithmetic of the local iterations. */
+enum {
+    MDZ_RHEO_FP64 = 0,      /* plain: reference initial guess, FP64 throughout */
+    MDZ_RHEO_MIXED = 1,     /* FP32 iterations + FP64 polish to the reference tolerance */
+    MDZ_RHEO_LEARNED = 2    /* MIXED with the surrogate network's initial guess */
+};
+
+typedef struct mdz_ctx mdz_ctx;
+
+/* Creates a device mirror of *mesh (all fields the path reads or writes),
+ * uploads every input field, numbers the equations and builds the Picard
+ * and Jacobian CSR patterns from the boundary tags. */
+int  mdz_ctx_create(mdz_ctx **out, const grid *mesh, const mat_prop *materials,
+                    const params *model, const scale *scaling);
+void mdz_ctx_destroy(mdz_ctx *ctx);
+/* Re-upload the per-time-step inputs (everything but u_in/v_in/p_in). */
+int  mdz_ctx_upload_step_fields(mdz_ctx *ctx, const grid *mesh);
+/* Upload / download the iterate u_in, v_in, p_in. */
+int  mdz_ctx_upload_solution(mdz_ctx *ctx, const double *u_in, const double *v_in, const double *p_in);
+int  mdz_ctx_download_solution(mdz_ctx *ctx, double *u_in, double *v_in, double *p_in);
+/* Copy every output field of the path back into *mesh. */
+int  mdz_ctx_download_grid(mdz_ctx *ctx, grid *mesh);
+int  mdz_ctx_set_rheology_mode(mdz_ctx *ctx, int mode);
+/* Loads surrogate-network weights (layout documented in mdz_mlp_*). */
+int  mdz_ctx_set_network(mdz_ctx *ctx, const float *weights, size_t n_floats);
+
+/* One Newton-iteration unit, all device-resident, on the context's stream:
+ * StrainRateComponents -> NonNewtonianViscosityGrid -> EvaluateRHS ->
+ * RheologicalOperators(0) -> Picard CSR values -> RheologicalOperators(1) ->
+ * Jacobian CSR values -> RheologicalOperators(0) -> residual and norms.
+ * Norms (resx, resz, resp) are written to res3 (host) when non-NULL; the
+ * call synchronises only in that case. */
+int  mdz_ctx_newton_unit(mdz_ctx *ctx, double *res3);
+
+/* CSR sizes and device pointers of the last assembled blocks
+ * (which: 0..3 = Picard A,B,C,D; 4..7 = Jacobian A,B,C,D). */
+int  mdz_ctx_block_info(mdz_ctx *ctx, int which, int *neq, int *nnz);
+int  mdz_ctx_download_block(mdz_ctx *ctx, int which, int *Ic, int *J, double *A, double *b,
+                            double *bbc);
+/* Residual vectors (F of the momentum and continuity blocks). */
+int  mdz_ctx_download_residual(mdz_ctx *ctx, double *Fmom, double *Fcont);
+/* Local-iteration counts accumulated since the last call (node x phase
+ * evaluations): hist[0..63] = FP64 visco-elastic Newton iterations,
+ * hist[64..127] = plastic return-map iterations, hist[128..191] = FP32
+ * pre-iterations (mixed / learned modes).  Resets the counters. */
+int  mdz_ctx_iteration_histogram(mdz_ctx *ctx, long long *hist192);
+/* Mean per-stage milliseconds over `reps` units, measured with HIP events
+ * on the context's stream: ms[0..8] = strain rates, rheology, right-hand
+ * sides, D(Picard), Picard values, D(Jacobian), Jacobian values,
+ * D(Picard), residual + norms; ms[9] = whole unit. */
+int  mdz_ctx_stage_times(mdz_ctx *ctx, int reps, double *ms10);
+/* Mean milliseconds per unit over `reps` back-to-back units. */
+int  mdz_ctx_time_units(mdz_ctx *ctx, int reps, double *ms_per_unit);
+/* The HIP stream (hipStream_t) the context launches on. */
+void *mdz_ctx_stream(mdz_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDOODZ_HOT_H */

@@ -1,0 +1,261 @@
+"""Python bindings of the MI355X implementation of the MDOODZ 6.0
+Newton-iteration grid path (ctypes over the C ABI in include/mdoodz_hot.h).
+
+The compute path is libmdz_hot.so (HIP, gfx950).  libmdz_model.so is the host
+set-up library (grid allocation in the reference layout, seeded synthetic
+models).  The ctypes structure classes are generated from the C header, so
+the Python view of the ABI structs cannot drift from the C one.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HEADER = os.path.join(ROOT, "include", "mdoodz_hot.h")
+LIB_HOT = os.path.join(HERE, "libmdz_hot.so")
+LIB_MODEL = os.path.join(HERE, "libmdz_model.so")
+
+MDZ_RHEO_FP64, MDZ_RHEO_MIXED, MDZ_RHEO_LEARNED = 0, 1, 2
+BC_PURE_SHEAR, BC_SIMPLE_SHEAR, BC_STRESS_WE, BC_FREE_SURFACE, BC_NEUMANN_TOP = 0, 1, 2, 3, 4
+
+
+# --------------------------------------------------------------------------
+# ctypes structures from the header
+# --------------------------------------------------------------------------
+_CTYPE = {"double": C.c_double, "DoodzFP": C.c_double, "int": C.c_int, "char": C.c_char}
+_STRUCTS = {"_scale": "scale", "_mat_prop": "mat_prop", "_BC": "BC", "_BCT": "BCT",
+            "_SparseMat": "SparseMat", "_params": "params", "_n_params": "Nparams", "_grid": "grid"}
+
+
+def _parse_struct(text, tag):
+    m = re.search(r"struct\s+%s\s*\{(.*?)\};" % re.escape(tag), text, re.S)
+    body = re.sub(r"/\*.*?\*/", "", m.group(1), flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = " ".join(decl.split())
+        if not decl:
+            continue
+        typ, rest = decl.split(None, 1)
+        for item in rest.split(","):
+            item = item.strip()
+            stars = len(item) - len(item.lstrip("*"))
+            name = item.lstrip("*").strip()
+            arr = re.search(r"\[(\d+)\]", name)
+            name = re.sub(r"\[.*\]", "", name).strip()
+            fields.append((name, typ, stars, int(arr.group(1)) if arr else 0))
+    return fields
+
+
+def _build_structs():
+    with open(HEADER) as f:
+        text = f.read()
+    classes = {}
+    for tag, tname in _STRUCTS.items():
+        cls = type(tname, (C.Structure,), {})
+        classes[tname] = cls
+    for tag, tname in _STRUCTS.items():
+        flds = []
+        for name, typ, stars, arr in _parse_struct(text, tag):
+            base = classes.get(typ) or _CTYPE[typ]
+            t = C.c_byte if (base is C.c_char and stars) else base
+            for _ in range(stars):
+                t = C.POINTER(t)
+            if arr:
+                t = t * arr
+            flds.append((name, t))
+        classes[tname]._fields_ = flds
+    return classes
+
+
+_S = _build_structs()
+scale, mat_prop, BC, BCT = _S["scale"], _S["mat_prop"], _S["BC"], _S["BCT"]
+SparseMat, params, Nparams, grid = _S["SparseMat"], _S["params"], _S["Nparams"], _S["grid"]
+
+
+class mdz_model(C.Structure):
+    _fields_ = [("mesh", C.POINTER(grid)), ("model", params), ("materials", mat_prop), ("scaling", scale),
+                ("Nmodel", Nparams), ("nph", C.c_int), ("bc_family", C.c_int), ("seed", C.c_ulonglong)]
+
+
+# --------------------------------------------------------------------------
+# libraries
+# --------------------------------------------------------------------------
+_libs = {}
+
+
+def lib_model():
+    if "model" not in _libs:
+        if not os.path.exists(LIB_MODEL):
+            raise RuntimeError("libmdz_model.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_MODEL)
+        L.mdz_model_inclusion.restype = C.POINTER(mdz_model)
+        L.mdz_model_inclusion.argtypes = [C.c_int, C.c_int, C.c_ulonglong]
+        L.mdz_model_lithosphere.restype = C.POINTER(mdz_model)
+        L.mdz_model_lithosphere.argtypes = [C.c_int, C.c_int, C.c_ulonglong, C.c_double, C.c_double, C.c_double]
+        L.mdz_model_edgecase.restype = C.POINTER(mdz_model)
+        L.mdz_model_edgecase.argtypes = [C.c_int, C.c_int, C.c_ulonglong, C.c_int, C.c_int]
+        L.mdz_model_clone.restype = C.POINTER(mdz_model)
+        L.mdz_model_clone.argtypes = [C.POINTER(mdz_model)]
+        L.mdz_model_free.argtypes = [C.POINTER(mdz_model)]
+        L.mdz_grid_dfield.restype = C.POINTER(C.c_double)
+        L.mdz_grid_dfield.argtypes = [C.POINTER(grid), C.c_char_p, C.POINTER(C.c_int)]
+        L.mdz_grid_tags.restype = C.POINTER(C.c_byte)
+        L.mdz_grid_tags.argtypes = [C.POINTER(grid), C.c_char_p, C.POINTER(C.c_int)]
+        L.mdz_grid_ifield.restype = C.POINTER(C.c_int)
+        L.mdz_grid_ifield.argtypes = [C.POINTER(grid), C.c_char_p, C.POINTER(C.c_int)]
+        L.mdz_grid_nfields.restype = C.c_int
+        L.mdz_grid_field_name.restype = C.c_char_p
+        L.mdz_grid_field_name.argtypes = [C.c_int]
+        L.mdz_grid_field_family.argtypes = [C.c_int]
+        L.mdz_grid_field_role.argtypes = [C.c_int]
+        L.mdz_sizeof.restype = C.c_size_t
+        L.mdz_sizeof.argtypes = [C.c_char_p]
+        L.mdz_offsetof.restype = C.c_long
+        L.mdz_offsetof.argtypes = [C.c_char_p, C.c_char_p]
+        _libs["model"] = L
+    return _libs["model"]
+
+
+HOT_SYMBOLS = ["Viscosity", "NonNewtonianViscosityGrid", "StrainRateComponents", "RheologicalOperators",
+               "EvaluateRHS", "UpdateNonLinearity", "InterpCentroidsToVerticesDouble", "EvalNumberOfEquations",
+               "BuildStokesOperatorDecoupled", "BuildJacobianOperatorDecoupled", "EvaluateStokesResidualDecoupled",
+               "AddCoeff2", "mdz_last_error", "mdz_set_exit_on_error", "mdz_last_status", "mdz_ctx_create",
+               "mdz_ctx_destroy", "mdz_ctx_upload_step_fields", "mdz_ctx_upload_solution",
+               "mdz_ctx_download_solution", "mdz_ctx_download_grid", "mdz_ctx_set_rheology_mode",
+               "mdz_ctx_set_network", "mdz_ctx_newton_unit", "mdz_ctx_block_info", "mdz_ctx_download_block",
+               "mdz_ctx_download_residual", "mdz_ctx_iteration_histogram", "mdz_ctx_stream",
+               "mdz_ctx_stage_times", "mdz_ctx_time_units"]
+
+
+def lib_hot():
+    """The HIP library.  Raises when it is not built: there is no CPU path."""
+    if "hot" not in _libs:
+        if not os.path.exists(LIB_HOT):
+            raise RuntimeError("libmdz_hot.so (HIP, gfx950) is not built; there is no CPU fallback")
+        L = C.CDLL(LIB_HOT)
+        G, SM, NP, SC, PR, MP = C.POINTER(grid), C.POINTER(SparseMat), Nparams, scale, params, mat_prop
+        L.mdz_last_error.restype = C.c_char_p
+        L.NonNewtonianViscosityGrid.argtypes = [G, C.POINTER(MP), C.POINTER(PR), NP, C.POINTER(SC)]
+        L.StrainRateComponents.argtypes = [G, SC, C.POINTER(PR)]
+        L.RheologicalOperators.argtypes = [G, C.POINTER(PR), C.POINTER(SC), C.c_int]
+        L.EvaluateRHS.argtypes = [G, PR, SC, C.c_double]
+        L.UpdateNonLinearity.argtypes = [G, C.c_void_p, C.c_void_p, C.c_void_p, MP, C.POINTER(PR), C.POINTER(NP),
+                                         SC, C.c_int, C.c_double]
+        L.InterpCentroidsToVerticesDouble.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), G, C.POINTER(PR)]
+        L.EvalNumberOfEquations.argtypes = [G, SM]
+        dp = C.POINTER(C.c_double)
+        L.BuildStokesOperatorDecoupled.argtypes = [G, PR, C.c_int, dp, dp, dp, dp, SM, SM, SM, SM, SM, C.c_int]
+        L.BuildJacobianOperatorDecoupled.argtypes = [G, PR, C.c_int, dp, dp, dp, dp, SM, SM, SM, SM, SM, C.c_int]
+        L.EvaluateStokesResidualDecoupled.argtypes = [SM, SM, SM, SM, SM, C.POINTER(NP), G, PR, SC, C.c_int]
+        L.mdz_ctx_create.argtypes = [C.POINTER(C.c_void_p), G, C.POINTER(MP), C.POINTER(PR), C.POINTER(SC)]
+        L.mdz_ctx_destroy.argtypes = [C.c_void_p]
+        L.mdz_ctx_newton_unit.argtypes = [C.c_void_p, dp]
+        L.mdz_ctx_block_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.mdz_ctx_download_block.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), dp, dp, dp]
+        L.mdz_ctx_download_grid.argtypes = [C.c_void_p, G]
+        L.mdz_ctx_download_residual.argtypes = [C.c_void_p, dp, dp]
+        L.mdz_ctx_upload_solution.argtypes = [C.c_void_p, dp, dp, dp]
+        L.mdz_ctx_download_solution.argtypes = [C.c_void_p, dp, dp, dp]
+        L.mdz_ctx_set_rheology_mode.argtypes = [C.c_void_p, C.c_int]
+        L.mdz_ctx_set_network.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_size_t]
+        L.mdz_ctx_iteration_histogram.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+        L.mdz_ctx_stage_times.argtypes = [C.c_void_p, C.c_int, dp]
+        L.mdz_ctx_time_units.argtypes = [C.c_void_p, C.c_int, dp]
+        L.mdz_ctx_stream.restype = C.c_void_p
+        L.mdz_ctx_stream.argtypes = [C.c_void_p]
+        L.mdz_set_exit_on_error(0)
+        _libs["hot"] = L
+    return _libs["hot"]
+
+
+def check(rc, L=None):
+    if rc != 0:
+        L = L or lib_hot()
+        raise RuntimeError("mdz error %d: %s" % (rc, L.mdz_last_error().decode()))
+
+
+# --------------------------------------------------------------------------
+# model wrapper
+# --------------------------------------------------------------------------
+class Model:
+    """A seeded synthetic model (grid + params + materials + scales)."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+        self.m = ptr.contents
+
+    @classmethod
+    def inclusion(cls, ncx=101, ncz=101, seed=1):
+        return cls(lib_model().mdz_model_inclusion(ncx, ncz, seed))
+
+    @classmethod
+    def lithosphere(cls, ncx=1001, ncz=501, seed=1, friction_deg=-1.0, cohesion=-1.0, rate=-1.0):
+        return cls(lib_model().mdz_model_lithosphere(ncx, ncz, seed, friction_deg, cohesion, rate))
+
+    @classmethod
+    def edgecase(cls, ncx, ncz, seed, bc_family, materials):
+        return cls(lib_model().mdz_model_edgecase(ncx, ncz, seed, bc_family, materials))
+
+    def clone(self):
+        return Model(lib_model().mdz_model_clone(self.ptr))
+
+    def free(self):
+        if self.ptr:
+            lib_model().mdz_model_free(self.ptr)
+            self.ptr = None
+
+    @property
+    def mesh(self):
+        return self.m.mesh
+
+    @property
+    def Nx(self):
+        return self.m.mesh.contents.Nx
+
+    @property
+    def Nz(self):
+        return self.m.mesh.contents.Nz
+
+    def field(self, name):
+        n = C.c_int(0)
+        p = lib_model().mdz_grid_dfield(self.m.mesh, name.encode(), C.byref(n))
+        if not p:
+            raise KeyError(name)
+        return np.ctypeslib.as_array(p, shape=(n.value,))
+
+    def tags(self, name):
+        n = C.c_int(0)
+        p = lib_model().mdz_grid_tags(self.m.mesh, name.encode(), C.byref(n))
+        return np.ctypeslib.as_array(p, shape=(n.value,))
+
+    def ifield(self, name):
+        n = C.c_int(0)
+        p = lib_model().mdz_grid_ifield(self.m.mesh, name.encode(), C.byref(n))
+        return np.ctypeslib.as_array(p, shape=(n.value,))
+
+
+def field_names(role_mask=3):
+    L = lib_model()
+    out = []
+    for i in range(L.mdz_grid_nfields()):
+        if L.mdz_grid_field_role(i) & role_mask:
+            out.append(L.mdz_grid_field_name(i).decode())
+    return out
+
+
+def new_sparsemat():
+    return SparseMat()
+
+
+def csr_arrays(S):
+    """numpy copies of (Ic, J, A, bbc, b) of a SparseMat filled by a builder."""
+    neq, nnz = S.neq, S.nnz
+    Ic = np.ctypeslib.as_array(S.Ic, shape=(neq + 1,)).copy()
+    J = np.ctypeslib.as_array(S.J, shape=(nnz,)).copy() if nnz else np.zeros(0, np.int32)
+    A = np.ctypeslib.as_array(S.A, shape=(nnz,)).copy() if nnz else np.zeros(0)
+    bbc = np.ctypeslib.as_array(S.bbc, shape=(neq,)).copy() if S.bbc else None
+    b = np.ctypeslib.as_array(S.b, shape=(neq,)).copy() if S.b else None
+    return Ic, J, A, bbc, b

@@ -1,0 +1,1086 @@
+/*
+ * mdz_api.hip -- C ABI of libmdz_hot.so: the reference-named drop-in entry
+ * points and the device-resident interface (include/mdoodz_hot.h).
+ *
+ * Drop-in entry points keep the reference's contract: host structs in,
+ * host structs out, CSR arrays returned in malloc() buffers the caller
+ * frees, errors reported by printing the reference's message and calling
+ * exit() with the reference's code (switchable with mdz_set_exit_on_error).
+ * Each call uploads the fields it reads and downloads the fields it writes;
+ * the device mirror of a host grid is kept between calls (keyed by the grid
+ * pointer and its dimensions) so that allocations are not repeated.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "mdz_kernels.h"
+#include "mdz_viscosity.h"
+
+/* ------------------------------------------------------------------ */
+/* errors                                                               */
+/* ------------------------------------------------------------------ */
+static char g_err[512];
+static int g_status = 0;
+static int g_exit_on_error = 1;
+
+extern "C" const char *mdz_last_error(void) { return g_err; }
+extern "C" void mdz_set_exit_on_error(int on) { g_exit_on_error = on; }
+extern "C" int mdz_last_status(void) { return g_status; }
+
+static int set_err(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    g_status = code;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t _e = (x);                                                                       \
+        if (_e != hipSuccess) return set_err(MDZ_EHIP, "%s: %s", #x, hipGetErrorString(_e));      \
+    } while (0)
+
+/* Drop-in failure: the reference prints and exits; kernels cannot, so the
+ * host wrapper reproduces its message and exit code. */
+static void dropin_fail(int code, int exit_code, const char *msg) {
+    if (msg) printf("%s", msg);
+    if (g_exit_on_error) {
+        fflush(stdout);
+        exit(exit_code);
+    }
+    if (!g_status) set_err(code, "%s", msg ? msg : "error");
+}
+
+static void dropin_check(int rc) {
+    if (rc != MDZ_OK) {
+        fprintf(stderr, "mdz: %s\n", g_err);
+        if (g_exit_on_error) exit(1);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* field table                                                          */
+/* ------------------------------------------------------------------ */
+enum {
+#define MDZ_FENUM(name, fam, role) FI_##name,
+    MDZ_ALL_DOUBLE_FIELDS(MDZ_FENUM)
+#undef MDZ_FENUM
+    FI_COUNT
+};
+
+struct FieldRef {
+    size_t host_off, dev_off;
+    int fam, role;
+};
+static const FieldRef FREF[] = {
+#define MDZ_FREF(name, fam, role) {offsetof(grid, name), offsetof(DevGrid, name), MDZ_FAM_##fam, MDZ_ROLE_##role},
+    MDZ_ALL_DOUBLE_FIELDS(MDZ_FREF)
+#undef MDZ_FREF
+};
+
+static size_t fam_len(int fam, int Nx, int Nz) {
+    switch (fam) {
+    case MDZ_FAM_C: return (size_t)(Nx - 1) * (Nz - 1);
+    case MDZ_FAM_V: return (size_t)Nx * Nz;
+    case MDZ_FAM_U: return (size_t)Nx * (Nz + 1);
+    default: return (size_t)(Nx + 1) * Nz;
+    }
+}
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+/* ------------------------------------------------------------------ */
+/* context                                                              */
+/* ------------------------------------------------------------------ */
+struct mdz_ctx {
+    int Nx, Nz, nph;
+    hipStream_t stream;
+    char *slab;          /* field arrays, tags, phases, coords, numbering */
+    size_t slab_bytes;
+    DevGrid g;
+    DevParams prm;
+    mat_prop *d_mat;
+    double *xg, *zg, *xvz, *zvx;
+    DevEqn e;
+    DevBlock blk[8];     /* 0..3 Picard A, B, C, D; 4..7 Jacobian A, B, C, D */
+    size_t blk_cap[8];   /* allocated nnz capacity */
+    int pattern_ready[2];
+    int *scratch;
+    size_t scratch_len;
+    double *norm_partial, *norm_out;
+    int *status;
+    unsigned long long *hist;
+    float *guess_n, *guess_s;
+    double *tmp_c, *tmp_v;  /* staging for InterpCentroidsToVerticesDouble */
+    params model;
+    mat_prop materials;
+    scale scaling;
+    /* surrogate network (set by mdz_ctx_set_network) */
+    float *d_net;
+    size_t net_floats;
+};
+
+static void fill_params(mdz_ctx *c, const params *m, const scale *s) {
+    DevParams &p = c->prm;
+    p.dt = m->dt; p.mineta = m->mineta; p.maxeta = m->maxeta; p.gx = m->gx; p.gz = m->gz;
+    p.free_surf_stab = m->free_surf_stab; p.compressible = m->compressible;
+    p.sT = s->T; p.sE = s->E; p.seta = s->eta; p.sS = s->S; p.sL = s->L;
+    p.step = m->step; p.iselastic = m->iselastic; p.eta_avg = m->eta_avg; p.aniso = m->aniso;
+    p.aniso_fstrain = m->aniso_fstrain; p.isperiodic_x = m->isperiodic_x; p.UnsplitDiffReac = m->UnsplitDiffReac;
+    p.diffuse_X = m->diffuse_X; p.diffuse_avg = m->diffuse_avg; p.polar = m->polar;
+    p.VolChangeReac = m->VolChangeReac; p.adiab_heat = m->adiab_heat; p.Nb_phases = m->Nb_phases;
+    p.Nx = m->Nx; p.Nz = m->Nz;
+    c->model = *m;
+    c->scaling = *s;
+}
+
+static int check_supported(const params *m, const mat_prop *mt) {
+    if (m->VolChangeReac == 1)
+        return set_err(MDZ_EUNSUPPORTED, "VolChangeReac = 1 (reaction volume changes) is not supported");
+    if (mt) {
+        for (int p = 0; p < m->Nb_phases; p++)
+            if (mt->reac_soft[p] == 1)
+                return set_err(MDZ_EUNSUPPORTED, "reac_soft = 1 (progressive reactions) is not supported (phase %d)", p);
+    }
+    if (m->Nb_phases < 1 || m->Nb_phases > 20) return set_err(MDZ_EINVAL, "Nb_phases = %d", m->Nb_phases);
+    return MDZ_OK;
+}
+
+static DevAsm make_asm(const mdz_ctx *c) {
+    DevAsm a;
+    memset(&a, 0, sizeof(a));
+    a.g = c->g;
+    a.e = c->e;
+    double dx = c->g.dx, dz = c->g.dz;
+    a.celvol = dx * dz;
+    a.one_dx = 1.0 / dx; a.one_dz = 1.0 / dz;
+    a.one_dx_dx = 1.0 / dx / dx; a.one_dz_dz = 1.0 / dz / dz; a.one_dx_dz = 1.0 / dx / dz;
+    a.om = c->model.free_surf_stab; a.dt = c->model.dt; a.gz_model = c->model.gz;
+    a.stab = c->model.free_surf_stab > 0 ? 1 : 0;
+    a.comp_model = c->model.compressible == 1 ? 1 : 0;
+    a.periodic = c->model.isperiodic_x;
+    a.Nz_model = c->model.Nz;
+    a.u = c->g.u_in; a.v = c->g.v_in; a.p_mom = c->g.p_corr; a.p_cont = c->g.p_in;
+    return a;
+}
+
+static void ctx_free_blocks(mdz_ctx *c) {
+    for (int i = 0; i < 8; i++) {
+        hipFree(c->blk[i].Ic); hipFree(c->blk[i].J); hipFree(c->blk[i].A);
+        hipFree(c->blk[i].b); hipFree(c->blk[i].bbc); hipFree(c->blk[i].F); hipFree(c->blk[i].d);
+        memset(&c->blk[i], 0, sizeof(DevBlock));
+        c->blk_cap[i] = 0;
+    }
+    c->pattern_ready[0] = c->pattern_ready[1] = 0;
+}
+
+/* Allocates the device mirror for an Nx x Nz grid with nph phases. */
+static int ctx_alloc(mdz_ctx **out, int Nx, int Nz, int nph) {
+    if (Nx < 4 || Nz < 4) return set_err(MDZ_EINVAL, "grid too small: %d x %d", Nx, Nz);
+    mdz_ctx *c = (mdz_ctx *)calloc(1, sizeof(mdz_ctx));
+    if (!c) return set_err(MDZ_ENOMEM, "host allocation");
+    c->Nx = Nx; c->Nz = Nz; c->nph = nph;
+    size_t nc = fam_len(MDZ_FAM_C, Nx, Nz), nv = fam_len(MDZ_FAM_V, Nx, Nz);
+    size_t nu = fam_len(MDZ_FAM_U, Nx, Nz), nw = fam_len(MDZ_FAM_W, Nx, Nz);
+    /* slab layout */
+    size_t off = 0;
+    std::vector<size_t> foff(FI_COUNT);
+    for (int i = 0; i < FI_COUNT; i++) { foff[i] = off; off += align256(fam_len(FREF[i].fam, Nx, Nz) * sizeof(double)); }
+    size_t o_uval = off; off += align256(nu * 8);
+    size_t o_vval = off; off += align256(nw * 8);
+    size_t o_pval = off; off += align256(nc * 8);
+    size_t o_tu = off; off += align256(nu);
+    size_t o_tv = off; off += align256(nw);
+    size_t o_tp = off; off += align256(nc);
+    size_t o_tg = off; off += align256(nv);
+    size_t o_comp = off; off += align256(nc * 4);
+    size_t o_phn = off; off += align256(nc * nph * 8);
+    size_t o_phs = off; off += align256(nv * nph * 8);
+    size_t o_xg = off; off += align256((size_t)Nx * 8);
+    size_t o_zg = off; off += align256((size_t)Nz * 8);
+    size_t o_xvz = off; off += align256((size_t)(Nx + 1) * 8);
+    size_t o_zvx = off; off += align256((size_t)(Nz + 1) * 8);
+    size_t o_eu = off; off += align256(nu * 4);
+    size_t o_ev = off; off += align256(nw * 4);
+    size_t o_ep = off; off += align256(nc * 4);
+    size_t o_mn = off; off += align256((nu + nw) * 4);
+    size_t o_cn = off; off += align256(nc * 4);
+    size_t o_tc = off; off += align256(nc * 8);
+    size_t o_tvv = off; off += align256(nv * 8);
+    c->slab_bytes = off;
+    if (hipMalloc((void **)&c->slab, off) != hipSuccess) { free(c); return set_err(MDZ_ENOMEM, "device slab of %zu bytes", off); }
+    hipMemset(c->slab, 0, off);
+    DevGrid &g = c->g;
+    g.Nx = Nx; g.Nz = Nz; g.ncx = Nx - 1; g.ncz = Nz - 1; g.nxvz = Nx + 1; g.nzvx = Nz + 1; g.nph = nph;
+    for (int i = 0; i < FI_COUNT; i++) *(double **)((char *)&g + FREF[i].dev_off) = (double *)(c->slab + foff[i]);
+    g.BCu_val = (double *)(c->slab + o_uval); g.BCv_val = (double *)(c->slab + o_vval); g.BCp_val = (double *)(c->slab + o_pval);
+    g.BCu = (int8_t *)(c->slab + o_tu); g.BCv = (int8_t *)(c->slab + o_tv); g.BCp = (int8_t *)(c->slab + o_tp);
+    g.BCg = (int8_t *)(c->slab + o_tg);
+    g.comp_cells = (int *)(c->slab + o_comp);
+    g.phase_n = (double *)(c->slab + o_phn); g.phase_s = (double *)(c->slab + o_phs);
+    c->xg = (double *)(c->slab + o_xg); c->zg = (double *)(c->slab + o_zg);
+    c->xvz = (double *)(c->slab + o_xvz); c->zvx = (double *)(c->slab + o_zvx);
+    c->e.eqn_u = (int *)(c->slab + o_eu); c->e.eqn_v = (int *)(c->slab + o_ev); c->e.eqn_p = (int *)(c->slab + o_ep);
+    c->e.mom_node = (int *)(c->slab + o_mn); c->e.cont_node = (int *)(c->slab + o_cn);
+    c->tmp_c = (double *)(c->slab + o_tc); c->tmp_v = (double *)(c->slab + o_tvv);
+    c->scratch_len = (nu + nw + nc) + 4096 + (nu + nw + nc) / 512;
+    if (hipMalloc((void **)&c->scratch, c->scratch_len * sizeof(int)) != hipSuccess) return set_err(MDZ_ENOMEM, "scratch");
+    size_t np = 0;
+    {
+        int nbu = mdz_div_up((int)nu, 256), nbv = mdz_div_up((int)nw, 256), nbp = mdz_div_up((int)nc, 256);
+        np = 2 * (size_t)(nbu + nbv + nbp);
+    }
+    if (hipMalloc((void **)&c->norm_partial, np * sizeof(double) + 256) != hipSuccess) return set_err(MDZ_ENOMEM, "norms");
+    if (hipMalloc((void **)&c->norm_out, 8 * sizeof(double)) != hipSuccess) return set_err(MDZ_ENOMEM, "norms");
+    if (hipMalloc((void **)&c->status, 4 * sizeof(int)) != hipSuccess) return set_err(MDZ_ENOMEM, "status");
+    if (hipMalloc((void **)&c->hist, 3 * MDZ_HIST_BINS * sizeof(unsigned long long)) != hipSuccess) return set_err(MDZ_ENOMEM, "hist");
+    if (hipMalloc((void **)&c->d_mat, sizeof(mat_prop)) != hipSuccess) return set_err(MDZ_ENOMEM, "materials");
+    hipMemset(c->status, 0, 4 * sizeof(int));
+    hipMemset(c->hist, 0, 3 * MDZ_HIST_BINS * sizeof(unsigned long long));
+    if (hipStreamCreate(&c->stream) != hipSuccess) return set_err(MDZ_EHIP, "stream");
+    *out = c;
+    return MDZ_OK;
+}
+
+static void ctx_release(mdz_ctx *c) {
+    if (!c) return;
+    hipStreamSynchronize(c->stream);
+    ctx_free_blocks(c);
+    hipFree(c->slab); hipFree(c->scratch); hipFree(c->norm_partial); hipFree(c->norm_out);
+    hipFree(c->status); hipFree(c->hist); hipFree(c->d_mat); hipFree(c->guess_n); hipFree(c->guess_s);
+    hipFree(c->d_net);
+    hipStreamDestroy(c->stream);
+    free(c);
+}
+
+/* ---- transfers ---- */
+static int up(mdz_ctx *c, const grid *m, int fi) {
+    const double *h = *(double *const *)((const char *)m + FREF[fi].host_off);
+    double *d = *(double **)((char *)&c->g + FREF[fi].dev_off);
+    if (!h) return set_err(MDZ_EINVAL, "host field %d is NULL", fi);
+    HIPCHK(hipMemcpyAsync(d, h, fam_len(FREF[fi].fam, c->Nx, c->Nz) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return MDZ_OK;
+}
+static int down(mdz_ctx *c, grid *m, int fi) {
+    double *h = *(double **)((char *)m + FREF[fi].host_off);
+    const double *d = *(double *const *)((const char *)&c->g + FREF[fi].dev_off);
+    if (!h) return set_err(MDZ_EINVAL, "host field %d is NULL", fi);
+    HIPCHK(hipMemcpyAsync(h, d, fam_len(FREF[fi].fam, c->Nx, c->Nz) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return MDZ_OK;
+}
+static int up_list(mdz_ctx *c, const grid *m, std::initializer_list<int> l) {
+    for (int fi : l) { int rc = up(c, m, fi); if (rc) return rc; }
+    return MDZ_OK;
+}
+static int down_list(mdz_ctx *c, grid *m, std::initializer_list<int> l) {
+    for (int fi : l) { int rc = down(c, m, fi); if (rc) return rc; }
+    return MDZ_OK;
+}
+
+static int up_tags(mdz_ctx *c, const grid *m) {
+    size_t nc = fam_len(MDZ_FAM_C, c->Nx, c->Nz), nv = fam_len(MDZ_FAM_V, c->Nx, c->Nz);
+    size_t nu = fam_len(MDZ_FAM_U, c->Nx, c->Nz), nw = fam_len(MDZ_FAM_W, c->Nx, c->Nz);
+    HIPCHK(hipMemcpyAsync(c->g.BCu, m->BCu.type, nu, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.BCv, m->BCv.type, nw, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.BCp, m->BCp.type, nc, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.BCg, m->BCg.type, nv, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.BCu_val, m->BCu.val, nu * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.BCv_val, m->BCv.val, nw * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.BCp_val, m->BCp.val, nc * 8, hipMemcpyHostToDevice, c->stream));
+    if (m->comp_cells) HIPCHK(hipMemcpyAsync(c->g.comp_cells, m->comp_cells, nc * 4, hipMemcpyHostToDevice, c->stream));
+    return MDZ_OK;
+}
+
+static int up_phases(mdz_ctx *c, const grid *m, int nph) {
+    size_t nc = fam_len(MDZ_FAM_C, c->Nx, c->Nz), nv = fam_len(MDZ_FAM_V, c->Nx, c->Nz);
+    for (int p = 0; p < nph; p++) {
+        HIPCHK(hipMemcpyAsync(c->g.phase_n + p * nc, m->phase_perc_n[p], nc * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->g.phase_s + p * nv, m->phase_perc_s[p], nv * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    return MDZ_OK;
+}
+
+static int up_coords(mdz_ctx *c, const grid *m) {
+    if (m->xg_coord) HIPCHK(hipMemcpyAsync(c->xg, m->xg_coord, c->Nx * 8, hipMemcpyHostToDevice, c->stream));
+    if (m->zg_coord) HIPCHK(hipMemcpyAsync(c->zg, m->zg_coord, c->Nz * 8, hipMemcpyHostToDevice, c->stream));
+    if (m->xvz_coord) HIPCHK(hipMemcpyAsync(c->xvz, m->xvz_coord, (c->Nx + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    if (m->zvx_coord) HIPCHK(hipMemcpyAsync(c->zvx, m->zvx_coord, (c->Nz + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    return MDZ_OK;
+}
+
+static int up_materials(mdz_ctx *c, const mat_prop *mt) {
+    c->materials = *mt;
+    HIPCHK(hipMemcpyAsync(c->d_mat, &c->materials, sizeof(mat_prop), hipMemcpyHostToDevice, c->stream));
+    return MDZ_OK;
+}
+
+static int sync(mdz_ctx *c) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipGetLastError());
+    return MDZ_OK;
+}
+
+/* ---- blocks ---- */
+static int block_alloc_rows(DevBlock *b, int neq) {
+    b->neq = neq;
+    size_t n = (size_t)neq + 1;
+    HIPCHK(hipMalloc((void **)&b->Ic, n * sizeof(int)));
+    HIPCHK(hipMalloc((void **)&b->b, n * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&b->bbc, n * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&b->F, n * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&b->d, n * sizeof(double)));
+    HIPCHK(hipMemset(b->Ic, 0, n * sizeof(int)));
+    HIPCHK(hipMemset(b->bbc, 0, n * sizeof(double)));
+    HIPCHK(hipMemset(b->F, 0, n * sizeof(double)));
+    return MDZ_OK;
+}
+
+__global__ void k_fill(double *p, double v, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+static int ensure_rows(mdz_ctx *c) {
+    int nm = c->e.neq_mom, ncn = c->e.neq_cont;
+    for (int i = 0; i < 8; i++) {
+        int neq = (i % 4) < 2 ? nm : ncn;
+        if (c->blk[i].Ic && c->blk[i].neq == neq) continue;
+        hipFree(c->blk[i].Ic); hipFree(c->blk[i].J); hipFree(c->blk[i].A);
+        hipFree(c->blk[i].b); hipFree(c->blk[i].bbc); hipFree(c->blk[i].F); hipFree(c->blk[i].d);
+        memset(&c->blk[i], 0, sizeof(DevBlock));
+        c->blk_cap[i] = 0;
+        int rc = block_alloc_rows(&c->blk[i], neq);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_fill, dim3(mdz_div_up(neq + 1, 256)), dim3(256), 0, c->stream, c->blk[i].d, 1.0, neq + 1);
+        c->pattern_ready[0] = c->pattern_ready[1] = 0;
+    }
+    return MDZ_OK;
+}
+
+/* builds the Picard (kind 0) or Jacobian (kind 1) pattern into blocks
+ * 4*kind + {0,1,2}; block 4*kind+3 (D) stays empty */
+static int build_pattern(mdz_ctx *c, int kind) {
+    int rc = ensure_rows(c);
+    if (rc) return rc;
+    DevAsm a = make_asm(c);
+    DevBlock *A = &c->blk[4 * kind], *B = &c->blk[4 * kind + 1], *C = &c->blk[4 * kind + 2], *D = &c->blk[4 * kind + 3];
+    HIPCHK(hipMemsetAsync(A->Ic, 0, ((size_t)A->neq + 1) * 4, c->stream));
+    HIPCHK(hipMemsetAsync(B->Ic, 0, ((size_t)B->neq + 1) * 4, c->stream));
+    HIPCHK(hipMemsetAsync(C->Ic, 0, ((size_t)C->neq + 1) * 4, c->stream));
+    HIPCHK(hipMemsetAsync(D->Ic, 0, ((size_t)D->neq + 1) * 4, c->stream));
+    rc = mdz_build_pattern(kind, a, A, B, C, c->scratch, c->scratch_len, c->stream);
+    if (rc) return rc;
+    DevBlock *blks[3] = {A, B, C};
+    for (int q = 0; q < 3; q++) {
+        DevBlock *b = blks[q];
+        int idx = 4 * kind + q;
+        HIPCHK(hipMemcpyAsync(b->Ic + b->neq, &b->nnz, sizeof(int), hipMemcpyHostToDevice, c->stream));
+        if ((size_t)b->nnz > c->blk_cap[idx] || !b->J) {
+            hipFree(b->J); hipFree(b->A);
+            size_t cap = (size_t)b->nnz + 64;
+            HIPCHK(hipMalloc((void **)&b->J, cap * sizeof(int)));
+            HIPCHK(hipMalloc((void **)&b->A, cap * sizeof(double)));
+            c->blk_cap[idx] = cap;
+        }
+    }
+    D->nnz = 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    mdz_fill_pattern(kind, a, A, B, C, c->stream);
+    c->pattern_ready[kind] = 1;
+    return sync(c);
+}
+
+/* ------------------------------------------------------------------ */
+/* drop-in registry                                                     */
+/* ------------------------------------------------------------------ */
+struct RegEntry { const grid *key; mdz_ctx *ctx; };
+static std::vector<RegEntry> g_reg;
+static std::mutex g_reg_mu;
+
+static mdz_ctx *dropin_ctx(const grid *mesh, int nph) {
+    std::lock_guard<std::mutex> lk(g_reg_mu);
+    for (auto &r : g_reg)
+        if (r.key == mesh) {
+            if (r.ctx->Nx == mesh->Nx && r.ctx->Nz == mesh->Nz && r.ctx->nph >= nph) return r.ctx;
+            ctx_release(r.ctx);
+            r.ctx = nullptr;
+            int rc = ctx_alloc(&r.ctx, mesh->Nx, mesh->Nz, nph);
+            if (rc) { dropin_check(rc); return nullptr; }
+            return r.ctx;
+        }
+    mdz_ctx *c = nullptr;
+    int rc = ctx_alloc(&c, mesh->Nx, mesh->Nz, nph);
+    if (rc) { dropin_check(rc); return nullptr; }
+    g_reg.push_back({mesh, c});
+    return c;
+}
+
+static void dropin_geometry(mdz_ctx *c, const grid *mesh) {
+    c->g.dx = mesh->dx;
+    c->g.dz = mesh->dz;
+}
+
+/* ------------------------------------------------------------------ */
+/* drop-in entry points                                                 */
+/* ------------------------------------------------------------------ */
+#define STRAIN_IN FI_u_in, FI_v_in
+#define STRAIN_OUT FI_div_u, FI_exxd, FI_ezzd, FI_exz, FI_exz_n, FI_exxd_s, FI_ezzd_s
+
+extern "C" void StrainRateComponents(grid *mesh, scale scaling, params *model) {
+    mdz_ctx *c = dropin_ctx(mesh, model->Nb_phases > 0 ? model->Nb_phases : 1);
+    if (!c) return;
+    dropin_geometry(c, mesh);
+    fill_params(c, model, &scaling);
+    dropin_check(up_tags(c, mesh));
+    dropin_check(up_list(c, mesh, {STRAIN_IN}));
+    mdz_launch_strain_rates(c->g, c->prm, c->stream);
+    dropin_check(down_list(c, mesh, {STRAIN_OUT}));
+    dropin_check(sync(c));
+}
+
+extern "C" void InterpCentroidsToVerticesDouble(double *CentroidArray, double *VertexArray, grid *mesh, params *model) {
+    mdz_ctx *c = dropin_ctx(mesh, model->Nb_phases > 0 ? model->Nb_phases : 1);
+    if (!c) return;
+    dropin_geometry(c, mesh);
+    size_t nc = fam_len(MDZ_FAM_C, c->Nx, c->Nz), nv = fam_len(MDZ_FAM_V, c->Nx, c->Nz);
+    if (hipMemcpyAsync(c->g.BCg, mesh->BCg.type, nv, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->tmp_c, CentroidArray, nc * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+        dropin_check(set_err(MDZ_EHIP, "upload"));
+        return;
+    }
+    DevParams prm = c->prm;
+    prm.isperiodic_x = model->isperiodic_x;
+    mdz_launch_interp(c->g, prm, c->tmp_c, c->tmp_v, c->stream);
+    if (hipMemcpyAsync(VertexArray, c->tmp_v, nv * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess) {
+        dropin_check(set_err(MDZ_EHIP, "download"));
+        return;
+    }
+    dropin_check(sync(c));
+}
+
+#define RHEO_IN                                                                                                \
+    FI_p_in, FI_T, FI_mu_n, FI_mu_s, FI_sxxd0, FI_szzd0, FI_sxz0, FI_sxz0_n, FI_sxxd0_s, FI_szzd0_s, FI_d0_n,  \
+        FI_phi0_n, FI_X0_n, FI_X0_s, FI_strain_n, FI_strain_s, FI_dil_n, FI_dil_s, FI_fric_n, FI_fric_s, FI_C_n, \
+        FI_C_s, FI_p0_n, FI_p0_s, FI_bet_n, FI_bet_s, FI_nx0_n, FI_nz0_n, FI_nx0_s, FI_nz0_s, FI_aniso_factor_n,  \
+        FI_aniso_factor_s, FI_FS_AR_n, FI_FS_AR_s, FI_X_n, FI_X_s, FI_div_u, FI_exxd, FI_ezzd, FI_exz, FI_exz_n, \
+        FI_exxd_s, FI_ezzd_s
+#define RHEO_OUT                                                                                                   \
+    FI_div_u_s, FI_T_s, FI_P_s, FI_d0_s, FI_phi0_s, FI_eta_n, FI_eta_phys_n, FI_VE_n, FI_sxxd, FI_szzd, FI_sxz_n,    \
+        FI_eII_el, FI_eII_pl, FI_eII_pwl, FI_eII_exp, FI_eII_lin, FI_eII_gbs, FI_eII_cst, FI_d_n, FI_exx_el,          \
+        FI_ezz_el, FI_exx_diss, FI_ezz_diss, FI_detadexx_n, FI_detadezz_n, FI_detadgxz_n, FI_detadp_n,              \
+        FI_ddivpdexx_n, FI_ddivpdezz_n, FI_ddivpdgxz_n, FI_ddivpdp_n, FI_p_corr, FI_div_u_el, FI_div_u_pl,           \
+        FI_div_u_r, FI_Wtot, FI_Wel, FI_Wdiss, FI_X_n, FI_OverS_n, FI_VE_s, FI_sxz, FI_eta_phys_s, FI_eta_s,         \
+        FI_exz_el, FI_exz_diss, FI_detadexx_s, FI_detadezz_s, FI_detadgxz_s, FI_detadp_s, FI_X_s, FI_OverS_s
+
+static int rheology_pass(mdz_ctx *c, int use_guess) {
+    const float *gn = use_guess ? c->guess_n : nullptr, *gs = use_guess ? c->guess_s : nullptr;
+    mdz_launch_rheology(c->g, c->prm, c->d_mat, gn, gs, c->hist, c->stream);
+    return MDZ_OK;
+}
+
+extern "C" void NonNewtonianViscosityGrid(grid *mesh, mat_prop *materials, params *model, Nparams Nmodel, scale *scaling) {
+    (void)Nmodel;
+    int rc = check_supported(model, materials);
+    if (rc) { dropin_check(rc); return; }
+    mdz_ctx *c = dropin_ctx(mesh, model->Nb_phases);
+    if (!c) return;
+    dropin_geometry(c, mesh);
+    fill_params(c, model, scaling);
+    c->prm.rheo_mode = MDZ_RHEO_FP64;
+    dropin_check(up_tags(c, mesh));
+    dropin_check(up_phases(c, mesh, model->Nb_phases));
+    dropin_check(up_materials(c, materials));
+    dropin_check(up_list(c, mesh, {RHEO_IN}));
+    rheology_pass(c, 0);
+    dropin_check(down_list(c, mesh, {RHEO_OUT}));
+    dropin_check(sync(c));
+}
+
+#define DT_IN                                                                                                   \
+    FI_eta_n, FI_eta_s, FI_detadexx_n, FI_detadezz_n, FI_detadgxz_n, FI_detadp_n, FI_ddivpdexx_n, FI_ddivpdezz_n, \
+        FI_ddivpdgxz_n, FI_ddivpdp_n, FI_detadexx_s, FI_detadezz_s, FI_detadgxz_s, FI_detadp_s, FI_exxd, FI_ezzd,  \
+        FI_exz_n, FI_exz, FI_exxd_s, FI_ezzd_s, FI_sxxd0, FI_szzd0, FI_sxz0_n, FI_sxz0, FI_sxxd0_s, FI_szzd0_s,    \
+        FI_mu_n, FI_mu_s, FI_bet_n, FI_nx0_n, FI_nz0_n, FI_nx0_s, FI_nz0_s, FI_aniso_factor_n, FI_aniso_factor_s,  \
+        FI_FS_AR_n, FI_FS_AR_s
+#define DT_OUT FI_D11_n, FI_D12_n, FI_D13_n, FI_D14_n, FI_D21_n, FI_D22_n, FI_D23_n, FI_D24_n, FI_D31_s, FI_D32_s, FI_D33_s, FI_D34_s
+
+static int dtensor_status(mdz_ctx *c) {
+    int st = 0;
+    HIPCHK(hipMemcpyAsync(&st, c->status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemsetAsync(c->status, 0, sizeof(int), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return st;
+}
+
+extern "C" void RheologicalOperators(grid *mesh, params *model, scale *scaling, int Jacobian) {
+    mdz_ctx *c = dropin_ctx(mesh, model->Nb_phases > 0 ? model->Nb_phases : 1);
+    if (!c) return;
+    dropin_geometry(c, mesh);
+    fill_params(c, model, scaling);
+    dropin_check(up_tags(c, mesh));
+    dropin_check(up_list(c, mesh, {DT_IN}));
+    mdz_launch_dtensor(c->g, c->prm, Jacobian ? 1 : 0, c->status, c->stream);
+    dropin_check(down_list(c, mesh, {DT_OUT}));
+    int st = dtensor_status(c);
+    if (Jacobian && st < 0) { dropin_check(st); return; }
+    if (Jacobian && (st & 1)) { dropin_fail(MDZ_ED34, 1, "EXIT: D34 is NAN!\n"); return; }
+    if (Jacobian && (st & 2)) { dropin_fail(MDZ_ED34, 1, "EXIT: D34 is INF!\n"); return; }
+}
+
+#define RHS_IN                                                                                                 \
+    FI_rho_n, FI_rho_s, FI_eta_n, FI_eta_s, FI_mu_n, FI_mu_s, FI_sxxd0, FI_szzd0, FI_sxz0, FI_T, FI_p0_n,       \
+        FI_bet_n, FI_rho0_n, FI_divth0_n, FI_rhs_t, FI_rhs_p, FI_Qrho
+#define RHS_OUT FI_roger_x, FI_roger_z, FI_gx, FI_gz, FI_rhs_p, FI_rhs_t, FI_Qrho
+
+extern "C" void EvaluateRHS(grid *mesh, params model, scale scaling, double RHO_REF) {
+    (void)RHO_REF;
+    mdz_ctx *c = dropin_ctx(mesh, model.Nb_phases > 0 ? model.Nb_phases : 1);
+    if (!c) return;
+    dropin_geometry(c, mesh);
+    fill_params(c, &model, &scaling);
+    dropin_check(up_tags(c, mesh));
+    dropin_check(up_coords(c, mesh));
+    dropin_check(up_list(c, mesh, {RHS_IN}));
+    mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, c->stream);
+    dropin_check(down_list(c, mesh, {RHS_OUT}));
+    dropin_check(sync(c));
+}
+
+extern "C" void UpdateNonLinearity(grid *mesh, markers *particles, markers *topo_chain, surface *topo,
+                                   mat_prop materials, params *model, Nparams *Nmodel, scale scaling, int mode,
+                                   double h_contin) {
+    (void)particles; (void)topo_chain; (void)topo; (void)Nmodel; (void)mode; (void)h_contin;
+    int rc = check_supported(model, &materials);
+    if (rc) { dropin_check(rc); return; }
+    mdz_ctx *c = dropin_ctx(mesh, model->Nb_phases);
+    if (!c) return;
+    dropin_geometry(c, mesh);
+    fill_params(c, model, &scaling);
+    c->prm.rheo_mode = MDZ_RHEO_FP64;
+    dropin_check(up_tags(c, mesh));
+    dropin_check(up_coords(c, mesh));
+    dropin_check(up_phases(c, mesh, model->Nb_phases));
+    dropin_check(up_materials(c, &materials));
+    dropin_check(up_list(c, mesh, {STRAIN_IN}));
+    dropin_check(up_list(c, mesh, {RHEO_IN}));
+    dropin_check(up_list(c, mesh, {RHS_IN}));
+    dropin_check(up_list(c, mesh, {DT_IN}));
+    mdz_launch_strain_rates(c->g, c->prm, c->stream);
+    rheology_pass(c, 0);
+    mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, c->stream);
+    mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
+    dropin_check(down_list(c, mesh, {STRAIN_OUT}));
+    dropin_check(down_list(c, mesh, {RHEO_OUT}));
+    dropin_check(down_list(c, mesh, {RHS_OUT}));
+    dropin_check(down_list(c, mesh, {DT_OUT}));
+    dropin_check(sync(c));
+}
+
+/* numbering on the device; host arrays malloc()ed as in the reference */
+__global__ void k_nodes_from_eqn(DevGrid g, DevEqn e) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    int nu = g.Nx * g.nzvx, nw = g.nxvz * g.Nz, nc = g.ncx * g.ncz;
+    if (i >= nu + nw + nc) return;
+    if (i < nu) {
+        int t = g.BCu[i], eq = e.eqn_u[i];
+        if ((t == -1 || t == 2 || t == -2) && eq >= 0 && eq < e.neq_mom) e.mom_node[eq] = i;
+    } else if (i < nu + nw) {
+        int c = i - nu, t = g.BCv[c], eq = e.eqn_v[c];
+        if ((t == -1 || t == 2) && eq >= 0 && eq < e.neq_mom) e.mom_node[eq] = i;
+    } else {
+        int c = i - nu - nw, eq = e.eqn_p[c];
+        if (g.BCp[c] == -1 && eq >= e.neq_mom && eq < e.neq_mom + e.neq_cont) e.cont_node[eq - e.neq_mom] = c;
+    }
+}
+
+/* tags the numbering and the stencils accept; anything else would make the
+ * right-hand-side order of the reference differ from the equation order */
+static int validate_tags(const grid *m) {
+    int Nx = m->Nx, Nz = m->Nz;
+    for (int i = 0; i < Nx * (Nz + 1); i++) {
+        int t = m->BCu.type[i];
+        if (!(t == -1 || t == 0 || t == 2 || t == -2 || t == -12 || t == 11 || t == 13 || t == 30))
+            return set_err(MDZ_EUNSUPPORTED, "Vx tag %d at node %d", t, i);
+    }
+    for (int i = 0; i < (Nx + 1) * Nz; i++) {
+        int t = m->BCv.type[i];
+        if (!(t == -1 || t == 0 || t == 2 || t == -12 || t == 11 || t == 13 || t == 30))
+            return set_err(MDZ_EUNSUPPORTED, "Vz tag %d at node %d", t, i);
+    }
+    for (int i = 0; i < (Nx - 1) * (Nz - 1); i++) {
+        int t = m->BCp.type[i];
+        if (!(t == -1 || t == 0 || t == 30 || t == 31)) return set_err(MDZ_EUNSUPPORTED, "P tag %d at cell %d", t, i);
+    }
+    return MDZ_OK;
+}
+
+extern "C" void EvalNumberOfEquations(grid *mesh, SparseMat *Stokes) {
+    mdz_ctx *c = dropin_ctx(mesh, 1);
+    if (!c) return;
+    dropin_geometry(c, mesh);
+    dropin_check(validate_tags(mesh));
+    dropin_check(up_tags(c, mesh));
+    dropin_check(mdz_number_equations(c->g, c->e, c->scratch, c->scratch_len, c->stream));
+    size_t nc = fam_len(MDZ_FAM_C, c->Nx, c->Nz), nu = fam_len(MDZ_FAM_U, c->Nx, c->Nz), nw = fam_len(MDZ_FAM_W, c->Nx, c->Nz);
+    Stokes->eqn_u = (int *)malloc(nu * sizeof(int));
+    Stokes->eqn_v = (int *)malloc(nw * sizeof(int));
+    Stokes->eqn_p = (int *)malloc(nc * sizeof(int));
+    hipMemcpyAsync(Stokes->eqn_u, c->e.eqn_u, nu * 4, hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(Stokes->eqn_v, c->e.eqn_v, nw * 4, hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(Stokes->eqn_p, c->e.eqn_p, nc * 4, hipMemcpyDeviceToHost, c->stream);
+    dropin_check(sync(c));
+    Stokes->neq_mom = c->e.neq_mom;
+    Stokes->neq_cont = c->e.neq_cont;
+    Stokes->neq = c->e.neq_mom + c->e.neq_cont;
+}
+
+/* uploads what the assembly / residual rows read, and the caller's numbering */
+static int asm_upload(mdz_ctx *c, grid *mesh, const params *model, SparseMat *Stokes, double *p_corr, double *p,
+                      double *u, double *v, int kind) {
+    int rc;
+    if ((rc = validate_tags(mesh))) return rc;
+    if ((rc = up_tags(c, mesh))) return rc;
+    size_t nc = fam_len(MDZ_FAM_C, c->Nx, c->Nz), nu = fam_len(MDZ_FAM_U, c->Nx, c->Nz), nw = fam_len(MDZ_FAM_W, c->Nx, c->Nz);
+    HIPCHK(hipMemcpyAsync(c->e.eqn_u, Stokes->eqn_u, nu * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->e.eqn_v, Stokes->eqn_v, nw * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->e.eqn_p, Stokes->eqn_p, nc * 4, hipMemcpyHostToDevice, c->stream));
+    c->e.neq_mom = Stokes->neq_mom;
+    c->e.neq_cont = Stokes->neq_cont;
+    int n = (int)(nu + nw + nc);
+    hipLaunchKernelGGL(k_nodes_from_eqn, dim3(mdz_div_up(n, 256)), dim3(256), 0, c->stream, c->g, c->e);
+    /* iterate arrays as passed (the reference passes mesh->u_in etc.) */
+    HIPCHK(hipMemcpyAsync(c->g.u_in, u, nu * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.v_in, v, nw * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.p_corr, p_corr, nc * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.p_in, p, nc * 8, hipMemcpyHostToDevice, c->stream));
+    if ((rc = up_list(c, mesh, {FI_roger_x, FI_roger_z, FI_rhs_p, FI_eta_n, FI_eta_s, FI_rho_n, FI_gx, FI_gz, FI_bet_n}))) return rc;
+    if (kind == 0) { if ((rc = up_list(c, mesh, {FI_D11_n, FI_D22_n, FI_D33_s}))) return rc; }
+    else if ((rc = up_list(c, mesh, {DT_OUT}))) return rc;
+    if (model->aniso == 1) { if ((rc = up_list(c, mesh, {FI_sxxd, FI_szzd, FI_sxz}))) return rc; }
+    return MDZ_OK;
+}
+
+/* host CSR of one block: malloc()ed Ic/J/A/bbc; J/A NULL when empty
+ * (what realloc(p, 0) leaves in the reference) */
+static int download_block(mdz_ctx *c, DevBlock *b, SparseMat *M, int with_b) {
+    M->neq = b->neq;
+    M->nnz = b->nnz;
+    M->Ic = (int *)malloc(((size_t)b->neq + 1) * sizeof(int));
+    M->J = b->nnz ? (int *)malloc((size_t)b->nnz * sizeof(int)) : NULL;
+    M->A = b->nnz ? (double *)malloc((size_t)b->nnz * sizeof(double)) : NULL;
+    M->bbc = (double *)malloc(((size_t)b->neq + 1) * sizeof(double));
+    if (!M->Ic || !M->bbc || (b->nnz && (!M->J || !M->A))) return set_err(MDZ_ENOMEM, "host CSR");
+    HIPCHK(hipMemcpyAsync(M->Ic, b->Ic, ((size_t)b->neq + 1) * 4, hipMemcpyDeviceToHost, c->stream));
+    if (b->nnz) {
+        HIPCHK(hipMemcpyAsync(M->J, b->J, (size_t)b->nnz * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(M->A, b->A, (size_t)b->nnz * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(M->bbc, b->bbc, (size_t)b->neq * 8, hipMemcpyDeviceToHost, c->stream));
+    if (with_b && M->b) HIPCHK(hipMemcpyAsync(M->b, b->b, (size_t)b->neq * 8, hipMemcpyDeviceToHost, c->stream));
+    return MDZ_OK;
+}
+
+static void build_operator_dropin(int kind, grid *mesh, params model, double *p_corr, double *p, double *u, double *v,
+                                  SparseMat *Stokes, SparseMat *SA, SparseMat *SB, SparseMat *SC, SparseMat *SD,
+                                  int Assemble) {
+    mdz_ctx *c = dropin_ctx(mesh, model.Nb_phases > 0 ? model.Nb_phases : 1);
+    if (!c) return;
+    dropin_geometry(c, mesh);
+    fill_params(c, &model, &c->scaling);
+    int rc = asm_upload(c, mesh, &model, Stokes, p_corr, p, u, v, kind);
+    if (rc) { dropin_check(rc); return; }
+    SA->neq = Stokes->neq_mom; SB->neq = Stokes->neq_mom;
+    SC->neq = Stokes->neq_cont; SC->neq_mom = Stokes->neq_mom;
+    SD->neq = Stokes->neq_cont; SD->neq_mom = Stokes->neq_mom;
+    rc = ensure_rows(c);
+    if (rc) { dropin_check(rc); return; }
+    DevBlock *A = &c->blk[4 * kind], *B = &c->blk[4 * kind + 1], *C = &c->blk[4 * kind + 2], *D = &c->blk[4 * kind + 3];
+    DevAsm a = make_asm(c);
+    if (Assemble == 1) {
+        rc = build_pattern(c, kind);
+        if (rc) { dropin_check(rc); return; }
+        a = make_asm(c);
+        mdz_launch_values(kind, a, A, B, C, D, c->stream);
+        dropin_check(download_block(c, A, SA, 1));
+        dropin_check(download_block(c, B, SB, 1));
+        dropin_check(download_block(c, C, SC, 1));
+        dropin_check(download_block(c, D, SD, 1));
+        /* b of B and D equal those of A and C (already downloaded through with_b) */
+        dropin_check(sync(c));
+    } else {
+        /* residual: b from the right-hand sides; Neumann rows read the bbc
+         * of the last assembly of this block */
+        if (SA->bbc) {
+            if (hipMemcpyAsync(A->bbc, SA->bbc, (size_t)SA->neq * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+                dropin_check(set_err(MDZ_EHIP, "bbc upload"));
+                return;
+            }
+        } else {
+            hipMemsetAsync(A->bbc, 0, (size_t)SA->neq * 8, c->stream);
+        }
+        mdz_launch_residual(kind, a, A, C, D, c->stream);
+        hipMemcpyAsync(SA->F, A->F, (size_t)SA->neq * 8, hipMemcpyDeviceToHost, c->stream);
+        hipMemcpyAsync(SA->b, A->b, (size_t)SA->neq * 8, hipMemcpyDeviceToHost, c->stream);
+        hipMemcpyAsync(SC->F, C->F, (size_t)SC->neq * 8, hipMemcpyDeviceToHost, c->stream);
+        hipMemcpyAsync(SC->b, C->b, (size_t)SC->neq * 8, hipMemcpyDeviceToHost, c->stream);
+        hipMemcpyAsync(SD->F, D->F, (size_t)SD->neq * 8, hipMemcpyDeviceToHost, c->stream);
+        dropin_check(sync(c));
+    }
+}
+
+extern "C" void BuildStokesOperatorDecoupled(grid *mesh, params model, int lev, double *p_corr, double *p, double *u,
+                                             double *v, SparseMat *Stokes, SparseMat *StokesA, SparseMat *StokesB,
+                                             SparseMat *StokesC, SparseMat *StokesD, int Assemble) {
+    (void)lev;
+    build_operator_dropin(0, mesh, model, p_corr, p, u, v, Stokes, StokesA, StokesB, StokesC, StokesD, Assemble);
+}
+
+extern "C" void BuildJacobianOperatorDecoupled(grid *mesh, params model, int lev, double *p_corr, double *p, double *u,
+                                               double *v, SparseMat *Stokes, SparseMat *StokesA, SparseMat *StokesB,
+                                               SparseMat *StokesC, SparseMat *StokesD, int Assemble) {
+    (void)lev;
+    build_operator_dropin(1, mesh, model, p_corr, p, u, v, Stokes, StokesA, StokesB, StokesC, StokesD, Assemble);
+}
+
+static void finish_norms(Nparams *Nmodel, const double *out6, scale scaling, int quiet) {
+    Nmodel->resx = sqrt(out6[0] / out6[3]);
+    Nmodel->resz = sqrt(out6[1] / out6[4]);
+    Nmodel->resp = sqrt(out6[2] / out6[5]);
+    if (Nmodel->nit == 0) {
+        Nmodel->resx0 = Nmodel->resx;
+        Nmodel->resz0 = Nmodel->resz;
+        Nmodel->resp0 = Nmodel->resp;
+    }
+    if (quiet == 0) {
+        printf("Fu abs. = %2.6e --- Fu rel. = %2.6e\n", Nmodel->resx, Nmodel->resx / Nmodel->resx0);
+        printf("Fv abs. = %2.6e --- Fv rel. = %2.6e\n", Nmodel->resz, Nmodel->resz / Nmodel->resz0);
+        printf("Fp abs. = %2.6e --- Fp rel. = %2.6e\n", Nmodel->resp, Nmodel->resp / Nmodel->resp0);
+    }
+    if (isnan(Nmodel->resx) || isnan(Nmodel->resz) || isnan(Nmodel->resp)) {
+        printf("Fu = %2.6e\n", Nmodel->resx * scaling.S * scaling.L);
+        printf("Fv = %2.6e\n", Nmodel->resz * scaling.S * scaling.L);
+        printf("Fp = %2.6e\n", Nmodel->resp * scaling.E * scaling.L * scaling.L);
+        dropin_fail(MDZ_ENAN, 122, "Solve went wrong - Nan residuals...\nExiting...\n");
+    }
+}
+
+extern "C" void EvaluateStokesResidualDecoupled(SparseMat *Stokes, SparseMat *StokesA, SparseMat *StokesB,
+                                                SparseMat *StokesC, SparseMat *StokesD, Nparams *Nmodel, grid *mesh,
+                                                params model, scale scaling, int quiet) {
+    int kind = model.aniso == 1 ? 1 : 0;
+    mdz_ctx *c = dropin_ctx(mesh, model.Nb_phases > 0 ? model.Nb_phases : 1);
+    if (!c) return;
+    c->scaling = scaling;
+    build_operator_dropin(kind, mesh, model, mesh->p_corr, mesh->p_in, mesh->u_in, mesh->v_in, Stokes, StokesA,
+                          StokesB, StokesC, StokesD, 0);
+    if (g_status && !g_exit_on_error) return;
+    DevBlock *A = &c->blk[4 * kind], *D = &c->blk[4 * kind + 3];
+    /* residual scaling vectors d */
+    hipMemcpyAsync(A->d, StokesA->d, (size_t)StokesA->neq * 8, hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(D->d, StokesD->d, (size_t)StokesD->neq * 8, hipMemcpyHostToDevice, c->stream);
+    DevAsm a = make_asm(c);
+    mdz_launch_norms(a, A, D, c->norm_partial, c->norm_out, c->stream);
+    double out6[6];
+    hipMemcpyAsync(out6, c->norm_out, 6 * 8, hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(StokesA->F, A->F, (size_t)StokesA->neq * 8, hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(StokesD->F, D->F, (size_t)StokesD->neq * 8, hipMemcpyDeviceToHost, c->stream);
+    dropin_check(down_list(c, mesh, {FI_ru, FI_rv, FI_rp}));
+    dropin_check(sync(c));
+    finish_norms(Nmodel, out6, scaling, quiet);
+}
+
+/* StokesAssemblyCoupled.c:47 -- append a coefficient, or fold a Dirichlet /
+ * surface-pressure / non-conforming / matching-Neumann neighbour into the
+ * right-hand side.  Host bookkeeping helper of the assembly interface. */
+extern "C" void AddCoeff2(int *J, double *A, int eqn, int jeq, int *nnzc, double coeff, int NODE_TYPE, double NODE_VAL,
+                          double *RHS) {
+    if (NODE_TYPE == 0 || NODE_TYPE == 31 || NODE_TYPE == 11 || NODE_TYPE == 13) {
+        RHS[eqn] -= coeff * NODE_VAL;
+    } else {
+        J[*nnzc] = jeq;
+        A[*nnzc] = coeff;
+        (*nnzc)++;
+    }
+}
+
+/* Pointwise Viscosity: evaluated by one GPU lane. */
+struct ViscArgs {
+    int phase;
+    double G, T, P, d, phi, X0, Exx, Ezz, Exz, Gxx, Gzz, Gxz, Txx0, Tzz0, Txz0, dil, fric, C, P0, beta;
+};
+__global__ void k_viscosity_point(ViscArgs a, const mat_prop *mt, DevParams prm, VOut *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    VGuess gs;
+    gs.mode = MDZ_RHEO_FP64;
+    gs.eta_ve0 = 0.0;
+    VOut o;
+    mdz_viscosity(a.phase, a.G, a.T, a.P, a.d, a.phi, a.X0, a.Exx, a.Ezz, a.Exz, a.Gxx, a.Gzz, a.Gxz, a.Txx0, a.Tzz0,
+                  a.Txz0, mt, prm, a.dil, a.fric, a.C, a.P0, a.beta, gs, o);
+    *out = o;
+}
+
+static mdz_ctx *g_point_ctx = nullptr;
+static VOut *g_point_out = nullptr;
+
+extern "C" double Viscosity(int phase, double G, double T, double P, double d, double phi, double X0, double Exx,
+                            double Ezz, double Exz, double Gxx, double Gzz, double Gxz, double Txx0, double Tzz0,
+                            double Txz0, mat_prop *materials, params *model, scale *scaling, double *txxn,
+                            double *tzzn, double *txzn, double *etaVE, double *VEcoeff, double *Eii_el, double *Eii_pl,
+                            double *Eii_pwl, double *Eii_exp, double *Eii_lin, double *Eii_gbs, double *Eii_cst,
+                            double *Exx_el, double *Ezz_el, double *Exz_el, double *Exx_diss, double *Ezz_diss,
+                            double *Exz_diss, double *d1, double strain_acc, double dil, double fric, double C,
+                            double *detadexx, double *detadezz, double *detadexz, double *detadp, double P0, double *X1,
+                            double *OverS, double *ddivpdexx, double *ddivpdezz, double *ddivpdexz, double *ddivpdp,
+                            double *Pcorr, double *drhodp, double *rho, double beta, double div, double *div_el,
+                            double *div_pl, double *div_r) {
+    (void)strain_acc; (void)div;
+    if (!g_point_ctx) {
+        if (ctx_alloc(&g_point_ctx, 4, 4, 1)) { dropin_check(g_status); return 0.0; }
+        if (hipMalloc((void **)&g_point_out, sizeof(VOut)) != hipSuccess) { dropin_check(set_err(MDZ_ENOMEM, "point")); return 0.0; }
+    }
+    mdz_ctx *c = g_point_ctx;
+    fill_params(c, model, scaling);
+    if (check_supported(model, materials)) { dropin_check(g_status); return 0.0; }
+    up_materials(c, materials);
+    ViscArgs a = {phase, G, T, P, d, phi, X0, Exx, Ezz, Exz, Gxx, Gzz, Gxz, Txx0, Tzz0, Txz0, dil, fric, C, P0, beta};
+    hipLaunchKernelGGL(k_viscosity_point, dim3(1), dim3(64), 0, c->stream, a, (const mat_prop *)c->d_mat, c->prm, g_point_out);
+    VOut o;
+    hipMemcpyAsync(&o, g_point_out, sizeof(VOut), hipMemcpyDeviceToHost, c->stream);
+    dropin_check(sync(c));
+    *txxn = o.txx; *tzzn = o.tzz; *txzn = o.txz; *etaVE = o.etaVE; *VEcoeff = o.VEcoeff;
+    *Eii_el = o.Eii_el; *Eii_pl = o.Eii_pl; *Eii_pwl = o.Eii_pwl; *Eii_exp = o.Eii_exp; *Eii_lin = o.Eii_lin;
+    *Eii_gbs = o.Eii_gbs; *Eii_cst = o.Eii_cst; *Exx_el = o.Exx_el; *Ezz_el = o.Ezz_el; *Exz_el = o.Exz_el;
+    *Exx_diss = o.Exx_diss; *Ezz_diss = o.Ezz_diss; *Exz_diss = o.Exz_diss; *d1 = o.d1;
+    *detadexx = o.detadexx; *detadezz = o.detadezz; *detadexz = o.detadexz; *detadp = o.detadp; *X1 = o.X1;
+    *OverS = o.OverS; *ddivpdexx = o.ddivpdexx; *ddivpdezz = o.ddivpdezz; *ddivpdexz = o.ddivpdexz;
+    *ddivpdp = o.ddivpdp; *Pcorr = o.Pcorr; *drhodp = o.drhodp; *rho = o.rho; *div_el = o.div_el;
+    *div_pl = o.div_pl; *div_r = o.div_r;
+    return o.eta;
+}
+
+/* ------------------------------------------------------------------ */
+/* device-resident interface                                            */
+/* ------------------------------------------------------------------ */
+extern "C" int mdz_ctx_create(mdz_ctx **out, const grid *mesh, const mat_prop *materials, const params *model,
+                              const scale *scaling) {
+    g_status = 0;
+    if (!out || !mesh || !materials || !model || !scaling) return set_err(MDZ_EINVAL, "NULL argument");
+    int rc = check_supported(model, materials);
+    if (rc) return rc;
+    if ((rc = validate_tags(mesh))) return rc;
+    mdz_ctx *c = nullptr;
+    if ((rc = ctx_alloc(&c, mesh->Nx, mesh->Nz, model->Nb_phases))) return rc;
+    c->g.dx = mesh->dx; c->g.dz = mesh->dz;
+    fill_params(c, model, scaling);
+    c->prm.rheo_mode = MDZ_RHEO_FP64;
+    if ((rc = up_tags(c, mesh)) || (rc = up_coords(c, mesh)) || (rc = up_phases(c, mesh, model->Nb_phases)) ||
+        (rc = up_materials(c, materials))) { ctx_release(c); return rc; }
+    for (int i = 0; i < FI_COUNT; i++)
+        if ((rc = up(c, mesh, i))) { ctx_release(c); return rc; }
+    if ((rc = mdz_number_equations(c->g, c->e, c->scratch, c->scratch_len, c->stream))) { ctx_release(c); return rc; }
+    if ((rc = build_pattern(c, 0)) || (rc = build_pattern(c, 1))) { ctx_release(c); return rc; }
+    *out = c;
+    return MDZ_OK;
+}
+
+extern "C" void mdz_ctx_destroy(mdz_ctx *ctx) { ctx_release(ctx); }
+
+extern "C" void *mdz_ctx_stream(mdz_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+extern "C" int mdz_ctx_upload_step_fields(mdz_ctx *c, const grid *mesh) {
+    if (!c || !mesh) return set_err(MDZ_EINVAL, "NULL argument");
+    int rc;
+    for (int i = 0; i < FI_COUNT; i++)
+        if (FREF[i].role == MDZ_ROLE_I && i != FI_u_in && i != FI_v_in && i != FI_p_in)
+            if ((rc = up(c, mesh, i))) return rc;
+    if ((rc = up_phases(c, mesh, c->nph))) return rc;
+    return sync(c);
+}
+
+extern "C" int mdz_ctx_upload_solution(mdz_ctx *c, const double *u, const double *v, const double *p) {
+    if (!c || !u || !v || !p) return set_err(MDZ_EINVAL, "NULL argument");
+    HIPCHK(hipMemcpyAsync(c->g.u_in, u, fam_len(MDZ_FAM_U, c->Nx, c->Nz) * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.v_in, v, fam_len(MDZ_FAM_W, c->Nx, c->Nz) * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.p_in, p, fam_len(MDZ_FAM_C, c->Nx, c->Nz) * 8, hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
+extern "C" int mdz_ctx_download_solution(mdz_ctx *c, double *u, double *v, double *p) {
+    if (!c || !u || !v || !p) return set_err(MDZ_EINVAL, "NULL argument");
+    HIPCHK(hipMemcpyAsync(u, c->g.u_in, fam_len(MDZ_FAM_U, c->Nx, c->Nz) * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(v, c->g.v_in, fam_len(MDZ_FAM_W, c->Nx, c->Nz) * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(p, c->g.p_in, fam_len(MDZ_FAM_C, c->Nx, c->Nz) * 8, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+extern "C" int mdz_ctx_download_grid(mdz_ctx *c, grid *mesh) {
+    if (!c || !mesh) return set_err(MDZ_EINVAL, "NULL argument");
+    int rc;
+    for (int i = 0; i < FI_COUNT; i++)
+        if (FREF[i].role & MDZ_ROLE_O)
+            if ((rc = down(c, mesh, i))) return rc;
+    return sync(c);
+}
+
+extern "C" int mdz_ctx_set_rheology_mode(mdz_ctx *c, int mode) {
+    if (!c || mode < MDZ_RHEO_FP64 || mode > MDZ_RHEO_LEARNED) return set_err(MDZ_EINVAL, "rheology mode %d", mode);
+    if (mode == MDZ_RHEO_LEARNED && !c->d_net) return set_err(MDZ_EINVAL, "learned mode needs mdz_ctx_set_network first");
+    c->prm.rheo_mode = mode;
+    return MDZ_OK;
+}
+
+extern "C" int mdz_ctx_newton_unit(mdz_ctx *c, double *res3) {
+    if (!c) return set_err(MDZ_EINVAL, "NULL context");
+    g_status = 0;
+    DevAsm a = make_asm(c);
+    mdz_launch_strain_rates(c->g, c->prm, c->stream);
+    rheology_pass(c, c->prm.rheo_mode == MDZ_RHEO_LEARNED);
+    mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, c->stream);
+    mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
+    mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->stream);
+    mdz_launch_dtensor(c->g, c->prm, 1, c->status, c->stream);
+    mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->stream);
+    mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
+    int kind = c->model.aniso == 1 ? 1 : 0;
+    mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], c->stream);
+    mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, c->stream);
+    if (res3) {
+        double out6[6];
+        HIPCHK(hipMemcpyAsync(out6, c->norm_out, 6 * 8, hipMemcpyDeviceToHost, c->stream));
+        int rc = sync(c);
+        if (rc) return rc;
+        res3[0] = sqrt(out6[0] / out6[3]);
+        res3[1] = sqrt(out6[1] / out6[4]);
+        res3[2] = sqrt(out6[2] / out6[5]);
+        if (isnan(res3[0]) || isnan(res3[1]) || isnan(res3[2])) return set_err(MDZ_ENAN, "NaN residual");
+    }
+    HIPCHK(hipGetLastError());
+    return MDZ_OK;
+}
+
+extern "C" int mdz_ctx_block_info(mdz_ctx *c, int which, int *neq, int *nnz) {
+    if (!c || which < 0 || which > 7) return set_err(MDZ_EINVAL, "block %d", which);
+    if (neq) *neq = c->blk[which].neq;
+    if (nnz) *nnz = c->blk[which].nnz;
+    return MDZ_OK;
+}
+
+extern "C" int mdz_ctx_download_block(mdz_ctx *c, int which, int *Ic, int *J, double *A, double *b, double *bbc) {
+    if (!c || which < 0 || which > 7) return set_err(MDZ_EINVAL, "block %d", which);
+    DevBlock *k = &c->blk[which];
+    if (Ic) HIPCHK(hipMemcpyAsync(Ic, k->Ic, ((size_t)k->neq + 1) * 4, hipMemcpyDeviceToHost, c->stream));
+    if (J && k->nnz) HIPCHK(hipMemcpyAsync(J, k->J, (size_t)k->nnz * 4, hipMemcpyDeviceToHost, c->stream));
+    if (A && k->nnz) HIPCHK(hipMemcpyAsync(A, k->A, (size_t)k->nnz * 8, hipMemcpyDeviceToHost, c->stream));
+    if (b) HIPCHK(hipMemcpyAsync(b, k->b, (size_t)k->neq * 8, hipMemcpyDeviceToHost, c->stream));
+    if (bbc) HIPCHK(hipMemcpyAsync(bbc, k->bbc, (size_t)k->neq * 8, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+extern "C" int mdz_ctx_download_residual(mdz_ctx *c, double *Fmom, double *Fcont) {
+    if (!c) return set_err(MDZ_EINVAL, "NULL context");
+    int kind = c->model.aniso == 1 ? 1 : 0;
+    if (Fmom) HIPCHK(hipMemcpyAsync(Fmom, c->blk[4 * kind].F, (size_t)c->e.neq_mom * 8, hipMemcpyDeviceToHost, c->stream));
+    if (Fcont) HIPCHK(hipMemcpyAsync(Fcont, c->blk[4 * kind + 3].F, (size_t)c->e.neq_cont * 8, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+extern "C" int mdz_ctx_iteration_histogram(mdz_ctx *c, long long *hist) {
+    if (!c || !hist) return set_err(MDZ_EINVAL, "NULL argument");
+    HIPCHK(hipMemcpyAsync(hist, c->hist, 3 * MDZ_HIST_BINS * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemsetAsync(c->hist, 0, 3 * MDZ_HIST_BINS * 8, c->stream));
+    return sync(c);
+}
+
+extern "C" int mdz_ctx_set_network(mdz_ctx *c, const float *w, size_t n) {
+    if (!c || !w || !n) return set_err(MDZ_EINVAL, "network weights");
+    hipFree(c->d_net);
+    c->d_net = nullptr;
+    HIPCHK(hipMalloc((void **)&c->d_net, n * sizeof(float)));
+    HIPCHK(hipMemcpy(c->d_net, w, n * sizeof(float), hipMemcpyHostToDevice));
+    c->net_floats = n;
+    return MDZ_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* timing                                                               */
+/* ------------------------------------------------------------------ */
+/* Runs `reps` Newton units with HIP events around each stage on the
+ * context's stream; ms[0..8] = mean milliseconds of the stages
+ * (strain rates, rheology, rhs, D(0), Picard values, D(1), Jacobian values,
+ * D(0), residual + norms), ms[9] = mean total per unit. */
+extern "C" int mdz_ctx_stage_times(mdz_ctx *c, int reps, double *ms) {
+    if (!c || reps < 1 || !ms) return set_err(MDZ_EINVAL, "stage times");
+    const int NS = 9;
+    hipEvent_t ev[NS + 1];
+    for (int i = 0; i <= NS; i++) HIPCHK(hipEventCreate(&ev[i]));
+    double acc[NS + 1] = {0};
+    DevAsm a = make_asm(c);
+    int kind = c->model.aniso == 1 ? 1 : 0;
+    for (int r = 0; r < reps; r++) {
+        HIPCHK(hipEventRecord(ev[0], c->stream));
+        mdz_launch_strain_rates(c->g, c->prm, c->stream);
+        HIPCHK(hipEventRecord(ev[1], c->stream));
+        rheology_pass(c, c->prm.rheo_mode == MDZ_RHEO_LEARNED);
+        HIPCHK(hipEventRecord(ev[2], c->stream));
+        mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, c->stream);
+        HIPCHK(hipEventRecord(ev[3], c->stream));
+        mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
+        HIPCHK(hipEventRecord(ev[4], c->stream));
+        mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->stream);
+        HIPCHK(hipEventRecord(ev[5], c->stream));
+        mdz_launch_dtensor(c->g, c->prm, 1, c->status, c->stream);
+        HIPCHK(hipEventRecord(ev[6], c->stream));
+        mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->stream);
+        HIPCHK(hipEventRecord(ev[7], c->stream));
+        mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
+        HIPCHK(hipEventRecord(ev[8], c->stream));
+        mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], c->stream);
+        mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, c->stream);
+        HIPCHK(hipEventRecord(ev[9], c->stream));
+        HIPCHK(hipEventSynchronize(ev[9]));
+        for (int i = 0; i < NS; i++) {
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+            acc[i] += t;
+        }
+        float tt = 0;
+        HIPCHK(hipEventElapsedTime(&tt, ev[0], ev[NS]));
+        acc[NS] += tt;
+    }
+    for (int i = 0; i <= NS; i++) ms[i] = acc[i] / reps;
+    for (int i = 0; i <= NS; i++) hipEventDestroy(ev[i]);
+    HIPCHK(hipGetLastError());
+    return MDZ_OK;
+}
+
+/* Times `reps` back-to-back Newton units (no host synchronisation inside)
+ * between two events; returns the mean milliseconds per unit. */
+extern "C" int mdz_ctx_time_units(mdz_ctx *c, int reps, double *ms_per_unit) {
+    if (!c || reps < 1 || !ms_per_unit) return set_err(MDZ_EINVAL, "time units");
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) {
+        int rc = mdz_ctx_newton_unit(c, nullptr);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms_per_unit = t / reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return MDZ_OK;
+}

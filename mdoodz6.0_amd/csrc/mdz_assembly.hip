@@ -1,0 +1,1299 @@
+/*
+ * mdz_assembly.hip -- gfx950 kernels of the decoupled Stokes assembly and
+ * residual of the Newton-iteration path.
+ *
+ * One lane per CSR row (equation).  Every row is emitted by one templated
+ * device function through a sink: SINK_COUNT counts the appended entries,
+ * SINK_PATTERN writes the column indices, SINK_VALUES writes the values and
+ * folds Dirichlet neighbours into bbc.  Which neighbours are appended, and
+ * in which order, depends only on the boundary tags, so the pattern built
+ * once per boundary-condition set stays valid for every Newton iteration
+ * and the value pass writes in place into it.
+ *
+ *   equation numbering              StokesRoutines.c:1134-1187
+ *   AddCoeff2 folding               StokesAssemblyCoupled.c:47-57
+ *   continuity rows                 StokesAssemblyDecoupled.c:62-103
+ *   Picard X rows, W/E Neumann      StokesAssemblyDecoupled.c:109-234, 476-790
+ *   Picard Z rows, S/N Neumann      StokesAssemblyDecoupled.c:796-972, 1168-1428
+ *   Jacobian X / Z rows             StokesAssemblyDecoupled.c:3034-3421, 3427-3825
+ *   row order / append order        StokesAssemblyCoupled.c:1446-1492
+ *   residual norms                  StokesRoutines.c:839-916
+ *
+ * The Jacobian rows are the composition of the linear operators they
+ * linearise (staggered strain rates, masked centroid<->vertex means with
+ * the weights wE/wW/wS/wN, the tangent blocks, the stress divergence);
+ * each neighbour's coefficient accumulates its contributions through it.
+ */
+#include "mdz_kernels.h"
+
+#define BLK 256
+#define OUTSIDE 99
+
+template <int MODE>
+struct Sink {
+    int n;
+    int *J;
+    double *A;
+    double bbc;
+    __device__ __forceinline__ void add(int jeq, double coeff, int type, double val) {
+        if (type == 0 || type == 31 || type == 11 || type == 13) {
+            if (MODE == SINK_VALUES) bbc -= coeff * val;
+        } else {
+            if (MODE == SINK_PATTERN) J[n] = jeq;
+            if (MODE == SINK_VALUES) A[n] = coeff;
+            n++;
+        }
+    }
+};
+
+struct Tags {
+    const int8_t *u, *v, *p, *gv;
+    int nu, nw, nc, nv;
+    __device__ __forceinline__ int U(int i) const { return (i >= 0 && i < nu) ? u[i] : OUTSIDE; }
+    __device__ __forceinline__ int V(int i) const { return (i >= 0 && i < nw) ? v[i] : OUTSIDE; }
+    __device__ __forceinline__ int P(int i) const { return (i >= 0 && i < nc) ? p[i] : OUTSIDE; }
+    __device__ __forceinline__ int G(int i) const { return (i >= 0 && i < nv) ? gv[i] : OUTSIDE; }
+};
+
+__device__ __forceinline__ Tags make_tags(const DevGrid &g) {
+    Tags t;
+    t.u = g.BCu; t.v = g.BCv; t.p = g.BCp; t.gv = g.BCg;
+    t.nu = g.Nx * g.nzvx; t.nw = g.nxvz * g.Nz; t.nc = g.ncx * g.ncz; t.nv = g.Nx * g.Nz;
+    return t;
+}
+
+/* ------------------------------------------------------------------ */
+/* continuity                                                           */
+/* ------------------------------------------------------------------ */
+template <bool ASM, int MODE>
+__device__ void continuity_row(const DevAsm &a, int comp, int c1, int c2, int c3, Sink<MODE> &rc, double &b, double &F) {
+    const DevGrid &g = a.g;
+    int nxvz = g.nxvz;
+    int tW = g.BCu[c1], tE = g.BCu[c1 + 1], tS = g.BCv[c3], tN = g.BCv[c3 + nxvz];
+    double uW = tW != 13 ? -a.one_dx * 1e0 : 0.0, uE = tE != 13 ? a.one_dx * 1e0 : 0.0;
+    double vS = tS != 13 ? -a.one_dz * 1e0 : 0.0, vN = tN != 13 ? a.one_dz * 1e0 : 0.0;
+    if (ASM) {
+        b *= a.celvol;
+        if (tW != 13) rc.add(a.e.eqn_u[c1], uW * a.celvol, tW, g.BCu_val[c1]);
+        if (tE != 13) rc.add(a.e.eqn_u[c1 + 1], uE * a.celvol, tE, g.BCu_val[c1 + 1]);
+        if (tS != 13) rc.add(a.e.eqn_v[c3], vS * a.celvol, tS, g.BCv_val[c3]);
+        if (tN != 13) rc.add(a.e.eqn_v[c3 + nxvz], vN * a.celvol, tN, g.BCv_val[c3 + nxvz]);
+    } else {
+        double pc = comp == 0 ? 0.0 : g.bet_n[c2] / a.dt;
+        double f = pc * a.p_cont[c2] + uW * a.u[c1] + uE * a.u[c1 + 1] + vS * a.v[c3] + vN * a.v[c3 + nxvz];
+        f -= b;
+        F = f * a.celvol;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Picard X rows                                                        */
+/* ------------------------------------------------------------------ */
+template <bool ASM, int MODE>
+__device__ void xmom_inner(const DevAsm &a, int c1, int c2, int c3, int eqn, Sink<MODE> &ra, Sink<MODE> &rb,
+                           double &b, double &F) {
+    const DevGrid &g = a.g;
+    int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz;
+    int iVxC = c1, iVxW = iVxC - 1, iVxE = iVxC + 1, iVxS = iVxC - nx, iVxN = iVxC + nx;
+    int iVzSW = c3 - nxvz, iVzSE = c3 - nxvz + 1, iVzNW = c3, iVzNE = c3 + 1;
+    int iPrW = c2, iPrE = c2 + 1, ixyN = c1, ixyS = c1 - nx;
+    if (g.BCu[iVxC] == -2) { iVxW = c1 + nx - 2; iPrW = c2 + ncx; iVzSW = c3 - 2; iVzNW = c3 + nxvz - 2; }
+    double D11E = g.D11_n[iPrE], D11W = g.D11_n[iPrW], D33N = g.D33_s[ixyN], D33S = g.D33_s[ixyS];
+    double inW = g.BCp[iPrW] == -1 ? 1.0 : 0.0, inE = g.BCp[iPrE] == -1 ? 1.0 : 0.0;
+    double inS = (g.BCg[ixyS] != 30 && g.BCu[ixyS] != 13) ? 1.0 : 0.0;
+    double inN = (g.BCg[c1] != 30 && g.BCu[c1 + nx] != 13) ? 1.0 : 0.0;
+    double dx = g.dx, dz = g.dz, comp = 1.0;
+    double uW = D11W * inW * ((1.0 / 3.0) * comp / dx - 1.0 / dx) / dx;
+    double uC = -(-D33N * inN / dz - D33S * inS / dz) / dz
+              - (D11E * inE * ((1.0 / 3.0) * comp / dx - 1 / dx) - D11W * inW * (-1.0 / 3.0 * comp / dx + 1.0 / dx)) / dx;
+    double uE = -D11E * inE * (-1.0 / 3.0 * comp / dx + 1.0 / dx) / dx;
+    double uS = -D33S * inS / (dz * dz);
+    double uN = -D33N * inN / (dz * dz);
+    double vSW = (1.0 / 3.0) * D11W * comp * inW / (dx * dz) - D33S * inS / (dx * dz);
+    double vSE = -1.0 / 3.0 * D11E * comp * inE / (dx * dz) + D33S * inS / (dx * dz);
+    double vNW = -1.0 / 3.0 * D11W * comp * inW / (dx * dz) + D33N * inN / (dx * dz);
+    double vNE = (1.0 / 3.0) * D11E * comp * inE / (dx * dz) - D33N * inN / (dx * dz);
+    double pW = 0, pE = 0;
+    if (a.stab == 1) {
+        double drhodx = (g.rho_n[c2 + 1] - g.rho_n[c2]) * a.one_dx;
+        double uC_corr = 1.00 * a.om * a.dt * g.gx[c1] * drhodx;
+        if (uC + uC_corr > 0.0) uC += uC_corr;
+    }
+    int tS = g.BCu[iVxS], tN = g.BCu[iVxN];
+    if (tS == 11) uC -= uS;
+    if (tN == 11) uC -= uN;
+    bool pgrad = g.BCp[iPrE] != 30 && g.BCp[iPrW] != 30;
+    if (pgrad) { pW = -a.one_dx * 1e0; pE = a.one_dx * 1e0; }
+    double cv = a.celvol;
+    if (ASM) {
+        b *= cv;
+        if (tS != 30) ra.add(a.e.eqn_u[iVxS], uS * cv, tS, (tS != 11 ? 1.0 : 2.0) * g.BCu_val[iVxS]);
+        if (g.BCu[iVxW] != 30) ra.add(a.e.eqn_u[iVxW], uW * cv, g.BCu[iVxW], g.BCu_val[iVxW]);
+        ra.add(eqn, uC * cv, g.BCu[iVxC], g.BCu_val[iVxC]);
+        if (g.BCu[iVxE] != 30) ra.add(a.e.eqn_u[iVxE], uE * cv, g.BCu[iVxE], g.BCu_val[iVxE]);
+        if (g.BCu[c1 + nx] != 30) ra.add(a.e.eqn_u[iVxN], uN * cv, tN, (tN != 11 ? 1.0 : 2.0) * g.BCu_val[iVxN]);
+        if (g.BCv[iVzSW] != 30) ra.add(a.e.eqn_v[iVzSW], vSW * cv, g.BCv[iVzSW], g.BCv_val[iVzSW]);
+        if (g.BCv[iVzSE] != 30) ra.add(a.e.eqn_v[iVzSE], vSE * cv, g.BCv[iVzSE], g.BCv_val[iVzSE]);
+        if (g.BCv[iVzNW] != 30) ra.add(a.e.eqn_v[iVzNW], vNW * cv, g.BCv[iVzNW], g.BCv_val[iVzNW]);
+        if (g.BCv[iVzNE] != 30) ra.add(a.e.eqn_v[iVzNE], vNE * cv, g.BCv[iVzNE], g.BCv_val[iVzNE]);
+        if (pgrad) {
+            rb.add(a.e.eqn_p[iPrW] - a.e.neq_mom, pW * cv, g.BCp[iPrW], g.BCp_val[iPrW]);
+            rb.add(a.e.eqn_p[iPrE] - a.e.neq_mom, pE * cv, g.BCp[iPrE], g.BCp_val[iPrE]);
+        }
+    } else {
+        const double *u = a.u, *v = a.v, *p = a.p_mom;
+        double f = uC * u[iVxC];
+        if (pgrad) f += pW * p[iPrW] + pE * p[iPrE];
+        if (g.BCv[iVzSE] != 30) f += vSE * v[iVzSE];
+        if (g.BCv[iVzSW] != 30) f += vSW * v[iVzSW];
+        if (g.BCv[iVzNE] != 30) f += vNE * v[iVzNE];
+        if (g.BCv[iVzNW] != 30) f += vNW * v[iVzNW];
+        if (tS != 30 && tS != 11) f += uS * u[iVxS];
+        if (tS == 11) f += 2.0 * uS * g.BCu_val[iVxS];
+        if (tN != 30 && tN != 11) f += uN * u[iVxN];
+        if (tN == 11) f += 2.0 * uN * g.BCu_val[iVxN];
+        if (g.BCu[iVxW] != 30) f += uW * u[iVxW];
+        if (g.BCu[iVxE] != 30) f += uE * u[iVxE];
+        f -= b;
+        F = f * cv;
+    }
+}
+
+template <bool ASM, int MODE>
+__device__ void xmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, int east, int comp, Sink<MODE> &ra,
+                             Sink<MODE> &rb, double &b, double bbc_in, double &F) {
+    const DevGrid &g = a.g;
+    int nx = g.Nx, nxvz = g.nxvz;
+    double etaE, etaW, etaN = g.eta_s[c1], etaS = g.eta_s[c1 - nx];
+    if (!east) { etaE = g.eta_n[c2 + 1]; etaW = etaE; }
+    else { etaW = g.eta_n[c2]; etaE = etaW; }
+    double uS = 0, uN = 0, uW = 0, uE = 0, uC = 0, vSW = 0, vSE = 0, vNW = 0, vNE = 0, pE = 0, pW = 0;
+    double dxx = a.one_dx_dx, dzz = a.one_dz_dz, dxz = a.one_dx_dz;
+    if (!east) {
+        b -= g.BCu_val[c1] * a.one_dx;
+        if (g.BCu[c1 + 1] != 30) {
+            uC = -2.0 * dxx * (etaE) + comp * 2.0 / 3.0 * dxx * etaE;
+            uE = 2.0 * dxx * etaE - comp * 2.0 / 3.0 * dxx * etaE;
+        }
+    } else {
+        b += g.BCu_val[c1] * a.one_dx;
+        if (g.BCu[c1 - 1] != 30) {
+            uW = 2.0 * dxx * etaW - comp * 2.0 / 3.0 * dxx * etaW;
+            uC = -2.0 * dxx * (etaW) + comp * 2.0 / 3.0 * dxx * etaW;
+        }
+    }
+    int tS = g.BCu[c1 - nx], tN = g.BCu[c1 + nx];
+    if (tS != 30 && tN != 30) {
+        if (tS != 13) { uS = dzz * etaS; uC += -dzz * etaS; }
+        if (tN != 13) { uN = dzz * etaN; uC += -dzz * etaN; }
+    }
+    if (tS == 30 && tN != 30) {
+        if (tN != 13) uC += -dzz * etaN;
+        if (tN != 13) uN = dzz * etaN;
+    }
+    if (tS != 30 && tN == 30) {
+        if (tS != 13) uC += -dzz * etaS;
+        if (tS != 13) uS = dzz * etaS;
+    }
+    if (tS == 11) uC += -dzz * etaS;
+    if (tN == 11) uC += -dzz * etaN;
+    bool vs = g.BCv[c3 - nxvz] != 30 && g.BCv[c3 - nxvz + 1] != 30;
+    bool vn = g.BCv[c3 + 1] != 30 && g.BCv[c3] != 30;
+    if (vs) {
+        vSE = -dxz * etaS + comp * 2.0 / 3.0 * dxz * etaE;
+        vSW = dxz * etaS - comp * 2.0 / 3.0 * dxz * etaW;
+    }
+    if (vn) {
+        vNE = dxz * etaN - comp * 2.0 / 3.0 * dxz * etaE;
+        vNW = -dxz * etaN + comp * 2.0 / 3.0 * dxz * etaW;
+    }
+    int ip = east ? c2 : c2 + 1;
+    if (!east) { if (g.BCp[c2 + 1] != 30) { pW = a.one_dx; pE = -pW; } }
+    else { if (g.BCp[c2] != 30) pW = a.one_dx; }
+    uS = -uS; uN = -uN; uW = -uW; uE = -uE; uC = -uC; vSW = -vSW; vSE = -vSE; vNW = -vNW; vNE = -vNE; pE = -pE; pW = -pW;
+    double pcoef = east ? pW : pE;
+    double cv = a.celvol;
+    if (ASM) {
+        b *= cv;
+        if (tS != 30) ra.add(a.e.eqn_u[c1 - nx], uS * cv, tS, (tS != 11 ? 1.0 : 2.0) * g.BCu_val[c1 - nx]);
+        if (east && g.BCu[c1 - 1] != 30) ra.add(a.e.eqn_u[c1 - 1], uW * cv, g.BCu[c1 - 1], g.BCu_val[c1 - 1]);
+        ra.add(eqn, uC * cv, g.BCu[c1], g.BCu_val[c1]);
+        if (!east && g.BCu[c1 + 1] != 30) ra.add(a.e.eqn_u[c1 + 1], uE * cv, g.BCu[c1 + 1], g.BCu_val[c1 + 1]);
+        if (tN != 30) ra.add(a.e.eqn_u[c1 + nx], uN * cv, tN, (tN != 11 ? 1.0 : 2.0) * g.BCu_val[c1 + nx]);
+        if (vs) {
+            ra.add(a.e.eqn_v[c3 - nxvz], vSW * cv, g.BCv[c3 - nxvz], g.BCv_val[c3 - nxvz]);
+            ra.add(a.e.eqn_v[c3 - nxvz + 1], vSE * cv, g.BCv[c3 - nxvz + 1], g.BCv_val[c3 - nxvz + 1]);
+        }
+        if (vn) {
+            ra.add(a.e.eqn_v[c3], vNW * cv, g.BCv[c3], g.BCv_val[c3]);
+            ra.add(a.e.eqn_v[c3 + 1], vNE * cv, g.BCv[c3 + 1], g.BCv_val[c3 + 1]);
+        }
+        if (g.BCp[ip] != 30) rb.add(a.e.eqn_p[ip] - a.e.neq_mom, pcoef * cv, g.BCp[ip], g.BCp_val[ip]);
+    } else {
+        const double *u = a.u, *v = a.v, *p = a.p_mom;
+        double f = -(b + bbc_in / cv);
+        f += uC * u[c1];
+        if (tS != 30 && tS != 11) f += uS * u[c1 - nx];
+        if (east && g.BCu[c1 - 1] != 30) f += uW * u[c1 - 1];
+        if (!east && g.BCu[c1 + 1] != 30) f += uE * u[c1 + 1];
+        if (tN != 30 && tN != 11) f += uN * u[c1 + nx];
+        if (vs) {
+            int t1 = g.BCv[c3 - nxvz + 1], t0 = g.BCv[c3 - nxvz];
+            if (t1 != 11 && t1 != 0) f += vSE * v[c3 - nxvz + 1];
+            if (t0 != 11 && t0 != 0) f += vSW * v[c3 - nxvz];
+        }
+        if (vn) {
+            int t1 = g.BCv[c3 + 1], t0 = g.BCv[c3];
+            if (t1 != 11 && t1 != 0) f += vNE * v[c3 + 1];
+            if (t0 != 11 && t0 != 0) f += vNW * v[c3];
+        }
+        if (g.BCp[ip] != 30) f += pcoef * p[ip];
+        F = f * cv;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Picard Z rows                                                        */
+/* ------------------------------------------------------------------ */
+template <bool ASM, int MODE>
+__device__ void zmom_inner(const DevAsm &a, int c1, int c2, int c3, int eqn, Sink<MODE> &ra, Sink<MODE> &rb,
+                           double &b, double &F) {
+    const DevGrid &g = a.g;
+    int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz;
+    double D22S = g.D22_n[c2], D22N = g.D22_n[c2 + ncx], D33E = g.D33_s[c1], D33W = g.D33_s[c1 - 1];
+    if (g.BCp[c2 + ncx] == 31) b += g.BCv_val[c3] * a.one_dz;
+    int tVS = g.BCv[c3 - nxvz], tVN = g.BCv[c3 + nxvz];
+    double inS = 0, inN = 0;
+    if (tVS != 30 && tVN != 30) { inS = 1.0; inN = 1.0; }
+    if (tVS == 30 && tVN != 30) inN = 1.0;
+    if (tVS != 30 && tVN == 30) inS = 1.0;
+    if (g.BCp[c2] == -1) inS = 1.0;
+    if (g.BCp[c2 + ncx] == -1) inN = 1.0;
+    double inW = (g.BCg[c1 - 1] != 30 && g.BCv[c3 - 1] != 13) ? 1.0 : 0.0;
+    double inE = (g.BCg[c1] != 30 && g.BCv[c3 + 1] != 13) ? 1.0 : 0.0;
+    double dx = g.dx, dz = g.dz, comp = 1.0;
+    double vW = -D33W * inW / (dx * dx);
+    double vC = -(D22N * inN * ((1.0 / 3.0) * comp / dz - 1 / dz) - D22S * inS * (-1.0 / 3.0 * comp / dz + 1.0 / dz)) / dz
+              - (-D33E * inE / dx - D33W * inW / dx) / dx;
+    double vE = -D33E * inE / (dx * dx);
+    double vS = D22S * inS * ((1.0 / 3.0) * comp / dz - 1 / dz) / dz;
+    double vN = -D22N * inN * (-1.0 / 3.0 * comp / dz + 1.0 / dz) / dz;
+    double uSW = (1.0 / 3.0) * D22S * comp * inS / (dx * dz) - D33W * inW / (dx * dz);
+    double uSE = -1.0 / 3.0 * D22S * comp * inS / (dx * dz) + D33E * inE / (dx * dz);
+    double uNW = -1.0 / 3.0 * D22N * comp * inN / (dx * dz) + D33W * inW / (dx * dz);
+    double uNE = (1.0 / 3.0) * D22N * comp * inN / (dx * dz) - D33E * inE / (dx * dz);
+    int tW = g.BCv[c3 - 1], tE = g.BCv[c3 + 1];
+    if (tW == 11) vC -= vW;
+    if (tE == 11) vC -= vE;
+    if (a.stab == 1) {
+        double drhodz = (g.rho_n[c2 + ncx] - g.rho_n[c2]) * a.one_dz;
+        double vC_corr = 1.00 * a.om * a.dt * g.gz[c3] * drhodz;
+        if (vC + vC_corr > 0.0) vC += vC_corr;
+    }
+    double pS = 0, pN = 0;
+    bool pgrad = g.BCp[c2] != 30 && g.BCp[c2 + ncx] != 30;
+    if (pgrad) { pS = -a.one_dz * 1e0; pN = a.one_dz * 1e0; }
+    double cv = a.celvol;
+    if (ASM) {
+        b *= cv;
+        if (g.BCu[c1 + nx] != 30) ra.add(a.e.eqn_u[c1 + nx], uNE * cv, g.BCu[c1 + nx], g.BCu_val[c1 + nx]);
+        if (g.BCu[c1] != 30) ra.add(a.e.eqn_u[c1], uSE * cv, g.BCu[c1], g.BCu_val[c1]);
+        if (g.BCu[c1 + nx - 1] != 30) ra.add(a.e.eqn_u[c1 + nx - 1], uNW * cv, g.BCu[c1 + nx - 1], g.BCu_val[c1 + nx - 1]);
+        if (g.BCu[c1 - 1] != 30) ra.add(a.e.eqn_u[c1 - 1], uSW * cv, g.BCu[c1 - 1], g.BCu_val[c1 - 1]);
+        if (tVS != 30) ra.add(a.e.eqn_v[c3 - nxvz], vS * cv, tVS, g.BCv_val[c3 - nxvz]);
+        if (tW != 30 && tW != -12) ra.add(a.e.eqn_v[c3 - 1], vW * cv, tW, (tW != 11 ? 1.0 : 2.0) * g.BCv_val[c3 - 1]);
+        if (tW == -12) ra.add(a.e.eqn_v[c3 + nxvz - 3], vW * cv, g.BCv[c3 + nxvz - 3], g.BCv_val[c3 + nxvz - 3]);
+        ra.add(eqn, vC * cv, g.BCv[c3], g.BCv_val[c3]);
+        if (tE != 30 && tE != -12) ra.add(a.e.eqn_v[c3 + 1], vE * cv, tE, (tE != 11 ? 1.0 : 2.0) * g.BCv_val[c3 + 1]);
+        if (tE == -12) ra.add(a.e.eqn_v[c3 - nxvz + 3], vE * cv, g.BCv[c3 - nxvz + 3], g.BCv_val[c3 - nxvz + 3]);
+        if (tVN != 30) ra.add(a.e.eqn_v[c3 + nxvz], vN * cv, tVN, g.BCv_val[c3 + nxvz]);
+        if (pgrad) {
+            rb.add(a.e.eqn_p[c2] - a.e.neq_mom, pS * cv, g.BCp[c2], g.BCp_val[c2]);
+            rb.add(a.e.eqn_p[c2 + ncx] - a.e.neq_mom, pN * cv, g.BCp[c2 + ncx], g.BCp_val[c2 + ncx]);
+        }
+    } else {
+        const double *u = a.u, *v = a.v, *p = a.p_mom;
+        double f = vC * v[c3];
+        if (pgrad) f += pS * p[c2] + pN * p[c2 + ncx];
+        if (tVS != 30) f += vS * v[c3 - nxvz];
+        if (tW != 30 && tW != 11 && tW != -12) f += vW * v[c3 - 1];
+        if (tE != 30 && tE != 11 && tE != -12) f += vE * v[c3 + 1];
+        if (tVN != 30) f += vN * v[c3 + nxvz];
+        if (g.BCu[c1 - 1] != 30) f += uSW * u[c1 - 1];
+        if (g.BCu[c1] != 30) f += uSE * u[c1];
+        if (g.BCu[c1 + nx - 1] != 30) f += uNW * u[c1 + nx - 1];
+        if (g.BCu[c1 + nx] != 30) f += uNE * u[c1 + nx];
+        if (tW == 11) f += -2 * D33W * a.one_dx_dx * g.BCv_val[c3 - 1];
+        if (tE == 11) f += -2 * D33E * a.one_dx_dx * g.BCv_val[c3 + 1];
+        if (tW == -12) f += vW * v[c3 + nxvz - 3];
+        if (tE == -12) f += vE * v[c3 - nxvz + 3];
+        f -= b;
+        F = f * cv;
+    }
+}
+
+template <bool ASM, int MODE>
+__device__ void zmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, int north, int comp, Sink<MODE> &ra,
+                             Sink<MODE> &rb, double &b, double bbc_in, double &F) {
+    const DevGrid &g = a.g;
+    int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz;
+    double etaS, etaN, etaE = g.eta_s[c1], etaW = g.eta_s[c1 - 1];
+    if (!north) { etaN = g.eta_n[c2 + ncx]; etaS = etaN; }
+    else { etaS = g.eta_n[c2]; etaN = etaS; }
+    double uSW = 0, uSE = 0, uNW = 0, uNE = 0, vS = 0, vW = 0, vC = 0, vE = 0, vN = 0, pN = 0, pS = 0;
+    double dxx = a.one_dx_dx, dzz = a.one_dz_dz, dxz = a.one_dx_dz;
+    if (!north) b -= g.BCv_val[c3] * a.one_dz;
+    else b += g.BCv_val[c3] * a.one_dz;
+    bool us = g.BCu[c1 - 1] != 30 && g.BCu[c1] != 30;
+    bool un = g.BCu[c1 + nx - 1] != 30 && g.BCu[c1 + nx] != 30;
+    if (us) {
+        uSW = dxz * etaW - comp * 2.0 / 3.0 * etaS * dxz;
+        uSE = -dxz * etaE + comp * 2.0 / 3.0 * etaS * dxz;
+    }
+    if (un) {
+        uNW = -dxz * etaW + comp * 2.0 / 3.0 * etaN * dxz;
+        uNE = dxz * etaE - comp * 2.0 / 3.0 * etaN * dxz;
+    }
+    if (!north) {
+        if (g.BCv[c3 + nxvz] != 30) {
+            vC = -2.0 * dzz * (etaN) + comp * 2.0 / 3.0 * etaS * dzz;
+            vN = 2.0 * dzz * etaN - comp * 2.0 / 3.0 * etaN * dzz;
+        }
+    } else {
+        if (g.BCv[c3 - nxvz] != 30) {
+            vS = 2.0 * dzz * etaS - comp * 2.0 / 3.0 * etaS * dzz;
+            vC = -2.0 * dzz * (etaS) + comp * 2.0 / 3.0 * etaS * dzz;
+        }
+    }
+    int tW = g.BCv[c3 - 1], tE = g.BCv[c3 + 1];
+    if (tW != 30 && tE != 30) {
+        if (tW != 13) { vW = dxx * etaW; vC += -dxx * etaW; }
+        if (tE != 13) { vE = dxx * etaE; vC += -dxx * etaE; }
+    }
+    if (tW == 30 && tE != 30) {
+        if (tE != 13) vE = dxx * etaE;
+        if (tE != 13) vC += -dxx * etaE;
+    }
+    if (tW != 30 && tE == 30) {
+        if (tW != 13) vW = dxx * etaW;
+        if (tW != 13) vC += -dxx * etaW;
+    }
+    if (tW == 11) vC += -dxx * etaW;
+    if (tE == 11) vC += -dxx * etaE;
+    if (a.stab == 1) {
+        /* density-gradient stabilisation with the model's scalar gz */
+        double drhodz = (g.rho_n[c2 + ncx] - g.rho_n[c2]) * a.one_dz;
+        if (!north) vC += 1.00 * a.om * a.dt * a.gz_model * drhodz;
+        else vC += -1.00 * a.om * a.dt * a.gz_model * drhodz;
+    }
+    if (!north) { if (g.BCp[c2 + ncx] != 30) pN = -a.one_dz; }
+    else { if (g.BCp[c2] != 30) pS = a.one_dz; }
+    uSW = -uSW; uSE = -uSE; uNW = -uNW; uNE = -uNE; vS = -vS; vW = -vW; vC = -vC; vE = -vE; vN = -vN; pN = -pN; pS = -pS;
+    double cv = a.celvol;
+    if (ASM) {
+        b *= cv;
+        if (us) {
+            ra.add(a.e.eqn_u[c1 - 1], uSW * cv, g.BCu[c1 - 1], g.BCu_val[c1 - 1]);
+            ra.add(a.e.eqn_u[c1], uSE * cv, g.BCu[c1], g.BCu_val[c1]);
+        }
+        if (un) {
+            ra.add(a.e.eqn_u[c1 + nx - 1], uNW * cv, g.BCu[c1 + nx - 1], g.BCu_val[c1 + nx - 1]);
+            ra.add(a.e.eqn_u[c1 + nx], uNE * cv, g.BCu[c1 + nx], g.BCu_val[c1 + nx]);
+        }
+        if (north && g.BCv[c3 - nxvz] != 30) ra.add(a.e.eqn_v[c3 - nxvz], vS * cv, g.BCv[c3 - nxvz], g.BCv_val[c3 - nxvz]);
+        if (tW != 30) ra.add(a.e.eqn_v[c3 - 1], vW * cv, tW, (tW != 11 ? 1.0 : 2.0) * g.BCv_val[c3 - 1]);
+        ra.add(eqn, vC * cv, g.BCv[c3], g.BCv_val[c3]);
+        if (tE != 30) ra.add(a.e.eqn_v[c3 + 1], vE * cv, tE, (tE != 11 ? 1.0 : 2.0) * g.BCv_val[c3 + 1]);
+        if (!north && g.BCv[c3 + nxvz] != 30) ra.add(a.e.eqn_v[c3 + nxvz], vN * cv, g.BCv[c3 + nxvz], g.BCv_val[c3 + nxvz]);
+        if (!north) {
+            if (g.BCp[c2 + ncx] != 30) rb.add(a.e.eqn_p[c2 + ncx] - a.e.neq_mom, pN * cv, g.BCp[c2 + ncx], g.BCp_val[c2 + ncx]);
+        } else {
+            if (g.BCp[c2] != 30) rb.add(a.e.eqn_p[c2] - a.e.neq_mom, pS * cv, g.BCp[c2], g.BCp_val[c2]);
+        }
+    } else {
+        const double *u = a.u, *v = a.v, *p = a.p_mom;
+        double f = -(b + bbc_in / cv);
+        f += vC * v[c3];
+        if (tW != 30 && tW != 11) f += vW * v[c3 - 1];
+        if (!north && g.BCv[c3 + nxvz] != 30) f += vN * v[c3 + nxvz];
+        if (north && g.BCv[c3 - nxvz] != 30) f += vS * v[c3 - nxvz];
+        if (tE != 30 && tE != 11) f += vE * v[c3 + 1];
+        if (us) {
+            int t0 = g.BCu[c1 - 1], t1 = g.BCu[c1];
+            if (t0 != 11 && t0 != 0) f += uSW * u[c1 - 1];
+            if (t1 != 11 && t1 != 0) f += uSE * u[c1];
+        }
+        if (un) {
+            int t0 = g.BCu[c1 + nx - 1], t1 = g.BCu[c1 + nx];
+            if (t0 != 11 && t0 != 0) f += uNW * u[c1 + nx - 1];
+            if (t1 != 11 && t1 != 0) f += uNE * u[c1 + nx];
+        }
+        if (!north) { if (g.BCp[c2 + ncx] != 30) f += pN * p[c2 + ncx]; }
+        else {
+            /* the reference reads the pressure one row below (c2 - ncx) here */
+            if (g.BCp[c2 - ncx] != 30) f += pS * p[c2 - ncx];
+        }
+        F = f * cv;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Jacobian X rows                                                      */
+/* ------------------------------------------------------------------ */
+/* u slots [dj+1][di+1] around the row's Vx node; v slots [b+2][a+1]
+ * around vz(k, l); p slots [b+1][a] for centroid (k-1+a, l-1+b). */
+struct XAcc { double u[3][3]; double v[4][4]; double p[3][2]; };
+
+__device__ __forceinline__ void xa_exx(XAcc &c, int a, int b, double w, double cmp, double dx, double dz) {
+    double f = w * (1.0 - cmp / 3.0) / dx, h = w * cmp / 3.0 / dz;
+    c.u[b + 1][a + 1] += f; c.u[b + 1][a] -= f;
+    c.v[b + 2][a + 1] -= h; c.v[b + 1][a + 1] += h;
+}
+__device__ __forceinline__ void xa_ezz(XAcc &c, int a, int b, double w, double cmp, double dx, double dz) {
+    double f = w * (1.0 - cmp / 3.0) / dz, h = w * cmp / 3.0 / dx;
+    c.v[b + 2][a + 1] += f; c.v[b + 1][a + 1] -= f;
+    c.u[b + 1][a + 1] -= h; c.u[b + 1][a] += h;
+}
+__device__ __forceinline__ void xa_gxz(XAcc &c, int a, int b, double w, double dx, double dz) {
+    c.u[b + 2][a + 1] += w / dz; c.u[b + 1][a + 1] -= w / dz;
+    c.v[b + 2][a + 2] += w / dx; c.v[b + 2][a + 1] -= w / dx;
+}
+
+template <bool ASM, int MODE>
+__device__ void xjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3, int eqn, int k, int l,
+                         Sink<MODE> &ra, Sink<MODE> &rb, double &b, double &F) {
+    const DevGrid &g = a.g;
+    int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz;
+    int iVxC = c1, iVxW = iVxC - 1, iVxE = iVxC + 1, iVxS = iVxC - nx, iVxN = iVxC + nx;
+    int iVxSW = iVxS - 1, iVxSE = iVxS + 1, iVxNW = iVxN - 1, iVxNE = iVxN + 1;
+    int iVzSW = c3 - nxvz, iVzSE = c3 - nxvz + 1, iVzNW = c3, iVzNE = c3 + 1;
+    int iVzSWW = iVzSW - 1, iVzSEE = iVzSE + 1, iVzNWW = iVzNW - 1, iVzNEE = iVzNE + 1;
+    int iVzNNW = iVzNW + nxvz, iVzSSW = iVzSW - nxvz, iVzNNE = iVzNE + nxvz, iVzSSE = iVzSE - nxvz;
+    int iPrW = c2, iPrE = c2 + 1, ixyN = c1, ixyS = c1 - nx;
+    int ixySW = ixyS - 1, ixySE = ixyS + 1, ixyNW = ixyN - 1, ixyNE = ixyN + 1;
+    int iPrSW = iPrW - ncx, iPrSE = iPrE - ncx, iPrNW = iPrW + ncx, iPrNE = iPrE + ncx;
+    if (T.V(iVzSWW) == -12) iVzSWW = iVzSW + (nx - 2);
+    if (T.V(iVzNWW) == -12) iVzNWW = iVzNW + (nx - 2);
+    if (T.V(iVzSEE) == -12) iVzSEE = iVzSE - (nx - 2);
+    if (T.V(iVzNEE) == -12) iVzNEE = iVzNE - (nx - 2);
+    if (g.BCu[iVxC] == -2) {
+        iVxW = c1 + nx - 2; iPrW = c2 + ncx; iVzSW = c3 - 2; iVzNW = c3 + nxvz - 2;
+        iPrNW = iPrW + ncx; iPrSW = iPrW - ncx;
+        iVxSW = iVxW - nx; iVxNW = iVxW + nx;
+        iVzNNW = iVzNW + nxvz; iVzSSW = iVzSW - nxvz;
+        iVzSWW = iVzSW - 1; iVzNWW = iVzNW - 1;
+    }
+    double inW = T.P(iPrW) == -1 ? 1.0 : 0.0, inE = T.P(iPrE) == -1 ? 1.0 : 0.0;
+    double inS = (T.G(ixyS) != 30 && T.U(iVxS) != 13) ? 1.0 : 0.0;
+    double inN = (T.G(ixyN) != 30 && T.U(iVxN) != 13) ? 1.0 : 0.0;
+    double inSWc = 0, inSEc = 0, inNWc = 0, inNEc = 0, inSWv = 0, inSEv = 0, inNWv = 0, inNEv = 0;
+    int nzvx = a.Nz_model + 1, nz = a.Nz_model;
+    if (l > 1 || (l == 1 && T.U(iVxS) == 11)) {
+        if (T.P(iPrSW) == -1) inSWc = 1.0;
+        if (T.P(iPrSE) == -1) inSEc = 1.0;
+    }
+    if (l < nzvx - 2 || (l == nzvx - 2 && T.U(iVxN) == 11)) {
+        if (T.P(iPrNW) == -1) inNWc = 1.0;
+        if (T.P(iPrNE) == -1) inNEc = 1.0;
+    }
+    if ((k > 0) || (k == 0 && (T.V(iVzSW) == -1 || T.V(iVzSW) == 0))) {
+        if (T.G(ixySW) != 30) inSWv = 1.0;
+        if (T.G(ixyNW) != 30) inNWv = 1.0;
+    }
+    if ((k < nx - 1) || (k == nx - 1 && (T.V(iVzSE) == -1 || T.V(iVzSE) == 0))) {
+        if (T.G(ixySE) != 30) inSEv = 1.0;
+        if (T.G(ixyNE) != 30) inNEv = 1.0;
+    }
+    double wE = inN + inS + inNEv + inSEv, wW = inN + inS + inNWv + inSWv;
+    double wS = inW + inE + inSWc + inSEc, wN = inW + inE + inNWc + inNEc;
+    if (wW > 1.0) wW = 1.0 / wW;
+    if (wE > 1.0) wE = 1.0 / wE;
+    if (wS > 1.0) wS = 1.0 / wS;
+    if (wN > 1.0) wN = 1.0 / wN;
+
+    double uSW = 0, uS = 0, uSE = 0, uW = 0, uC = 0, uE = 0, uNW = 0, uN = 0, uNE = 0;
+    double vSSW = 0, vSSE = 0, vSWW = 0, vSW = 0, vSE = 0, vSEE = 0, vNWW = 0, vNW = 0, vNE = 0, vNEE = 0, vNNW = 0, vNNE = 0;
+    double pSW = 0, pSE = 0, pW = 0, pE = 0, pNW = 0, pNE = 0;
+    if (MODE == SINK_VALUES) {
+        double dx = g.dx, dz = g.dz, cmp = 1.0;
+        double D11E = g.D11_n[iPrE], D12E = g.D12_n[iPrE], D13E = g.D13_n[iPrE], D14E = g.D14_n[iPrE];
+        double D11W = g.D11_n[iPrW], D12W = g.D12_n[iPrW], D13W = g.D13_n[iPrW], D14W = g.D14_n[iPrW];
+        double D31N = g.D31_s[ixyN], D32N = g.D32_s[ixyN], D33N = g.D33_s[ixyN], D34N = g.D34_s[ixyN];
+        double D31S = g.D31_s[ixyS], D32S = g.D32_s[ixyS], D33S = g.D33_s[ixyS], D34S = g.D34_s[ixyS];
+        XAcc c;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) c.u[i][j] = 0.0;
+#pragma unroll
+            for (int j = 0; j < 2; j++) c.p[i][j] = 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) c.v[i][j] = 0.0;
+        /* normal stress at centroids W (aa=0) and E (aa=1) */
+#pragma unroll
+        for (int aa = 0; aa < 2; aa++) {
+            double f = aa == 0 ? inW / dx : -inE / dx;
+            double D1 = aa == 0 ? D11W : D11E, D2 = aa == 0 ? D12W : D12E, D3 = aa == 0 ? D13W : D13E, D4 = aa == 0 ? D14W : D14E;
+            double w = aa == 0 ? wW : wE;
+            xa_exx(c, aa, 0, f * D1, cmp, dx, dz);
+            xa_ezz(c, aa, 0, f * D2, cmp, dx, dz);
+            double fSo = aa == 0 ? inSWv : inSEv, fNo = aa == 0 ? inNWv : inNEv;
+            xa_gxz(c, 0, -1, f * D3 * w * inS, dx, dz);
+            xa_gxz(c, 0, 0, f * D3 * w * inN, dx, dz);
+            xa_gxz(c, aa == 0 ? -1 : 1, -1, f * D3 * w * fSo, dx, dz);
+            xa_gxz(c, aa == 0 ? -1 : 1, 0, f * D3 * w * fNo, dx, dz);
+            c.p[1][aa] += f * D4;
+        }
+        /* shear stress at vertices S (bb=0) and N (bb=1) */
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++) {
+            int bv = bb == 0 ? -1 : 0;
+            double f = bb == 0 ? inS / dz : -inN / dz;
+            double D1 = bb == 0 ? D31S : D31N, D2 = bb == 0 ? D32S : D32N, D3 = bb == 0 ? D33S : D33N, D4 = bb == 0 ? D34S : D34N;
+            double w = bb == 0 ? wS : wN;
+            int cb = bb == 0 ? -1 : 1;
+            double fl[4] = {inW, inE, bb == 0 ? inSWc : inNWc, bb == 0 ? inSEc : inNEc};
+            const int ca[4] = {0, 1, 0, 1};
+            int cr[4] = {0, 0, cb, cb};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                double wq = w * fl[q];
+                xa_exx(c, ca[q], cr[q], f * D1 * wq, cmp, dx, dz);
+                xa_ezz(c, ca[q], cr[q], f * D2 * wq, cmp, dx, dz);
+                c.p[cr[q] + 1][ca[q]] += f * D4 * wq;
+            }
+            xa_gxz(c, 0, bv, f * D3, dx, dz);
+        }
+        c.p[1][0] += -inW / dx;
+        c.p[1][1] += inE / dx;
+        uSW = c.u[0][0]; uS = c.u[0][1]; uSE = c.u[0][2]; uW = c.u[1][0]; uC = c.u[1][1]; uE = c.u[1][2];
+        uNW = c.u[2][0]; uN = c.u[2][1]; uNE = c.u[2][2];
+        vSSW = c.v[0][1]; vSSE = c.v[0][2]; vSWW = c.v[1][0]; vSW = c.v[1][1]; vSE = c.v[1][2]; vSEE = c.v[1][3];
+        vNWW = c.v[2][0]; vNW = c.v[2][1]; vNE = c.v[2][2]; vNEE = c.v[2][3]; vNNW = c.v[3][1]; vNNE = c.v[3][2];
+        pSW = c.p[0][0]; pSE = c.p[0][1]; pW = c.p[1][0]; pE = c.p[1][1]; pNW = c.p[2][0]; pNE = c.p[2][1];
+        if (a.stab == 1) {
+            double drhodx = (g.rho_n[c2 + 1] - g.rho_n[c2]) * a.one_dx;
+            double uC_corr = 1.00 * a.om * a.dt * g.gx[c1] * drhodx;
+            if (uC + uC_corr > 0.0) uC += uC_corr;
+        }
+        if (T.U(iVxS) == 11) uC -= uS;
+        if (T.U(iVxN) == 11) uC -= uN;
+        if (T.U(iVxSW) == 11) uW -= uSW;
+        if (T.U(iVxSE) == 11) uE -= uSE;
+        if (T.U(iVxNW) == 11) uW -= uNW;
+        if (T.U(iVxNE) == 11) uE -= uNE;
+        if (T.V(iVzNWW) == 11) vNW -= vNWW;
+        if (T.V(iVzNEE) == 11) vNE -= vNEE;
+        if (T.V(iVzSWW) == 11) vSW -= vSWW;
+        if (T.V(iVzSEE) == 11) vSE -= vSEE;
+    }
+    double cv = a.celvol;
+    if (ASM) {
+        b *= cv;
+        const int *eu = a.e.eqn_u, *ev = a.e.eqn_v, *ep = a.e.eqn_p;
+        const double *uv = g.BCu_val, *vv = g.BCv_val, *pv = g.BCp_val;
+#define UADD11(idx, coef) \
+    if (g.BCu[idx] != 30) ra.add(eu[idx], (coef)*cv, g.BCu[idx], (g.BCu[idx] != 11 ? 1.0 : 2.0) * uv[idx]);
+        UADD11(iVxSW, uSW)
+        UADD11(iVxS, uS)
+        UADD11(iVxSE, uSE)
+        if (g.BCu[iVxW] != 30) ra.add(eu[iVxW], uW * cv, g.BCu[iVxW], uv[iVxW]);
+        ra.add(eqn, uC * cv, g.BCu[iVxC], uv[iVxC]);
+        if (g.BCu[iVxE] != 30) ra.add(eu[iVxE], uE * cv, g.BCu[iVxE], uv[iVxE]);
+        UADD11(iVxNW, uNW)
+        UADD11(iVxN, uN)
+        UADD11(iVxNE, uNE)
+#undef UADD11
+#define VADD(idx, coef) ra.add(ev[idx], (coef)*cv, g.BCv[idx], vv[idx])
+        if (l > 1 && g.BCv[iVzSSW] != 30) VADD(iVzSSW, vSSW);
+        if (l > 1 && g.BCv[iVzSSE] != 30) VADD(iVzSSE, vSSE);
+        if ((k > 1) || ((k == 0 || k == 1) && g.BCv[iVzSWW] == -1))
+            if (g.BCv[iVzSWW] != 30) VADD(iVzSWW, vSWW);
+        if (g.BCv[iVzSW] != 30) VADD(iVzSW, vSW);
+        if (g.BCv[iVzSE] != 30) VADD(iVzSE, vSE);
+        if ((k < nx - 2) || ((k == nx - 2 || k == nx - 1) && g.BCv[iVzSEE] == -1))
+            if (g.BCv[iVzSEE] != 30) VADD(iVzSEE, vSEE);
+        if ((k > 1) || ((k == 0 || k == 1) && g.BCv[iVzNWW] == -1))
+            if (g.BCv[iVzNWW] != 30) VADD(iVzNWW, vNWW);
+        if (g.BCv[iVzNW] != 30) VADD(iVzNW, vNW);
+        if (g.BCv[iVzNE] != 30) VADD(iVzNE, vNE);
+        if ((k < nx - 2) || ((k == nx - 2 || k == nx - 1) && g.BCv[iVzNEE] == -1))
+            if (g.BCv[iVzNEE] != 30) VADD(iVzNEE, vNEE);
+        if (l < nz - 1 && g.BCv[iVzNNW] != 30) VADD(iVzNNW, vNNW);
+        if (l < nz - 1 && g.BCv[iVzNNE] != 30) VADD(iVzNNE, vNNE);
+#undef VADD
+#define PADD(idx, coef) rb.add(ep[idx] - a.e.neq_mom, (coef)*cv, g.BCp[idx], pv[idx])
+        if (l > 1) {
+            if (g.BCp[iPrSW] != 30) PADD(iPrSW, pSW);
+            if (g.BCp[iPrSE] != 30) PADD(iPrSE, pSE);
+        }
+        if (g.BCp[iPrE] != 30 && g.BCp[c2] != 30) { PADD(iPrW, pW); PADD(iPrE, pE); }
+        if (l < nz - 1) {
+            if (g.BCp[iPrNW] != 30) PADD(iPrNW, pNW);
+            if (g.BCp[iPrNE] != 30) PADD(iPrNE, pNE);
+        }
+#undef PADD
+    } else {
+        double f = 0.0;
+        f += (g.sxxd[iPrE] - g.sxxd[iPrW]) / g.dx;
+        f -= (g.p_in[iPrE] - g.p_in[iPrW]) / g.dx;
+        f += (g.sxz[ixyN] - g.sxz[ixyS]) / g.dz;
+        f *= -1.0;
+        f -= b;
+        F = f * cv;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Jacobian Z rows                                                      */
+/* ------------------------------------------------------------------ */
+/* v slots [dj+1][di+1] around the row's Vz node; u slots [b+1][a+2]
+ * around u(k, l); p slots [b][a+1] for centroid (k-1+a, l-1+b). */
+struct ZAcc { double v[3][3]; double u[4][4]; double p[2][3]; };
+
+__device__ __forceinline__ void za_exx(ZAcc &c, int a, int b, double w, double cmp, double dx, double dz) {
+    double f = w * (1.0 - cmp / 3.0) / dx, h = w * cmp / 3.0 / dz;
+    c.u[b + 1][a + 2] += f; c.u[b + 1][a + 1] -= f;
+    c.v[b + 1][a + 1] -= h; c.v[b][a + 1] += h;
+}
+__device__ __forceinline__ void za_ezz(ZAcc &c, int a, int b, double w, double cmp, double dx, double dz) {
+    double f = w * (1.0 - cmp / 3.0) / dz, h = w * cmp / 3.0 / dx;
+    c.v[b + 1][a + 1] += f; c.v[b][a + 1] -= f;
+    c.u[b + 1][a + 2] -= h; c.u[b + 1][a + 1] += h;
+}
+__device__ __forceinline__ void za_gxz(ZAcc &c, int a, int b, double w, double dx, double dz) {
+    c.u[b + 2][a + 1] += w / dz; c.u[b + 1][a + 1] -= w / dz;
+    c.v[b + 1][a + 1] += w / dx; c.v[b + 1][a] -= w / dx;
+}
+
+template <bool ASM, int MODE>
+__device__ void zjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3, int eqn, int k, int l,
+                         Sink<MODE> &ra, Sink<MODE> &rb, double &b, double &F) {
+    const DevGrid &g = a.g;
+    int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz;
+    int periodix = a.periodic;
+    int iVzC = c3, iVzW = iVzC - 1, iVzE = iVzC + 1, iVzS = c3 - nxvz, iVzN = c3 + nxvz;
+    int iVzSW = iVzS - 1, iVzSE = iVzS + 1, iVzNW = iVzN - 1, iVzNE = iVzN + 1;
+    int iVxNW = c1 + nx - 1, iVxNE = c1 + nx, iVxSW = c1 - 1, iVxSE = c1;
+    int iVxNWW = iVxNW - 1, iVxNEE = iVxNE + 1, iVxSWW = iVxSW - 1, iVxSEE = iVxSE + 1;
+    int iVxNNW = iVxNW + nx, iVxNNE = iVxNE + nx, iVxSSW = iVxSW - nx, iVxSSE = iVxSE - nx;
+    int iPrS = c2, iPrSW = iPrS - 1, iPrSE = iPrS + 1, iPrN = c2 + ncx, iPrNW = iPrN - 1, iPrNE = iPrN + 1;
+    int ixyW = c1 - 1, ixySW = ixyW - nx, ixyNW = ixyW + nx, ixyE = c1, ixySE = ixyE - nx, ixyNE = ixyE + nx;
+    if (k == 1 && g.BCv[iVzW] == -12) {
+        iVxNWW = iVxNW + (nx - 2); iVxSWW = iVxSW + (nx - 2);
+        iPrNW = iPrN + (ncx - 1); iPrSW = iPrS + (ncx - 1);
+        iVzW = iVzC + (nx - 2); iVzSW = iVzW - nxvz; iVzNW = iVzW + nxvz;
+    }
+    if (k == nx - 1 && g.BCv[iVzE] == -12) {
+        iVxNEE = iVxNE - (nx - 2); iVxSEE = iVxSE - (nx - 2);
+        iPrNE = iPrN - (ncx - 1); iPrSE = iPrS - (ncx - 1);
+        iVzE = iVzC - (nx - 2); iVzNE = iVzE + nxvz; iVzSE = iVzE - nxvz;
+    }
+    double inS = T.P(iPrS) == -1 ? 1.0 : 0.0, inN = T.P(iPrN) == -1 ? 1.0 : 0.0;
+    double inW = (T.G(ixyW) != 30 && T.V(iVzW) != 13) ? 1.0 : 0.0;
+    double inE = (T.G(ixyE) != 30 && T.V(iVzE) != 13) ? 1.0 : 0.0;
+    double inSWc = 0, inSEc = 0, inNWc = 0, inNEc = 0, inSWv = 0, inSEv = 0, inNWv = 0, inNEv = 0;
+    int nz = a.Nz_model;
+    if ((k > 1) || (k == 1 && T.V(iVzW) == -1)) {
+        if (T.P(iPrSW) == -1) inSWc = 1.0;
+        if (T.P(iPrNW) == -1) inNWc = 1.0;
+    }
+    if ((k < nxvz - 2) || (k == nxvz - 2 && T.V(iVzE) == -1)) {
+        if (T.P(iPrSE) == -1) inSEc = 1.0;
+        if (T.P(iPrNE) == -1) inNEc = 1.0;
+    }
+    if (l > 0) {
+        if (T.G(ixySW) != 30) inSWv = 1.0;
+        if (T.G(ixySE) != 30) inSEv = 1.0;
+    }
+    if (l < nz - 1) {
+        if (T.G(ixyNW) != 30) inNWv = 1.0;
+        if (T.G(ixyNE) != 30) inNEv = 1.0;
+    }
+    double wE = inN + inS + inNEc + inSEc, wW = inN + inS + inNWc + inSWc;
+    double wS = inW + inE + inSWv + inSEv, wN = inW + inE + inNWv + inNEv;
+    if (wW > 1.0) wW = 1.0 / wW;
+    if (wE > 1.0) wE = 1.0 / wE;
+    if (wS > 1.0) wS = 1.0 / wS;
+    if (wN > 1.0) wN = 1.0 / wN;
+
+    double vSW = 0, vS = 0, vSE = 0, vW = 0, vC = 0, vE = 0, vNW = 0, vN = 0, vNE = 0;
+    double uSSW = 0, uSSE = 0, uSWW = 0, uSW = 0, uSE = 0, uSEE = 0, uNWW = 0, uNW = 0, uNE = 0, uNEE = 0, uNNW = 0, uNNE = 0;
+    double pSW = 0, pS = 0, pSE = 0, pNW = 0, pN = 0, pNE = 0;
+    if (MODE == SINK_VALUES) {
+        double dx = g.dx, dz = g.dz, cmp = 1.0;
+        double D31W = g.D31_s[ixyW], D32W = g.D32_s[ixyW], D33W = g.D33_s[ixyW], D34W = g.D34_s[ixyW];
+        double D31E = g.D31_s[ixyE], D32E = g.D32_s[ixyE], D33E = g.D33_s[ixyE], D34E = g.D34_s[ixyE];
+        double D21S = g.D21_n[iPrS], D22S = g.D22_n[iPrS], D23S = g.D23_n[iPrS], D24S = g.D24_n[iPrS];
+        double D21N = g.D21_n[iPrN], D22N = g.D22_n[iPrN], D23N = g.D23_n[iPrN], D24N = g.D24_n[iPrN];
+        ZAcc c;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) c.v[i][j] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) c.u[i][j] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) c.p[i][j] = 0.0;
+        /* normal stress at centroids S (bb=0) and N (bb=1) */
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++) {
+            double f = bb == 0 ? inS / dz : -inN / dz;
+            double D1 = bb == 0 ? D21S : D21N, D2 = bb == 0 ? D22S : D22N, D3 = bb == 0 ? D23S : D23N, D4 = bb == 0 ? D24S : D24N;
+            double w = bb == 0 ? wS : wN;
+            za_exx(c, 0, bb, f * D1, cmp, dx, dz);
+            za_ezz(c, 0, bb, f * D2, cmp, dx, dz);
+            int vb = bb == 0 ? -1 : 1;
+            double fo_w = bb == 0 ? inSWv : inNWv, fo_e = bb == 0 ? inSEv : inNEv;
+            za_gxz(c, 0, 0, f * D3 * w * inW, dx, dz);
+            za_gxz(c, 1, 0, f * D3 * w * inE, dx, dz);
+            za_gxz(c, 0, vb, f * D3 * w * fo_w, dx, dz);
+            za_gxz(c, 1, vb, f * D3 * w * fo_e, dx, dz);
+            c.p[bb][1] += f * D4;
+        }
+        /* shear stress at vertices W (aa=0) and E (aa=1) */
+#pragma unroll
+        for (int aa = 0; aa < 2; aa++) {
+            double f = aa == 0 ? inW / dx : -inE / dx;
+            double D1 = aa == 0 ? D31W : D31E, D2 = aa == 0 ? D32W : D32E, D3 = aa == 0 ? D33W : D33E, D4 = aa == 0 ? D34W : D34E;
+            double w = aa == 0 ? wW : wE;
+            int co = aa == 0 ? -1 : 1;
+            double fl[4] = {inS, inN, aa == 0 ? inSWc : inSEc, aa == 0 ? inNWc : inNEc};
+            int ca[4] = {0, 0, co, co};
+            const int cb[4] = {0, 1, 0, 1};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                double wq = w * fl[q];
+                za_exx(c, ca[q], cb[q], f * D1 * wq, cmp, dx, dz);
+                za_ezz(c, ca[q], cb[q], f * D2 * wq, cmp, dx, dz);
+                c.p[cb[q]][ca[q] + 1] += f * D4 * wq;
+            }
+            za_gxz(c, aa, 0, f * D3, dx, dz);
+        }
+        c.p[0][1] += -inS / dz;
+        c.p[1][1] += inN / dz;
+        vSW = c.v[0][0]; vS = c.v[0][1]; vSE = c.v[0][2]; vW = c.v[1][0]; vC = c.v[1][1]; vE = c.v[1][2];
+        vNW = c.v[2][0]; vN = c.v[2][1]; vNE = c.v[2][2];
+        uSSW = c.u[0][1]; uSSE = c.u[0][2]; uSWW = c.u[1][0]; uSW = c.u[1][1]; uSE = c.u[1][2]; uSEE = c.u[1][3];
+        uNWW = c.u[2][0]; uNW = c.u[2][1]; uNE = c.u[2][2]; uNEE = c.u[2][3]; uNNW = c.u[3][1]; uNNE = c.u[3][2];
+        pSW = c.p[0][0]; pS = c.p[0][1]; pSE = c.p[0][2]; pNW = c.p[1][0]; pN = c.p[1][1]; pNE = c.p[1][2];
+        if (a.stab == 1) {
+            double drhodz = (g.rho_n[c2 + ncx] - g.rho_n[c2]) * a.one_dz;
+            double vC_corr = 1.00 * a.om * a.dt * g.gz[c3] * drhodz;
+            if (vC + vC_corr > 0.0) vC += vC_corr;
+        }
+        if (T.V(iVzW) == 11) vC -= vW;
+        if (T.V(iVzE) == 11) vC -= vE;
+        if (T.V(iVzSW) == 11) vW -= vSW;
+        if (T.V(iVzSE) == 11) vE -= vSE;
+        if (T.V(iVzNW) == 11) vW -= vNW;
+        if (T.V(iVzNE) == 11) vE -= vNE;
+        if (T.U(iVxNNW) == 11) uNW -= uNNW;
+        if (T.U(iVxNNE) == 11) uNE -= uNNE;
+        if (T.U(iVxSSW) == 11) uSW -= uSSW;
+        if (T.U(iVxSSE) == 11) uSE -= uSSE;
+    }
+    double cv = a.celvol;
+    if (ASM) {
+        b *= cv;
+        const int *eu = a.e.eqn_u, *ev = a.e.eqn_v, *ep = a.e.eqn_p;
+        const double *uv = g.BCu_val, *vv = g.BCv_val, *pv = g.BCp_val;
+#define UADD(idx, coef) ra.add(eu[idx], (coef)*cv, g.BCu[idx], uv[idx])
+#define VADD(idx, coef) ra.add(ev[idx], (coef)*cv, g.BCv[idx], vv[idx])
+        if (g.BCu[iVxSSW] != 30) UADD(iVxSSW, uSSW);
+        if (g.BCu[iVxSSE] != 30) UADD(iVxSSE, uSSE);
+        if ((k > 1) || (k == 1 && g.BCu[iVxSWW] == -1))
+            if (g.BCu[iVxSWW] != 30) UADD(iVxSWW, uSWW);
+        if (g.BCu[iVxSW] != 30) UADD(iVxSW, uSW);
+        if (g.BCu[iVxSE] != 30) UADD(iVxSE, uSE);
+        if ((k < nx - 1) || (k == nx - 1 && g.BCu[iVxSEE] == -1))
+            if (g.BCu[c1 + 1] != 30) UADD(iVxSEE, uSEE);
+        if ((k > 1) || (k == 1 && g.BCu[iVxNWW] == -1))
+            if (g.BCu[c1 + nx - 2] != 30) UADD(iVxNWW, uNWW);
+        if (g.BCu[iVxNW] != 30) UADD(iVxNW, uNW);
+        if (g.BCu[iVxNE] != 30) UADD(iVxNE, uNE);
+        if ((k < nx - 1) || (k == nx - 1 && g.BCu[iVxNEE] == -1))
+            if (g.BCu[iVxNEE] != 30) UADD(iVxNEE, uNEE);
+        if (g.BCu[iVxNNW] != 30) UADD(iVxNNW, uNNW);
+        if (g.BCu[iVxNNE] != 30) UADD(iVxNNE, uNNE);
+        if (g.BCv[iVzSW] == -1) VADD(iVzSW, vSW);
+        if (g.BCv[iVzS] != 30) VADD(iVzS, vS);
+        if (g.BCv[iVzSE] == -1) VADD(iVzSE, vSE);
+        if (g.BCv[iVzW] == -1) VADD(iVzW, vW);
+        ra.add(eqn, vC * cv, g.BCv[iVzC], vv[iVzC]);
+        if (g.BCv[iVzE] == -1) VADD(iVzE, vE);
+        if (g.BCv[iVzNW] == -1) VADD(iVzNW, vNW);
+        if (g.BCv[iVzN] != 30) VADD(iVzN, vN);
+        if (g.BCv[iVzNE] == -1) VADD(iVzNE, vNE);
+#undef UADD
+#undef VADD
+#define PADD(idx, coef, vidx) rb.add(ep[idx] - a.e.neq_mom, (coef)*cv, g.BCp[idx], pv[vidx])
+        if (k > 1 || (k == 1 && periodix == 1))
+            if (g.BCp[iPrSW] != 30) PADD(iPrSW, pSW, iPrSW);
+        if (g.BCp[iPrS] != 30 && g.BCp[iPrN] != 30) PADD(iPrS, pS, iPrS);
+        if (k < nx - 1 || (k == nx - 1 && periodix == 1))
+            if (g.BCp[iPrSE] != 30) PADD(iPrSE, pSE, iPrSE);
+        if (k > 1 || (k == 1 && periodix == 1))
+            if (g.BCp[iPrNW] != 30) PADD(iPrNW, pNW, iPrNW);
+        if (g.BCp[iPrS] != 30 && g.BCp[iPrN] != 30) PADD(iPrN, pN, c2 + ncx);
+        if (k < nx - 1 || (k == nx - 1 && periodix == 1))
+            if (g.BCp[iPrNE] != 30) PADD(iPrNE, pNE, iPrNE);
+#undef PADD
+    } else {
+        double f = 0.0;
+        f += (g.szzd[iPrN] - g.szzd[iPrS]) / g.dz;
+        f -= (g.p_in[iPrN] - g.p_in[iPrS]) / g.dz;
+        f += (g.sxz[ixyE] - g.sxz[ixyW]) / g.dx;
+        f *= -1.0;
+        f -= b;
+        F = f * cv;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* row dispatch                                                         */
+/* ------------------------------------------------------------------ */
+
+/* One momentum row: which stencil applies (StokesAssemblyDecoupled.c:1691-1729,
+ * 1763-1800, 3952-4060). */
+template <bool ASM, int MODE>
+__device__ void momentum_row(int kind, const DevAsm &a, const Tags &T, int eqn, Sink<MODE> &ra, Sink<MODE> &rb,
+                             double &b, double bbc_in, double &F) {
+    const DevGrid &g = a.g;
+    int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz, nz = g.Nz, nzvx = g.nzvx;
+    int node = a.e.mom_node[eqn];
+    int nu = nx * nzvx;
+    int comp_builder = kind == 1 ? 1 : a.comp_model;
+    if (node < nu) {
+        int c1 = node, k = c1 % nx, l = c1 / nx;
+        int c2 = k - 1 + (l - 1) * ncx, c3 = k + l * nxvz;
+        int t = g.BCu[c1];
+        bool inner = l > 0 && l < nzvx - 1 && k > 0 && k < nx - 1;
+        if (inner || (k == 0 && t == -2)) {
+            if (kind == 0) xmom_inner<ASM, MODE>(a, c1, c2, c3, eqn, ra, rb, b, F);
+            else xjac_row<ASM, MODE>(a, T, c1, c2, c3, eqn, k, l, ra, rb, b, F);
+        }
+        if (k == 0 && t == 2) xmom_neumann<ASM, MODE>(a, c1, c2, c3, eqn, 0, comp_builder, ra, rb, b, bbc_in, F);
+        if (k == nx - 1 && t == 2) xmom_neumann<ASM, MODE>(a, c1, c2, c3, eqn, 1, comp_builder, ra, rb, b, bbc_in, F);
+    } else {
+        int c3 = node - nu, k = c3 % nxvz, l = c3 / nxvz;
+        int c1 = k + l * nx, c2 = k - 1 + (l - 1) * ncx;
+        int t = g.BCv[c3];
+        bool inner = k > 0 && k < nxvz - 1 && l > 0 && l < nz - 1;
+        if (inner) {
+            if (kind == 0) zmom_inner<ASM, MODE>(a, c1, c2, c3, eqn, ra, rb, b, F);
+            else zjac_row<ASM, MODE>(a, T, c1, c2, c3, eqn, k, l, ra, rb, b, F);
+        }
+        if (l == 0 && t == 2) zmom_neumann<ASM, MODE>(a, c1, c2, c3, eqn, 0, comp_builder, ra, rb, b, bbc_in, F);
+        if (l == nz - 1 && t == 2) zmom_neumann<ASM, MODE>(a, c1, c2, c3, eqn, 1, comp_builder, ra, rb, b, bbc_in, F);
+    }
+}
+
+__device__ __forceinline__ double mom_rhs(const DevAsm &a, int eqn) {
+    int node = a.e.mom_node[eqn], nu = a.g.Nx * a.g.nzvx;
+    return node < nu ? a.g.roger_x[node] : a.g.roger_z[node - nu];
+}
+
+/* pattern: per-row counts */
+__global__ void __launch_bounds__(BLK) k_count_mom(int kind, DevAsm a, int *cntA, int *cntB) {
+    int eqn = blockIdx.x * BLK + threadIdx.x;
+    if (eqn >= a.e.neq_mom) return;
+    Tags T = make_tags(a.g);
+    Sink<SINK_COUNT> ra{0, nullptr, nullptr, 0.0}, rb{0, nullptr, nullptr, 0.0};
+    double b = 0.0, F = 0.0;
+    momentum_row<true, SINK_COUNT>(kind, a, T, eqn, ra, rb, b, 0.0, F);
+    cntA[eqn] = ra.n;
+    cntB[eqn] = rb.n;
+}
+
+__device__ __forceinline__ void cont_geom(const DevAsm &a, int eqn, int &c1, int &c2, int &c3) {
+    c2 = a.e.cont_node[eqn];
+    int k = c2 % a.g.ncx, l = c2 / a.g.ncx;
+    c1 = k + (l + 1) * a.g.Nx;
+    c3 = k + l * a.g.nxvz + 1;
+}
+
+__global__ void __launch_bounds__(BLK) k_count_cont(DevAsm a, int *cntC) {
+    int eqn = blockIdx.x * BLK + threadIdx.x;
+    if (eqn >= a.e.neq_cont) return;
+    int c1, c2, c3;
+    cont_geom(a, eqn, c1, c2, c3);
+    Sink<SINK_COUNT> rc{0, nullptr, nullptr, 0.0};
+    double b = 0.0, F = 0.0;
+    continuity_row<true, SINK_COUNT>(a, 0, c1, c2, c3, rc, b, F);
+    cntC[eqn] = rc.n;
+}
+
+__global__ void __launch_bounds__(BLK) k_pattern_mom(int kind, DevAsm a, DevBlock A, DevBlock B) {
+    int eqn = blockIdx.x * BLK + threadIdx.x;
+    if (eqn >= a.e.neq_mom) return;
+    Tags T = make_tags(a.g);
+    Sink<SINK_PATTERN> ra{0, A.J + A.Ic[eqn], nullptr, 0.0}, rb{0, B.J + B.Ic[eqn], nullptr, 0.0};
+    double b = 0.0, F = 0.0;
+    momentum_row<true, SINK_PATTERN>(kind, a, T, eqn, ra, rb, b, 0.0, F);
+}
+
+__global__ void __launch_bounds__(BLK) k_pattern_cont(DevAsm a, DevBlock C) {
+    int eqn = blockIdx.x * BLK + threadIdx.x;
+    if (eqn >= a.e.neq_cont) return;
+    int c1, c2, c3;
+    cont_geom(a, eqn, c1, c2, c3);
+    Sink<SINK_PATTERN> rc{0, C.J + C.Ic[eqn], nullptr, 0.0};
+    double b = 0.0, F = 0.0;
+    continuity_row<true, SINK_PATTERN>(a, 0, c1, c2, c3, rc, b, F);
+}
+
+/* values (Assemble = 1): A/B values, bbc, final b = rhs*celvol (+/- Neumann) + bbc */
+__global__ void __launch_bounds__(BLK) k_values_mom(int kind, DevAsm a, DevBlock A, DevBlock B) {
+    int eqn = blockIdx.x * BLK + threadIdx.x;
+    if (eqn >= a.e.neq_mom) return;
+    Tags T = make_tags(a.g);
+    Sink<SINK_VALUES> ra{0, nullptr, A.A + A.Ic[eqn], 0.0}, rb{0, nullptr, B.A + B.Ic[eqn], 0.0};
+    double b = mom_rhs(a, eqn), F = 0.0;
+    momentum_row<true, SINK_VALUES>(kind, a, T, eqn, ra, rb, b, 0.0, F);
+    A.bbc[eqn] = ra.bbc;
+    B.bbc[eqn] = rb.bbc;
+    double bf = b + ra.bbc;
+    A.b[eqn] = bf;
+    B.b[eqn] = bf;
+}
+
+__global__ void __launch_bounds__(BLK) k_values_cont(int kind, DevAsm a, DevBlock C, DevBlock D) {
+    int eqn = blockIdx.x * BLK + threadIdx.x;
+    if (eqn >= a.e.neq_cont) return;
+    int c1, c2, c3;
+    cont_geom(a, eqn, c1, c2, c3);
+    Sink<SINK_VALUES> rc{0, nullptr, C.A + C.Ic[eqn], 0.0};
+    double b = a.g.rhs_p[c2], F = 0.0;
+    continuity_row<true, SINK_VALUES>(a, 0, c1, c2, c3, rc, b, F);
+    C.bbc[eqn] = rc.bbc;
+    D.bbc[eqn] = 0.0;
+    double bf = b + rc.bbc;
+    C.b[eqn] = bf;
+    D.b[eqn] = bf;
+    (void)kind;
+}
+
+/* residual (Assemble = 0): b = rhs (Neumann rows adjusted), F */
+__global__ void __launch_bounds__(BLK) k_residual_mom(int kind, DevAsm a, DevBlock A) {
+    int eqn = blockIdx.x * BLK + threadIdx.x;
+    if (eqn >= a.e.neq_mom) return;
+    Tags T = make_tags(a.g);
+    Sink<SINK_COUNT> ra{0, nullptr, nullptr, 0.0}, rb{0, nullptr, nullptr, 0.0};
+    double b = mom_rhs(a, eqn), F = A.F[eqn];
+    momentum_row<false, SINK_COUNT>(kind, a, T, eqn, ra, rb, b, A.bbc[eqn], F);
+    A.b[eqn] = b;
+    A.F[eqn] = F;
+}
+
+__global__ void __launch_bounds__(BLK) k_residual_cont(int kind, DevAsm a, DevBlock C, DevBlock D) {
+    int eqn = blockIdx.x * BLK + threadIdx.x;
+    if (eqn >= a.e.neq_cont) return;
+    int c1, c2, c3;
+    cont_geom(a, eqn, c1, c2, c3);
+    /* the Jacobian builder reads comp_cells at the vertex-family index c1 */
+    int comp = kind == 1 ? a.g.comp_cells[c1] : a.g.comp_cells[c2];
+    Sink<SINK_COUNT> rc{0, nullptr, nullptr, 0.0};
+    double b = a.g.rhs_p[c2], F = 0.0;
+    continuity_row<false, SINK_COUNT>(a, comp, c1, c2, c3, rc, b, F);
+    C.b[eqn] = b;
+    C.F[eqn] = F;
+    D.F[eqn] = F;
+}
+
+/* ------------------------------------------------------------------ */
+/* scan, numbering                                                      */
+/* ------------------------------------------------------------------ */
+#define SCAN_BLK 1024
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+/* block-local exclusive scan; writes the block total */
+__global__ void __launch_bounds__(SCAN_BLK) k_scan_local(int *data, int n, int *block_sums) {
+    __shared__ int wsum[SCAN_BLK / 64];
+    int i = blockIdx.x * SCAN_BLK + threadIdx.x;
+    int v = i < n ? data[i] : 0;
+    int inc = wave_incl_scan(v);
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        int s = lane < SCAN_BLK / 64 ? wsum[lane] : 0;
+        int si = wave_incl_scan(s);
+        if (lane < SCAN_BLK / 64) wsum[lane] = si - s;
+    }
+    __syncthreads();
+    int excl = inc - v + wsum[w];
+    if (i < n) data[i] = excl;
+    if (threadIdx.x == SCAN_BLK - 1) block_sums[blockIdx.x] = excl + v;
+}
+
+/* single-block exclusive scan of the block sums (any count) */
+__global__ void __launch_bounds__(SCAN_BLK) k_scan_sums(int *sums, int nb, int *total) {
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += SCAN_BLK) {
+        __shared__ int wsum[SCAN_BLK / 64];
+        int i = base + threadIdx.x;
+        int v = i < nb ? sums[i] : 0;
+        int inc = wave_incl_scan(v);
+        int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        if (w == 0) {
+            int s = lane < SCAN_BLK / 64 ? wsum[lane] : 0;
+            int si = wave_incl_scan(s);
+            if (lane < SCAN_BLK / 64) wsum[lane] = si - s;
+        }
+        __syncthreads();
+        int excl = inc - v + wsum[w] + carry;
+        if (i < nb) sums[i] = excl;
+        __syncthreads();
+        if (threadIdx.x == SCAN_BLK - 1) carry = excl + v;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_BLK) k_scan_add(int *data, int n, const int *block_sums) {
+    int i = blockIdx.x * SCAN_BLK + threadIdx.x;
+    if (i < n) data[i] += block_sums[blockIdx.x];
+}
+
+int mdz_scan_exclusive(int *data, int n, int *scratch, size_t scratch_len, hipStream_t s, int *total) {
+    int nb = mdz_div_up(n > 0 ? n : 1, SCAN_BLK);
+    if ((size_t)nb + 1 > scratch_len) return MDZ_ENOMEM;
+    int *sums = scratch, *dtot = scratch + nb;
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(SCAN_BLK), 0, s, data, n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLK), 0, s, sums, nb, dtot);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(SCAN_BLK), 0, s, data, n, (const int *)sums);
+    if (total) {
+        if (hipMemcpyAsync(total, dtot, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return MDZ_EHIP;
+        if (hipStreamSynchronize(s) != hipSuccess) return MDZ_EHIP;
+    }
+    return MDZ_OK;
+}
+
+/* flags of u, v (momentum) and p (continuity) DOFs, concatenated */
+__global__ void __launch_bounds__(BLK) k_eqn_flags(DevGrid g, int *flags) {
+    int i = blockIdx.x * BLK + threadIdx.x;
+    int nu = g.Nx * g.nzvx, nw = g.nxvz * g.Nz, nc = g.ncx * g.ncz;
+    if (i >= nu + nw + nc) return;
+    int f = 0;
+    if (i < nu) { int t = g.BCu[i]; f = (t == -1 || t == 2 || t == -2); }
+    else if (i < nu + nw) { int t = g.BCv[i - nu]; f = (t == -1 || t == 2); }
+    else { f = g.BCp[i - nu - nw] == -1; }
+    flags[i] = f;
+}
+
+/* eqn from the scanned flags; row->node maps */
+__global__ void __launch_bounds__(BLK) k_eqn_assign(DevGrid g, const int *scan, DevEqn e) {
+    int i = blockIdx.x * BLK + threadIdx.x;
+    int nu = g.Nx * g.nzvx, nw = g.nxvz * g.Nz, nc = g.ncx * g.ncz;
+    if (i >= nu + nw + nc) return;
+    if (i < nu) {
+        int t = g.BCu[i];
+        int eq = (t == -1 || t == 2 || t == -2) ? scan[i] : -1;
+        e.eqn_u[i] = eq;
+        if (eq >= 0) e.mom_node[eq] = i;
+    } else if (i < nu + nw) {
+        int c = i - nu, t = g.BCv[c];
+        int eq = (t == -1 || t == 2) ? scan[i] : -1;
+        e.eqn_v[c] = eq;
+        if (eq >= 0) e.mom_node[eq] = i;
+    } else {
+        int c = i - nu - nw;
+        int eq = g.BCp[c] == -1 ? scan[i] : -1;
+        e.eqn_p[c] = eq;
+        if (eq >= 0) e.cont_node[eq - e.neq_mom] = c;
+    }
+}
+
+/* -12 nodes copy the equation number of the node Nx-1 to their left
+ * (StokesRoutines.c:1156); sequential dependence along a row resolved by
+ * walking left until a non-copy node */
+__global__ void __launch_bounds__(BLK) k_eqn_periodic(DevGrid g, DevEqn e) {
+    int i = blockIdx.x * BLK + threadIdx.x;
+    int nu = g.Nx * g.nzvx;
+    if (i >= nu) return;
+    if (g.BCu[i] != -12) return;
+    int j = i - g.Nx + 1;
+    e.eqn_u[i] = j >= 0 ? (g.BCu[j] == -12 ? -1 : e.eqn_u[j]) : -1;
+}
+
+int mdz_number_equations(const DevGrid &g, DevEqn &e, int *scratch, size_t scratch_len, hipStream_t s) {
+    int nu = g.Nx * g.nzvx, nw = g.nxvz * g.Nz, nc = g.ncx * g.ncz, n = nu + nw + nc;
+    if ((size_t)n + (size_t)mdz_div_up(n, SCAN_BLK) + 2 > scratch_len) return MDZ_ENOMEM;
+    int *flags = scratch, *rest = scratch + n;
+    hipLaunchKernelGGL(k_eqn_flags, dim3(mdz_div_up(n, BLK)), dim3(BLK), 0, s, g, flags);
+    /* counts of momentum DOFs: scan of the u+v part gives neq_mom */
+    int total = 0;
+    int rc = mdz_scan_exclusive(flags, n, rest, scratch_len - n, s, &total);
+    if (rc) return rc;
+    /* neq_mom = scanned value at the first p flag */
+    int neq_mom = 0;
+    if (hipMemcpyAsync(&neq_mom, flags + nu + nw, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return MDZ_EHIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return MDZ_EHIP;
+    e.neq_mom = neq_mom;
+    e.neq_cont = total - neq_mom;
+    hipLaunchKernelGGL(k_eqn_assign, dim3(mdz_div_up(n, BLK)), dim3(BLK), 0, s, g, (const int *)flags, e);
+    hipLaunchKernelGGL(k_eqn_periodic, dim3(mdz_div_up(nu, BLK)), dim3(BLK), 0, s, g, e);
+    return MDZ_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* pattern, values, residual launchers                                  */
+/* ------------------------------------------------------------------ */
+int mdz_build_pattern(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, int *scratch,
+                      size_t scratch_len, hipStream_t s) {
+    int nm = a.e.neq_mom, ncnt = a.e.neq_cont;
+    int gm = mdz_div_up(nm > 0 ? nm : 1, BLK), gc = mdz_div_up(ncnt > 0 ? ncnt : 1, BLK);
+    /* counts go into Ic[0..neq-1], scanned in place; Ic[neq] = nnz */
+    hipLaunchKernelGGL(k_count_mom, dim3(gm), dim3(BLK), 0, s, kind, a, A->Ic, B->Ic);
+    if (C) hipLaunchKernelGGL(k_count_cont, dim3(gc), dim3(BLK), 0, s, a, C->Ic);
+    int tot = 0, rc;
+    if ((rc = mdz_scan_exclusive(A->Ic, nm, scratch, scratch_len, s, &tot))) return rc;
+    A->nnz = tot;
+    if ((rc = mdz_scan_exclusive(B->Ic, nm, scratch, scratch_len, s, &tot))) return rc;
+    B->nnz = tot;
+    if (C) {
+        if ((rc = mdz_scan_exclusive(C->Ic, ncnt, scratch, scratch_len, s, &tot))) return rc;
+        C->nnz = tot;
+    }
+    return MDZ_OK;
+}
+
+/* second half of the pattern build, once J has been allocated for the nnz */
+void mdz_fill_pattern(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, hipStream_t s) {
+    int nm = a.e.neq_mom, ncnt = a.e.neq_cont;
+    if (nm > 0) hipLaunchKernelGGL(k_pattern_mom, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, kind, a, *A, *B);
+    if (C && ncnt > 0) hipLaunchKernelGGL(k_pattern_cont, dim3(mdz_div_up(ncnt, BLK)), dim3(BLK), 0, s, a, *C);
+}
+
+void mdz_launch_values(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, hipStream_t s) {
+    int nm = a.e.neq_mom, ncnt = a.e.neq_cont;
+    if (nm > 0) hipLaunchKernelGGL(k_values_mom, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, kind, a, *A, *B);
+    if (ncnt > 0) hipLaunchKernelGGL(k_values_cont, dim3(mdz_div_up(ncnt, BLK)), dim3(BLK), 0, s, kind, a, *C, *D);
+}
+
+void mdz_launch_residual(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, DevBlock *D, hipStream_t s) {
+    int nm = a.e.neq_mom, ncnt = a.e.neq_cont;
+    if (nm > 0) hipLaunchKernelGGL(k_residual_mom, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, kind, a, *A);
+    if (ncnt > 0) hipLaunchKernelGGL(k_residual_cont, dim3(mdz_div_up(ncnt, BLK)), dim3(BLK), 0, s, kind, a, *C, *D);
+}
+
+/* ------------------------------------------------------------------ */
+/* norms                                                                */
+/* ------------------------------------------------------------------ */
+#define NRM_BLK 256
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+/* per node family: F^2 sums and DOF counts, residual fields, F *= d.
+ * block partials: [3 families][2 quantities][nblocks] */
+__global__ void __launch_bounds__(NRM_BLK) k_norms(DevAsm a, DevBlock A, DevBlock D, double *partial, int nbu,
+                                                   int nbv, int nbp) {
+    const DevGrid &g = a.g;
+    int nu = g.Nx * g.nzvx, nw = g.nxvz * g.Nz, nc = g.ncx * g.ncz;
+    int bid = blockIdx.x, fam, base;
+    if (bid < nbu) { fam = 0; base = bid; }
+    else if (bid < nbu + nbv) { fam = 1; base = bid - nbu; }
+    else { fam = 2; base = bid - nbu - nbv; }
+    int i = base * NRM_BLK + threadIdx.x;
+    double s = 0.0, cnt = 0.0;
+    if (fam == 0 && i < nu) {
+        int t = g.BCu[i];
+        if (t != 0 && t != 30 && t != 11 && t != 13 && t != -12) {
+            int e = a.e.eqn_u[i];
+            double f = A.F[e];
+            s = f * f; cnt = 1.0;
+            g.ru[i] = f;
+            A.F[e] = f * A.d[e];
+        }
+    } else if (fam == 1 && i < nw) {
+        int t = g.BCv[i];
+        if (t != 0 && t != 30 && t != 11 && t != 13 && t != -12) {
+            int e = a.e.eqn_v[i];
+            double f = A.F[e];
+            s = f * f; cnt = 1.0;
+            g.rv[i] = f;
+            A.F[e] = f * A.d[e];
+        }
+    } else if (fam == 2 && i < nc) {
+        int t = g.BCp[i];
+        if (t != 0 && t != 30 && t != 31) {
+            int e = a.e.eqn_p[i] - a.e.neq_mom;
+            double f = D.F[e];
+            s = f * f; cnt = 1.0;
+            g.rp[i] = f;
+            D.F[e] = f * D.d[e];
+        }
+    }
+    __shared__ double ws[NRM_BLK / 64][2];
+    s = wave_sum(s);
+    cnt = wave_sum(cnt);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ws[w][0] = s; ws[w][1] = cnt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ts = 0, tc = 0;
+        for (int q = 0; q < NRM_BLK / 64; q++) { ts += ws[q][0]; tc += ws[q][1]; }
+        int nbf = fam == 0 ? nbu : (fam == 1 ? nbv : nbp);
+        int off = fam == 0 ? 0 : (fam == 1 ? 2 * nbu : 2 * (nbu + nbv));
+        partial[off + base] = ts;
+        partial[off + nbf + base] = tc;
+    }
+}
+
+__global__ void __launch_bounds__(NRM_BLK) k_norms_final(const double *partial, int nbu, int nbv, int nbp, double *out6) {
+    int nb[3] = {nbu, nbv, nbp};
+    int off = 0;
+    for (int f = 0; f < 3; f++) {
+        double s = 0, c = 0;
+        for (int i = threadIdx.x; i < nb[f]; i += NRM_BLK) { s += partial[off + i]; c += partial[off + nb[f] + i]; }
+        __shared__ double ws[NRM_BLK / 64][2];
+        s = wave_sum(s);
+        c = wave_sum(c);
+        if ((threadIdx.x & 63) == 0) { ws[threadIdx.x >> 6][0] = s; ws[threadIdx.x >> 6][1] = c; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double ts = 0, tc = 0;
+            for (int q = 0; q < NRM_BLK / 64; q++) { ts += ws[q][0]; tc += ws[q][1]; }
+            out6[f] = ts;
+            out6[3 + f] = tc;
+        }
+        __syncthreads();
+        off += 2 * nb[f];
+    }
+}
+
+size_t mdz_norm_partials(const DevGrid &g) {
+    int nbu = mdz_div_up(g.Nx * g.nzvx, NRM_BLK), nbv = mdz_div_up(g.nxvz * g.Nz, NRM_BLK), nbp = mdz_div_up(g.ncx * g.ncz, NRM_BLK);
+    return 2 * (size_t)(nbu + nbv + nbp);
+}
+
+void mdz_launch_norms(const DevAsm &a, DevBlock *A, DevBlock *D, double *partial, double *out6, hipStream_t s) {
+    const DevGrid &g = a.g;
+    int nbu = mdz_div_up(g.Nx * g.nzvx, NRM_BLK), nbv = mdz_div_up(g.nxvz * g.Nz, NRM_BLK), nbp = mdz_div_up(g.ncx * g.ncz, NRM_BLK);
+    hipLaunchKernelGGL(k_norms, dim3(nbu + nbv + nbp), dim3(NRM_BLK), 0, s, a, *A, *D, partial, nbu, nbv, nbp);
+    hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(NRM_BLK), 0, s, (const double *)partial, nbu, nbv, nbp, out6);
+}

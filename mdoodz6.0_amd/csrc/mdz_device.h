@@ -1,0 +1,64 @@
+/*
+ * mdz_device.h -- device-side view of the grid and the scalar parameters of
+ * the Newton-iteration path (gfx950).  Shared by the rheology, assembly and
+ * API translation units.
+ *
+ * Layout in HBM: one array per field, exactly the host SoA layout
+ * (row-major in x, k + l * row_length), each 256-byte aligned inside one
+ * slab; tags are int8, phase fractions [nph][n] contiguous.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mdoodz_hot.h"
+#include "mdz_fields.h"
+
+/* All double fields of the path, by name (pointers into the device slab). */
+struct DevGrid {
+    int Nx, Nz, ncx, ncz, nxvz, nzvx, nph;
+    double dx, dz;
+#define MDZ_DEVPTR(name, fam, role) double *name;
+    MDZ_ALL_DOUBLE_FIELDS(MDZ_DEVPTR)
+#undef MDZ_DEVPTR
+    double *BCu_val, *BCv_val, *BCp_val;
+    int8_t *BCu, *BCv, *BCp, *BCg;
+    int *comp_cells;
+    double *phase_n; /* [nph][ncx*ncz] */
+    double *phase_s; /* [nph][Nx*Nz] */
+};
+
+/* Scalar model parameters the kernels read (from params and scale). */
+struct DevParams {
+    double dt, mineta, maxeta, gx, gz, free_surf_stab, compressible;
+    double sT, sE, seta, sS, sL;
+    int step, iselastic, eta_avg, aniso, aniso_fstrain, isperiodic_x, UnsplitDiffReac;
+    int diffuse_X, diffuse_avg, polar, VolChangeReac, adiab_heat, Nb_phases, Nx, Nz;
+    int rheo_mode; /* MDZ_RHEO_* */
+};
+
+/* Histogram bins of the local-iteration counters. */
+#define MDZ_HIST_BINS 64
+
+/* Equation numbering and the row -> node maps built with the pattern. */
+struct DevEqn {
+    int *eqn_u, *eqn_v, *eqn_p;
+    int neq_mom, neq_cont;
+    int *mom_node;  /* per momentum row: node id; u nodes as is, v nodes + Nx*(Nz+1) */
+    int *cont_node; /* per continuity row: centroid id */
+};
+
+/* One CSR block on the device. */
+struct DevBlock {
+    int neq, nnz;
+    int *Ic, *J;
+    double *A, *b, *bbc, *F, *d;
+};
+
+/* Sinks through which one stencil row is emitted.  The emission order and
+ * the Dirichlet folding depend only on the boundary tags, so the same row
+ * code produces the pattern (COUNT, then J) and the values (A, bbc). */
+enum { SINK_COUNT = 0, SINK_PATTERN = 1, SINK_VALUES = 2 };
+
+static inline __host__ __device__ int mdz_div_up(int a, int b) { return (a + b - 1) / b; }

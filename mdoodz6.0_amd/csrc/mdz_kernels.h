@@ -1,0 +1,45 @@
+/*
+ * mdz_kernels.h -- host-side launch interface of the gfx950 kernels.
+ */
+#pragma once
+
+#include "mdz_device.h"
+
+/* rheology half (mdz_rheology.hip) */
+void mdz_launch_strain_rates(const DevGrid &g, const DevParams &prm, hipStream_t s);
+void mdz_launch_interp(const DevGrid &g, const DevParams &prm, const double *src, double *dst, hipStream_t s);
+void mdz_launch_rheology(const DevGrid &g, const DevParams &prm, const mat_prop *mt, const float *guess_n,
+                         const float *guess_s, unsigned long long *hist, hipStream_t s);
+void mdz_launch_dtensor(const DevGrid &g, const DevParams &prm, int jac, int *status, hipStream_t s);
+void mdz_launch_rhs(const DevGrid &g, const DevParams &prm, const double *xg, const double *zvx,
+                    const double *xvz, const double *zg, hipStream_t s);
+
+/* assembly half (mdz_assembly.hip) */
+struct DevAsm {
+    DevGrid g;
+    DevEqn e;
+    double celvol, one_dx, one_dz, one_dx_dx, one_dz_dz, one_dx_dz, om, dt, gz_model;
+    int stab, comp_model, periodic, Nz_model;
+    const double *u, *v, *p_mom, *p_cont; /* iterate: p_corr for momentum rows, p_in for continuity */
+};
+
+/* numbering: flags -> exclusive scan -> eqn_u/v/p, -12 copies, row->node maps */
+int mdz_number_equations(const DevGrid &g, DevEqn &e, int *scratch_int, size_t scratch_len, hipStream_t s);
+/* pattern of a block family: kind 0 = Picard, 1 = Jacobian; fills Ic/J and nnz */
+int mdz_build_pattern(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, int *scratch_int,
+                      size_t scratch_len, hipStream_t s);
+/* second half of the pattern build (after J is allocated to the nnz counts) */
+void mdz_fill_pattern(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, hipStream_t s);
+/* values of the assembled blocks (Assemble = 1) */
+void mdz_launch_values(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, hipStream_t s);
+/* residual rows (Assemble = 0) through the Picard (kind 0) or Jacobian (kind 1) stencils;
+ * A/C carry b, F, bbc; D carries F and d */
+void mdz_launch_residual(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, DevBlock *D, hipStream_t s);
+/* norms: sums of F^2 over the u, v, p DOF nodes, ru/rv/rp, F *= d.
+ * partial: device buffer of at least mdz_norm_partials(g) doubles; out3: device double[6]
+ * (sum_x, sum_z, sum_p, ndof_x, ndof_z, ndof_p) */
+size_t mdz_norm_partials(const DevGrid &g);
+void mdz_launch_norms(const DevAsm &a, DevBlock *A, DevBlock *D, double *partial, double *out6, hipStream_t s);
+
+/* device-wide exclusive scan of n ints (in place), returns the total on the host (synchronises) */
+int mdz_scan_exclusive(int *data, int n, int *scratch, size_t scratch_len, hipStream_t s, int *total);

@@ -1,0 +1,795 @@
+/*
+ * mdz_model.c -- host-side grid allocation and seeded synthetic set-ups for
+ * the MDOODZ Newton-iteration path.  See mdz_model.h.
+ */
+#include "mdz_model.h"
+#include "mdz_fields.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ZEROC 273.15
+#define RGAS 8.314510
+
+/* ------------------------------------------------------------------ */
+/* Counter-based RNG: splitmix64 of (seed, stream, index).             */
+/* ------------------------------------------------------------------ */
+static uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+static double urand(uint64_t seed, uint64_t stream, uint64_t i) {
+    uint64_t h = mix64(mix64(seed ^ (stream * 0xD1B54A32D192ED03ULL)) + i);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+static double nrand(uint64_t seed, uint64_t stream, uint64_t i) {
+    double u1 = urand(seed, stream, 2 * i), u2 = urand(seed, stream, 2 * i + 1);
+    if (u1 < 1e-300) u1 = 1e-300;
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+/* ------------------------------------------------------------------ */
+/* Field table                                                          */
+/* ------------------------------------------------------------------ */
+typedef struct { const char *name; size_t off; int fam; int role; } fdesc;
+
+#define FD(name, fam, role) {#name, offsetof(grid, name), MDZ_FAM_##fam, MDZ_ROLE_##role},
+static const fdesc FIELDS[] = { MDZ_ALL_DOUBLE_FIELDS(FD) };
+#undef FD
+#define NFIELDS ((int)(sizeof(FIELDS) / sizeof(FIELDS[0])))
+
+static int fam_len(int fam, int Nx, int Nz) {
+    switch (fam) {
+    case MDZ_FAM_C: return (Nx - 1) * (Nz - 1);
+    case MDZ_FAM_V: return Nx * Nz;
+    case MDZ_FAM_U: return Nx * (Nz + 1);
+    default: return (Nx + 1) * Nz;
+    }
+}
+
+int mdz_grid_nfields(void) { return NFIELDS; }
+const char *mdz_grid_field_name(int i) { return (i >= 0 && i < NFIELDS) ? FIELDS[i].name : NULL; }
+int mdz_grid_field_family(int i) { return (i >= 0 && i < NFIELDS) ? FIELDS[i].fam : -1; }
+int mdz_grid_field_role(int i) { return (i >= 0 && i < NFIELDS) ? FIELDS[i].role : -1; }
+
+static void *xcalloc(size_t n, size_t s) {
+    void *p = calloc(n ? n : 1, s);
+    if (!p) { fprintf(stderr, "mdz_model: out of host memory\n"); abort(); }
+    return p;
+}
+
+grid *mdz_grid_alloc(int Nx, int Nz, int nph) {
+    grid *g = (grid *)xcalloc(1, sizeof(grid));
+    int nc = (Nx - 1) * (Nz - 1), nv = Nx * Nz, nu = Nx * (Nz + 1), nw = (Nx + 1) * Nz;
+    g->Nx = Nx; g->Nz = Nz; g->NN = nv; g->NC = nc;
+    for (int i = 0; i < NFIELDS; i++) {
+        double **slot = (double **)((char *)g + FIELDS[i].off);
+        *slot = (double *)xcalloc((size_t)fam_len(FIELDS[i].fam, Nx, Nz), sizeof(double));
+    }
+    g->BCu.type = (char *)xcalloc(nu, 1); g->BCu.val = (double *)xcalloc(nu, sizeof(double));
+    g->BCv.type = (char *)xcalloc(nw, 1); g->BCv.val = (double *)xcalloc(nw, sizeof(double));
+    g->BCp.type = (char *)xcalloc(nc, 1); g->BCp.val = (double *)xcalloc(nc, sizeof(double));
+    g->BCg.type = (char *)xcalloc(nv, 1); g->BCg.val = (double *)xcalloc(nv, sizeof(double));
+    g->comp_cells = (int *)xcalloc(nc, sizeof(int));
+    g->kp = (int *)xcalloc(nc, sizeof(int)); g->lp = (int *)xcalloc(nc, sizeof(int));
+    g->kn = (int *)xcalloc(nv, sizeof(int)); g->ln = (int *)xcalloc(nv, sizeof(int));
+    g->kvx = (int *)xcalloc(nu, sizeof(int)); g->lvx = (int *)xcalloc(nu, sizeof(int));
+    g->kvz = (int *)xcalloc(nw, sizeof(int)); g->lvz = (int *)xcalloc(nw, sizeof(int));
+    g->xg_coord = (double *)xcalloc(Nx, sizeof(double));
+    g->zg_coord = (double *)xcalloc(Nz, sizeof(double));
+    g->xc_coord = (double *)xcalloc(Nx - 1, sizeof(double));
+    g->zc_coord = (double *)xcalloc(Nz - 1, sizeof(double));
+    g->xvz_coord = (double *)xcalloc(Nx + 1, sizeof(double));
+    g->zvx_coord = (double *)xcalloc(Nz + 1, sizeof(double));
+    g->phase_perc_n = (double **)xcalloc(nph, sizeof(double *));
+    g->phase_perc_s = (double **)xcalloc(nph, sizeof(double *));
+    for (int p = 0; p < nph; p++) {
+        g->phase_perc_n[p] = (double *)xcalloc(nc, sizeof(double));
+        g->phase_perc_s[p] = (double *)xcalloc(nv, sizeof(double));
+    }
+    /* Index maps: GridRoutines.c:481 GridIndices */
+    for (int l = 0; l < Nz; l++) for (int k = 0; k < Nx; k++) { g->kn[k + l * Nx] = k; g->ln[k + l * Nx] = l; }
+    for (int l = 0; l < Nz + 1; l++) for (int k = 0; k < Nx; k++) { g->kvx[k + l * Nx] = k; g->lvx[k + l * Nx] = l; }
+    for (int l = 0; l < Nz; l++) for (int k = 0; k < Nx + 1; k++) { g->kvz[k + l * (Nx + 1)] = k; g->lvz[k + l * (Nx + 1)] = l; }
+    for (int l = 0; l < Nz - 1; l++) for (int k = 0; k < Nx - 1; k++) { g->kp[k + l * (Nx - 1)] = k; g->lp[k + l * (Nx - 1)] = l; }
+    return g;
+}
+
+void mdz_grid_free(grid *g, int nph) {
+    if (!g) return;
+    for (int i = 0; i < NFIELDS; i++) free(*(double **)((char *)g + FIELDS[i].off));
+    free(g->BCu.type); free(g->BCu.val); free(g->BCv.type); free(g->BCv.val);
+    free(g->BCp.type); free(g->BCp.val); free(g->BCg.type); free(g->BCg.val);
+    free(g->comp_cells); free(g->kp); free(g->lp); free(g->kn); free(g->ln);
+    free(g->kvx); free(g->lvx); free(g->kvz); free(g->lvz);
+    free(g->xg_coord); free(g->zg_coord); free(g->xc_coord); free(g->zc_coord);
+    free(g->xvz_coord); free(g->zvx_coord);
+    for (int p = 0; p < nph; p++) { free(g->phase_perc_n[p]); free(g->phase_perc_s[p]); }
+    free(g->phase_perc_n); free(g->phase_perc_s);
+    free(g);
+}
+
+grid *mdz_grid_clone(const grid *src, int nph) {
+    int Nx = src->Nx, Nz = src->Nz;
+    grid *g = mdz_grid_alloc(Nx, Nz, nph);
+    int nc = (Nx - 1) * (Nz - 1), nv = Nx * Nz, nu = Nx * (Nz + 1), nw = (Nx + 1) * Nz;
+    g->dx = src->dx; g->dz = src->dz;
+    for (int i = 0; i < NFIELDS; i++) {
+        double *d = *(double **)((char *)g + FIELDS[i].off);
+        const double *s = *(double *const *)((const char *)src + FIELDS[i].off);
+        memcpy(d, s, sizeof(double) * (size_t)fam_len(FIELDS[i].fam, Nx, Nz));
+    }
+    memcpy(g->BCu.type, src->BCu.type, nu); memcpy(g->BCu.val, src->BCu.val, nu * sizeof(double));
+    memcpy(g->BCv.type, src->BCv.type, nw); memcpy(g->BCv.val, src->BCv.val, nw * sizeof(double));
+    memcpy(g->BCp.type, src->BCp.type, nc); memcpy(g->BCp.val, src->BCp.val, nc * sizeof(double));
+    memcpy(g->BCg.type, src->BCg.type, nv); memcpy(g->BCg.val, src->BCg.val, nv * sizeof(double));
+    memcpy(g->comp_cells, src->comp_cells, nc * sizeof(int));
+    memcpy(g->xg_coord, src->xg_coord, Nx * sizeof(double));
+    memcpy(g->zg_coord, src->zg_coord, Nz * sizeof(double));
+    memcpy(g->xc_coord, src->xc_coord, (Nx - 1) * sizeof(double));
+    memcpy(g->zc_coord, src->zc_coord, (Nz - 1) * sizeof(double));
+    memcpy(g->xvz_coord, src->xvz_coord, (Nx + 1) * sizeof(double));
+    memcpy(g->zvx_coord, src->zvx_coord, (Nz + 1) * sizeof(double));
+    for (int p = 0; p < nph; p++) {
+        memcpy(g->phase_perc_n[p], src->phase_perc_n[p], nc * sizeof(double));
+        memcpy(g->phase_perc_s[p], src->phase_perc_s[p], nv * sizeof(double));
+    }
+    return g;
+}
+
+double *mdz_grid_dfield(grid *g, const char *name, int *len) {
+    int Nx = g->Nx, Nz = g->Nz;
+    for (int i = 0; i < NFIELDS; i++)
+        if (!strcmp(name, FIELDS[i].name)) {
+            if (len) *len = fam_len(FIELDS[i].fam, Nx, Nz);
+            return *(double **)((char *)g + FIELDS[i].off);
+        }
+    if (!strcmp(name, "BCu.val")) { if (len) *len = fam_len(MDZ_FAM_U, Nx, Nz); return g->BCu.val; }
+    if (!strcmp(name, "BCv.val")) { if (len) *len = fam_len(MDZ_FAM_W, Nx, Nz); return g->BCv.val; }
+    if (!strcmp(name, "BCp.val")) { if (len) *len = fam_len(MDZ_FAM_C, Nx, Nz); return g->BCp.val; }
+    if (!strncmp(name, "phase_perc_n:", 13)) { if (len) *len = fam_len(MDZ_FAM_C, Nx, Nz); return g->phase_perc_n[atoi(name + 13)]; }
+    if (!strncmp(name, "phase_perc_s:", 13)) { if (len) *len = fam_len(MDZ_FAM_V, Nx, Nz); return g->phase_perc_s[atoi(name + 13)]; }
+    if (!strcmp(name, "xg_coord")) { if (len) *len = Nx; return g->xg_coord; }
+    if (!strcmp(name, "zg_coord")) { if (len) *len = Nz; return g->zg_coord; }
+    if (!strcmp(name, "xc_coord")) { if (len) *len = Nx - 1; return g->xc_coord; }
+    if (!strcmp(name, "zc_coord")) { if (len) *len = Nz - 1; return g->zc_coord; }
+    return NULL;
+}
+
+char *mdz_grid_tags(grid *g, const char *name, int *len) {
+    int Nx = g->Nx, Nz = g->Nz;
+    if (!strcmp(name, "BCu")) { if (len) *len = fam_len(MDZ_FAM_U, Nx, Nz); return g->BCu.type; }
+    if (!strcmp(name, "BCv")) { if (len) *len = fam_len(MDZ_FAM_W, Nx, Nz); return g->BCv.type; }
+    if (!strcmp(name, "BCp")) { if (len) *len = fam_len(MDZ_FAM_C, Nx, Nz); return g->BCp.type; }
+    if (!strcmp(name, "BCg")) { if (len) *len = fam_len(MDZ_FAM_V, Nx, Nz); return g->BCg.type; }
+    return NULL;
+}
+
+int *mdz_grid_ifield(grid *g, const char *name, int *len) {
+    int Nx = g->Nx, Nz = g->Nz;
+    int nc = fam_len(MDZ_FAM_C, Nx, Nz), nv = fam_len(MDZ_FAM_V, Nx, Nz);
+    int nu = fam_len(MDZ_FAM_U, Nx, Nz), nw = fam_len(MDZ_FAM_W, Nx, Nz);
+    struct { const char *n; int *p; int l; } t[] = {
+        {"comp_cells", g->comp_cells, nc}, {"kp", g->kp, nc}, {"lp", g->lp, nc}, {"kn", g->kn, nv},
+        {"ln", g->ln, nv}, {"kvx", g->kvx, nu}, {"lvx", g->lvx, nu}, {"kvz", g->kvz, nw}, {"lvz", g->lvz, nw}};
+    for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); i++)
+        if (!strcmp(name, t[i].n)) { if (len) *len = t[i].l; return t[i].p; }
+    return NULL;
+}
+
+/* ------------------------------------------------------------------ */
+/* ABI layout tables                                                    */
+/* ------------------------------------------------------------------ */
+typedef struct { const char *type; const char *field; size_t off; } abient;
+static const abient ABI[] = {
+#include "mdz_abi_table.inc"
+};
+
+size_t mdz_sizeof(const char *t) {
+    if (!strcmp(t, "scale")) return sizeof(scale);
+    if (!strcmp(t, "mat_prop")) return sizeof(mat_prop);
+    if (!strcmp(t, "BC")) return sizeof(BC);
+    if (!strcmp(t, "BCT")) return sizeof(BCT);
+    if (!strcmp(t, "SparseMat")) return sizeof(SparseMat);
+    if (!strcmp(t, "params")) return sizeof(params);
+    if (!strcmp(t, "Nparams")) return sizeof(Nparams);
+    if (!strcmp(t, "grid")) return sizeof(grid);
+    return 0;
+}
+
+long mdz_offsetof(const char *t, const char *f) {
+    for (size_t i = 0; i < sizeof(ABI) / sizeof(ABI[0]); i++)
+        if (!strcmp(ABI[i].type, t) && !strcmp(ABI[i].field, f)) return (long)ABI[i].off;
+    return -1;
+}
+
+/* ------------------------------------------------------------------ */
+/* Scaling, defaults and flow-law tables                               */
+/* ------------------------------------------------------------------ */
+
+/* InputOutput.c:1526 ScaleMe */
+static void scale_me(scale *s) {
+    s->t = s->L / s->V;
+    s->a = s->V / s->t;
+    s->E = 1.0 / s->t;
+    s->S = s->eta / s->t;
+    s->m = s->S * s->L * pow(s->t, 2.0);
+    s->rho = s->m / pow(s->L, 3.0);
+    s->F = s->m * s->L / pow(s->t, 2.0);
+    s->J = s->m * pow(s->L, 2.0) / pow(s->t, 2.0);
+    s->W = s->J / s->t;
+    s->Cv = s->J / s->m / s->T;
+    s->k = s->W / s->L / s->T;
+    s->rhoE = s->J / s->m * s->rho;
+}
+
+/* Material defaults of InputOutput.c:1194-1301 (ReadMatProps defaults),
+ * already divided by the scales. */
+static void phase_defaults(mat_prop *m, int k, const scale *s) {
+    m->rho[k] = 2700.0 / s->rho;
+    m->mu[k] = 1.0e10 / s->S;
+    m->C[k] = 1.0e7 / s->S;
+    m->phi[k] = 30.0 * M_PI / 180.0;
+    m->psi[k] = 0.0;
+    m->Slim[k] = 1.0e10 / s->S;
+    m->alp[k] = 0.0;
+    m->bet[k] = 1.0e-40 / (1.0 / s->S);
+    m->T0[k] = ZEROC / s->T;
+    m->P0[k] = 1e5 / s->S;
+    m->cstv[k] = 0; m->pwlv[k] = 0; m->linv[k] = 0; m->gbsv[k] = 0; m->expv[k] = 0;
+    m->eta0[k] = 1.0e20 / s->eta;
+    m->pref_pwl[k] = 1.0;
+    m->gs[k] = 0;
+    m->gs_ref[k] = 2.0e-3 / s->L;
+    m->is_tensile[k] = 0;
+    m->n_vp[k] = 1.0;
+    m->eta_vp[k] = 0.0;
+    m->reac_soft[k] = 0;
+    m->phase_mix[k] = 0;
+    m->Fpwl[k] = 1.0; m->Flin[k] = 1.0; m->Fgbs[k] = 1.0;
+}
+
+/* Power-law table entries (FlowLaws.c:44 ReadDataPowerLaw, codes 13, 23, 40)
+ * and its nondimensionalisation. */
+static void set_power_law(mat_prop *m, int k, int code, const scale *s) {
+    double t = 1, n = 1, Q = 0, V = 0, A = 1;
+    switch (code) {
+    case 13: n = 2.3; Q = 154.0e3; V = 0.0; A = 5.0717e-18; break;   /* wet quartzite, Ranalli 1995 */
+    case 23: n = 3.2; Q = 238.0e3; V = 0.0; A = 2.0632e-23; break;   /* plagioclase, Shelton & Tullis 1981 */
+    case 40: n = 3.5; Q = 530.0e3; V = 11.0e-6; A = 1.1000e-16; break; /* dry olivine, Hirth & Kohlstedt 2003 */
+    default: fprintf(stderr, "mdz_model: unknown power law %d\n", code); abort();
+    }
+    m->pwlv[k] = code;
+    m->tpwl[k] = t; m->npwl[k] = n; m->mpwl[k] = 0.0; m->rpwl[k] = 0.0;
+    m->Qpwl[k] = Q / s->J;
+    m->Vpwl[k] = V / pow(s->L, 3);
+    m->Apwl[k] = A / (pow(s->S, -n) * pow(s->S, -0.0) * pow(s->L, 0.0) / s->t);
+    m->fpwl[k] = 0.0 / s->S;
+    m->apwl[k] = 0.0;
+    m->Fpwl[k] = 1.0 / 6.0 * pow(2.0, 1.0 / n) * pow(3.0, (n - 1.0) / 2.0 / n);
+}
+
+/* Linear (diffusion) creep, FlowLaws.c:558 ReadDataLinear code 40. */
+static void set_linear_law(mat_prop *m, int k, const scale *s) {
+    double n = 1.0, mm = 3.0;
+    m->linv[k] = 40;
+    m->tlin[k] = 1; m->nlin[k] = n; m->mlin[k] = mm; m->rlin[k] = 0.0;
+    m->Qlin[k] = 375e3 / s->J;
+    m->Vlin[k] = 4.0e-6 / pow(s->L, 3.0);
+    m->Alin[k] = 1.5e-15 / (pow(s->S, -n) * pow(s->S, -0.0) * pow(s->L, mm) / s->t);
+    m->flin[k] = 0.0; m->alin[k] = 0.0;
+    m->Flin[k] = 1.0 / 6.0 * pow(2.0, 1.0 / n) * pow(3.0, (n - 1) / 2.0 / n);
+}
+
+/* Regularised Peierls creep, FlowLaws.c:742 ReadDataExponential code 40. */
+static void set_peierls_law(mat_prop *m, int k, const scale *s) {
+    m->expv[k] = 40;
+    m->texp[k] = 1; m->qexp[k] = 2.0; m->Gexp[k] = 0.1;
+    m->Sexp[k] = 8.5e9 / s->S;
+    m->Qexp[k] = 5.4e5 / s->J;
+    m->Vexp[k] = 0.0;
+    m->nexp[k] = 0.0;
+    m->Eexp[k] = 5.7e11 / (s->E * pow(s->S, -m->nexp[k]));
+}
+
+static void common_params(params *p, int Nx, int Nz, double xmin, double xmax, double zmin, double zmax) {
+    memset(p, 0, sizeof(*p));
+    p->Nx = Nx; p->Nz = Nz;
+    p->xmin = xmin; p->xmax = xmax; p->zmin = zmin; p->zmax = zmax;
+    p->xmin0 = xmin; p->xmax0 = xmax; p->zmin0 = zmin; p->zmax0 = zmax;
+    p->step = 1; p->nit = 0; p->Newton = 1;
+    p->eta_avg = 0; p->ismechanical = 1; p->iselastic = 1; p->isnonnewtonian = 1;
+    p->decoupled_solve = 1; p->lsolver = 2; p->diag_scaling = 0;
+    p->compressible = 0; p->free_surf_stab = 0.0; p->aniso = 0;
+    p->line_search = 0; p->safe_mode = 0; p->nstagmax = 3;
+}
+
+/* GridRoutines.c:230 SetGridCoordinates (regular spacing) */
+static void set_coords(grid *g, params *p) {
+    int Nx = g->Nx, Nz = g->Nz;
+    g->dx = (p->xmax - p->xmin) / (Nx - 1);
+    g->dz = (p->zmax - p->zmin) / (Nz - 1);
+    p->dx = g->dx; p->dz = g->dz;
+    g->xg_coord[0] = p->xmin; g->zg_coord[0] = p->zmin;
+    for (int i = 1; i < Nx; i++) g->xg_coord[i] = g->xg_coord[i - 1] + g->dx;
+    for (int i = 1; i < Nz; i++) g->zg_coord[i] = g->zg_coord[i - 1] + g->dz;
+    g->xc_coord[0] = p->xmin + g->dx / 2.0; g->zc_coord[0] = p->zmin + g->dz / 2.0;
+    for (int i = 1; i < Nx - 1; i++) g->xc_coord[i] = g->xc_coord[i - 1] + g->dx;
+    for (int i = 1; i < Nz - 1; i++) g->zc_coord[i] = g->zc_coord[i - 1] + g->dz;
+    g->xvz_coord[0] = p->xmin - g->dx / 2.0; g->zvx_coord[0] = p->zmin - g->dz / 2.0;
+    for (int i = 1; i < Nx + 1; i++) g->xvz_coord[i] = g->xvz_coord[i - 1] + g->dx;
+    for (int i = 1; i < Nz + 1; i++) g->zvx_coord[i] = g->zvx_coord[i - 1] + g->dz;
+}
+
+/* ------------------------------------------------------------------ */
+/* Boundary tags                                                        */
+/* ------------------------------------------------------------------ */
+static void set_bcs(grid *g, params *p, int family, double Lx, double Lz) {
+    int Nx = g->Nx, Nz = g->Nz, ncx = Nx - 1, ncz = Nz - 1;
+    double E = p->EpsBG;
+    /* vertices and centroids default to interior */
+    for (int i = 0; i < Nx * Nz; i++) { g->BCg.type[i] = -1; g->BCg.val[i] = 0.0; }
+    for (int i = 0; i < ncx * ncz; i++) { g->BCp.type[i] = -1; g->BCp.val[i] = 0.0; }
+    /* Vx */
+    for (int l = 0; l < Nz + 1; l++)
+        for (int k = 0; k < Nx; k++) {
+            int c = k + l * Nx;
+            char t = -1; double v = 0.0;
+            switch (family) {
+            case MDZ_BC_SIMPLE_SHEAR:
+                if (k == 0) { t = -2; v = 0.0; }
+                if (k == Nx - 1) { t = -12; v = 0.0; }
+                if (l == 0) { t = 11; v = -1.0 * E * Lz; }
+                if (l == Nz) { t = 11; v = 1.0 * E * Lz; }
+                break;
+            case MDZ_BC_STRESS_WE:
+                if ((k == 0 || k == Nx - 1) && l > 0 && l < Nz) {
+                    t = 2;
+                    int c1 = (k == 0) ? (k + (l - 1) * ncx) : (k + (l - 1) * ncx - 1);
+                    v = -g->p_in[c1];
+                }
+                if (l == 0 || l == Nz) { t = 13; v = 0.0; }
+                break;
+            default: /* pure shear, free surface, Neumann top */
+                if (k == 0 || k == Nx - 1) { t = 0; v = -g->xg_coord[k] * E; }
+                if (l == 0 || l == Nz) { t = 13; v = 0.0; }
+                break;
+            }
+            g->BCu.type[c] = t; g->BCu.val[c] = v;
+        }
+    /* Vz */
+    for (int l = 0; l < Nz; l++)
+        for (int k = 0; k < Nx + 1; k++) {
+            int c = k + l * (Nx + 1);
+            char t = -1; double v = 0.0;
+            switch (family) {
+            case MDZ_BC_SIMPLE_SHEAR:
+                if (l == 0 || l == Nz - 1) { t = 0; v = 0.0; }
+                if (k == 0 || k == Nx) { t = -12; v = 0.0; }
+                break;
+            case MDZ_BC_STRESS_WE:
+                if ((l == 0 || l == Nz - 1) && k > 0 && k < Nx) { t = 0; v = g->zg_coord[l] * E; }
+                if (k == 0 || k == Nx) { t = 11; v = 0.0; }
+                break;
+            case MDZ_BC_NEUMANN_TOP:
+                if (l == 0 && k > 0 && k < Nx) { t = 0; v = g->zg_coord[l] * E; }
+                if (l == Nz - 1 && k > 0 && k < Nx) { t = 2; v = 0.0; }
+                if (k == 0 || k == Nx) { t = 13; v = 0.0; }
+                break;
+            default:
+                if (l == 0 || l == Nz - 1) { t = 0; v = g->zg_coord[l] * E; }
+                if (k == 0 || k == Nx) { t = 13; v = 0.0; }
+                break;
+            }
+            g->BCv.type[c] = t; g->BCv.val[c] = v;
+        }
+    if (family == MDZ_BC_FREE_SURFACE) {
+        /* Flat surface on the vertex row ncz-2: the centroid row just above
+         * it carries the surface pressure (31, Dirichlet p = 0), the row
+         * above that is air (30). */
+        int lsurf = ncz - 2;
+        for (int l = lsurf + 1; l < ncz; l++)
+            for (int k = 0; k < ncx; k++) g->BCp.type[k + l * ncx] = 30;
+        for (int k = 0; k < ncx; k++) { g->BCp.type[k + lsurf * ncx] = 31; g->BCp.val[k + lsurf * ncx] = 0.0; }
+        /* vertices above the surface are air */
+        for (int l = lsurf + 1; l < Nz; l++)
+            for (int k = 0; k < Nx; k++) g->BCg.type[k + l * Nx] = 30;
+        /* velocity nodes above the surface are air */
+        for (int l = lsurf + 1; l < Nz + 1; l++)
+            for (int k = 0; k < Nx; k++) { g->BCu.type[k + l * Nx] = 30; g->BCu.val[k + l * Nx] = 0.0; }
+        for (int l = lsurf + 1; l < Nz; l++)
+            for (int k = 0; k < Nx + 1; k++) { g->BCv.type[k + l * (Nx + 1)] = 30; g->BCv.val[k + l * (Nx + 1)] = 0.0; }
+    }
+}
+
+/* GridRoutines.c:316 InitialiseSolutionFields: Dirichlet values forced,
+ * air zeroed. */
+static void force_dirichlet(grid *g) {
+    int Nx = g->Nx, Nz = g->Nz;
+    for (int c = 0; c < Nx * (Nz + 1); c++) {
+        if (g->BCu.type[c] == 30) g->u_in[c] = 0.0;
+        if (g->BCu.type[c] == 0) g->u_in[c] = g->BCu.val[c];
+    }
+    for (int c = 0; c < (Nx + 1) * Nz; c++) {
+        if (g->BCv.type[c] == 30) g->v_in[c] = 0.0;
+        if (g->BCv.type[c] == 0) g->v_in[c] = g->BCv.val[c];
+    }
+    for (int c = 0; c < (Nx - 1) * (Nz - 1); c++)
+        if (g->BCp.type[c] == 30 || g->BCp.type[c] == 31) g->p_in[c] = 0.0;
+}
+
+void mdz_lithostatic_pressure(grid *g, const params *p, double *pl) {
+    int ncx = g->Nx - 1, ncz = g->Nz - 1;
+    for (int c = 0; c < ncx * ncz; c++) pl[c] = 0.0;
+    for (int l = ncz - 2; l >= 0; l--)
+        for (int k = 0; k < ncx; k++) {
+            int c = k + l * ncx;
+            pl[c] = 0.0;
+            if (g->BCp.type[c] != 30 && g->BCp.type[c] != 31)
+                pl[c] = pl[c + ncx] - p->gz * g->dz * g->rho_n[c];
+        }
+    for (int c = 0; c < ncx * ncz; c++)
+        if (g->BCp.type[c] != 30 && g->BCp.type[c] != 31) {
+            pl[c] += p->PrBG;
+            pl[c] += 0.5 * p->gz * g->dz * g->rho_n[c];
+        }
+}
+
+static mdz_model *model_new(int Nx, int Nz, int nph, unsigned long long seed, int fam) {
+    mdz_model *m = (mdz_model *)xcalloc(1, sizeof(mdz_model));
+    m->mesh = mdz_grid_alloc(Nx, Nz, nph);
+    m->nph = nph; m->seed = seed; m->bc_family = fam;
+    m->Nmodel.nit = 0; m->Nmodel.nit_max = 10;
+    m->Nmodel.abs_tol_u = 1e-10; m->Nmodel.rel_tol_u = 1e-10;
+    m->Nmodel.abs_tol_p = 1e-10; m->Nmodel.rel_tol_p = 1e-10;
+    return m;
+}
+
+void mdz_model_free(mdz_model *m) {
+    if (!m) return;
+    mdz_grid_free(m->mesh, m->nph);
+    free(m);
+}
+
+mdz_model *mdz_model_clone(const mdz_model *src) {
+    mdz_model *m = (mdz_model *)xcalloc(1, sizeof(mdz_model));
+    *m = *src;
+    m->mesh = mdz_grid_clone(src->mesh, src->nph);
+    return m;
+}
+
+/* Phase-weighted vertex/centroid averages of per-phase properties. */
+static void phase_average(grid *g, int nph, const double *prop, double *cen, double *ver) {
+    int nc = (g->Nx - 1) * (g->Nz - 1), nv = g->Nx * g->Nz;
+    for (int c = 0; c < nc; c++) { double s = 0; for (int p = 0; p < nph; p++) s += g->phase_perc_n[p][c] * prop[p]; cen[c] = s; }
+    if (ver) for (int c = 0; c < nv; c++) { double s = 0; for (int p = 0; p < nph; p++) s += g->phase_perc_s[p][c] * prop[p]; ver[c] = s; }
+}
+
+/* Interpolate a centroid field to vertices by the padded four-point mean
+ * (same rule as the path's own interpolation; used to fill vertex inputs). */
+static void c2v(const grid *g, const double *cen, double *ver) {
+    int Nx = g->Nx, Nz = g->Nz, ncx = Nx - 1, ncz = Nz - 1;
+    for (int l = 0; l < Nz; l++)
+        for (int k = 0; k < Nx; k++) {
+            int kw = k - 1 < 0 ? 0 : k - 1, ke = k > ncx - 1 ? ncx - 1 : k;
+            int ls = l - 1 < 0 ? 0 : l - 1, ln = l > ncz - 1 ? ncz - 1 : l;
+            ver[k + l * Nx] = 0.25 * (cen[kw + ls * ncx] + cen[ke + ls * ncx] + cen[kw + ln * ncx] + cen[ke + ln * ncx]);
+        }
+}
+
+static void fill_common_fields(mdz_model *m) {
+    grid *g = m->mesh;
+    int nc = (g->Nx - 1) * (g->Nz - 1);
+    mat_prop *mt = &m->materials;
+    double tmp[20] = {0};
+    for (int p = 0; p < m->nph; p++) tmp[p] = mt->mu[p];
+    phase_average(g, m->nph, tmp, g->mu_n, g->mu_s);
+    for (int p = 0; p < m->nph; p++) tmp[p] = mt->rho[p];
+    phase_average(g, m->nph, tmp, g->rho_n, g->rho_s);
+    for (int c = 0; c < nc; c++) g->rho0_n[c] = g->rho_n[c];
+    for (int p = 0; p < m->nph; p++) tmp[p] = mt->bet[p];
+    phase_average(g, m->nph, tmp, g->bet_n, g->bet_s);
+    for (int p = 0; p < m->nph; p++) tmp[p] = mt->C[p];
+    phase_average(g, m->nph, tmp, g->C_n, g->C_s);
+    for (int p = 0; p < m->nph; p++) tmp[p] = mt->phi[p];
+    phase_average(g, m->nph, tmp, g->fric_n, g->fric_s);
+    for (int p = 0; p < m->nph; p++) tmp[p] = mt->psi[p];
+    phase_average(g, m->nph, tmp, g->dil_n, g->dil_s);
+    for (int p = 0; p < m->nph; p++) tmp[p] = mt->gs_ref[p];
+    phase_average(g, m->nph, tmp, g->d0_n, NULL);
+    for (int c = 0; c < nc; c++) g->comp_cells[c] = (int)m->model.compressible;
+    for (int c = 0; c < g->Nx * g->Nz; c++) { g->aniso_factor_s[c] = 1.0; g->FS_AR_s[c] = 1.0; g->nz0_s[c] = 1.0; }
+    for (int c = 0; c < nc; c++) { g->aniso_factor_n[c] = 1.0; g->FS_AR_n[c] = 1.0; g->nz0_n[c] = 1.0; }
+}
+
+/* ------------------------------------------------------------------ */
+/* Set-ups                                                              */
+/* ------------------------------------------------------------------ */
+
+mdz_model *mdz_model_inclusion(int ncx, int ncz, unsigned long long seed) {
+    int Nx = ncx + 1, Nz = ncz + 1;
+    mdz_model *m = model_new(Nx, Nz, 2, seed, MDZ_BC_PURE_SHEAR);
+    grid *g = m->mesh;
+    scale *s = &m->scaling;
+    s->eta = 1.0; s->L = 1.0; s->V = 1.0; s->T = 1.0;
+    scale_me(s);
+    params *p = &m->model;
+    common_params(p, Nx, Nz, -0.5, 0.5, -0.5, 0.5);
+    p->dt = 1e-4; p->dt0 = p->dt; p->EpsBG = 1.0; p->gx = 0.0; p->gz = 0.0;
+    p->Nb_phases = 2; p->mineta = 1e-6; p->maxeta = 1e6; p->user1 = 0.05;
+    mat_prop *mt = &m->materials;
+    memset(mt, 0, sizeof(*mt));
+    mt->Nb_phases = 2;
+    for (int k = 0; k < 2; k++) {
+        phase_defaults(mt, k, s);
+        mt->cstv[k] = 1; mt->mu[k] = 1.0; mt->C[k] = 1.75; mt->phi[k] = 30.0 * M_PI / 180.0;
+        mt->psi[k] = 0.0; mt->rho[k] = 1.0;
+    }
+    mt->eta0[0] = 1e2; mt->eta0[1] = 1e-3;
+    mt->R = RGAS / (s->J / s->T);
+    set_coords(g, p);
+    /* GridRoutines.c:530 SetUpModel_NoMarkers geometry (binary fractions) */
+    double r = p->user1 / s->L, z0 = 0.5 * (p->zmax + p->zmin);
+    for (int l = 0; l < Nz - 1; l++)
+        for (int k = 0; k < Nx - 1; k++) {
+            int c = k + l * (Nx - 1);
+            double x = g->xc_coord[k], z = g->zc_coord[l] - z0;
+            int inc = x * x + z * z < r * r;
+            g->phase_perc_n[0][c] = inc ? 0.0 : 1.0; g->phase_perc_n[1][c] = inc ? 1.0 : 0.0;
+            g->T[c] = 0.05;
+        }
+    for (int l = 0; l < Nz; l++)
+        for (int k = 0; k < Nx; k++) {
+            int c = k + l * Nx;
+            double x = g->xg_coord[k], z = g->zg_coord[l];
+            int inc = x * x + z * z < r * r;
+            g->phase_perc_s[0][c] = inc ? 0.0 : 1.0; g->phase_perc_s[1][c] = inc ? 1.0 : 0.0;
+        }
+    fill_common_fields(m);
+    /* pressure N(0, 0.1) */
+    for (int c = 0; c < (Nx - 1) * (Nz - 1); c++) { g->p_in[c] = 0.1 * nrand(seed, 3, c); g->p0_n[c] = g->p_in[c]; }
+    c2v(g, g->p0_n, g->p0_s);
+    set_bcs(g, p, MDZ_BC_PURE_SHEAR, 1.0, 1.0);
+    /* pure shear + U[-1e-3, 1e-3] noise on free nodes */
+    for (int l = 0; l < Nz + 1; l++)
+        for (int k = 0; k < Nx; k++) {
+            int c = k + l * Nx;
+            g->u_in[c] = -g->xg_coord[k] * p->EpsBG;
+            if (g->BCu.type[c] == -1) g->u_in[c] += 1e-3 * (2.0 * urand(seed, 1, c) - 1.0);
+        }
+    for (int l = 0; l < Nz; l++)
+        for (int k = 0; k < Nx + 1; k++) {
+            int c = k + l * (Nx + 1);
+            g->v_in[c] = g->zg_coord[l] * p->EpsBG;
+            if (g->BCv.type[c] == -1) g->v_in[c] += 1e-3 * (2.0 * urand(seed, 2, c) - 1.0);
+        }
+    force_dirichlet(g);
+    return m;
+}
+
+/* Layer fractions of the interval [z0, z1] (depths positive down, km). */
+static void layer_fractions(double ztop, double zbot, double *f) {
+    const double b[4] = {-1e30, 20e3, 35e3, 1e30};
+    double len = zbot - ztop;
+    for (int p = 0; p < 3; p++) {
+        double a = ztop > b[p] ? ztop : b[p], e = zbot < b[p + 1] ? zbot : b[p + 1];
+        f[p] = e > a ? (e - a) / len : 0.0;
+    }
+}
+
+/* Geotherm: 273 K at the surface -> 1603 K at 120 km, then 0.5 K/km. */
+static double geotherm(double depth_m) {
+    if (depth_m < 0) depth_m = 0;
+    if (depth_m <= 120e3) return 273.0 + (1603.0 - 273.0) * depth_m / 120e3;
+    return 1603.0 + 0.5e-3 * (depth_m - 120e3);
+}
+
+static void lithosphere_materials(mdz_model *m, double fric_deg, double coh) {
+    scale *s = &m->scaling;
+    mat_prop *mt = &m->materials;
+    memset(mt, 0, sizeof(*mt));
+    mt->Nb_phases = 3;
+    const double rho[3] = {2700.0, 2900.0, 3300.0};
+    const int pwl[3] = {13, 23, 40};
+    for (int k = 0; k < 3; k++) {
+        phase_defaults(mt, k, s);
+        mt->rho[k] = rho[k] / s->rho;
+        mt->mu[k] = 3e10 / s->S;
+        mt->C[k] = coh / s->S;
+        mt->phi[k] = fric_deg * M_PI / 180.0;
+        mt->psi[k] = 0.0;
+        set_power_law(mt, k, pwl[k], s);
+    }
+    set_peierls_law(mt, 2, s);
+    mt->R = RGAS / (s->J / s->T);
+}
+
+mdz_model *mdz_model_lithosphere(int ncx, int ncz, unsigned long long seed, double fric_deg,
+                                 double coh, double rate) {
+    if (fric_deg < 0) fric_deg = 30.0;
+    if (coh < 0) coh = 10e6;
+    if (rate < 0) rate = 1e-15;
+    int Nx = ncx + 1, Nz = ncz + 1;
+    mdz_model *m = model_new(Nx, Nz, 3, seed, MDZ_BC_PURE_SHEAR);
+    grid *g = m->mesh;
+    scale *s = &m->scaling;
+    s->eta = 1e22; s->L = 1e4; s->V = 1e-9; s->T = 100.0;
+    scale_me(s);
+    params *p = &m->model;
+    common_params(p, Nx, Nz, -500e3 / s->L, 500e3 / s->L, -250e3 / s->L, 0.0);
+    p->dt = 1e11 / s->t; p->dt0 = p->dt;
+    p->EpsBG = -rate / s->E; /* negative EpsBG = extension (u = -x EpsBG) */
+    p->gx = 0.0; p->gz = -9.81 / s->a;
+    p->Nb_phases = 3; p->mineta = 1e18 / s->eta; p->maxeta = 1e24 / s->eta;
+    lithosphere_materials(m, fric_deg, coh);
+    set_coords(g, p);
+    int nc = (Nx - 1) * (Nz - 1);
+    /* phases: fractional overlap of each cell with the layers */
+    double f[3];
+    for (int l = 0; l < Nz - 1; l++)
+        for (int k = 0; k < Nx - 1; k++) {
+            int c = k + l * (Nx - 1);
+            double zt = -(g->zg_coord[l + 1]) * s->L, zb = -(g->zg_coord[l]) * s->L;
+            layer_fractions(zt, zb, f);
+            for (int q = 0; q < 3; q++) g->phase_perc_n[q][c] = f[q];
+        }
+    for (int l = 0; l < Nz; l++)
+        for (int k = 0; k < Nx; k++) {
+            int c = k + l * Nx;
+            double zc = -g->zg_coord[l] * s->L, h = 0.5 * g->dz * s->L;
+            double zt = zc - h < 0 ? 0 : zc - h, zb = zc + h;
+            layer_fractions(zt, zb, f);
+            for (int q = 0; q < 3; q++) g->phase_perc_s[q][c] = f[q];
+        }
+    /* temperature: geotherm + seeded Gaussian anomaly (+50 K, 20 km) */
+    double xa = (-400e3 + 800e3 * urand(seed, 10, 0));
+    double za = -(40e3 + 140e3 * urand(seed, 10, 1));
+    for (int l = 0; l < Nz - 1; l++)
+        for (int k = 0; k < Nx - 1; k++) {
+            int c = k + l * (Nx - 1);
+            double x = g->xc_coord[k] * s->L, z = g->zc_coord[l] * s->L;
+            double dT = 50.0 * exp(-((x - xa) * (x - xa) + (z - za) * (z - za)) / (20e3 * 20e3));
+            g->T[c] = (geotherm(-z) + dT) / s->T;
+        }
+    fill_common_fields(m);
+    set_bcs(g, p, MDZ_BC_PURE_SHEAR, (p->xmax - p->xmin), (p->zmax - p->zmin));
+    mdz_lithostatic_pressure(g, p, g->p_in);
+    for (int c = 0; c < nc; c++) g->p0_n[c] = g->p_in[c];
+    c2v(g, g->p0_n, g->p0_s);
+    /* velocity: pure-shear extension + 1 % seeded noise on free nodes */
+    double vmax = fabs(p->EpsBG) * 0.5 * (p->xmax - p->xmin);
+    for (int l = 0; l < Nz + 1; l++)
+        for (int k = 0; k < Nx; k++) {
+            int c = k + l * Nx;
+            g->u_in[c] = -g->xg_coord[k] * p->EpsBG;
+            if (g->BCu.type[c] == -1) g->u_in[c] += 0.01 * vmax * (2.0 * urand(seed, 1, c) - 1.0);
+        }
+    for (int l = 0; l < Nz; l++)
+        for (int k = 0; k < Nx + 1; k++) {
+            int c = k + l * (Nx + 1);
+            g->v_in[c] = g->zg_coord[l] * p->EpsBG;
+            if (g->BCv.type[c] == -1) g->v_in[c] += 0.01 * vmax * (2.0 * urand(seed, 2, c) - 1.0);
+        }
+    force_dirichlet(g);
+    return m;
+}
+
+mdz_model *mdz_model_edgecase(int ncx, int ncz, unsigned long long seed, int fam, int which) {
+    int Nx = ncx + 1, Nz = ncz + 1;
+    int nph = which == 0 ? 2 : 3;
+    mdz_model *m = model_new(Nx, Nz, nph, seed, fam);
+    grid *g = m->mesh;
+    params *p = &m->model;
+    scale *s = &m->scaling;
+    int nc = (Nx - 1) * (Nz - 1), nv = Nx * Nz;
+    if (which == 0) {
+        s->eta = 1.0; s->L = 1.0; s->V = 1.0; s->T = 1.0;
+        scale_me(s);
+        common_params(p, Nx, Nz, -0.5, 0.5, -0.5, 0.5);
+        p->dt = 1e-1; p->EpsBG = 1.0; p->gz = -1.0;
+        p->mineta = 1e-6; p->maxeta = 1e6;
+        mat_prop *mt = &m->materials;
+        memset(mt, 0, sizeof(*mt));
+        mt->Nb_phases = 2;
+        for (int k = 0; k < 2; k++) {
+            phase_defaults(mt, k, s);
+            mt->cstv[k] = 1; mt->mu[k] = 1.0 + k; mt->C[k] = 1.0 + 0.5 * k; mt->phi[k] = (30.0 - 10 * k) * M_PI / 180.0;
+            mt->rho[k] = 1.0 + 0.2 * k;
+        }
+        mt->eta0[0] = 1.0; mt->eta0[1] = 1e-2;
+        mt->bet[0] = 1e-3; mt->bet[1] = 2e-3; /* bulk modulus 1e3 G: nearly incompressible */
+        mt->psi[1] = 5.0 * M_PI / 180.0;
+        mt->eta_vp[1] = 1e-2;
+        mt->R = RGAS / (s->J / s->T);
+    } else {
+        s->eta = 1e22; s->L = 1e4; s->V = 1e-9; s->T = 100.0;
+        scale_me(s);
+        common_params(p, Nx, Nz, -50e3 / s->L, 50e3 / s->L, -40e3 / s->L, 0.0);
+        p->dt = 1e11 / s->t; p->EpsBG = -3e-15 / s->E; p->gz = -9.81 / s->a;
+        p->mineta = 1e18 / s->eta; p->maxeta = 1e24 / s->eta;
+        lithosphere_materials(m, 30.0, 10e6);
+        mat_prop *mt = &m->materials;
+        set_linear_law(mt, 2, s);             /* olivine diffusion creep */
+        mt->eta_vp[1] = 1e20 / s->S / pow(s->t, 1.0); /* regularised plasticity */
+        mt->psi[1] = 10.0 * M_PI / 180.0;     /* dilatant lower crust */
+        mt->is_tensile[0] = 1;                 /* tension cut-off */
+        mt->Slim[2] = 300e6 / s->S;            /* von-Mises cap */
+        for (int k = 0; k < 3; k++) mt->bet[k] = 1e-11 / (1.0 / s->S); /* K = 100 GPa */
+    }
+    p->Nb_phases = nph;
+    m->materials.Nb_phases = nph;
+    set_coords(g, p);
+    /* random mixtures of phases */
+    for (int c = 0; c < nc; c++) {
+        double tot = 0, w[3];
+        for (int q = 0; q < nph; q++) { w[q] = urand(seed, 20 + q, c); w[q] = w[q] * w[q] * w[q]; tot += w[q]; }
+        for (int q = 0; q < nph; q++) g->phase_perc_n[q][c] = w[q] / tot;
+        if (urand(seed, 19, c) < 0.5) { /* pure cells */
+            int q = (int)(urand(seed, 18, c) * nph); if (q >= nph) q = nph - 1;
+            for (int r = 0; r < nph; r++) g->phase_perc_n[r][c] = r == q ? 1.0 : 0.0;
+        }
+    }
+    for (int c = 0; c < nv; c++) {
+        double tot = 0, w[3];
+        for (int q = 0; q < nph; q++) { w[q] = urand(seed, 30 + q, c); w[q] = w[q] * w[q] * w[q]; tot += w[q]; }
+        for (int q = 0; q < nph; q++) g->phase_perc_s[q][c] = w[q] / tot;
+        if (urand(seed, 29, c) < 0.5) {
+            int q = (int)(urand(seed, 28, c) * nph); if (q >= nph) q = nph - 1;
+            for (int r = 0; r < nph; r++) g->phase_perc_s[r][c] = r == q ? 1.0 : 0.0;
+        }
+    }
+    fill_common_fields(m);
+    for (int c = 0; c < nc; c++) {
+        if (which == 0) {
+            g->T[c] = 1.0;
+            g->p_in[c] = 0.5 + 0.5 * nrand(seed, 40, c);
+        } else {
+            g->T[c] = (573.0 + 1100.0 * urand(seed, 41, c)) / s->T;
+            g->p_in[c] = (3e8 * urand(seed, 42, c) - 3e7) / s->S; /* a few tensile cells */
+        }
+        g->p0_n[c] = g->p_in[c] * (1.0 + 0.01 * nrand(seed, 43, c));
+    }
+    c2v(g, g->p0_n, g->p0_s);
+    /* old deviatoric stresses: up to about the yield stress */
+    double smax = which == 0 ? 1.5 : 300e6 / s->S;
+    for (int c = 0; c < nc; c++) {
+        g->sxxd0[c] = smax * (2.0 * urand(seed, 50, c) - 1.0);
+        g->szzd0[c] = -g->sxxd0[c] + 0.1 * smax * (2.0 * urand(seed, 51, c) - 1.0);
+        g->sxz0_n[c] = smax * (2.0 * urand(seed, 52, c) - 1.0);
+    }
+    for (int c = 0; c < nv; c++) {
+        g->sxz0[c] = smax * (2.0 * urand(seed, 53, c) - 1.0);
+        g->sxxd0_s[c] = smax * (2.0 * urand(seed, 54, c) - 1.0);
+        g->szzd0_s[c] = -g->sxxd0_s[c];
+    }
+    set_bcs(g, p, fam, (p->xmax - p->xmin), (p->zmax - p->zmin));
+    /* velocities: background + 20 % noise so that strain rates vary widely */
+    double vmax = fabs(p->EpsBG) * 0.5 * (p->xmax - p->xmin);
+    for (int l = 0; l < Nz + 1; l++)
+        for (int k = 0; k < Nx; k++) {
+            int c = k + l * Nx;
+            g->u_in[c] = fam == MDZ_BC_SIMPLE_SHEAR ? 2.0 * g->zvx_coord[l] * p->EpsBG : -g->xg_coord[k] * p->EpsBG;
+            if (g->BCu.type[c] != 0 && g->BCu.type[c] != 30) g->u_in[c] += 0.2 * vmax * (2.0 * urand(seed, 1, c) - 1.0);
+        }
+    for (int l = 0; l < Nz; l++)
+        for (int k = 0; k < Nx + 1; k++) {
+            int c = k + l * (Nx + 1);
+            g->v_in[c] = fam == MDZ_BC_SIMPLE_SHEAR ? 0.0 : g->zg_coord[l] * p->EpsBG;
+            if (g->BCv.type[c] != 0 && g->BCv.type[c] != 30) g->v_in[c] += 0.2 * vmax * (2.0 * urand(seed, 2, c) - 1.0);
+        }
+    if (fam == MDZ_BC_SIMPLE_SHEAR) p->isperiodic_x = 1;
+    if (fam == MDZ_BC_STRESS_WE) {
+        /* traction values: minus the cell pressure next to the wall */
+        for (int l = 1; l < Nz; l++) {
+            g->BCu.val[0 + l * Nx] = -g->p_in[0 + (l - 1) * (Nx - 1)];
+            g->BCu.val[Nx - 1 + l * Nx] = -g->p_in[(Nx - 2) + (l - 1) * (Nx - 1)];
+        }
+    }
+    force_dirichlet(g);
+    return m;
+}

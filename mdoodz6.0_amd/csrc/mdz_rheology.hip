@@ -1,0 +1,564 @@
+/*
+ * mdz_rheology.hip -- gfx950 kernels of the rheology half of the
+ * Newton-iteration path: strain rates, centroid->vertex interpolation,
+ * grid rheology (one lane per centroid / vertex, phase loop, averaging),
+ * D-tensor assembly (Picard and consistent tangent) and the momentum /
+ * continuity right-hand sides.
+ *
+ *   k_strain_cells / k_strain_vertices / k_exz_n   RheologyDensity.c:1902-1982
+ *   k_interp_c2v                                    GridRoutines.c:106-196
+ *   k_rheo_cells / k_rheo_vertices                  RheologyDensity.c:961-1415
+ *   k_dtensor_cells / k_dtensor_vertices            RheologyDensity.c:713-954
+ *   k_rhs_u / k_rhs_v / k_rhs_p                     StokesRoutines.c:922-1128
+ *
+ * All HBM-bound except the rheology, which is FP64-VALU bound
+ * (transcendentals of the local iterations).
+ */
+#include "mdz_kernels.h"
+#include "mdz_viscosity.h"
+
+#define BLK 256
+
+/* ---------------------------------------------------------------- */
+/* strain rates                                                      */
+/* ---------------------------------------------------------------- */
+__global__ void __launch_bounds__(BLK) k_strain_cells(DevGrid g) {
+    int c0 = blockIdx.x * BLK + threadIdx.x;
+    int Ncx = g.ncx, Nx = g.Nx;
+    if (c0 >= Ncx * g.ncz) return;
+    int k = c0 % Ncx, l = c0 / Ncx;
+    int c1 = k + l * Nx, c2 = k + l * (Nx + 1);
+    double div = 0.0, exx = 0.0, ezz = 0.0;
+    int t = g.BCp[c0];
+    if (t != 30 && t != 31) {
+        double dudx = (g.u_in[c1 + 1 + Nx] - g.u_in[c1 + Nx]) / g.dx;
+        double dvdz = (g.v_in[c2 + Nx + 1 + 1] - g.v_in[c2 + 1]) / g.dz;
+        div = dudx + dvdz;
+        exx = dudx - 1.0 / 3.0 * div;
+        ezz = (g.v_in[c2 + 1 + (Nx + 1)] - g.v_in[c2 + 1]) / g.dz - 1.0 / 3.0 * div;
+    }
+    g.div_u[c0] = div;
+    g.exxd[c0] = exx;
+    g.ezzd[c0] = ezz;
+}
+
+__global__ void __launch_bounds__(BLK) k_strain_vertices(DevGrid g) {
+    int c1 = blockIdx.x * BLK + threadIdx.x;
+    int Nx = g.Nx;
+    if (c1 >= Nx * g.Nz) return;
+    int k = c1 % Nx, l = c1 / Nx;
+    int c2 = k + l * (Nx + 1);
+    double e = 0.0;
+    if (g.BCg[c1] != 30) {
+        if (g.BCu[c1] != 30 && g.BCu[c1 + Nx] != 30) e += 0.5 * (g.u_in[c1 + Nx] - g.u_in[c1]) / g.dz;
+        if (g.BCv[c2] != 30 && g.BCv[c2 + 1] != 30) e += 0.5 * (g.v_in[c2 + 1] - g.v_in[c2]) / g.dx;
+    }
+    g.exz[c1] = e;
+}
+
+__global__ void __launch_bounds__(BLK) k_exz_n(DevGrid g) {
+    int c0 = blockIdx.x * BLK + threadIdx.x;
+    int Ncx = g.ncx, Nx = g.Nx;
+    if (c0 >= Ncx * g.ncz) return;
+    int k = c0 % Ncx, l = c0 / Ncx;
+    int c1 = k + l * Nx;
+    double s = 0.0, n = 0.0;
+    int t = g.BCp[c0];
+    if (t != 30 && t != 31) {
+        if (g.BCg[c1] != 30) { s += g.exz[c1]; n++; }
+        if (g.BCg[c1 + 1] != 30) { s += g.exz[c1 + 1]; n++; }
+        if (g.BCg[c1 + Nx] != 30) { s += g.exz[c1 + Nx]; n++; }
+        if (g.BCg[c1 + Nx + 1] != 30) { s += g.exz[c1 + Nx + 1]; n++; }
+        if (n > 0) s /= n;
+    }
+    g.exz_n[c0] = s;
+}
+
+/* padded-centroid value at padded coordinates (i, j), i in [0, ncx+1],
+ * j in [0, ncz+1]: zero-flux (or periodic in x) ghost ring */
+__device__ __forceinline__ double padded(const double *__restrict__ c, int i, int j, int ncx, int ncz, int per) {
+    int ci, cj = j - 1;
+    if (i == 0) ci = per ? ncx - 1 : 0;
+    else if (i == ncx + 1) ci = per ? 0 : ncx - 1;
+    else ci = i - 1;
+    cj = cj < 0 ? 0 : (cj > ncz - 1 ? ncz - 1 : cj);
+    return c[ci + cj * ncx];
+}
+
+/* Up to four centroid fields interpolated to vertices in one pass. */
+__global__ void __launch_bounds__(BLK) k_interp_c2v(DevGrid g, int per, int nf, const double *__restrict__ s0,
+                                                    double *__restrict__ d0, const double *__restrict__ s1,
+                                                    double *__restrict__ d1, const double *__restrict__ s2,
+                                                    double *__restrict__ d2, const double *__restrict__ s3,
+                                                    double *__restrict__ d3, const double *__restrict__ s4,
+                                                    double *__restrict__ d4) {
+    int c1 = blockIdx.x * BLK + threadIdx.x;
+    int Nx = g.Nx;
+    if (c1 >= Nx * g.Nz) return;
+    int k = c1 % Nx, l = c1 / Nx;
+    bool air = g.BCg[c1] == 30;
+    const double *src[5] = {s0, s1, s2, s3, s4};
+    double *dst[5] = {d0, d1, d2, d3, d4};
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+        if (f >= nf) break;
+        double v = 0.0;
+        if (!air) {
+            const double *c = src[f];
+            /* same summation order as the padded-array formulation */
+            v = 0.25 * (padded(c, k, l, g.ncx, g.ncz, per) + padded(c, k + 1, l, g.ncx, g.ncz, per)
+                        + padded(c, k, l + 1, g.ncx, g.ncz, per) + padded(c, k + 1, l + 1, g.ncx, g.ncz, per));
+        }
+        dst[f][c1] = v;
+    }
+}
+
+/* ---------------------------------------------------------------- */
+/* grid rheology                                                     */
+/* ---------------------------------------------------------------- */
+
+/* Effective visco-elastic strain rates through the inverse anisotropy
+ * tensor (RheologyDensity.c:1061-1094). */
+__device__ __forceinline__ void ve_rates(const DevParams &prm, double nx, double nz, double af, double fs,
+                                         double exx, double ezz, double exz, double sxx0, double szz0,
+                                         double sxz0, double etae, double &Exx, double &Ezz, double &Exz,
+                                         double &Gxx, double &Gzz, double &Gxz) {
+    double ani = 0.0, d0 = 0.0, d1 = 0.0;
+    if (prm.aniso != 0) {
+        if (prm.aniso_fstrain == 0) ani = 1.0 - 1.0 / af;
+        if (prm.aniso_fstrain == 1) ani = 1.0 - 1.0 / fs;
+        d0 = 2.0 * pow(nx, 2.0) * pow(nz, 2.0);
+        d1 = nx * nz * (-pow(nx, 2.0) + pow(nz, 2.0));
+    }
+    double Da11 = 2.0 - 2.0 * ani * d0, Da12 = 2.0 * ani * d0, Da13 = 2.0 * ani * d1;
+    double Da22 = 2.0 - 2.0 * ani * d0, Da23 = -2.0 * ani * d1, Da33 = 1.0 + 2.0 * ani * (d0 - 0.5);
+    double a11 = Da33 * Da22 - Da23 * Da23, a12 = Da13 * Da23 - Da33 * Da12, a13 = Da12 * Da23 - Da13 * Da22;
+    double a22 = Da33 * Da11 - Da13 * Da13, a23 = Da12 * Da13 - Da11 * Da23, a33 = Da11 * Da22 - Da12 * Da12;
+    double det = (Da11 * a11) + (Da12 * a12) + (Da13 * a13);
+    double i11 = a11 / det, i12 = a12 / det, i13 = a13 / det, i22 = a22 / det, i23 = a23 / det, i33 = a33 / det;
+    Exx = exx + (i11 * sxx0 + i12 * szz0 + i13 * sxz0) / etae;
+    Ezz = ezz + (i12 * sxx0 + i22 * szz0 + i23 * sxz0) / etae;
+    Exz = exz + (i13 * sxx0 + i23 * szz0 + i33 * sxz0) / 2.0 / etae;
+    double gxz = 2.0 * exz + (i13 * sxx0 + i23 * szz0 + i33 * sxz0) / etae;
+    Gxx = Exx * (1.0 - ani * d0) + Ezz * ani * d0 + gxz * ani * d1;
+    Gzz = Ezz * (1.0 - ani * d0) + Exx * ani * d0 - gxz * ani * d1;
+    Gxz = Exx * ani * d1 - Exx * ani * d1 + gxz * (ani * (d0 - 0.5) + 0.5);
+}
+
+__device__ __forceinline__ void hist_add(unsigned long long *hist, int bin) {
+    if (hist) atomicAdd(&hist[bin < MDZ_HIST_BINS ? bin : MDZ_HIST_BINS - 1], 1ULL);
+}
+
+__global__ void __launch_bounds__(BLK) k_rheo_cells(DevGrid g, DevParams prm, const mat_prop *__restrict__ mt,
+                                                    const float *__restrict__ guess_n, unsigned long long *hist) {
+    int c0 = blockIdx.x * BLK + threadIdx.x;
+    int nc = g.ncx * g.ncz;
+    if (c0 >= nc) return;
+    const int average = prm.eta_avg;
+    double eta_n = 0, eta_phys_n = 0, VE_n = 0, sxxd = 0, szzd = 0, sxz_n = 0;
+    double eII_el = 0, eII_pl = 0, eII_pwl = 0, eII_exp = 0, eII_lin = 0, eII_gbs = 0, eII_cst = 0, d_n = 0;
+    double exx_el = 0, ezz_el = 0, exx_diss = 0, ezz_diss = 0;
+    double dexx = 0, dezz = 0, dgxz = 0, dp = 0, ddxx = 0, ddzz = 0, ddgxz = 0, ddp = 0;
+    double p_corr = 0, div_el = 0, div_pl = 0, div_r = 0, Wtot = 0, Wel = 0, Wdiss = 0, X_n = 0, OverS = 0;
+    int t = g.BCp[c0];
+    bool active = t != 30 && t != 31;
+    if (active) {
+        double etae = prm.iselastic == 1 ? prm.dt * g.mu_n[c0] : 1.0;
+        double Exx, Ezz, Exz, Gxx, Gzz, Gxz;
+        double exxd = g.exxd[c0], ezzd = g.ezzd[c0], exz_c = g.exz_n[c0];
+        ve_rates(prm, g.nx0_n[c0], g.nz0_n[c0], g.aniso_factor_n[c0], g.FS_AR_n[c0], exxd, ezzd, exz_c,
+                 g.sxxd0[c0], g.szzd0[c0], g.sxz0_n[c0], etae, Exx, Ezz, Exz, Gxx, Gzz, Gxz);
+        const double mu = g.mu_n[c0], T = g.T[c0], P = g.p_in[c0], d0 = g.d0_n[c0], phi0 = g.phi0_n[c0];
+        const double X0 = g.X0_n[c0], s0xx = g.sxxd0[c0], s0zz = g.szzd0[c0], s0xz = g.sxz0_n[c0];
+        const double dil = g.dil_n[c0], fric = g.fric_n[c0], C = g.C_n[c0], P0 = g.p0_n[c0], bet = g.bet_n[c0];
+        for (int p = 0; p < prm.Nb_phases; p++) {
+            double fr = g.phase_n[(size_t)p * nc + c0];
+            if (!(fabs(fr) > 1.0e-13)) continue;
+            VGuess gs;
+            gs.mode = prm.rheo_mode;
+            gs.eta_ve0 = guess_n ? (double)guess_n[(size_t)p * nc + c0] : 0.0;
+            VOut o;
+            mdz_viscosity(p, mu, T, P, d0, phi0, X0, Exx, Ezz, Exz, Gxx, Gzz, Gxz, s0xx, s0zz, s0xz, mt, prm, dil,
+                          fric, C, P0, bet, gs, o);
+            hist_add(hist, o.it_ve & 0xff);
+            hist_add(hist ? hist + MDZ_HIST_BINS : nullptr, o.it_pl);
+            hist_add(hist ? hist + 2 * MDZ_HIST_BINS : nullptr, o.it_ve >> 8);
+            if (average == 0) {
+                eta_n += fr * o.etaVE;
+                eta_phys_n += fr * o.eta;
+                dexx += fr * o.detadexx;
+                dezz += fr * o.detadezz;
+                dgxz += fr * o.detadexz / 2.0;
+                dp += fr * o.detadp;
+            }
+            if (average == 0 || average == 2) {
+                sxxd += fr * o.txx;
+                szzd += fr * o.tzz;
+                sxz_n += fr * o.txz;
+            }
+            VE_n += fr * o.VEcoeff;
+            eII_el += fr * o.Eii_el;
+            eII_pl += fr * o.Eii_pl;
+            eII_pwl += fr * o.Eii_pwl;
+            eII_exp += fr * o.Eii_exp;
+            eII_lin += fr * o.Eii_lin;
+            eII_gbs += fr * o.Eii_gbs;
+            eII_cst += fr * o.Eii_cst;
+            d_n += fr * 1.0 / o.d1;
+            exx_el += fr * o.Exx_el;
+            exx_diss += fr * o.Exx_diss;
+            ezz_el += fr * o.Ezz_el;
+            ezz_diss += fr * o.Ezz_diss;
+            double txx1 = o.txx, tzz1 = o.tzz, txz1 = o.txz;
+            Wtot += fr * (txx1 * exxd + tzz1 * ezzd + (txx1 + tzz1) * (exxd + ezzd) + 2.0 * txz1 * exz_c);
+            Wdiss += fr * (txx1 * o.Exx_diss + tzz1 * o.Ezz_diss + (txx1 + tzz1) * (o.Exx_diss + o.Ezz_diss) + 2.0 * txz1 * o.Exz_diss);
+            Wel += fr * (txx1 * o.Exx_el + tzz1 * o.Ezz_el + (txx1 + tzz1) * (o.Exx_el + o.Ezz_el) + 2.0 * txz1 * o.Exz_el);
+            ddxx += fr * o.ddivpdexx;
+            ddzz += fr * o.ddivpdezz;
+            ddgxz += fr * o.ddivpdexz / 2.0;
+            ddp += fr * o.ddivpdp;
+            p_corr += fr * o.Pcorr;
+            div_el += fr * o.div_el;
+            div_pl += fr * o.div_pl;
+            div_r += fr * o.div_r;
+            X_n += fr * o.X1;
+            OverS += fr * o.OverS;
+            if (average == 1) {
+                sxxd += fr * 1.0 / txx1;
+                szzd += fr * 1.0 / tzz1;
+                sxz_n += fr * 1.0 / txz1;
+                eta_n += fr * 1.0 / o.etaVE;
+                eta_phys_n += fr * 1.0 / o.eta;
+                double e2 = pow(o.etaVE, 2.0);
+                dexx += fr * o.detadexx / e2;
+                dezz += fr * o.detadezz / e2;
+                dgxz += fr * o.detadexz / 2.0 / e2;
+                dp += fr * o.detadp / e2;
+            }
+            if (average == 2) {
+                eta_n += fr * log(o.etaVE);
+                eta_phys_n += fr * log(o.eta);
+                dexx += fr * o.detadexx / o.etaVE;
+                dezz += fr * o.detadezz / o.etaVE;
+                dgxz += fr * o.detadexz / 2.0 / o.etaVE;
+                dp += fr * o.detadp / o.etaVE;
+            }
+        }
+        d_n = 1.0 / d_n;
+        if (average == 1) {
+            sxxd = 1.0 / sxxd; szzd = 1.0 / szzd; sxz_n = 1.0 / sxz_n;
+            eta_n = 1.0 / eta_n; eta_phys_n = 1.0 / eta_phys_n;
+            double e2 = pow(eta_n, 2.0);
+            dexx *= e2; dezz *= e2; dgxz *= e2; dp *= e2;
+        }
+        if (average == 2) {
+            eta_n = exp(eta_n); eta_phys_n = exp(eta_phys_n);
+            dexx *= eta_n; dezz *= eta_n; dgxz *= eta_n; dp *= eta_n;
+        }
+    }
+    g.eta_n[c0] = eta_n; g.eta_phys_n[c0] = eta_phys_n; g.VE_n[c0] = VE_n;
+    g.sxxd[c0] = sxxd; g.szzd[c0] = szzd; g.sxz_n[c0] = sxz_n;
+    g.eII_el[c0] = eII_el; g.eII_pl[c0] = eII_pl; g.eII_pwl[c0] = eII_pwl; g.eII_exp[c0] = eII_exp;
+    g.eII_lin[c0] = eII_lin; g.eII_gbs[c0] = eII_gbs; g.eII_cst[c0] = eII_cst; g.d_n[c0] = d_n;
+    g.exx_el[c0] = exx_el; g.exx_diss[c0] = exx_diss; g.ezz_el[c0] = ezz_el; g.ezz_diss[c0] = ezz_diss;
+    g.detadexx_n[c0] = dexx; g.detadezz_n[c0] = dezz; g.detadgxz_n[c0] = dgxz; g.detadp_n[c0] = dp;
+    g.ddivpdexx_n[c0] = ddxx; g.ddivpdezz_n[c0] = ddzz; g.ddivpdgxz_n[c0] = ddgxz; g.ddivpdp_n[c0] = ddp;
+    g.p_corr[c0] = p_corr; g.div_u_el[c0] = div_el; g.div_u_pl[c0] = div_pl; g.div_u_r[c0] = div_r;
+    g.Wtot[c0] = Wtot; g.Wel[c0] = Wel; g.Wdiss[c0] = Wdiss;
+    if (prm.UnsplitDiffReac == 0) g.X_n[c0] = X_n;
+    g.OverS_n[c0] = OverS;
+}
+
+__global__ void __launch_bounds__(BLK) k_rheo_vertices(DevGrid g, DevParams prm, const mat_prop *__restrict__ mt,
+                                                       const float *__restrict__ guess_s, unsigned long long *hist) {
+    int c1 = blockIdx.x * BLK + threadIdx.x;
+    int nv = g.Nx * g.Nz;
+    if (c1 >= nv) return;
+    const int average = prm.eta_avg;
+    double VE_s = 0, sxz = 0, eta_phys_s = 0, eta_s = 0, exz_el = 0, exz_diss = 0;
+    double dexx = 0, dezz = 0, dgxz = 0, dp = 0, X_s = 0, OverS = 0;
+    if (g.BCg[c1] != 30) {
+        double etae = prm.iselastic == 1 ? prm.dt * g.mu_s[c1] : 1.0;
+        double Exx, Ezz, Exz, Gxx, Gzz, Gxz;
+        ve_rates(prm, g.nx0_s[c1], g.nz0_s[c1], g.aniso_factor_s[c1], g.FS_AR_s[c1], g.exxd_s[c1], g.ezzd_s[c1],
+                 g.exz[c1], g.sxxd0_s[c1], g.szzd0_s[c1], g.sxz0[c1], etae, Exx, Ezz, Exz, Gxx, Gzz, Gxz);
+        const double mu = g.mu_s[c1], T = g.T_s[c1], P = g.P_s[c1], d0 = g.d0_s[c1], phi0 = g.phi0_s[c1];
+        const double X0 = g.X0_s[c1], s0xx = g.sxxd0_s[c1], s0zz = g.szzd0_s[c1], s0xz = g.sxz0[c1];
+        const double dil = g.dil_s[c1], fric = g.fric_s[c1], C = g.C_s[c1], P0 = g.p0_s[c1], bet = g.bet_s[c1];
+        for (int p = 0; p < prm.Nb_phases; p++) {
+            double fr = g.phase_s[(size_t)p * nv + c1];
+            if (!(fabs(fr) > 1.0e-13)) continue;
+            VGuess gs;
+            gs.mode = prm.rheo_mode;
+            gs.eta_ve0 = guess_s ? (double)guess_s[(size_t)p * nv + c1] : 0.0;
+            VOut o;
+            mdz_viscosity(p, mu, T, P, d0, phi0, X0, Exx, Ezz, Exz, Gxx, Gzz, Gxz, s0xx, s0zz, s0xz, mt, prm, dil,
+                          fric, C, P0, bet, gs, o);
+            hist_add(hist, o.it_ve & 0xff);
+            hist_add(hist ? hist + MDZ_HIST_BINS : nullptr, o.it_pl);
+            hist_add(hist ? hist + 2 * MDZ_HIST_BINS : nullptr, o.it_ve >> 8);
+            if (average == 0) {
+                eta_s += fr * o.etaVE;
+                eta_phys_s += fr * o.eta;
+                dexx += fr * o.detadexx;
+                dezz += fr * o.detadezz;
+                dgxz += fr * o.detadexz / 2.0;
+                dp += fr * o.detadp;
+            }
+            if (average == 0 || average == 2) sxz += fr * o.txz;
+            VE_s += fr * o.VEcoeff;
+            exz_el += fr * o.Exz_el;
+            exz_diss += fr * o.Exz_diss;
+            OverS += fr * o.OverS;
+            X_s += fr * o.X1;
+            if (average == 1) {
+                sxz += fr * 1.0 / o.txz;
+                eta_s += fr * 1.0 / o.etaVE;
+                eta_phys_s += fr * 1.0 / o.eta;
+                double e2 = pow(o.etaVE, 2.0);
+                dexx += fr * o.detadexx / e2;
+                dezz += fr * o.detadezz / e2;
+                dgxz += fr * o.detadexz / 2.0 / e2;
+                dp += fr * o.detadp / e2;
+            }
+            if (average == 2) {
+                eta_s += fr * log(o.etaVE);
+                eta_phys_s += fr * log(o.eta);
+                dexx += fr * o.detadexx / o.etaVE;
+                dezz += fr * o.detadezz / o.etaVE;
+                dgxz += fr * o.detadexz / 2.0 / o.etaVE;
+                dp += fr * o.detadp / o.etaVE;
+            }
+        }
+        if (average == 1) {
+            sxz = 1.0 / sxz; eta_s = 1.0 / eta_s; eta_phys_s = 1.0 / eta_phys_s;
+            double e2 = pow(eta_s, 2.0);
+            dexx *= e2; dezz *= e2; dgxz *= e2; dp *= e2;
+        }
+        if (average == 2) {
+            eta_s = exp(eta_s); eta_phys_s = exp(eta_phys_s);
+            dexx *= eta_s; dezz *= eta_s; dgxz *= eta_s; dp *= eta_s;
+        }
+    }
+    g.VE_s[c1] = VE_s; g.sxz[c1] = sxz; g.eta_phys_s[c1] = eta_phys_s; g.eta_s[c1] = eta_s;
+    g.exz_el[c1] = exz_el; g.exz_diss[c1] = exz_diss;
+    g.detadexx_s[c1] = dexx; g.detadezz_s[c1] = dezz; g.detadgxz_s[c1] = dgxz; g.detadp_s[c1] = dp;
+    if (prm.UnsplitDiffReac == 0) g.X_s[c1] = X_s;
+    g.OverS_s[c1] = OverS;
+}
+
+/* ---------------------------------------------------------------- */
+/* D tensor                                                          */
+/* ---------------------------------------------------------------- */
+__device__ __forceinline__ void aniso_factors(const DevParams &prm, double nx, double nz, double af, double fs,
+                                              double &ani, double &d0, double &d1) {
+    ani = 0.0; d0 = 0.0; d1 = 0.0;
+    if (prm.aniso != 0) {
+        if (prm.aniso_fstrain == 0) ani = 1.0 - 1.0 / af;
+        if (prm.aniso_fstrain == 1) ani = 1.0 - 1.0 / fs;
+        d0 = 2.0 * pow(nx, 2.0) * pow(nz, 2.0);
+        d1 = nx * nz * (-pow(nx, 2.0) + pow(nz, 2.0));
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_dtensor_cells(DevGrid g, DevParams prm, int jac) {
+    int k = blockIdx.x * BLK + threadIdx.x;
+    if (k >= g.ncx * g.ncz) return;
+    double D11 = 0, D12 = 0, D13 = 0, D14 = 0, D21 = 0, D22 = 0, D23 = 0, D24 = 0;
+    int t = g.BCp[k];
+    if (t != 30 && t != 31) {
+        double ani, d0, d1, eta = g.eta_n[k];
+        aniso_factors(prm, g.nx0_n[k], g.nz0_n[k], g.aniso_factor_n[k], g.FS_AR_n[k], ani, d0, d1);
+        D11 = 2.0 * eta - 2.0 * ani * d0 * eta;
+        D12 = 2.0 * ani * d0 * eta;
+        D13 = 2.0 * ani * d1 * eta;
+        D21 = 2.0 * ani * d0 * eta;
+        D22 = 2.0 * eta - 2.0 * ani * d0 * eta;
+        D23 = -2.0 * ani * d1 * eta;
+        if (jac) {
+            double etae = prm.iselastic == 1 ? prm.dt * g.mu_n[k] : 1.0;
+            double K = prm.compressible == 1 ? 1.0 / g.bet_n[k] : 0.0;
+            double Exx, Ezz, Exz, Gxx, Gzz, Gxz;
+            ve_rates(prm, g.nx0_n[k], g.nz0_n[k], g.aniso_factor_n[k], g.FS_AR_n[k], g.exxd[k], g.ezzd[k],
+                     g.exz_n[k], g.sxxd0[k], g.szzd0[k], g.sxz0_n[k], etae, Exx, Ezz, Exz, Gxx, Gzz, Gxz);
+            double a = g.detadexx_n[k], b = g.detadezz_n[k], c = g.detadgxz_n[k], dpp = g.detadp_n[k];
+            double Kdt = K * prm.dt;
+            D11 = D11 + 2.0 * a * Gxx - Kdt * g.ddivpdexx_n[k];
+            D12 = D12 + 2.0 * b * Gxx - Kdt * g.ddivpdezz_n[k];
+            D13 = D13 + 2.0 * c * Gxx - Kdt * g.ddivpdgxz_n[k];
+            D14 = 2.0 * dpp * Exx - Kdt * g.ddivpdp_n[k];
+            D21 = D21 + 2.0 * a * Gzz - Kdt * g.ddivpdexx_n[k];
+            D22 = D22 + 2.0 * b * Gzz - Kdt * g.ddivpdezz_n[k];
+            D23 = D23 + 2.0 * c * Gzz - Kdt * g.ddivpdgxz_n[k];
+            D24 = 2.0 * dpp * Ezz - Kdt * g.ddivpdp_n[k];
+        }
+    }
+    g.D11_n[k] = D11; g.D12_n[k] = D12; g.D13_n[k] = D13; g.D14_n[k] = D14;
+    g.D21_n[k] = D21; g.D22_n[k] = D22; g.D23_n[k] = D23; g.D24_n[k] = D24;
+}
+
+__global__ void __launch_bounds__(BLK) k_dtensor_vertices(DevGrid g, DevParams prm, int jac, int *status) {
+    int k = blockIdx.x * BLK + threadIdx.x;
+    if (k >= g.Nx * g.Nz) return;
+    double D31 = 0, D32 = 0, D33 = 0, D34 = 0;
+    if (g.BCg[k] != 30) {
+        double ani, d0, d1, eta = g.eta_s[k];
+        aniso_factors(prm, g.nx0_s[k], g.nz0_s[k], g.aniso_factor_s[k], g.FS_AR_s[k], ani, d0, d1);
+        D31 = 2.0 * ani * d1 * eta;
+        D32 = -2.0 * ani * d1 * eta;
+        D33 = eta + 2.0 * ani * (d0 - 0.5) * eta;
+        if (jac) {
+            double etae = prm.iselastic == 1 ? prm.dt * g.mu_s[k] : 1.0;
+            double Exx, Ezz, Exz, Gxx, Gzz, Gxz;
+            ve_rates(prm, g.nx0_s[k], g.nz0_s[k], g.aniso_factor_s[k], g.FS_AR_s[k], g.exxd_s[k], g.ezzd_s[k],
+                     g.exz[k], g.sxxd0_s[k], g.szzd0_s[k], g.sxz0[k], etae, Exx, Ezz, Exz, Gxx, Gzz, Gxz);
+            D31 = D31 + 2.0 * g.detadexx_s[k] * Gxz;
+            D32 = D32 + 2.0 * g.detadezz_s[k] * Gxz;
+            D33 = D33 + 2.0 * g.detadgxz_s[k] * Gxz;
+            D34 = 2.0 * g.detadp_s[k] * Exz;
+        }
+    }
+    g.D31_s[k] = D31; g.D32_s[k] = D32; g.D33_s[k] = D33; g.D34_s[k] = D34;
+    if (jac && status) {
+        if (isnan(D34)) atomicOr(status, 1);
+        if (isinf(D34)) atomicOr(status, 2);
+    }
+}
+
+/* ---------------------------------------------------------------- */
+/* right-hand sides                                                  */
+/* ---------------------------------------------------------------- */
+__global__ void __launch_bounds__(BLK) k_rhs_u(DevGrid g, DevParams prm, const double *__restrict__ xg,
+                                               const double *__restrict__ zvx) {
+    int c = blockIdx.x * BLK + threadIdx.x;
+    int Nx = g.Nx, NzVx = g.nzvx, Ncx = g.ncx;
+    if (c >= Nx * NzVx) return;
+    int k = c % Nx, l = c / Nx;
+    int c2 = k + (l - 1) * Ncx;
+    double gx = prm.gx, r = 0.0;
+    if (prm.polar == 1) {
+        double x = xg[k], z = zvx[l], tet = atan(z / x);
+        if (tet > 0.0) gx = prm.gz * cos(tet);
+        if (tet < 0.0) gx = -prm.gz * cos(tet);
+    }
+    g.gx[c] = gx;
+    int t = g.BCu[c];
+    if (l > 0 && l < NzVx - 1 && (t == -1 || t == 2 || t == -2)) {
+        int iPrW = c2 - 1, iPrE = c2, ixyN = c, ixyS = c - Nx, iVxS = c - Nx, iVxN = c + Nx;
+        if (t == -2) iPrW = c2 + Ncx - 1;
+        /* on wall nodes the reference reads one cell outside the centroid
+         * array here (unused there: the elastic terms skip k = 0, Nx-1) */
+        int nc = Ncx * g.ncz;
+        bool inW = iPrW >= 0 && iPrW < nc && g.BCp[iPrW] == -1;
+        bool inE = iPrE >= 0 && iPrE < nc && g.BCp[iPrE] == -1;
+        bool inS = g.BCg[ixyS] != 30 && g.BCu[iVxS] != 13;
+        bool inN = g.BCg[ixyN] != 30 && g.BCu[iVxN] != 13;
+        double rhoVx = prm.polar == 0 ? 0.5 * (g.rho_s[ixyS] + g.rho_s[ixyN])
+                                      : 0.5 * (g.rho_n[iPrW < 0 ? 0 : iPrW] + g.rho_n[iPrE >= nc ? nc - 1 : iPrE]);
+        r = -gx * rhoVx;
+        if (prm.iselastic == 1 && k > 0 && k < Nx - 1) {
+            double dx = g.dx, dz = g.dz, dt = prm.dt;
+            if (inE) r -= 1.0 / dx * (g.eta_n[iPrE] / (g.mu_n[iPrE] * dt) * g.sxxd0[iPrE]);
+            if (inW) r -= 1.0 / dx * (-g.eta_n[iPrW] / (g.mu_n[iPrW] * dt) * g.sxxd0[iPrW]);
+            if (inN) r -= 1.0 / dz * (g.eta_s[ixyN] / (g.mu_s[ixyN] * dt) * g.sxz0[ixyN]);
+            if (inS) r -= 1.0 / dz * (-g.eta_s[ixyS] / (g.mu_s[ixyS] * dt) * g.sxz0[ixyS]);
+        }
+    }
+    g.roger_x[c] = -r;
+}
+
+__global__ void __launch_bounds__(BLK) k_rhs_v(DevGrid g, DevParams prm, const double *__restrict__ xvz,
+                                               const double *__restrict__ zg) {
+    int c = blockIdx.x * BLK + threadIdx.x;
+    int Nx = g.Nx, NxVz = g.nxvz, Nz = g.Nz, Ncx = g.ncx;
+    if (c >= NxVz * Nz) return;
+    int k = c % NxVz, l = c / NxVz;
+    int c1 = k + l * Nx - 1, c2 = k - 1 + (l - 1) * Ncx;
+    double gz = prm.gz, r = 0.0;
+    if (prm.polar == 1) {
+        double x = xvz[k], z = zg[l], tet = atan(z / x);
+        if (tet > 0.0) gz = prm.gz * sin(tet);
+        if (tet < 0.0) gz = -prm.gz * sin(tet);
+    }
+    g.gz[c] = gz;
+    if (k > 0 && k < NxVz - 1 && g.BCv[c] == -1) {
+        int iPrS = c2, iPrN = c2 + Ncx, ixyE = c1 + 1, ixyW = c1, iVyW = c - 1, iVyE = c + 1;
+        int nc = Ncx * g.ncz;
+        if (iPrS < 0 || iPrN >= nc) { g.roger_z[c] = 0.0; return; } /* no interior Vz row outside the cells */
+        bool inS = g.BCp[iPrS] == -1, inN = g.BCp[iPrN] == -1;
+        bool inW = g.BCg[ixyW] != 30 && g.BCv[iVyW] != 13;
+        bool inE = g.BCg[ixyE] != 30 && g.BCv[iVyE] != 13;
+        double rhoVz = 0.5 * (g.rho_n[iPrS] + g.rho_n[iPrN]);
+        r = -gz * rhoVz;
+        if (prm.iselastic == 1 && l > 0 && l < Nz - 1) {
+            double dx = g.dx, dz = g.dz, dt = prm.dt;
+            if (inN) r -= 1.0 / dz * (g.eta_n[iPrN] / (g.mu_n[iPrN] * dt) * (g.szzd0[iPrN]));
+            if (inS) r -= 1.0 / dz * (-g.eta_n[iPrS] / (g.mu_n[iPrS] * dt) * (g.szzd0[iPrS]));
+            if (inE) r -= 1.0 / dx * (g.eta_s[ixyE] / (g.mu_s[ixyE] * dt) * g.sxz0[ixyE]);
+            if (inW) r -= 1.0 / dx * (-g.eta_s[ixyW] / (g.mu_s[ixyW] * dt) * g.sxz0[ixyW]);
+        }
+    }
+    g.roger_z[c] = -r;
+}
+
+__global__ void __launch_bounds__(BLK) k_rhs_p(DevGrid g, DevParams prm) {
+    int c = blockIdx.x * BLK + threadIdx.x;
+    if (c >= g.ncx * g.ncz) return;
+    int t = g.BCp[c];
+    if (t != 30 && t != 31) {
+        g.rhs_t[c] = g.T[c];
+        g.Qrho[c] = 0.0;
+        double r = 0.0;
+        if (prm.compressible == 1 && g.comp_cells[c] == 1) {
+            if (prm.VolChangeReac == 0) r += g.p0_n[c] * g.bet_n[c] / prm.dt;
+            if (prm.VolChangeReac == 1) r += log(g.rho0_n[c]) / prm.dt;
+            if (prm.adiab_heat > 0) r += g.divth0_n[c];
+        }
+        g.rhs_p[c] = r;
+    }
+}
+
+/* ---------------------------------------------------------------- */
+/* launchers                                                         */
+/* ---------------------------------------------------------------- */
+void mdz_launch_strain_rates(const DevGrid &g, const DevParams &prm, hipStream_t s) {
+    int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
+    hipLaunchKernelGGL(k_strain_cells, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, g);
+    hipLaunchKernelGGL(k_strain_vertices, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g);
+    hipLaunchKernelGGL(k_exz_n, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, g);
+    hipLaunchKernelGGL(k_interp_c2v, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm.isperiodic_x, 2,
+                       (const double *)g.exxd, g.exxd_s, (const double *)g.ezzd, g.ezzd_s, (const double *)nullptr,
+                       (double *)nullptr, (const double *)nullptr, (double *)nullptr, (const double *)nullptr,
+                       (double *)nullptr);
+}
+
+void mdz_launch_interp(const DevGrid &g, const DevParams &prm, const double *src, double *dst, hipStream_t s) {
+    int nv = g.Nx * g.Nz;
+    hipLaunchKernelGGL(k_interp_c2v, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm.isperiodic_x, 1, src, dst,
+                       (const double *)nullptr, (double *)nullptr, (const double *)nullptr, (double *)nullptr,
+                       (const double *)nullptr, (double *)nullptr, (const double *)nullptr, (double *)nullptr);
+}
+
+void mdz_launch_rheology(const DevGrid &g, const DevParams &prm, const mat_prop *mt, const float *guess_n,
+                         const float *guess_s, unsigned long long *hist, hipStream_t s) {
+    int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
+    hipLaunchKernelGGL(k_interp_c2v, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm.isperiodic_x, 5,
+                       (const double *)g.div_u, g.div_u_s, (const double *)g.T, g.T_s, (const double *)g.p_in, g.P_s,
+                       (const double *)g.d0_n, g.d0_s, (const double *)g.phi0_n, g.phi0_s);
+    hipLaunchKernelGGL(k_rheo_cells, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, g, prm, mt, guess_n, hist);
+    hipLaunchKernelGGL(k_rheo_vertices, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm, mt, guess_s, hist);
+}
+
+void mdz_launch_dtensor(const DevGrid &g, const DevParams &prm, int jac, int *status, hipStream_t s) {
+    int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
+    hipLaunchKernelGGL(k_dtensor_cells, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, g, prm, jac);
+    hipLaunchKernelGGL(k_dtensor_vertices, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm, jac, status);
+}
+
+void mdz_launch_rhs(const DevGrid &g, const DevParams &prm, const double *xg, const double *zvx,
+                    const double *xvz, const double *zg, hipStream_t s) {
+    int nu = g.Nx * g.nzvx, nw = g.nxvz * g.Nz, nc = g.ncx * g.ncz;
+    hipLaunchKernelGGL(k_rhs_u, dim3(mdz_div_up(nu, BLK)), dim3(BLK), 0, s, g, prm, xg, zvx);
+    hipLaunchKernelGGL(k_rhs_v, dim3(mdz_div_up(nw, BLK)), dim3(BLK), 0, s, g, prm, xvz, zg);
+    hipLaunchKernelGGL(k_rhs_p, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, g, prm);
+}

@@ -1,0 +1,412 @@
+/*
+ * mdz_viscosity.h -- device local constitutive update of one node and one
+ * phase: composite dislocation / diffusion / GBS / Peierls creep, Maxwell
+ * elasticity, Drucker-Prager return map with von-Mises and tension
+ * cut-offs, and the consistent tangent (SOURCE/RheologyDensity.c:44-707).
+ *
+ * The visco-elastic local Newton iteration on eta_ve can run in FP32 with
+ * an FP64 polish (the FP64 polish always finishes to the reference's
+ * tolerance |r/Eii| < 1e-13), and can start from a supplied initial guess
+ * instead of the reference's min-of-isolated-viscosities guess.
+ */
+#pragma once
+
+#include "mdz_device.h"
+
+#define MDZ_ZEROC 273.15
+
+struct VOut {
+    double eta, txx, tzz, txz, etaVE, VEcoeff;
+    double Eii_el, Eii_pl, Eii_pwl, Eii_exp, Eii_lin, Eii_gbs, Eii_cst;
+    double Exx_el, Ezz_el, Exz_el, Exx_diss, Ezz_diss, Exz_diss, d1;
+    double detadexx, detadezz, detadexz, detadp, X1, OverS;
+    double ddivpdexx, ddivpdezz, ddivpdexz, ddivpdp, Pcorr, drhodp, rho;
+    double div_el, div_pl, div_r;
+    int it_ve, it_pl;
+};
+
+/* Optional initial guess and arithmetic of the VE iteration. */
+struct VGuess {
+    int mode;        /* MDZ_RHEO_FP64 / MIXED / LEARNED */
+    double eta_ve0;  /* > 0: initial guess for eta_ve (learned mode) */
+};
+
+__device__ __forceinline__ double mdz_min(double a, double b) { return a <= b ? a : b; }
+
+/* FP32 Newton on the VE residual from eta0 until |r/Eii| < 1e-6 or 20
+ * iterations; returns the FP32 iterate (only used as a start for FP64). */
+__device__ inline float ve_newton_f32(float eta0, float Eii, float f_ani, float eta_el, float eta_cst,
+                                      float lC_pwl, float n_pwl, float lC_exp, float nST, float lC_lin, float n_lin,
+                                      float lC_gbs, float n_gbs, int elastic, int constant,
+                                      int dislocation, int peierls, int diffusion, int gbs, int *its) {
+    float eta = eta0;
+    int it;
+    for (it = 0; it < 20; it++) {
+        float Tii = 2.0f * f_ani * eta * Eii;
+        float lT = __logf(Tii);
+        float Ecst = constant ? Tii / f_ani / 2.0f / eta_cst : 0.0f;
+        /* creep rates as exp(log C + n log Tii): no FP32 overflow of C */
+        float Epwl = dislocation ? __expf(lC_pwl + n_pwl * lT) : 0.0f;
+        float Egbs = gbs ? __expf(lC_gbs + n_gbs * lT) : 0.0f;
+        float Eexp = peierls ? __expf(lC_exp + nST * lT) : 0.0f;
+        float Elin = diffusion ? __expf(lC_lin + n_lin * lT) : 0.0f;
+        float r = Eii - (elastic ? Tii / (2.0f * eta_el) : 0.0f) - (Epwl + Eexp + Elin + Egbs + Ecst);
+        if (fabsf(r / Eii) < 1e-6f) break;
+        float df = 0.0f;
+        if (elastic) df += -Eii / eta_el;
+        if (peierls) df += -Eexp * nST / eta;
+        if (diffusion) df += -Elin * n_lin / eta;
+        if (dislocation) df += -Epwl * n_pwl / eta;
+        if (constant) df += -Eii / eta_cst;
+        float en = eta - r / df;
+        /* stay in the bracket where the residual is monotone */
+        eta = en > 0.0f ? en : 0.5f * eta;
+    }
+    *its = it;
+    return eta;
+}
+
+__device__ inline void mdz_viscosity(int phase, double G, double T, double P, double d, double phi, double X0,
+                                     double Exx, double Ezz, double Exz, double Gxx, double Gzz, double Gxz,
+                                     double Txx0, double Tzz0, double Txz0, const mat_prop *__restrict__ mt,
+                                     const DevParams &prm, double dil, double fric, double C, double P0,
+                                     double beta, const VGuess &guess, VOut &o) {
+    const double R = mt->R, dt = prm.dt;
+    const double TmaxPeierls = (1200.0 + MDZ_ZEROC) / prm.sT;
+    const int nitmax = 20;
+    const double tol = 1.0e-11;
+    int it;
+    int constant = 0, dislocation = 0, peierls = 0, diffusion = 0, gbs = 0;
+    int elastic = prm.iselastic;
+    double K = 1.0 / beta;
+    double Tii0 = sqrt(Txx0 * Txx0 + Txz0 * Txz0);
+
+    const double Ea_pwl = mt->Qpwl[phase], Va_pwl = mt->Vpwl[phase], n_pwl = mt->npwl[phase];
+    const double m_pwl = mt->mpwl[phase], r_pwl = mt->rpwl[phase], A_pwl = mt->Apwl[phase];
+    const double f_pwl = mt->fpwl[phase], a_pwl = mt->apwl[phase], F_pwl = mt->Fpwl[phase];
+    const double pre_factor = mt->pref_pwl[phase];
+    const double Ea_lin = mt->Qlin[phase], Va_lin = mt->Vlin[phase], n_lin = mt->nlin[phase];
+    const double m_lin = mt->mlin[phase], r_lin = mt->rlin[phase], A_lin = mt->Alin[phase];
+    const double f_lin = mt->flin[phase], a_lin = mt->alin[phase], F_lin = mt->Flin[phase];
+    const double Ea_gbs = mt->Qgbs[phase], Va_gbs = mt->Vgbs[phase], n_gbs = mt->ngbs[phase];
+    const double m_gbs = mt->mgbs[phase], r_gbs = mt->rgbs[phase], A_gbs = mt->Agbs[phase];
+    const double f_gbs = mt->fgbs[phase], a_gbs = mt->agbs[phase], F_gbs = mt->Fgbs[phase];
+    const double Ea_exp = mt->Qexp[phase], S_exp = mt->Sexp[phase], E_exp = mt->Eexp[phase];
+    const double gamma = mt->Gexp[phase], n_exp = mt->nexp[phase];
+    int gs = mt->gs[phase];
+    const double pg = mt->ppzm[phase], Kg = mt->Kpzm[phase], Qg = mt->Qpzm[phase], gam = mt->Gpzm[phase];
+    const double cg = mt->cpzm[phase], lambda = mt->Lpzm[phase];
+    const double eta_vp0 = mt->eta_vp[phase], n_vp = mt->n_vp[phase];
+    int phase_two = mt->phase_two[phase], constant_mix = mt->phase_mix[phase];
+    const int mix_avg = prm.diffuse_avg;
+    const double alpha = mt->alp[phase];
+    double eta_vp = 0.0;
+
+    double B_pwl = 0, B_lin = 0, B_exp = 0, B_gbs = 0, C_pwl = 0, C_lin = 0, C_exp = 0, C_gbs = 0;
+    double dCpwldP = 0, dClindP = 0, ST = 0;
+    double eta_pwl = 0, eta_exp = 0, eta_lin = 0, eta_el, eta_gbs = 0, eta_cst = 0, eta_vep = 0;
+    double Txx, Tzz, Txz, Tyy, Tii, Eii, Gii, f_ani, res = 0, res0 = 0;
+    double Tyield, F_trial, gdot = 0, dQdtxx = 0, dQdtzz = 0, dQdtxz = 0, dQdP = 0;
+    double Exx_lin = 0, Ezz_lin = 0, Exz_lin = 0, Exx_exp = 0, Ezz_exp = 0, Exz_exp = 0;
+    double Exx_cst = 0, Ezz_cst = 0, Exz_cst = 0;
+    double divp = 0.0, Pc = P, F_trial0 = 0.0;
+    double deta_vep_dExx = 0, deta_vep_dEzz = 0, deta_vep_dExz = 0, deta_vep_dP = 0;
+    double dlamdExx = 0, dlamdEzz = 0, dlamdExz = 0, dlamdP = 0;
+    const double ddivrdpc = 0.0, divr = 0.0;
+    int is_pl = 0;
+
+    if (prm.diffuse_X == 0) constant_mix = 0;
+
+    o.Eii_exp = 0; o.Eii_lin = 0; o.Eii_pl = 0; o.Eii_pwl = 0; o.Eii_el = 0; o.Eii_gbs = 0; o.Eii_cst = 0;
+    o.Exx_el = 0; o.Ezz_el = 0; o.Exz_el = 0;
+    o.OverS = 0; o.it_pl = 0;
+
+    double Eyy = -(Exx + Ezz), Gyy = -(Gxx + Gzz);
+    Eii = sqrt(1.0 / 2.0 * (Exx * Exx + Ezz * Ezz + Eyy * Eyy) + Exz * Exz);
+    Gii = sqrt(1.0 / 2.0 * (Gxx * Gxx + Gzz * Gzz + Gyy * Gyy) + Gxz * Gxz);
+    if (Eii * prm.sE < 1e-30) Eii = 1e-30 / prm.sE;
+    if (Gii * prm.sE < 1e-30) Gii = 1e-30 / prm.sE;
+    f_ani = Gii / Eii;
+    o.Pcorr = P;
+
+    if (mt->cstv[phase] != 0) constant = 1;
+    if (mt->pwlv[phase] != 0) dislocation = 1;
+    if (mt->expv[phase] != 0 && T < TmaxPeierls) peierls = 1;
+    if (mt->linv[phase] != 0) diffusion = 1;
+    if (mt->gbsv[phase] != 0) gbs = 1;
+    if (mt->gs[phase] != 0) gs = 1;
+    if (prm.step == 0) elastic = 0;
+    double d1 = mt->gs_ref[phase];
+    if (elastic == 0) { G = 1e1; K = 1e1; dil = 0.0; }
+    if (T < MDZ_ZEROC / prm.sT) T = MDZ_ZEROC / prm.sT;
+
+    if (dislocation == 1) {
+        B_pwl = pre_factor * F_pwl * pow(A_pwl, -1.0 / n_pwl) * exp((Ea_pwl + P * Va_pwl) / R / n_pwl / T)
+              * pow(d, m_pwl / n_pwl) * pow(f_pwl, -r_pwl / n_pwl) * exp(-a_pwl * phi / n_pwl);
+        C_pwl = pow(2.0 * f_ani * B_pwl, -n_pwl);
+        dCpwldP = -C_pwl * Va_pwl / R / T;
+    }
+    if (diffusion == 1) {
+        B_lin = F_lin * pow(A_lin, -1.0 / n_lin) * exp((Ea_lin + P * Va_lin) / R / n_lin / T)
+              * pow(f_lin, -r_lin / n_lin) * exp(-a_lin * phi / n_lin);
+        C_lin = pow(2.0 * f_ani * B_lin, -n_lin);
+        dClindP = -C_lin * Va_lin / R / T;
+    }
+    if (gbs == 1) {
+        B_gbs = F_gbs * pow(A_gbs, -1.0 / n_gbs) * exp((Ea_gbs + P * Va_gbs) / R / n_gbs / T)
+              * pow(d, m_gbs / n_gbs) * pow(f_gbs, -r_gbs / n_gbs) * exp(-a_gbs * phi / n_gbs);
+        C_gbs = pow(2.0 * f_ani * B_gbs, -n_gbs);
+    }
+    if (peierls == 1) {
+        ST = Ea_exp / R / T * 2.0 * gamma * (1 - gamma);
+        double Arr_exp = exp(-Ea_exp / R / T * pow(1.0 - gamma, 2.0));
+        double F_exp = pow(pow(2.0, 1.0 - ST - n_exp) / pow(sqrt(3.0), ST + n_exp + 1.0), 1.0 / (ST + n_exp));
+        B_exp = F_exp * (pow(gamma * S_exp, ST / (ST + n_exp)) / pow(E_exp * Arr_exp, 1.0 / (ST + n_exp)));
+        C_exp = pow(2.0 * f_ani * B_exp, -(ST + n_exp));
+    }
+
+    double cos_fric = cos(fric), sin_fric = sin(fric), sin_dil = sin(dil);
+    Tyield = C * cos_fric + P * sin_fric;
+    if (mt->Slim[phase] < Tyield) {
+        sin_fric = 0.0; cos_fric = 1.0; sin_dil = 0.0;
+        C = mt->Slim[phase]; Tyield = mt->Slim[phase];
+    }
+    int tens = 0;
+    {
+        double P_tens = -25e6 / prm.sS;
+        double sin_fric_tens = -C * cos_fric / P_tens;
+        if (C * cos_fric + P * sin_fric_tens < Tyield && mt->is_tensile[phase] == 1) {
+            sin_fric = sin_fric_tens;
+            Tyield = C * cos_fric + P * sin_fric_tens;
+            tens = 1;
+        }
+    }
+    o.X1 = X0;
+
+    eta_el = G * dt;
+    if (constant == 1) eta_cst = mt->eta0[phase];
+    if (constant_mix == 1 && mix_avg == 0) eta_cst = X0 * mt->eta0[phase] + (1 - X0) * mt->eta0[phase_two];
+    if (constant_mix == 1 && mix_avg == 1) eta_cst = pow(X0 / mt->eta0[phase] + (1 - X0) / mt->eta0[phase_two], -1.0);
+    if (constant_mix == 1 && mix_avg == 2) eta_cst = exp(X0 * log(mt->eta0[phase]) + (1 - X0) * log(mt->eta0[phase_two]));
+    if (dislocation == 1) eta_pwl = B_pwl * pow(Eii, 1.0 / n_pwl - 1.0);
+    if (diffusion == 1) eta_lin = B_lin * pow(Eii, 1.0 / n_lin - 1.0) * pow(d, m_lin / n_lin);
+    if (gbs == 1) eta_gbs = B_gbs * pow(Eii, 1.0 / n_gbs - 1.0);
+    if (peierls == 1) eta_exp = B_exp * pow(Eii, 1.0 / (ST + n_exp) - 1.0);
+
+    double eta_up = 1.0e100 / prm.seta;
+    if (constant == 1) eta_up = mdz_min(eta_up, eta_cst);
+    if (dislocation == 1) eta_up = mdz_min(eta_up, eta_pwl);
+    if (elastic == 1) eta_up = mdz_min(eta_up, eta_el);
+    if (peierls == 1) eta_up = mdz_min(eta_up, eta_exp);
+    if (diffusion == 1) eta_up = mdz_min(eta_up, eta_lin);
+    if (gbs == 1) eta_up = mdz_min(eta_up, eta_gbs);
+    double eta_ve = eta_up;
+
+    /* reduced-precision pre-iterations: same residual in FP32 with the
+     * exponents applied as exp(n log Tii); grain-size evolution stays FP64 */
+    int it32 = 0;
+    if (guess.mode != MDZ_RHEO_FP64 && gs == 0) {
+        double e0 = (guess.eta_ve0 > 0.0) ? guess.eta_ve0 : eta_ve;
+        float lcp = dislocation ? (float)log(C_pwl) : 0.0f;
+        float lce = peierls ? (float)log(C_exp) : 0.0f;
+        float lcl = diffusion ? (float)(log(C_lin) - m_lin * log(d1)) : 0.0f;
+        float lcg = gbs ? (float)log(C_gbs) : 0.0f;
+        float ef = ve_newton_f32((float)e0, (float)Eii, (float)f_ani, (float)eta_el, (float)eta_cst,
+                                 lcp, (float)n_pwl, lce, (float)(ST + n_exp), lcl,
+                                 (float)n_lin, lcg, (float)n_gbs, elastic, constant, dislocation,
+                                 peierls, diffusion, gbs, &it32);
+        /* a finite, positive FP32 iterate replaces the reference guess */
+        if (ef > 0.0f && ef < 3.0e38f) eta_ve = (double)ef;
+    } else if (guess.eta_ve0 > 0.0) {
+        eta_ve = guess.eta_ve0;
+    }
+
+    double Eii_pwl = 0, Eii_gbs = 0, Eii_exp = 0, Eii_lin = 0, Eii_cst = 0;
+    for (it = 0; it < nitmax; it++) {
+        Tii = 2.0 * f_ani * eta_ve * Eii;
+        if (constant == 1) Eii_cst = Tii / f_ani / 2.0 / eta_cst;
+        if (dislocation == 1) Eii_pwl = C_pwl * pow(Tii, n_pwl);
+        if (gbs == 1) Eii_gbs = C_gbs * pow(Tii, n_gbs);
+        if (peierls == 1) Eii_exp = C_exp * pow(Tii, ST + n_exp);
+        if (gs == 1) d1 = exp(log(Kg * exp(-Qg / R / T) * gam / (lambda * (1.0 / cg) * Tii * (Eii_pwl + Eii_exp + Eii_gbs + 0.0) * pg)) / (1.0 + pg));
+        if (diffusion == 1) Eii_lin = C_lin * pow(Tii, n_lin) * pow(d1, -m_lin);
+        double Eii_vis = Eii_pwl + Eii_exp + Eii_lin + Eii_gbs + Eii_cst;
+        double r_eta_ve = Eii - elastic * Tii / (2.0 * eta_el) - Eii_vis;
+        res = fabs(r_eta_ve / Eii);
+        if (it == 0) res0 = res;
+        if (res < tol / 100) break;
+        double dfdeta = 0.0;
+        if (elastic == 1) dfdeta += -Eii / eta_el;
+        if (peierls == 1) dfdeta += -(Eii_exp) * (ST + n_exp) / eta_ve;
+        if (diffusion == 1) dfdeta += -(Eii_lin) * n_lin / eta_ve;
+        if (dislocation == 1) dfdeta += -(Eii_pwl) * n_pwl / eta_ve;
+        if (constant == 1) dfdeta += -Eii / eta_cst;
+        eta_ve -= r_eta_ve / dfdeta;
+    }
+    o.it_ve = it + (it32 << 8);
+    o.Eii_pwl = Eii_pwl; o.Eii_gbs = Eii_gbs; o.Eii_exp = Eii_exp; o.Eii_lin = Eii_lin; o.Eii_cst = Eii_cst;
+
+    Tii = 2.0 * eta_ve * f_ani * Eii;
+    Txx = Gxx * Tii / f_ani / Eii;
+    Tyy = Gxx * Tii / f_ani / Eii;
+    Tzz = Gzz * Tii / f_ani / Eii;
+    Txz = Gxz * Tii / f_ani / Eii;
+
+    if (dislocation == 1) eta_pwl = pow(2.0 * C_pwl, -1.0) * pow(Tii, 1.0 - n_pwl);
+    if (diffusion == 1) eta_lin = B_lin * pow(Eii_lin, 1.0 / n_lin - 1.0) * pow(d1, m_lin / n_lin);
+    if (peierls == 1) eta_exp = B_exp * pow(Eii_exp, 1.0 / (ST + n_exp) - 1.0);
+    if (diffusion == 1) { Exx_lin = Txx / 2.0 / eta_lin; Ezz_lin = Tzz / 2.0 / eta_lin; Exz_lin = Txz / 2.0 / eta_lin; }
+    if (peierls == 1) { Exx_exp = Txx / 2.0 / eta_exp; Ezz_exp = Tzz / 2.0 / eta_exp; Exz_exp = Txz / 2.0 / eta_exp; }
+    if (constant == 1) { Exx_cst = Txx / 2.0 / eta_cst; Ezz_cst = Tzz / 2.0 / eta_cst; Exz_cst = Txz / 2.0 / eta_cst; }
+
+    double deta = 0.0;
+    if (dislocation == 1) deta += -C_pwl * pow(Tii, n_pwl - 3.0) * (n_pwl - 1.0);
+    if (peierls == 1) deta += -C_exp * pow(Tii, n_exp + ST - 3.0) * (n_exp + ST - 1.0);
+    if (diffusion == 1) deta += -C_lin * pow(Tii, n_lin - 3.0) * (n_lin - 1.0) * pow(d1, -m_lin);
+    double eve2 = pow(eta_ve, 2.0);
+    double detadTxx = deta * (2.0 * Txx + Tzz) * eve2;
+    double detadTzz = deta * (2.0 * Tzz + Txx) * eve2;
+    double detadTxz = 2.0 * deta * Txz * eve2;
+    double den = 1.0 - 2.0 * (detadTxx * Exx + detadTzz * Ezz + detadTxz * Exz);
+    double deta_ve_dExx = detadTxx * 2.0 * eta_ve / den;
+    double deta_ve_dEzz = detadTzz * 2.0 * eta_ve / den;
+    double deta_ve_dExz = detadTxz * 2.0 * eta_ve / den;
+    double dummy = 0.0, Jii = Tii * Tii;
+    if (dislocation == 1) dummy += -2.0 * pow(Jii, (1.0 / 2.0) * n_pwl - 1.0 / 2.0) * dCpwldP;
+    if (diffusion == 1) dummy += -2.0 * pow(Jii, (1.0 / 2.0) * n_lin - 1.0 / 2.0) * pow(d, -m_lin) * dClindP;
+    double deta_ve_dP = pow(eta_ve, 2) * dummy;
+
+    F_trial = Tii - Tyield;
+    if (tens == 1) {
+        eta_vp = eta_vp0 * pow(Eii, 1.0 / n_vp - 1);
+        double Pc_chk = -(C * cos_fric) / (-Tii / P + sin_fric);
+        sin_dil = (P * eta_ve + P * eta_vp - Pc_chk * eta_ve - Pc_chk * eta_vp) / (K * dt * (C * cos_fric + Pc_chk * sin_fric - Tii));
+    }
+    if (F_trial > 1e-17) {
+        is_pl = 1;
+        eta_vp = eta_vp0 * pow(Eii, 1.0 / n_vp - 1);
+        gdot = F_trial / (eta_ve + eta_vp + K * dt * sin_fric * sin_dil);
+        dQdtxx = Txx / 2.0 / Tii;
+        dQdtzz = Tzz / 2.0 / Tii;
+        dQdtxz = Txz / 1.0 / Tii;
+        dQdP = -sin_dil;
+        res0 = F_trial;
+        F_trial0 = F_trial;
+        for (it = 0; it < nitmax; it++) {
+            dQdP = -sin_dil;
+            divp = -gdot * dQdP;
+            Pc = P + K * dt * divp;
+            eta_vp = eta_vp0 * pow(fabs(gdot), 1.0 / n_vp - 1.0);
+            Tyield = C * cos_fric + Pc * sin_fric + gdot * eta_vp;
+            F_trial = Tii - eta_ve * gdot - Tyield;
+            res = fabs(F_trial);
+            if (res < tol || res / F_trial0 < tol) break;
+            double dFdgdot = -eta_ve - eta_vp / n_vp - K * dt * sin_fric * sin_dil;
+            gdot -= F_trial / dFdgdot;
+        }
+        o.it_pl = it + 1;
+        Txx = 2.0 * eta_ve * (Exx - gdot * dQdtxx);
+        Tzz = 2.0 * eta_ve * (Ezz - gdot * dQdtzz);
+        Tyy = -(Txx + Tzz);
+        Txz = 2.0 * eta_ve * (Exz - gdot * dQdtxz / 2.0);
+        Tii = sqrt(0.5 * (pow(Txx, 2.0) + pow(Tzz, 2.0) + pow(Tyy, 2.0)) + pow(Txz, 2.0));
+        eta_vep = Tii / (2.0 * Eii);
+        o.Eii_pl = gdot / 2.0;
+        Tii = 2.0 * eta_vep * Eii;
+        o.OverS = eta_vp * gdot;
+        o.Pcorr = Pc;
+    }
+
+    double dsin_dildExx = 0.0, dsin_dildEzz = 0.0, dsin_dildExz = 0.0, dsin_dildP = 0.0;
+    if (is_pl == 1) {
+        if (tens == 1) {
+            double dE[3] = {(2.0 * Exx + Ezz) / 2.0 / Eii, (2.0 * Ezz + Exx) / 2.0 / Eii, Exz / 1.0 / Eii};
+            double de[3] = {deta_ve_dExx, deta_ve_dEzz, deta_ve_dExz};
+            double a2 = 2 * Eii * eta_ve - P * sin_fric;
+            double bden = K * dt * a2 * (C * P * cos_fric * sin_fric + (C * cos_fric - 2 * Eii * eta_ve) * a2);
+            double q1 = C * P * cos_fric * sin_fric + a2 * a2;
+            double od[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                double dT = Eii * de[i] + dE[i] * eta_ve;
+                od[i] = (2 * K * dt * sin_dil * dT * q1
+                         + P * (-C * cos_fric * de[i] * a2 + 2 * C * cos_fric * (eta_ve + eta_vp) * dT + de[i] * a2 * a2)) / bden;
+            }
+            dsin_dildExx = od[0]; dsin_dildEzz = od[1]; dsin_dildExz = od[2];
+            dsin_dildP = (2 * C * Eii * cos_fric * (eta_ve + eta_vp) * (P * deta_ve_dP - eta_ve)
+                          - C * P * cos_fric * deta_ve_dP * a2
+                          + 2 * Eii * K * dt * sin_dil * (C * cos_fric * sin_fric * (P * deta_ve_dP - eta_ve) + deta_ve_dP * a2 * a2)
+                          + a2 * a2 * (P * deta_ve_dP + eta_ve + eta_vp)) / bden;
+        }
+        double dFdExx = (2.0 * Exx + Ezz) * eta_ve / Eii + 2.0 * Eii * deta_ve_dExx;
+        double dFdEzz = (2.0 * Ezz + Exx) * eta_ve / Eii + 2.0 * Eii * deta_ve_dEzz;
+        double dFdExz = 2.0 * Exz * eta_ve / Eii + 2.0 * Eii * deta_ve_dExz;
+        double dFdP = -sin_fric + 2.0 * Eii * deta_ve_dP;
+        double g = 1.0 / (eta_ve + eta_vp / n_vp + K * dt * sin_fric * sin_dil);
+        dlamdExx = g * (dFdExx - gdot * (deta_ve_dExx + K * dt * sin_fric * dsin_dildExx));
+        dlamdEzz = g * (dFdEzz - gdot * (deta_ve_dEzz + K * dt * sin_fric * dsin_dildEzz));
+        dlamdExz = g * (dFdExz - gdot * (deta_ve_dExz + K * dt * sin_fric * dsin_dildExz));
+        dlamdP = g * (dFdP - gdot * (deta_ve_dP + K * dt * sin_fric * dsin_dildP));
+        double deta_vp_dExx = eta_vp / gdot * dlamdExx * (1.0 / n_vp - 1.0);
+        double deta_vp_dEzz = eta_vp / gdot * dlamdEzz * (1.0 / n_vp - 1.0);
+        double deta_vp_dExz = eta_vp / gdot * dlamdExz * (1.0 / n_vp - 1.0);
+        double deta_vp_dP = eta_vp / gdot * dlamdP * (1.0 / n_vp - 1.0);
+        double a = eta_vp + K * dt * sin_fric * sin_dil;
+        double E3 = 2.0 * pow(Eii, 3.0);
+        deta_vep_dExx = -0.5 * (2.0 * Exx + Ezz) * Tyield / E3 + (a * dlamdExx + gdot * deta_vp_dExx + gdot * K * dt * sin_fric * dsin_dildExx) / (2.0 * Eii);
+        deta_vep_dEzz = -0.5 * (2.0 * Ezz + Exx) * Tyield / E3 + (a * dlamdEzz + gdot * deta_vp_dEzz + gdot * K * dt * sin_fric * dsin_dildEzz) / (2.0 * Eii);
+        deta_vep_dExz = -Exz * Tyield / E3 + (a * dlamdExz + gdot * deta_vp_dExz + gdot * K * dt * sin_fric * dsin_dildExz) / (2.0 * Eii);
+        deta_vep_dP = (sin_fric + a * dlamdP + gdot * deta_vp_dP + gdot * K * dt * sin_fric * dsin_dildP) / (2.0 * Eii);
+    }
+
+    {
+        double rho_ref = mt->rho[phase];
+        o.rho = rho_ref * exp(1.0 / K * Pc - alpha * T);
+        o.drhodp = o.rho / K;
+    }
+
+    double inv_eta_diss = 0.0;
+    if (peierls == 1) inv_eta_diss += (1.0 / eta_exp);
+    if (dislocation == 1) inv_eta_diss += (1.0 / eta_pwl);
+    if (diffusion == 1) inv_eta_diss += (1.0 / eta_lin);
+    if (constant == 1) inv_eta_diss += (1.0 / eta_cst);
+    if (is_pl == 0) {
+        o.detadexx = deta_ve_dExx; o.detadezz = deta_ve_dEzz; o.detadexz = deta_ve_dExz; o.detadp = deta_ve_dP;
+        o.etaVE = eta_ve;
+        o.ddivpdexx = 0.0; o.ddivpdezz = 0.0; o.ddivpdexz = 0.0; o.ddivpdp = ddivrdpc;
+    } else {
+        o.detadexx = deta_vep_dExx; o.detadezz = deta_vep_dEzz; o.detadexz = deta_vep_dExz; o.detadp = deta_vep_dP;
+        o.etaVE = eta_vep;
+        o.ddivpdexx = -dQdP * dlamdExx + gdot * dsin_dildExx;
+        o.ddivpdezz = -dQdP * dlamdEzz + gdot * dsin_dildEzz;
+        o.ddivpdexz = -dQdP * dlamdExz + gdot * dsin_dildExz;
+        o.ddivpdp = -dQdP * dlamdP + gdot * dsin_dildP + ddivrdpc;
+        inv_eta_diss += 1.0 / eta_vep;
+    }
+    eta_pwl = pow(2.0 * C_pwl, -1.0) * pow(Tii, 1.0 - n_pwl);
+    o.Exx_el = (double)elastic * (Txx - Txx0) / 2.0 / eta_el;
+    o.Ezz_el = (double)elastic * (Tzz - Tzz0) / 2.0 / eta_el;
+    o.Exz_el = (double)elastic * (Txz - Txz0) / 2.0 / eta_el;
+    double Exx_pl = gdot * dQdtxx, Ezz_pl = gdot * dQdtzz, Exz_pl = gdot * dQdtxz / 2.0;
+    double Exx_pwl = Txx / 2.0 / eta_pwl, Ezz_pwl = Tzz / 2.0 / eta_pwl, Exz_pwl = Txz / 2.0 / eta_pwl;
+    o.Exx_diss = Exx_pl + Exx_lin + Exx_pwl + Exx_exp + 0.0 + Exx_cst;
+    o.Ezz_diss = Ezz_pl + Ezz_lin + Ezz_pwl + Ezz_exp + 0.0 + Ezz_cst;
+    o.Exz_diss = Exz_pl + Exz_lin + Exz_pwl + Exz_exp + 0.0 + Exz_cst;
+    o.Eii_el = (double)elastic * fabs(Tii - Tii0) / 2.0 / eta_el;
+    o.Eii_pwl = Tii / 2.0 / eta_pwl;
+    o.div_pl = divp;
+    o.div_el = -(Pc - P0) / (K * dt);
+    o.div_r = divr;
+    double eta = 1.0 / (inv_eta_diss);
+    o.VEcoeff = eta_ve / eta_el;
+    if (elastic == 0) o.VEcoeff = 0.0;
+    if (prm.step == 0) o.etaVE = eta;
+    o.txx = Txx; o.tzz = Tzz; o.txz = Txz;
+    o.d1 = d1;
+    if (eta > prm.maxeta) eta = prm.maxeta;
+    if (eta < prm.mineta) eta = prm.mineta;
+    if (o.etaVE > prm.maxeta) o.etaVE = prm.maxeta;
+    if (o.etaVE < prm.mineta) o.etaVE = prm.mineta;
+    o.eta = eta;
+    (void)res0;
+}

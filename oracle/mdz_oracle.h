@@ -1,0 +1,76 @@
+/*
+ * mdz_oracle.h -- CPU restatement (C99 + OpenMP) of the MDOODZ 6.0
+ * Newton-iteration grid path.  TEST INFRASTRUCTURE ONLY: the checker that the
+ * HIP implementation is compared against, and the CPU baseline that bench.py
+ * times.  Nothing in the product library links or calls this.
+ *
+ * Every function restates the reference function it is named after
+ * (orc_X  <->  reference X), with file:line citations at the definition.
+ */
+#ifndef MDZ_ORACLE_H
+#define MDZ_ORACLE_H
+
+#include "../include/mdoodz_hot.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+double orc_Viscosity(int phase, double G, double T, double P, double d, double phi, double X0,
+                     double Exx, double Ezz, double Exz, double Gxx, double Gzz, double Gxz,
+                     double Txx0, double Tzz0, double Txz0, mat_prop *materials, params *model,
+                     scale *scaling, double *txxn, double *tzzn, double *txzn, double *etaVE,
+                     double *VEcoeff, double *Eii_el, double *Eii_pl, double *Eii_pwl,
+                     double *Eii_exp, double *Eii_lin, double *Eii_gbs, double *Eii_cst,
+                     double *Exx_el, double *Ezz_el, double *Exz_el, double *Exx_diss,
+                     double *Ezz_diss, double *Exz_diss, double *d1, double strain_acc, double dil,
+                     double fric, double C, double *detadexx, double *detadezz, double *detadexz,
+                     double *detadp, double P0, double *X1, double *OverS, double *ddivpdexx,
+                     double *ddivpdezz, double *ddivpdexz, double *ddivpdp, double *Pcorr,
+                     double *drhodp, double *rho, double beta, double div, double *div_el,
+                     double *div_pl, double *div_r);
+/* Local iteration counts of the last orc_Viscosity call (VE, plastic). */
+void orc_last_iterations(int *it_ve, int *it_pl);
+
+void orc_InterpCentroidsToVerticesDouble(double *cen, double *ver, grid *mesh, params *model);
+void orc_StrainRateComponents(grid *mesh, scale scaling, params *model);
+void orc_NonNewtonianViscosityGrid(grid *mesh, mat_prop *materials, params *model, Nparams Nmodel,
+                                   scale *scaling);
+void orc_RheologicalOperators(grid *mesh, params *model, scale *scaling, int Jacobian);
+void orc_EvaluateRHS(grid *mesh, params model, scale scaling, double RHO_REF);
+void orc_EvalNumberOfEquations(grid *mesh, SparseMat *Stokes);
+void orc_BuildStokesOperatorDecoupled(grid *mesh, params model, int lev, double *p_corr, double *p,
+                                      double *u, double *v, SparseMat *Stokes, SparseMat *StokesA,
+                                      SparseMat *StokesB, SparseMat *StokesC, SparseMat *StokesD,
+                                      int Assemble);
+void orc_BuildJacobianOperatorDecoupled(grid *mesh, params model, int lev, double *p_corr,
+                                        double *p, double *u, double *v, SparseMat *Stokes,
+                                        SparseMat *StokesA, SparseMat *StokesB, SparseMat *StokesC,
+                                        SparseMat *StokesD, int Assemble);
+void orc_EvaluateStokesResidualDecoupled(SparseMat *Stokes, SparseMat *StokesA, SparseMat *StokesB,
+                                         SparseMat *StokesC, SparseMat *StokesD, Nparams *Nmodel,
+                                         grid *mesh, params model, scale scaling, int quiet);
+void orc_UpdateNonLinearity(grid *mesh, mat_prop materials, params *model, Nparams *Nmodel,
+                            scale scaling);
+
+/* One Newton-iteration unit (the measured sequence), all on the CPU:
+ * StrainRateComponents -> NonNewtonianViscosityGrid -> EvaluateRHS ->
+ * RheologicalOperators(0) -> BuildStokesOperatorDecoupled(1) ->
+ * RheologicalOperators(1) -> BuildJacobianOperatorDecoupled(1) ->
+ * RheologicalOperators(0) -> EvaluateStokesResidualDecoupled.
+ * The SparseMat blocks must have b/F/d allocated (SAlloc) and are
+ * (re)assembled; their CSR arrays are malloc()ed and must be freed by the
+ * caller (orc_free_block). */
+void orc_newton_unit(grid *mesh, mat_prop *materials, params *model, Nparams *Nmodel, scale *scaling,
+                     SparseMat *Stokes, SparseMat *SA, SparseMat *SB, SparseMat *SC, SparseMat *SD,
+                     SparseMat *JA, SparseMat *JB, SparseMat *JC, SparseMat *JD);
+void orc_free_block(SparseMat *M);
+/* Allocation of b, x, F, d (d = 1), like MemoryAllocFree.c:159 SAlloc. */
+void orc_salloc(SparseMat *M, int neq);
+void orc_sfree(SparseMat *M);
+int  orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1244 @@
+/*
+ * orc_assembly.c -- oracle restatement of the decoupled Stokes assembly and
+ * residual of the MDOODZ Newton-iteration path (TEST INFRASTRUCTURE:
+ * checker and CPU baseline).
+ *
+ *   orc_EvalNumberOfEquations            StokesRoutines.c:1134-1187
+ *   add_coeff (AddCoeff2)                StokesAssemblyCoupled.c:47-57
+ *   continuity row                       StokesAssemblyDecoupled.c:62-103
+ *   Picard X rows (inner, W/E Neumann)   StokesAssemblyDecoupled.c:109-234, 476-790
+ *   Picard Z rows (inner, S/N Neumann)   StokesAssemblyDecoupled.c:796-972, 1168-1428
+ *   Jacobian X / Z rows                  StokesAssemblyDecoupled.c:3034-3421, 3427-3825
+ *   orc_BuildStokesOperatorDecoupled     StokesAssemblyDecoupled.c:1573-2145
+ *   orc_BuildJacobianOperatorDecoupled   StokesAssemblyDecoupled.c:3834-4376
+ *   CSR merge (row = equation order,     StokesAssemblyCoupled.c:1446-1492,
+ *   entries in append order)             StokesAssemblyDecoupled.c:1429-1476
+ *   orc_EvaluateStokesResidualDecoupled  StokesRoutines.c:839-916
+ *
+ * The Jacobian rows are written as the composition of the discrete
+ * operators they linearise: strain rates of the staggered velocity field,
+ * masked averages centroid<->vertex with the reference's weights
+ * (wE, wW, wS, wN and the in-domain flags), the 2x4 / 1x4 tangent blocks
+ * D1x, D2x, D3x, and the stress divergence.  Each neighbour's coefficient
+ * is the sum of its contributions through that chain.
+ *
+ * Index arithmetic and in-domain flags follow the reference.  Where the
+ * reference reads a tag outside its array (row -1 or row Nz-1 of the
+ * centroids next to a non-conforming Dirichlet wall, vz index -1 on the
+ * first periodic row), the read is undefined behaviour there; here such a
+ * tag is treated as "outside the domain" (flag 0, no remap).
+ */
+#include "mdz_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OUTSIDE 99
+
+typedef struct {
+    const grid *m;
+    int nx, nz, ncx, ncz, nxvz, nzvx;
+} geo;
+
+static inline int tU(const geo *g, int i) { return (i >= 0 && i < g->nx * g->nzvx) ? g->m->BCu.type[i] : OUTSIDE; }
+static inline int tV(const geo *g, int i) { return (i >= 0 && i < g->nxvz * g->nz) ? g->m->BCv.type[i] : OUTSIDE; }
+static inline int tP(const geo *g, int i) { return (i >= 0 && i < g->ncx * g->ncz) ? g->m->BCp.type[i] : OUTSIDE; }
+static inline int tG(const geo *g, int i) { return (i >= 0 && i < g->nx * g->nz) ? g->m->BCg.type[i] : OUTSIDE; }
+
+/* one CSR row under construction */
+typedef struct {
+    int n;
+    int J[32];
+    double A[32];
+    double bbc;
+} rowbuf;
+
+/* StokesAssemblyCoupled.c:47 AddCoeff2: Dirichlet / surface-pressure /
+ * non-conforming / matching-Neumann neighbours go to the right-hand side,
+ * every other neighbour is appended (zeros kept). */
+static inline void add_coeff(rowbuf *r, int jeq, double coeff, int type, double val) {
+    if (type == 0 || type == 31 || type == 11 || type == 13) r->bbc -= coeff * val;
+    else { r->J[r->n] = jeq; r->A[r->n] = coeff; r->n++; }
+}
+
+typedef struct {
+    const SparseMat *S;       /* equation numbers */
+    double *bA, *FA;          /* b and F of the A block (momentum) or C block (continuity) */
+    double *bB, *FD;          /* b of B (momentum) / F of D (continuity) */
+    const double *u, *v, *p;  /* iterate (p = p_corr for momentum, p_in for continuity) */
+    double celvol, one_dx, one_dz, one_dx_dx, one_dz_dz, one_dx_dz;
+    int comp, stab, assemble;
+    double om;
+    const params *model;
+} rowctx;
+
+/* ---------------------------------------------------------------- */
+/* Continuity (StokesAssemblyDecoupled.c:62)                         */
+/* ---------------------------------------------------------------- */
+static void continuity_row(const geo *g, const rowctx *x, int c1, int c2, int c3, int eqn, rowbuf *rc) {
+    const grid *m = g->m;
+    int nxvz = g->nxvz;
+    double pc = x->comp == 0 ? 0.0 : m->bet_n[c2] / x->model->dt;
+    double uW = 0, uE = 0, vN = 0, vS = 0;
+    if (m->BCu.type[c1] != 13) uW = -x->one_dx * 1e0;
+    if (m->BCu.type[c1 + 1] != 13) uE = x->one_dx * 1e0;
+    if (m->BCv.type[c3] != 13) vS = -x->one_dz * 1e0;
+    if (m->BCv.type[c3 + nxvz] != 13) vN = x->one_dz * 1e0;
+    if (x->assemble == 1) {
+        x->bA[eqn] *= x->celvol;
+        x->bB[eqn] *= x->celvol;
+        if (m->BCu.type[c1] != 13) add_coeff(rc, x->S->eqn_u[c1], uW * x->celvol, m->BCu.type[c1], m->BCu.val[c1]);
+        if (m->BCu.type[c1 + 1] != 13) add_coeff(rc, x->S->eqn_u[c1 + 1], uE * x->celvol, m->BCu.type[c1 + 1], m->BCu.val[c1 + 1]);
+        if (m->BCv.type[c3] != 13) add_coeff(rc, x->S->eqn_v[c3], vS * x->celvol, m->BCv.type[c3], m->BCv.val[c3]);
+        if (m->BCv.type[c3 + nxvz] != 13) add_coeff(rc, x->S->eqn_v[c3 + nxvz], vN * x->celvol, m->BCv.type[c3 + nxvz], m->BCv.val[c3 + nxvz]);
+    } else {
+        const double *u = x->u, *v = x->v, *p = x->p;
+        double F = pc * p[c2] + uW * u[c1] + uE * u[c1 + 1] + vS * v[c3] + vN * v[c3 + nxvz];
+        F -= x->bA[eqn];
+        F *= x->celvol;
+        x->FA[eqn] = F;
+        x->FD[eqn] = F;
+    }
+}
+
+/* ---------------------------------------------------------------- */
+/* Picard X-momentum, inner and periodic (StokesAssemblyDecoupled.c:109) */
+/* ---------------------------------------------------------------- */
+static void xmom_inner(const geo *g, const rowctx *x, int c1, int c2, int c3, int eqn, rowbuf *ra, rowbuf *rb) {
+    const grid *m = g->m;
+    int nx = g->nx, ncx = g->ncx, nxvz = g->nxvz;
+    int iVxC = c1, iVxW = iVxC - 1, iVxE = iVxC + 1, iVxS = iVxC - nx, iVxN = iVxC + nx;
+    int iVzSW = c3 - nxvz, iVzSE = c3 - nxvz + 1, iVzNW = c3, iVzNE = c3 + 1;
+    int iPrW = c2, iPrE = c2 + 1, ixyN = c1, ixyS = c1 - nx;
+    if (m->BCu.type[iVxC] == -2) { iVxW = c1 + nx - 2; iPrW = c2 + ncx; iVzSW = c3 - 2; iVzNW = c3 + nxvz - 2; }
+    double D11E = m->D11_n[iPrE], D11W = m->D11_n[iPrW], D33N = m->D33_s[ixyN], D33S = m->D33_s[ixyS];
+    double inE = 0, inW = 0, inS = 0, inN = 0;
+    if (m->BCp.type[iPrW] == -1) inW = 1.0;
+    if (m->BCp.type[iPrE] == -1) inE = 1.0;
+    if (m->BCg.type[ixyS] != 30 && m->BCu.type[ixyS] != 13) inS = 1.0;
+    if (m->BCg.type[c1] != 30 && m->BCu.type[c1 + nx] != 13) inN = 1.0;
+    double dx = m->dx, dz = m->dz, comp = 1.0;
+    /* -(d/dx (D11 Exx) + d/dz (D33 2 Exz)) with deviatoric Exx, masked */
+    double uW = D11W * inW * ((1.0 / 3.0) * comp / dx - 1.0 / dx) / dx;
+    double uC = -(-D33N * inN / dz - D33S * inS / dz) / dz
+              - (D11E * inE * ((1.0 / 3.0) * comp / dx - 1 / dx) - D11W * inW * (-1.0 / 3.0 * comp / dx + 1.0 / dx)) / dx;
+    double uE = -D11E * inE * (-1.0 / 3.0 * comp / dx + 1.0 / dx) / dx;
+    double uS = -D33S * inS / pow(dz, 2.0);
+    double uN = -D33N * inN / pow(dz, 2.0);
+    double vSW = (1.0 / 3.0) * D11W * comp * inW / (dx * dz) - D33S * inS / (dx * dz);
+    double vSE = -1.0 / 3.0 * D11E * comp * inE / (dx * dz) + D33S * inS / (dx * dz);
+    double vNW = -1.0 / 3.0 * D11W * comp * inW / (dx * dz) + D33N * inN / (dx * dz);
+    double vNE = (1.0 / 3.0) * D11E * comp * inE / (dx * dz) - D33N * inN / (dx * dz);
+    double pW = 0, pE = 0;
+    if (x->stab == 1) {
+        double drhodx = (m->rho_n[c2 + 1] - m->rho_n[c2]) * x->one_dx;
+        double uC_corr = 1.00 * x->om * x->model->dt * m->gx[c1] * drhodx;
+        if (uC + uC_corr > 0.0) uC += uC_corr;
+    }
+    if (m->BCu.type[iVxS] == 11) uC -= uS;
+    if (m->BCu.type[iVxN] == 11) uC -= uN;
+    if (m->BCp.type[iPrE] != 30 && m->BCp.type[iPrW] != 30) { pW = -x->one_dx * 1e0; pE = x->one_dx * 1e0; }
+    double cv = x->celvol;
+    if (x->assemble == 1) {
+        x->bA[eqn] *= cv;
+        x->bB[eqn] *= cv;
+        if (m->BCu.type[iVxS] != 30)
+            add_coeff(ra, x->S->eqn_u[iVxS], uS * cv, m->BCu.type[iVxS], (m->BCu.type[iVxS] != 11 ? 1.0 : 2.0) * m->BCu.val[iVxS]);
+        if (m->BCu.type[iVxW] != 30) add_coeff(ra, x->S->eqn_u[iVxW], uW * cv, m->BCu.type[iVxW], m->BCu.val[iVxW]);
+        add_coeff(ra, eqn, uC * cv, m->BCu.type[iVxC], m->BCu.val[iVxC]);
+        if (m->BCu.type[iVxE] != 30) add_coeff(ra, x->S->eqn_u[iVxE], uE * cv, m->BCu.type[iVxE], m->BCu.val[iVxE]);
+        if (m->BCu.type[c1 + nx] != 30)
+            add_coeff(ra, x->S->eqn_u[iVxN], uN * cv, m->BCu.type[iVxN], (m->BCu.type[iVxN] != 11 ? 1.0 : 2.0) * m->BCu.val[iVxN]);
+        if (m->BCv.type[iVzSW] != 30) add_coeff(ra, x->S->eqn_v[iVzSW], vSW * cv, m->BCv.type[iVzSW], m->BCv.val[iVzSW]);
+        if (m->BCv.type[iVzSE] != 30) add_coeff(ra, x->S->eqn_v[iVzSE], vSE * cv, m->BCv.type[iVzSE], m->BCv.val[iVzSE]);
+        if (m->BCv.type[iVzNW] != 30) add_coeff(ra, x->S->eqn_v[iVzNW], vNW * cv, m->BCv.type[iVzNW], m->BCv.val[iVzNW]);
+        if (m->BCv.type[iVzNE] != 30) add_coeff(ra, x->S->eqn_v[iVzNE], vNE * cv, m->BCv.type[iVzNE], m->BCv.val[iVzNE]);
+        if (m->BCp.type[iPrE] != 30 && m->BCp.type[iPrW] != 30) {
+            add_coeff(rb, x->S->eqn_p[iPrW] - x->S->neq_mom, pW * cv, m->BCp.type[iPrW], m->BCp.val[iPrW]);
+            add_coeff(rb, x->S->eqn_p[iPrE] - x->S->neq_mom, pE * cv, m->BCp.type[iPrE], m->BCp.val[iPrE]);
+        }
+    } else {
+        const double *u = x->u, *v = x->v, *p = x->p;
+        double F = uC * u[iVxC];
+        if (m->BCp.type[iPrE] != 30 && m->BCp.type[iPrW] != 30) F += pW * p[iPrW] + pE * p[iPrE];
+        if (m->BCv.type[iVzSE] != 30) F += vSE * v[iVzSE];
+        if (m->BCv.type[iVzSW] != 30) F += vSW * v[iVzSW];
+        if (m->BCv.type[iVzNE] != 30) F += vNE * v[iVzNE];
+        if (m->BCv.type[iVzNW] != 30) F += vNW * v[iVzNW];
+        if (m->BCu.type[iVxS] != 30 && m->BCu.type[iVxS] != 11) F += uS * u[iVxS];
+        if (m->BCu.type[iVxS] == 11) F += 2.0 * uS * m->BCu.val[iVxS];
+        if (m->BCu.type[iVxN] != 30 && m->BCu.type[iVxN] != 11) F += uN * u[iVxN];
+        if (m->BCu.type[iVxN] == 11) F += 2.0 * uN * m->BCu.val[iVxN];
+        if (m->BCu.type[iVxW] != 30) F += uW * u[iVxW];
+        if (m->BCu.type[iVxE] != 30) F += uE * u[iVxE];
+        F -= x->bA[eqn];
+        F *= cv;
+        x->FA[eqn] = F;
+    }
+}
+
+/* Tangential contributions shared by the W and E Neumann X rows
+ * (StokesAssemblyDecoupled.c:476-790): eta dvx/dz and eta dvz/dx terms. */
+static void xmom_neumann(const geo *g, const rowctx *x, int c1, int c2, int c3, int eqn, int east,
+                         int comp, rowbuf *ra, rowbuf *rb) {
+    const grid *m = g->m;
+    int nx = g->nx, nxvz = g->nxvz;
+    double etaE, etaW, etaN = m->eta_s[c1], etaS = m->eta_s[c1 - nx];
+    if (!east) { etaE = m->eta_n[c2 + 1]; etaW = etaE; }
+    else { etaW = m->eta_n[c2]; etaE = etaW; }
+    double uS = 0, uN = 0, uW = 0, uE = 0, uC = 0, vSW = 0, vSE = 0, vNW = 0, vNE = 0, pE = 0, pW = 0;
+    double dxx = x->one_dx_dx, dzz = x->one_dz_dz, dxz = x->one_dx_dz;
+    if (!east) {
+        x->bA[eqn] -= m->BCu.val[c1] * x->one_dx;
+        if (m->BCu.type[c1 + 1] != 30) {
+            uC = -2.0 * dxx * (etaE) + comp * 2.0 / 3.0 * dxx * etaE;
+            uE = 2.0 * dxx * etaE - comp * 2.0 / 3.0 * dxx * etaE;
+        }
+    } else {
+        x->bA[eqn] += m->BCu.val[c1] * x->one_dx;
+        if (m->BCu.type[c1 - 1] != 30) {
+            uW = 2.0 * dxx * etaW - comp * 2.0 / 3.0 * dxx * etaW;
+            uC = -2.0 * dxx * (etaW) + comp * 2.0 / 3.0 * dxx * etaW;
+        }
+    }
+    int tS = m->BCu.type[c1 - nx], tN = m->BCu.type[c1 + nx];
+    if (tS != 30 && tN != 30) {
+        if (tS != 13) { uS = dzz * etaS; uC += -dzz * etaS; }
+        if (tN != 13) { uN = dzz * etaN; uC += -dzz * etaN; }
+    }
+    if (tS == 30 && tN != 30) {
+        if (tN != 13) uC += -dzz * etaN;
+        if (tN != 13) uN = dzz * etaN;
+    }
+    if (tS != 30 && tN == 30) {
+        if (tS != 13) uC += -dzz * etaS;
+        if (tS != 13) uS = dzz * etaS;
+    }
+    if (tS == 11) uC += -dzz * etaS;
+    if (tN == 11) uC += -dzz * etaN;
+    if (m->BCv.type[c3 - nxvz] != 30 && m->BCv.type[c3 - nxvz + 1] != 30) {
+        vSE = -dxz * etaS + comp * 2.0 / 3.0 * dxz * etaE;
+        vSW = dxz * etaS - comp * 2.0 / 3.0 * dxz * etaW;
+    }
+    if (m->BCv.type[c3 + 1] != 30 && m->BCv.type[c3] != 30) {
+        vNE = dxz * etaN - comp * 2.0 / 3.0 * dxz * etaE;
+        vNW = -dxz * etaN + comp * 2.0 / 3.0 * dxz * etaW;
+    }
+    int ip = east ? c2 : c2 + 1;
+    if (!east) { if (m->BCp.type[c2 + 1] != 30) { pW = x->one_dx; pE = -pW; } }
+    else { if (m->BCp.type[c2] != 30) pW = x->one_dx; }
+    uS = -uS; uN = -uN; uW = -uW; uE = -uE; uC = -uC; vSW = -vSW; vSE = -vSE; vNW = -vNW; vNE = -vNE; pE = -pE; pW = -pW;
+    double pcoef = east ? pW : pE;
+    double cv = x->celvol;
+    if (x->assemble == 1) {
+        x->bA[eqn] *= cv;
+        x->bB[eqn] *= cv;
+        if (tS != 30) add_coeff(ra, x->S->eqn_u[c1 - nx], uS * cv, tS, (tS != 11 ? 1.0 : 2.0) * m->BCu.val[c1 - nx]);
+        if (east && m->BCu.type[c1 - 1] != 30) add_coeff(ra, x->S->eqn_u[c1 - 1], uW * cv, m->BCu.type[c1 - 1], m->BCu.val[c1 - 1]);
+        add_coeff(ra, eqn, uC * cv, m->BCu.type[c1], m->BCu.val[c1]);
+        if (!east && m->BCu.type[c1 + 1] != 30) add_coeff(ra, x->S->eqn_u[c1 + 1], uE * cv, m->BCu.type[c1 + 1], m->BCu.val[c1 + 1]);
+        if (tN != 30) add_coeff(ra, x->S->eqn_u[c1 + nx], uN * cv, tN, (tN != 11 ? 1.0 : 2.0) * m->BCu.val[c1 + nx]);
+        if (m->BCv.type[c3 - nxvz] != 30 && m->BCv.type[c3 - nxvz + 1] != 30) {
+            add_coeff(ra, x->S->eqn_v[c3 - nxvz], vSW * cv, m->BCv.type[c3 - nxvz], m->BCv.val[c3 - nxvz]);
+            add_coeff(ra, x->S->eqn_v[c3 - nxvz + 1], vSE * cv, m->BCv.type[c3 - nxvz + 1], m->BCv.val[c3 - nxvz + 1]);
+        }
+        if (m->BCv.type[c3] != 30 && m->BCv.type[c3 + 1] != 30) {
+            add_coeff(ra, x->S->eqn_v[c3], vNW * cv, m->BCv.type[c3], m->BCv.val[c3]);
+            add_coeff(ra, x->S->eqn_v[c3 + 1], vNE * cv, m->BCv.type[c3 + 1], m->BCv.val[c3 + 1]);
+        }
+        if (m->BCp.type[ip] != 30)
+            add_coeff(rb, x->S->eqn_p[ip] - x->S->neq_mom, pcoef * cv, m->BCp.type[ip], m->BCp.val[ip]);
+    } else {
+        const double *u = x->u, *v = x->v, *p = x->p;
+        double F = -(x->bA[eqn] + ra->bbc / cv);
+        F += uC * u[c1];
+        if (tS != 30 && tS != 11) F += uS * u[c1 - nx];
+        if (east && m->BCu.type[c1 - 1] != 30) F += uW * u[c1 - 1];
+        if (!east && m->BCu.type[c1 + 1] != 30) F += uE * u[c1 + 1];
+        if (tN != 30 && tN != 11) F += uN * u[c1 + nx];
+        if (m->BCv.type[c3 - nxvz] != 30 && m->BCv.type[c3 - nxvz + 1] != 30) {
+            if (m->BCv.type[c3 - nxvz + 1] != 11 && m->BCv.type[c3 - nxvz + 1] != 0) F += vSE * v[c3 - nxvz + 1];
+            if (m->BCv.type[c3 - nxvz] != 11 && m->BCv.type[c3 - nxvz] != 0) F += vSW * v[c3 - nxvz];
+        }
+        if (m->BCv.type[c3] != 30 && m->BCv.type[c3 + 1] != 30) {
+            if (m->BCv.type[c3 + 1] != 11 && m->BCv.type[c3 + 1] != 0) F += vNE * v[c3 + 1];
+            if (m->BCv.type[c3] != 11 && m->BCv.type[c3] != 0) F += vNW * v[c3];
+        }
+        if (m->BCp.type[ip] != 30) F += pcoef * p[ip];
+        F *= cv;
+        x->FA[eqn] = F;
+    }
+}
+
+/* ---------------------------------------------------------------- */
+/* Picard Z-momentum inner (StokesAssemblyDecoupled.c:796)            */
+/* ---------------------------------------------------------------- */
+static void zmom_inner(const geo *g, const rowctx *x, int c1, int c2, int c3, int eqn, rowbuf *ra, rowbuf *rb) {
+    const grid *m = g->m;
+    int nx = g->nx, ncx = g->ncx, nxvz = g->nxvz;
+    double D22S = m->D22_n[c2], D22N = m->D22_n[c2 + ncx], D33E = m->D33_s[c1], D33W = m->D33_s[c1 - 1];
+    if (m->BCp.type[c2 + ncx] == 31) x->bA[eqn] += m->BCv.val[c3] * x->one_dz;
+    double inS = 0, inN = 0;
+    int tVS = m->BCv.type[c3 - nxvz], tVN = m->BCv.type[c3 + nxvz];
+    if (tVS != 30 && tVN != 30) { inS = 1.0; inN = 1.0; }
+    if (tVS == 30 && tVN != 30) inN = 1.0;
+    if (tVS != 30 && tVN == 30) inS = 1.0;
+    if (m->BCp.type[c2] == -1) inS = 1.0;
+    if (m->BCp.type[c2 + ncx] == -1) inN = 1.0;
+    double inW = 0, inE = 0;
+    if (m->BCg.type[c1 - 1] != 30 && m->BCv.type[c3 - 1] != 13) inW = 1.0;
+    if (m->BCg.type[c1] != 30 && m->BCv.type[c3 + 1] != 13) inE = 1.0;
+    double dx = m->dx, dz = m->dz, comp = 1.0;
+    double vW = -D33W * inW / pow(dx, 2);
+    double vC = -(D22N * inN * ((1.0 / 3.0) * comp / dz - 1 / dz) - D22S * inS * (-1.0 / 3.0 * comp / dz + 1.0 / dz)) / dz
+              - (-D33E * inE / dx - D33W * inW / dx) / dx;
+    double vE = -D33E * inE / pow(dx, 2);
+    double vS = D22S * inS * ((1.0 / 3.0) * comp / dz - 1 / dz) / dz;
+    double vN = -D22N * inN * (-1.0 / 3.0 * comp / dz + 1.0 / dz) / dz;
+    double uSW = (1.0 / 3.0) * D22S * comp * inS / (dx * dz) - D33W * inW / (dx * dz);
+    double uSE = -1.0 / 3.0 * D22S * comp * inS / (dx * dz) + D33E * inE / (dx * dz);
+    double uNW = -1.0 / 3.0 * D22N * comp * inN / (dx * dz) + D33W * inW / (dx * dz);
+    double uNE = (1.0 / 3.0) * D22N * comp * inN / (dx * dz) - D33E * inE / (dx * dz);
+    int tW = m->BCv.type[c3 - 1], tE = m->BCv.type[c3 + 1];
+    if (tW == 11) vC -= vW;
+    if (tE == 11) vC -= vE;
+    if (x->stab == 1) {
+        double drhodz = (m->rho_n[c2 + ncx] - m->rho_n[c2]) * x->one_dz;
+        double vC_corr = 1.00 * x->om * x->model->dt * m->gz[c3] * drhodz;
+        if (vC + vC_corr > 0.0) vC += vC_corr;
+    }
+    double pS = 0, pN = 0;
+    if (m->BCp.type[c2] != 30 && m->BCp.type[c2 + ncx] != 30) { pS = -x->one_dz * 1e0; pN = x->one_dz * 1e0; }
+    double cv = x->celvol;
+    if (x->assemble == 1) {
+        x->bA[eqn] *= cv;
+        x->bB[eqn] *= cv;
+        if (m->BCu.type[c1 + nx] != 30) add_coeff(ra, x->S->eqn_u[c1 + nx], uNE * cv, m->BCu.type[c1 + nx], m->BCu.val[c1 + nx]);
+        if (m->BCu.type[c1] != 30) add_coeff(ra, x->S->eqn_u[c1], uSE * cv, m->BCu.type[c1], m->BCu.val[c1]);
+        if (m->BCu.type[c1 + nx - 1] != 30) add_coeff(ra, x->S->eqn_u[c1 + nx - 1], uNW * cv, m->BCu.type[c1 + nx - 1], m->BCu.val[c1 + nx - 1]);
+        if (m->BCu.type[c1 - 1] != 30) add_coeff(ra, x->S->eqn_u[c1 - 1], uSW * cv, m->BCu.type[c1 - 1], m->BCu.val[c1 - 1]);
+        if (tVS != 30) add_coeff(ra, x->S->eqn_v[c3 - nxvz], vS * cv, tVS, m->BCv.val[c3 - nxvz]);
+        if (tW != 30 && tW != -12) add_coeff(ra, x->S->eqn_v[c3 - 1], vW * cv, tW, (tW != 11 ? 1.0 : 2.0) * m->BCv.val[c3 - 1]);
+        if (tW == -12) add_coeff(ra, x->S->eqn_v[c3 + nxvz - 3], vW * cv, m->BCv.type[c3 + nxvz - 3], m->BCv.val[c3 + nxvz - 3]);
+        add_coeff(ra, eqn, vC * cv, m->BCv.type[c3], m->BCv.val[c3]);
+        if (tE != 30 && tE != -12) add_coeff(ra, x->S->eqn_v[c3 + 1], vE * cv, tE, (tE != 11 ? 1.0 : 2.0) * m->BCv.val[c3 + 1]);
+        if (tE == -12) add_coeff(ra, x->S->eqn_v[c3 - nxvz + 3], vE * cv, m->BCv.type[c3 - nxvz + 3], m->BCv.val[c3 - nxvz + 3]);
+        if (tVN != 30) add_coeff(ra, x->S->eqn_v[c3 + nxvz], vN * cv, tVN, m->BCv.val[c3 + nxvz]);
+        if (m->BCp.type[c2] != 30 && m->BCp.type[c2 + ncx] != 30) {
+            add_coeff(rb, x->S->eqn_p[c2] - x->S->neq_mom, pS * cv, m->BCp.type[c2], m->BCp.val[c2]);
+            add_coeff(rb, x->S->eqn_p[c2 + ncx] - x->S->neq_mom, pN * cv, m->BCp.type[c2 + ncx], m->BCp.val[c2 + ncx]);
+        }
+    } else {
+        const double *u = x->u, *v = x->v, *p = x->p;
+        double F = vC * v[c3];
+        if (m->BCp.type[c2] != 30 && m->BCp.type[c2 + ncx] != 30) F += pS * p[c2] + pN * p[c2 + ncx];
+        if (tVS != 30) F += vS * v[c3 - nxvz];
+        if (tW != 30 && tW != 11 && tW != -12) F += vW * v[c3 - 1];
+        if (tE != 30 && tE != 11 && tE != -12) F += vE * v[c3 + 1];
+        if (tVN != 30) F += vN * v[c3 + nxvz];
+        if (m->BCu.type[c1 - 1] != 30) F += uSW * u[c1 - 1];
+        if (m->BCu.type[c1] != 30) F += uSE * u[c1];
+        if (m->BCu.type[c1 + nx - 1] != 30) F += uNW * u[c1 + nx - 1];
+        if (m->BCu.type[c1 + nx] != 30) F += uNE * u[c1 + nx];
+        if (tW == 11) F += -2 * D33W * x->one_dx_dx * m->BCv.val[c3 - 1];
+        if (tE == 11) F += -2 * D33E * x->one_dx_dx * m->BCv.val[c3 + 1];
+        if (tW == -12) F += vW * v[c3 + nxvz - 3];
+        if (tE == -12) F += vE * v[c3 - nxvz + 3];
+        F -= x->bA[eqn];
+        F *= cv;
+        x->FA[eqn] = F;
+    }
+}
+
+/* Z-momentum S/N Neumann rows (StokesAssemblyDecoupled.c:1168-1428) */
+static void zmom_neumann(const geo *g, const rowctx *x, int c1, int c2, int c3, int eqn, int north,
+                         int comp, rowbuf *ra, rowbuf *rb) {
+    const grid *m = g->m;
+    int nx = g->nx, ncx = g->ncx, nxvz = g->nxvz;
+    double etaS, etaN, etaE = m->eta_s[c1], etaW = m->eta_s[c1 - 1];
+    if (!north) { etaN = m->eta_n[c2 + ncx]; etaS = etaN; }
+    else { etaS = m->eta_n[c2]; etaN = etaS; }
+    double uSW = 0, uSE = 0, uNW = 0, uNE = 0, vS = 0, vW = 0, vC = 0, vE = 0, vN = 0, pN = 0, pS = 0;
+    double dxx = x->one_dx_dx, dzz = x->one_dz_dz, dxz = x->one_dx_dz;
+    if (!north) x->bA[eqn] -= m->BCv.val[c3] * x->one_dz;
+    else x->bA[eqn] += m->BCv.val[c3] * x->one_dz;
+    if (m->BCu.type[c1 - 1] != 30 && m->BCu.type[c1] != 30) {
+        uSW = dxz * etaW - comp * 2.0 / 3.0 * etaS * dxz;
+        uSE = -dxz * etaE + comp * 2.0 / 3.0 * etaS * dxz;
+    }
+    if (m->BCu.type[c1 + nx - 1] != 30 && m->BCu.type[c1 + nx] != 30) {
+        uNW = -dxz * etaW + comp * 2.0 / 3.0 * etaN * dxz;
+        uNE = dxz * etaE - comp * 2.0 / 3.0 * etaN * dxz;
+    }
+    if (!north) {
+        if (m->BCv.type[c3 + nxvz] != 30) {
+            vC = -2.0 * dzz * (etaN) + comp * 2.0 / 3.0 * etaS * dzz;
+            vN = 2.0 * dzz * etaN - comp * 2.0 / 3.0 * etaN * dzz;
+        }
+    } else {
+        if (m->BCv.type[c3 - nxvz] != 30) {
+            vS = 2.0 * dzz * etaS - comp * 2.0 / 3.0 * etaS * dzz;
+            vC = -2.0 * dzz * (etaS) + comp * 2.0 / 3.0 * etaS * dzz;
+        }
+    }
+    int tW = m->BCv.type[c3 - 1], tE = m->BCv.type[c3 + 1];
+    if (tW != 30 && tE != 30) {
+        if (tW != 13) { vW = dxx * etaW; vC += -dxx * etaW; }
+        if (tE != 13) { vE = dxx * etaE; vC += -dxx * etaE; }
+    }
+    if (tW == 30 && tE != 30) {
+        if (tE != 13) vE = dxx * etaE;
+        if (tE != 13) vC += -dxx * etaE;
+    }
+    if (tW != 30 && tE == 30) {
+        if (tW != 13) vW = dxx * etaW;
+        if (tW != 13) vC += -dxx * etaW;
+    }
+    if (tW == 11) vC += -dxx * etaW;
+    if (tE == 11) vC += -dxx * etaE;
+    if (x->stab == 1) {
+        double drhodz = (m->rho_n[c2 + ncx] - m->rho_n[c2]) * x->one_dz;
+        if (!north) vC += 1.00 * x->om * x->model->dt * x->model->gz * drhodz;
+        else vC += -1.00 * x->om * x->model->dt * x->model->gz * drhodz;
+    }
+    if (!north) { if (m->BCp.type[c2 + ncx] != 30) pN = -x->one_dz; }
+    else { if (m->BCp.type[c2] != 30) pS = x->one_dz; }
+    uSW = -uSW; uSE = -uSE; uNW = -uNW; uNE = -uNE; vS = -vS; vW = -vW; vC = -vC; vE = -vE; vN = -vN; pN = -pN; pS = -pS;
+    double cv = x->celvol;
+    if (x->assemble == 1) {
+        x->bA[eqn] *= cv;
+        x->bB[eqn] *= cv;
+        if (m->BCu.type[c1 - 1] != 30 && m->BCu.type[c1] != 30) {
+            add_coeff(ra, x->S->eqn_u[c1 - 1], uSW * cv, m->BCu.type[c1 - 1], m->BCu.val[c1 - 1]);
+            add_coeff(ra, x->S->eqn_u[c1], uSE * cv, m->BCu.type[c1], m->BCu.val[c1]);
+        }
+        if (m->BCu.type[c1 + nx - 1] != 30 && m->BCu.type[c1 + nx] != 30) {
+            add_coeff(ra, x->S->eqn_u[c1 + nx - 1], uNW * cv, m->BCu.type[c1 + nx - 1], m->BCu.val[c1 + nx - 1]);
+            add_coeff(ra, x->S->eqn_u[c1 + nx], uNE * cv, m->BCu.type[c1 + nx], m->BCu.val[c1 + nx]);
+        }
+        if (north && m->BCv.type[c3 - nxvz] != 30) add_coeff(ra, x->S->eqn_v[c3 - nxvz], vS * cv, m->BCv.type[c3 - nxvz], m->BCv.val[c3 - nxvz]);
+        if (tW != 30) add_coeff(ra, x->S->eqn_v[c3 - 1], vW * cv, tW, (tW != 11 ? 1.0 : 2.0) * m->BCv.val[c3 - 1]);
+        add_coeff(ra, eqn, vC * cv, m->BCv.type[c3], m->BCv.val[c3]);
+        if (tE != 30) add_coeff(ra, x->S->eqn_v[c3 + 1], vE * cv, tE, (tE != 11 ? 1.0 : 2.0) * m->BCv.val[c3 + 1]);
+        if (!north && m->BCv.type[c3 + nxvz] != 30) add_coeff(ra, x->S->eqn_v[c3 + nxvz], vN * cv, m->BCv.type[c3 + nxvz], m->BCv.val[c3 + nxvz]);
+        if (!north) {
+            if (m->BCp.type[c2 + ncx] != 30) add_coeff(rb, x->S->eqn_p[c2 + ncx] - x->S->neq_mom, pN * cv, m->BCp.type[c2 + ncx], m->BCp.val[c2 + ncx]);
+        } else {
+            if (m->BCp.type[c2] != 30) add_coeff(rb, x->S->eqn_p[c2] - x->S->neq_mom, pS * cv, m->BCp.type[c2], m->BCp.val[c2]);
+        }
+    } else {
+        const double *u = x->u, *v = x->v, *p = x->p;
+        double F = -(x->bA[eqn] + ra->bbc / cv);
+        F += vC * v[c3];
+        if (tW != 30 && tW != 11) F += vW * v[c3 - 1];
+        if (!north && m->BCv.type[c3 + nxvz] != 30) F += vN * v[c3 + nxvz];
+        if (north && m->BCv.type[c3 - nxvz] != 30) F += vS * v[c3 - nxvz];
+        if (tE != 30 && tE != 11) F += vE * v[c3 + 1];
+        if (m->BCu.type[c1 - 1] != 30 && m->BCu.type[c1] != 30) {
+            if (m->BCu.type[c1 - 1] != 11 && m->BCu.type[c1 - 1] != 0) F += uSW * u[c1 - 1];
+            if (m->BCu.type[c1] != 11 && m->BCu.type[c1] != 0) F += uSE * u[c1];
+        }
+        if (m->BCu.type[c1 + nx - 1] != 30 && m->BCu.type[c1 + nx] != 30) {
+            if (m->BCu.type[c1 + nx - 1] != 11 && m->BCu.type[c1 + nx - 1] != 0) F += uNW * u[c1 + nx - 1];
+            if (m->BCu.type[c1 + nx] != 11 && m->BCu.type[c1 + nx] != 0) F += uNE * u[c1 + nx];
+        }
+        if (!north) { if (m->BCp.type[c2 + ncx] != 30) F += pN * p[c2 + ncx]; }
+        else {
+            /* StokesAssemblyDecoupled.c:1422 reads the pressure one row
+             * below the cell it assembles against (c2 - ncx); kept. */
+            if (m->BCp.type[c2 - ncx] != 30) F += pS * p[c2 - ncx];
+        }
+        F *= cv;
+        x->FA[eqn] = F;
+    }
+}
+
+/* ---------------------------------------------------------------- */
+/* Jacobian rows as an operator composition                          */
+/* ---------------------------------------------------------------- */
+
+/* Coefficient accumulator of one X-momentum Jacobian row.
+ * u slots (di, dj) in [-1,1]^2 around the row's Vx node;
+ * v slots (a, b), a in [-1,2], b in [-2,1], around vz(k, l) (the NW one);
+ * p slots (a, b), a in {0,1}, b in [-1,1], for centroid (k-1+a, l-1+b). */
+typedef struct { double u[3][3]; double v[4][4]; double p[3][2]; } xacc;
+
+/* centroid (k-1+a, l-1+b): u nodes (a-1, b) W and (a, b) E; v nodes (a, b-1) S and (a, b) N */
+static inline void x_exx(xacc *c, int a, int b, double w, double cmp, double dx, double dz) {
+    double f = w * (1.0 - cmp / 3.0) / dx, h = w * cmp / 3.0 / dz;
+    c->u[b + 1][a + 1] += f; c->u[b + 1][a] -= f;
+    c->v[b + 2][a + 1] -= h; c->v[b + 1][a + 1] += h;
+}
+static inline void x_ezz(xacc *c, int a, int b, double w, double cmp, double dx, double dz) {
+    double f = w * (1.0 - cmp / 3.0) / dz, h = w * cmp / 3.0 / dx;
+    c->v[b + 2][a + 1] += f; c->v[b + 1][a + 1] -= f;
+    c->u[b + 1][a + 1] -= h; c->u[b + 1][a] += h;
+}
+/* vertex (k+a, l+b): u (a, b+1) - u (a, b) over dz; v (a+1, b) - v (a, b) over dx */
+static inline void x_gxz(xacc *c, int a, int b, double w, double dx, double dz) {
+    c->u[b + 2][a + 1] += w / dz; c->u[b + 1][a + 1] -= w / dz;
+    c->v[b + 2][a + 2] += w / dx; c->v[b + 2][a + 1] -= w / dx;
+}
+static inline void x_p(xacc *c, int a, int b, double w) { c->p[b + 1][a] += w; }
+
+static void xjac_row(const geo *g, const rowctx *x, int c1, int c2, int c3, int eqn, int k, int l,
+                     rowbuf *ra, rowbuf *rb) {
+    const grid *m = g->m;
+    int nx = g->nx, ncx = g->ncx, nxvz = g->nxvz;
+    int iVxC = c1, iVxW = iVxC - 1, iVxE = iVxC + 1, iVxS = iVxC - nx, iVxN = iVxC + nx;
+    int iVxSW = iVxS - 1, iVxSE = iVxS + 1, iVxNW = iVxN - 1, iVxNE = iVxN + 1;
+    int iVzSW = c3 - nxvz, iVzSE = c3 - nxvz + 1, iVzNW = c3, iVzNE = c3 + 1;
+    int iVzSWW = iVzSW - 1, iVzSEE = iVzSE + 1, iVzNWW = iVzNW - 1, iVzNEE = iVzNE + 1;
+    int iVzNNW = iVzNW + nxvz, iVzSSW = iVzSW - nxvz, iVzNNE = iVzNE + nxvz, iVzSSE = iVzSE - nxvz;
+    int iPrW = c2, iPrE = c2 + 1, ixyN = c1, ixyS = c1 - nx;
+    int ixySW = ixyS - 1, ixySE = ixyS + 1, ixyNW = ixyN - 1, ixyNE = ixyN + 1;
+    int iPrSW = iPrW - ncx, iPrSE = iPrE - ncx, iPrNW = iPrW + ncx, iPrNE = iPrE + ncx;
+    if (tV(g, iVzSWW) == -12) iVzSWW = iVzSW + (nx - 2);
+    if (tV(g, iVzNWW) == -12) iVzNWW = iVzNW + (nx - 2);
+    if (tV(g, iVzSEE) == -12) iVzSEE = iVzSE - (nx - 2);
+    if (tV(g, iVzNEE) == -12) iVzNEE = iVzNE - (nx - 2);
+    if (m->BCu.type[iVxC] == -2) {
+        iVxW = c1 + nx - 2; iPrW = c2 + ncx; iVzSW = c3 - 2; iVzNW = c3 + nxvz - 2;
+        iPrNW = iPrW + ncx; iPrSW = iPrW - ncx;
+        iVxSW = iVxW - nx; iVxNW = iVxW + nx;
+        iVzNNW = iVzNW + nxvz; iVzSSW = iVzSW - nxvz;
+        iVzSWW = iVzSW - 1; iVzNWW = iVzNW - 1;
+    }
+    double D11E = m->D11_n[iPrE], D12E = m->D12_n[iPrE], D13E = m->D13_n[iPrE], D14E = m->D14_n[iPrE];
+    double D11W = m->D11_n[iPrW], D12W = m->D12_n[iPrW], D13W = m->D13_n[iPrW], D14W = m->D14_n[iPrW];
+    double D31N = m->D31_s[ixyN], D32N = m->D32_s[ixyN], D33N = m->D33_s[ixyN], D34N = m->D34_s[ixyN];
+    double D31S = m->D31_s[ixyS], D32S = m->D32_s[ixyS], D33S = m->D33_s[ixyS], D34S = m->D34_s[ixyS];
+    double inE = 0, inW = 0, inS = 0, inN = 0;
+    if (tP(g, iPrW) == -1) inW = 1.0;
+    if (tP(g, iPrE) == -1) inE = 1.0;
+    if (tG(g, ixyS) != 30 && tU(g, iVxS) != 13) inS = 1.0;
+    if (tG(g, ixyN) != 30 && tU(g, iVxN) != 13) inN = 1.0;
+    double inSWc = 0, inSEc = 0, inNWc = 0, inNEc = 0, inSWv = 0, inSEv = 0, inNWv = 0, inNEv = 0;
+    int nzvx = x->model->Nz + 1, nz = x->model->Nz;
+    if (l > 1 || (l == 1 && tU(g, iVxS) == 11)) {
+        if (tP(g, iPrSW) == -1) inSWc = 1.0;
+        if (tP(g, iPrSE) == -1) inSEc = 1.0;
+    }
+    if (l < nzvx - 2 || (l == nzvx - 2 && tU(g, iVxN) == 11)) {
+        if (tP(g, iPrNW) == -1) inNWc = 1.0;
+        if (tP(g, iPrNE) == -1) inNEc = 1.0;
+    }
+    if ((k > 0) || (k == 0 && (tV(g, iVzSW) == -1 || tV(g, iVzSW) == 0))) {
+        if (tG(g, ixySW) != 30) inSWv = 1.0;
+        if (tG(g, ixyNW) != 30) inNWv = 1.0;
+    }
+    if ((k < nx - 1) || (k == nx - 1 && (tV(g, iVzSE) == -1 || tV(g, iVzSE) == 0))) {
+        if (tG(g, ixySE) != 30) inSEv = 1.0;
+        if (tG(g, ixyNE) != 30) inNEv = 1.0;
+    }
+    double wE = inN + inS + inNEv + inSEv, wW = inN + inS + inNWv + inSWv;
+    double wS = inW + inE + inSWc + inSEc, wN = inW + inE + inNWc + inNEc;
+    if (wW > 1.0) wW = 1.0 / wW;
+    if (wE > 1.0) wE = 1.0 / wE;
+    if (wS > 1.0) wS = 1.0 / wS;
+    if (wN > 1.0) wN = 1.0 / wN;
+    double dx = m->dx, dz = m->dz, cmp = x->comp;
+
+    xacc c;
+    memset(&c, 0, sizeof(c));
+    /* normal stress at centroids W (a=0) and E (a=1): +/- in/dx * Sxx */
+    for (int a = 0; a < 2; a++) {
+        double f = a == 0 ? inW / dx : -inE / dx;
+        double D1 = a == 0 ? D11W : D11E, D2 = a == 0 ? D12W : D12E, D3 = a == 0 ? D13W : D13E, D4 = a == 0 ? D14W : D14E;
+        double w = a == 0 ? wW : wE;
+        x_exx(&c, a, 0, f * D1, cmp, dx, dz);
+        x_ezz(&c, a, 0, f * D2, cmp, dx, dz);
+        /* centroid shear rate: masked mean of its four vertices */
+        double fS = inS, fN = inN, fSo = a == 0 ? inSWv : inSEv, fNo = a == 0 ? inNWv : inNEv;
+        x_gxz(&c, 0, -1, f * D3 * w * fS, dx, dz);
+        x_gxz(&c, 0, 0, f * D3 * w * fN, dx, dz);
+        x_gxz(&c, a == 0 ? -1 : 1, -1, f * D3 * w * fSo, dx, dz);
+        x_gxz(&c, a == 0 ? -1 : 1, 0, f * D3 * w * fNo, dx, dz);
+        x_p(&c, a, 0, f * D4);
+    }
+    /* shear stress at vertices S (b=-1) and N (b=0): +/- in/dz * Sxz */
+    for (int bb = 0; bb < 2; bb++) {
+        int b = bb == 0 ? -1 : 0;
+        double f = bb == 0 ? inS / dz : -inN / dz;
+        double D1 = bb == 0 ? D31S : D31N, D2 = bb == 0 ? D32S : D32N, D3 = bb == 0 ? D33S : D33N, D4 = bb == 0 ? D34S : D34N;
+        double w = bb == 0 ? wS : wN;
+        /* vertex normal rates and pressure: masked mean of the four centroids */
+        int cb = bb == 0 ? -1 : 1; /* row of the outer centroids */
+        double fl[4] = {inW, inE, bb == 0 ? inSWc : inNWc, bb == 0 ? inSEc : inNEc};
+        int ca[4] = {0, 1, 0, 1}, cr[4] = {0, 0, cb, cb};
+        for (int q = 0; q < 4; q++) {
+            double wq = w * fl[q];
+            x_exx(&c, ca[q], cr[q], f * D1 * wq, cmp, dx, dz);
+            x_ezz(&c, ca[q], cr[q], f * D2 * wq, cmp, dx, dz);
+            x_p(&c, ca[q], cr[q], f * D4 * wq);
+        }
+        x_gxz(&c, 0, b, f * D3, dx, dz);
+    }
+    /* pressure gradient */
+    x_p(&c, 0, 0, -inW / dx);
+    x_p(&c, 1, 0, inE / dx);
+
+    double uSW = c.u[0][0], uS = c.u[0][1], uSE = c.u[0][2], uW = c.u[1][0], uC = c.u[1][1], uE = c.u[1][2];
+    double uNW = c.u[2][0], uN = c.u[2][1], uNE = c.u[2][2];
+    double vSSW = c.v[0][1], vSSE = c.v[0][2], vSWW = c.v[1][0], vSW = c.v[1][1], vSE = c.v[1][2], vSEE = c.v[1][3];
+    double vNWW = c.v[2][0], vNW = c.v[2][1], vNE = c.v[2][2], vNEE = c.v[2][3], vNNW = c.v[3][1], vNNE = c.v[3][2];
+    double pSW = c.p[0][0], pSE = c.p[0][1], pW = c.p[1][0], pE = c.p[1][1], pNW = c.p[2][0], pNE = c.p[2][1];
+
+    if (x->stab == 1) {
+        double drhodx = (m->rho_n[c2 + 1] - m->rho_n[c2]) * x->one_dx;
+        double uC_corr = 1.00 * x->om * x->model->dt * m->gx[c1] * drhodx;
+        if (uC + uC_corr > 0.0) uC += uC_corr;
+    }
+    /* non-conforming Dirichlet neighbours (ghost = 2 val - inner) */
+    if (tU(g, iVxS) == 11) uC -= uS;
+    if (tU(g, iVxN) == 11) uC -= uN;
+    if (tU(g, iVxSW) == 11) uW -= uSW;
+    if (tU(g, iVxSE) == 11) uE -= uSE;
+    if (tU(g, iVxNW) == 11) uW -= uNW;
+    if (tU(g, iVxNE) == 11) uE -= uNE;
+    if (tV(g, iVzNWW) == 11) vNW -= vNWW;
+    if (tV(g, iVzNEE) == 11) vNE -= vNEE;
+    if (tV(g, iVzSWW) == 11) vSW -= vSWW;
+    if (tV(g, iVzSEE) == 11) vSE -= vSEE;
+
+    double cv = x->celvol;
+    if (x->assemble == 1) {
+        const SparseMat *S = x->S;
+        x->bA[eqn] *= cv;
+        x->bB[eqn] *= cv;
+#define UADD11(idx, coef)                                                                                     \
+    if (m->BCu.type[idx] != 30)                                                                               \
+        add_coeff(ra, S->eqn_u[idx], (coef) * cv, m->BCu.type[idx], (m->BCu.type[idx] != 11 ? 1.0 : 2.0) * m->BCu.val[idx]);
+        UADD11(iVxSW, uSW)
+        UADD11(iVxS, uS)
+        UADD11(iVxSE, uSE)
+        if (m->BCu.type[iVxW] != 30) add_coeff(ra, S->eqn_u[iVxW], uW * cv, m->BCu.type[iVxW], m->BCu.val[iVxW]);
+        add_coeff(ra, eqn, uC * cv, m->BCu.type[iVxC], m->BCu.val[iVxC]);
+        if (m->BCu.type[iVxE] != 30) add_coeff(ra, S->eqn_u[iVxE], uE * cv, m->BCu.type[iVxE], m->BCu.val[iVxE]);
+        UADD11(iVxNW, uNW)
+        UADD11(iVxN, uN)
+        UADD11(iVxNE, uNE)
+#undef UADD11
+#define VADD(idx, coef) add_coeff(ra, S->eqn_v[idx], (coef) * cv, m->BCv.type[idx], m->BCv.val[idx])
+        if (l > 1 && m->BCv.type[iVzSSW] != 30) VADD(iVzSSW, vSSW);
+        if (l > 1 && m->BCv.type[iVzSSE] != 30) VADD(iVzSSE, vSSE);
+        if ((k > 1) || ((k == 0 || k == 1) && m->BCv.type[iVzSWW] == -1))
+            if (m->BCv.type[iVzSWW] != 30) VADD(iVzSWW, vSWW);
+        if (m->BCv.type[iVzSW] != 30) VADD(iVzSW, vSW);
+        if (m->BCv.type[iVzSE] != 30) VADD(iVzSE, vSE);
+        if ((k < nx - 2) || ((k == nx - 2 || k == nx - 1) && m->BCv.type[iVzSEE] == -1))
+            if (m->BCv.type[iVzSEE] != 30) VADD(iVzSEE, vSEE);
+        if ((k > 1) || ((k == 0 || k == 1) && m->BCv.type[iVzNWW] == -1))
+            if (m->BCv.type[iVzNWW] != 30) VADD(iVzNWW, vNWW);
+        if (m->BCv.type[iVzNW] != 30) VADD(iVzNW, vNW);
+        if (m->BCv.type[iVzNE] != 30) VADD(iVzNE, vNE);
+        if ((k < nx - 2) || ((k == nx - 2 || k == nx - 1) && m->BCv.type[iVzNEE] == -1))
+            if (m->BCv.type[iVzNEE] != 30) VADD(iVzNEE, vNEE);
+        if (l < nz - 1 && m->BCv.type[iVzNNW] != 30) VADD(iVzNNW, vNNW);
+        if (l < nz - 1 && m->BCv.type[iVzNNE] != 30) VADD(iVzNNE, vNNE);
+#undef VADD
+#define PADD(idx, coef) add_coeff(rb, S->eqn_p[idx] - S->neq_mom, (coef) * cv, m->BCp.type[idx], m->BCp.val[idx])
+        if (l > 1) {
+            if (m->BCp.type[iPrSW] != 30) PADD(iPrSW, pSW);
+            if (m->BCp.type[iPrSE] != 30) PADD(iPrSE, pSE);
+        }
+        if (m->BCp.type[iPrE] != 30 && m->BCp.type[c2] != 30) { PADD(iPrW, pW); PADD(iPrE, pE); }
+        if (l < nz - 1) {
+            if (m->BCp.type[iPrNW] != 30) PADD(iPrNW, pNW);
+            if (m->BCp.type[iPrNE] != 30) PADD(iPrNE, pNE);
+        }
+#undef PADD
+    } else {
+        /* stress-divergence residual (used with anisotropy) */
+        double F = 0.0;
+        F += (m->sxxd[iPrE] - m->sxxd[iPrW]) / dx;
+        F -= (m->p_in[iPrE] - m->p_in[iPrW]) / dx;
+        F += (m->sxz[ixyN] - m->sxz[ixyS]) / dz;
+        F *= -1.0;
+        F -= x->bA[eqn];
+        F *= cv;
+        x->FA[eqn] = F;
+    }
+    (void)D14E;
+}
+
+/* Coefficient accumulator of one Z-momentum Jacobian row.
+ * v slots (di, dj) in [-1,1]^2 around the row's Vz node;
+ * u slots (a, b), a in [-2,1], b in [-1,2], around u(k, l) (the SE one);
+ * p slots (a, b), a in [-1,1], b in {0,1}, for centroid (k-1+a, l-1+b). */
+typedef struct { double v[3][3]; double u[4][4]; double p[2][3]; } zacc;
+
+/* centroid (k-1+a, l-1+b): u (a-1, b) W, (a, b) E; v (a, b-1) S, (a, b) N */
+static inline void z_exx(zacc *c, int a, int b, double w, double cmp, double dx, double dz) {
+    double f = w * (1.0 - cmp / 3.0) / dx, h = w * cmp / 3.0 / dz;
+    c->u[b + 1][a + 2] += f; c->u[b + 1][a + 1] -= f;
+    c->v[b + 1][a + 1] -= h; c->v[b][a + 1] += h;
+}
+static inline void z_ezz(zacc *c, int a, int b, double w, double cmp, double dx, double dz) {
+    double f = w * (1.0 - cmp / 3.0) / dz, h = w * cmp / 3.0 / dx;
+    c->v[b + 1][a + 1] += f; c->v[b][a + 1] -= f;
+    c->u[b + 1][a + 2] -= h; c->u[b + 1][a + 1] += h;
+}
+/* vertex (k-1+a, l+b): u (a-1, b+1) - u (a-1, b) over dz; v (a, b) - v (a-1, b) over dx */
+static inline void z_gxz(zacc *c, int a, int b, double w, double dx, double dz) {
+    c->u[b + 2][a + 1] += w / dz; c->u[b + 1][a + 1] -= w / dz;
+    c->v[b + 1][a + 1] += w / dx; c->v[b + 1][a] -= w / dx;
+}
+static inline void z_p(zacc *c, int a, int b, double w) { c->p[b][a + 1] += w; }
+
+static void zjac_row(const geo *g, const rowctx *x, int c1, int c2, int c3, int eqn, int k, int l,
+                     rowbuf *ra, rowbuf *rb) {
+    const grid *m = g->m;
+    int nx = g->nx, ncx = g->ncx, nxvz = g->nxvz;
+    int periodix = x->model->isperiodic_x;
+    int iVzC = c3, iVzW = iVzC - 1, iVzE = iVzC + 1, iVzS = c3 - nxvz, iVzN = c3 + nxvz;
+    int iVzSW = iVzS - 1, iVzSE = iVzS + 1, iVzNW = iVzN - 1, iVzNE = iVzN + 1;
+    int iVxNW = c1 + nx - 1, iVxNE = c1 + nx, iVxSW = c1 - 1, iVxSE = c1;
+    int iVxNWW = iVxNW - 1, iVxNEE = iVxNE + 1, iVxSWW = iVxSW - 1, iVxSEE = iVxSE + 1;
+    int iVxNNW = iVxNW + nx, iVxNNE = iVxNE + nx, iVxSSW = iVxSW - nx, iVxSSE = iVxSE - nx;
+    int iPrS = c2, iPrSW = iPrS - 1, iPrSE = iPrS + 1, iPrN = c2 + ncx, iPrNW = iPrN - 1, iPrNE = iPrN + 1;
+    int ixyW = c1 - 1, ixySW = ixyW - nx, ixyNW = ixyW + nx, ixyE = c1, ixySE = ixyE - nx, ixyNE = ixyE + nx;
+    if (k == 1 && m->BCv.type[iVzW] == -12) {
+        iVxNWW = iVxNW + (nx - 2); iVxSWW = iVxSW + (nx - 2);
+        iPrNW = iPrN + (ncx - 1); iPrSW = iPrS + (ncx - 1);
+        iVzW = iVzC + (nx - 2); iVzSW = iVzW - nxvz; iVzNW = iVzW + nxvz;
+    }
+    if (k == nx - 1 && m->BCv.type[iVzE] == -12) {
+        iVxNEE = iVxNE - (nx - 2); iVxSEE = iVxSE - (nx - 2);
+        iPrNE = iPrN - (ncx - 1); iPrSE = iPrS - (ncx - 1);
+        iVzE = iVzC - (nx - 2); iVzNE = iVzE + nxvz; iVzSE = iVzE - nxvz;
+    }
+    double D31W = m->D31_s[ixyW], D32W = m->D32_s[ixyW], D33W = m->D33_s[ixyW], D34W = m->D34_s[ixyW];
+    double D31E = m->D31_s[ixyE], D32E = m->D32_s[ixyE], D33E = m->D33_s[ixyE], D34E = m->D34_s[ixyE];
+    double D21S = m->D21_n[iPrS], D22S = m->D22_n[iPrS], D23S = m->D23_n[iPrS], D24S = m->D24_n[iPrS];
+    double D21N = m->D21_n[iPrN], D22N = m->D22_n[iPrN], D23N = m->D23_n[iPrN], D24N = m->D24_n[iPrN];
+    double inS = 0, inN = 0, inW = 0, inE = 0;
+    if (tP(g, iPrS) == -1) inS = 1.0;
+    if (tP(g, iPrN) == -1) inN = 1.0;
+    if (tG(g, ixyW) != 30 && tV(g, iVzW) != 13) inW = 1.0;
+    if (tG(g, ixyE) != 30 && tV(g, iVzE) != 13) inE = 1.0;
+    double inSWc = 0, inSEc = 0, inNWc = 0, inNEc = 0, inSWv = 0, inSEv = 0, inNWv = 0, inNEv = 0;
+    int nz = x->model->Nz;
+    if ((k > 1) || (k == 1 && tV(g, iVzW) == -1)) {
+        if (tP(g, iPrSW) == -1) inSWc = 1.0;
+        if (tP(g, iPrNW) == -1) inNWc = 1.0;
+    }
+    if ((k < nxvz - 2) || (k == nxvz - 2 && tV(g, iVzE) == -1)) {
+        if (tP(g, iPrSE) == -1) inSEc = 1.0;
+        if (tP(g, iPrNE) == -1) inNEc = 1.0;
+    }
+    if (l > 0) {
+        if (tG(g, ixySW) != 30) inSWv = 1.0;
+        if (tG(g, ixySE) != 30) inSEv = 1.0;
+    }
+    if (l < nz - 1) {
+        if (tG(g, ixyNW) != 30) inNWv = 1.0;
+        if (tG(g, ixyNE) != 30) inNEv = 1.0;
+    }
+    double wE = inN + inS + inNEc + inSEc, wW = inN + inS + inNWc + inSWc;
+    double wS = inW + inE + inSWv + inSEv, wN = inW + inE + inNWv + inNEv;
+    if (wW > 1.0) wW = 1.0 / wW;
+    if (wE > 1.0) wE = 1.0 / wE;
+    if (wS > 1.0) wS = 1.0 / wS;
+    if (wN > 1.0) wN = 1.0 / wN;
+    double dx = m->dx, dz = m->dz, cmp = x->comp;
+
+    zacc c;
+    memset(&c, 0, sizeof(c));
+    /* normal stress at centroids S (b=0) and N (b=1): +/- in/dz * Szz */
+    for (int b = 0; b < 2; b++) {
+        double f = b == 0 ? inS / dz : -inN / dz;
+        double D1 = b == 0 ? D21S : D21N, D2 = b == 0 ? D22S : D22N, D3 = b == 0 ? D23S : D23N, D4 = b == 0 ? D24S : D24N;
+        double w = b == 0 ? wS : wN;
+        z_exx(&c, 0, b, f * D1, cmp, dx, dz);
+        z_ezz(&c, 0, b, f * D2, cmp, dx, dz);
+        int vb = b == 0 ? -1 : 1; /* row of the outer vertices */
+        double fo_w = b == 0 ? inSWv : inNWv, fo_e = b == 0 ? inSEv : inNEv;
+        z_gxz(&c, 0, 0, f * D3 * w * inW, dx, dz);
+        z_gxz(&c, 1, 0, f * D3 * w * inE, dx, dz);
+        z_gxz(&c, 0, vb, f * D3 * w * fo_w, dx, dz);
+        z_gxz(&c, 1, vb, f * D3 * w * fo_e, dx, dz);
+        z_p(&c, 0, b, f * D4);
+    }
+    /* shear stress at vertices W (a=0) and E (a=1): +/- in/dx * Sxz */
+    for (int a = 0; a < 2; a++) {
+        double f = a == 0 ? inW / dx : -inE / dx;
+        double D1 = a == 0 ? D31W : D31E, D2 = a == 0 ? D32W : D32E, D3 = a == 0 ? D33W : D33E, D4 = a == 0 ? D34W : D34E;
+        double w = a == 0 ? wW : wE;
+        int ca_o = a == 0 ? -1 : 1; /* column of the outer centroids */
+        double fl[4] = {inS, inN, a == 0 ? inSWc : inSEc, a == 0 ? inNWc : inNEc};
+        int ca[4] = {0, 0, ca_o, ca_o}, cb[4] = {0, 1, 0, 1};
+        for (int q = 0; q < 4; q++) {
+            double wq = w * fl[q];
+            z_exx(&c, ca[q], cb[q], f * D1 * wq, cmp, dx, dz);
+            z_ezz(&c, ca[q], cb[q], f * D2 * wq, cmp, dx, dz);
+            z_p(&c, ca[q], cb[q], f * D4 * wq);
+        }
+        z_gxz(&c, a, 0, f * D3, dx, dz);
+    }
+    z_p(&c, 0, 0, -inS / dz);
+    z_p(&c, 0, 1, inN / dz);
+
+    double vSW = c.v[0][0], vS = c.v[0][1], vSE = c.v[0][2], vW = c.v[1][0], vC = c.v[1][1], vE = c.v[1][2];
+    double vNW = c.v[2][0], vN = c.v[2][1], vNE = c.v[2][2];
+    double uSSW = c.u[0][1], uSSE = c.u[0][2], uSWW = c.u[1][0], uSW = c.u[1][1], uSE = c.u[1][2], uSEE = c.u[1][3];
+    double uNWW = c.u[2][0], uNW = c.u[2][1], uNE = c.u[2][2], uNEE = c.u[2][3], uNNW = c.u[3][1], uNNE = c.u[3][2];
+    double pSW = c.p[0][0], pS = c.p[0][1], pSE = c.p[0][2], pNW = c.p[1][0], pN = c.p[1][1], pNE = c.p[1][2];
+
+    if (x->stab == 1) {
+        double drhodz = (m->rho_n[c2 + ncx] - m->rho_n[c2]) * x->one_dz;
+        double vC_corr = 1.00 * x->om * x->model->dt * m->gz[c3] * drhodz;
+        if (vC + vC_corr > 0.0) vC += vC_corr;
+    }
+    if (tV(g, iVzW) == 11) vC -= vW;
+    if (tV(g, iVzE) == 11) vC -= vE;
+    if (tV(g, iVzSW) == 11) vW -= vSW;
+    if (tV(g, iVzSE) == 11) vE -= vSE;
+    if (tV(g, iVzNW) == 11) vW -= vNW;
+    if (tV(g, iVzNE) == 11) vE -= vNE;
+    if (tU(g, iVxNNW) == 11) uNW -= uNNW;
+    if (tU(g, iVxNNE) == 11) uNE -= uNNE;
+    if (tU(g, iVxSSW) == 11) uSW -= uSSW;
+    if (tU(g, iVxSSE) == 11) uSE -= uSSE;
+
+    double cv = x->celvol;
+    if (x->assemble == 1) {
+        const SparseMat *S = x->S;
+        x->bA[eqn] *= cv;
+        x->bB[eqn] *= cv;
+#define UADD(idx, coef) add_coeff(ra, S->eqn_u[idx], (coef) * cv, m->BCu.type[idx], m->BCu.val[idx])
+#define VADD(idx, coef) add_coeff(ra, S->eqn_v[idx], (coef) * cv, m->BCv.type[idx], m->BCv.val[idx])
+        if (m->BCu.type[iVxSSW] != 30) UADD(iVxSSW, uSSW);
+        if (m->BCu.type[iVxSSE] != 30) UADD(iVxSSE, uSSE);
+        if ((k > 1) || (k == 1 && m->BCu.type[iVxSWW] == -1))
+            if (m->BCu.type[iVxSWW] != 30) UADD(iVxSWW, uSWW);
+        if (m->BCu.type[iVxSW] != 30) UADD(iVxSW, uSW);
+        if (m->BCu.type[iVxSE] != 30) UADD(iVxSE, uSE);
+        if ((k < nx - 1) || (k == nx - 1 && m->BCu.type[iVxSEE] == -1))
+            if (m->BCu.type[c1 + 1] != 30) UADD(iVxSEE, uSEE);
+        if ((k > 1) || (k == 1 && m->BCu.type[iVxNWW] == -1))
+            if (m->BCu.type[c1 + nx - 2] != 30) UADD(iVxNWW, uNWW);
+        if (m->BCu.type[iVxNW] != 30) UADD(iVxNW, uNW);
+        if (m->BCu.type[iVxNE] != 30) UADD(iVxNE, uNE);
+        if ((k < nx - 1) || (k == nx - 1 && m->BCu.type[iVxNEE] == -1))
+            if (m->BCu.type[iVxNEE] != 30) UADD(iVxNEE, uNEE);
+        if (m->BCu.type[iVxNNW] != 30) UADD(iVxNNW, uNNW);
+        if (m->BCu.type[iVxNNE] != 30) UADD(iVxNNE, uNNE);
+        if (m->BCv.type[iVzSW] == -1) VADD(iVzSW, vSW);
+        if (m->BCv.type[iVzS] != 30) VADD(iVzS, vS);
+        if (m->BCv.type[iVzSE] == -1) VADD(iVzSE, vSE);
+        if (m->BCv.type[iVzW] == -1) VADD(iVzW, vW);
+        add_coeff(ra, eqn, vC * cv, m->BCv.type[iVzC], m->BCv.val[iVzC]);
+        if (m->BCv.type[iVzE] == -1) VADD(iVzE, vE);
+        if (m->BCv.type[iVzNW] == -1) VADD(iVzNW, vNW);
+        if (m->BCv.type[iVzN] != 30) VADD(iVzN, vN);
+        if (m->BCv.type[iVzNE] == -1) VADD(iVzNE, vNE);
+#undef UADD
+#undef VADD
+#define PADD(idx, coef, vidx) add_coeff(rb, S->eqn_p[idx] - S->neq_mom, (coef) * cv, m->BCp.type[idx], m->BCp.val[vidx])
+        if (k > 1 || (k == 1 && periodix == 1))
+            if (m->BCp.type[iPrSW] != 30) PADD(iPrSW, pSW, iPrSW);
+        if (m->BCp.type[iPrS] != 30 && m->BCp.type[iPrN] != 30) PADD(iPrS, pS, iPrS);
+        if (k < nx - 1 || (k == nx - 1 && periodix == 1))
+            if (m->BCp.type[iPrSE] != 30) PADD(iPrSE, pSE, iPrSE);
+        if (k > 1 || (k == 1 && periodix == 1))
+            if (m->BCp.type[iPrNW] != 30) PADD(iPrNW, pNW, iPrNW);
+        if (m->BCp.type[iPrS] != 30 && m->BCp.type[iPrN] != 30) PADD(iPrN, pN, c2 + ncx);
+        if (k < nx - 1 || (k == nx - 1 && periodix == 1))
+            if (m->BCp.type[iPrNE] != 30) PADD(iPrNE, pNE, iPrNE);
+#undef PADD
+    } else {
+        double F = 0.0;
+        F += (m->szzd[iPrN] - m->szzd[iPrS]) / dz;
+        F -= (m->p_in[iPrN] - m->p_in[iPrS]) / dz;
+        F += (m->sxz[ixyE] - m->sxz[ixyW]) / dx;
+        F *= -1.0;
+        F -= x->bA[eqn];
+        F *= cv;
+        x->FA[eqn] = F;
+    }
+}
+
+/* ---------------------------------------------------------------- */
+/* Equation numbering (StokesRoutines.c:1134)                         */
+/* ---------------------------------------------------------------- */
+void orc_EvalNumberOfEquations(grid *mesh, SparseMat *Stokes) {
+    int inc = 0, incP = 0, Nx = mesh->Nx, Nz = mesh->Nz;
+    Stokes->eqn_u = (int *)malloc((size_t)(Nz + 1) * Nx * sizeof(int));
+    Stokes->eqn_v = (int *)malloc((size_t)Nz * (Nx + 1) * sizeof(int));
+    Stokes->eqn_p = (int *)malloc((size_t)(Nz - 1) * (Nx - 1) * sizeof(int));
+    for (int l = 0; l < Nz + 1; l++)
+        for (int k = 0; k < Nx; k++) {
+            int kk = k + l * Nx, t = mesh->BCu.type[kk];
+            Stokes->eqn_u[kk] = -1;
+            if (t == -1 || t == 2 || t == -2) Stokes->eqn_u[kk] = inc++;
+            if (t == -12) Stokes->eqn_u[kk] = Stokes->eqn_u[kk - Nx + 1];
+        }
+    for (int l = 0; l < Nz; l++)
+        for (int k = 0; k < Nx + 1; k++) {
+            int kk = k + l * (Nx + 1), t = mesh->BCv.type[kk];
+            Stokes->eqn_v[kk] = -1;
+            if (t == -1 || t == 2) Stokes->eqn_v[kk] = inc++;
+        }
+    Stokes->neq_mom = inc;
+    for (int l = 0; l < Nz - 1; l++)
+        for (int k = 0; k < Nx - 1; k++) {
+            int kk = k + l * (Nx - 1);
+            Stokes->eqn_p[kk] = -1;
+            if (mesh->BCp.type[kk] == -1) { Stokes->eqn_p[kk] = inc++; incP++; }
+        }
+    Stokes->neq = inc;
+    Stokes->neq_cont = incP;
+}
+
+/* ---------------------------------------------------------------- */
+/* Builders                                                          */
+/* ---------------------------------------------------------------- */
+
+/* growable per-thread CSR fragment */
+typedef struct { int n, cap; int *J; double *A; } frag;
+static void frag_push(frag *f, const rowbuf *r) {
+    if (f->n + r->n > f->cap) {
+        f->cap = (f->n + r->n) * 2 + 1024;
+        f->J = (int *)realloc(f->J, sizeof(int) * f->cap);
+        f->A = (double *)realloc(f->A, sizeof(double) * f->cap);
+    }
+    memcpy(f->J + f->n, r->J, sizeof(int) * r->n);
+    memcpy(f->A + f->n, r->A, sizeof(double) * r->n);
+    f->n += r->n;
+}
+
+/* Assembled block under construction: row starts (relative to the owning
+ * fragment) and the fragments, one per (node pass, thread); merged pass by
+ * pass and thread by thread, which is equation order
+ * (StokesAssemblyDecoupled.c:1733-1736, 1804-1807). */
+typedef struct {
+    int neq;
+    int *rowstart;  /* per equation: start offset within its thread fragment */
+    int *rowthr;    /* per equation: owning thread, -1 if never visited */
+    frag *fr;
+    int nth;
+} blockbuild;
+
+static void bb_init(blockbuild *b, int neq, int nfrag) {
+    b->neq = neq; b->nth = nfrag;
+    b->rowstart = (int *)calloc((size_t)neq + 1, sizeof(int));
+    b->rowthr = (int *)malloc(sizeof(int) * ((size_t)neq + 1));
+    for (int i = 0; i < neq; i++) b->rowthr[i] = -1;
+    b->fr = (frag *)calloc((size_t)nfrag, sizeof(frag));
+}
+
+/* Merge into M (Ic/J/A malloc()ed at their final sizes). */
+static void bb_merge(blockbuild *b, SparseMat *M) {
+    int *base = (int *)calloc((size_t)b->nth + 1, sizeof(int));
+    for (int t = 0; t < b->nth; t++) base[t + 1] = base[t] + b->fr[t].n;
+    int nnz = base[b->nth];
+    M->Ic = (int *)calloc((size_t)b->neq + 1, sizeof(int));
+    M->J = nnz ? (int *)malloc(sizeof(int) * nnz) : NULL;
+    M->A = nnz ? (double *)malloc(sizeof(double) * nnz) : NULL;
+    for (int t = 0; t < b->nth; t++) {
+        if (b->fr[t].n) {
+            memcpy(M->J + base[t], b->fr[t].J, sizeof(int) * b->fr[t].n);
+            memcpy(M->A + base[t], b->fr[t].A, sizeof(double) * b->fr[t].n);
+        }
+        free(b->fr[t].J); free(b->fr[t].A);
+    }
+    /* rows never visited keep Ic = 0, as with the reference's calloc'ed Ic */
+    for (int e = 0; e < b->neq; e++)
+        if (b->rowthr[e] >= 0) M->Ic[e] = b->rowstart[e] + base[b->rowthr[e]];
+    M->Ic[b->neq] = nnz;
+    M->nnz = nnz;
+    free(base); free(b->rowstart); free(b->rowthr); free(b->fr);
+}
+
+typedef enum { KIND_PICARD = 0, KIND_JACOBIAN = 1 } opkind;
+
+static void build_operator(opkind kind, grid *mesh, params model, double *p_corr, double *p, double *u,
+                           double *v, SparseMat *Stokes, SparseMat *SA, SparseMat *SB, SparseMat *SC,
+                           SparseMat *SD, int Assemble) {
+    geo g = {mesh, mesh->Nx, mesh->Nz, mesh->Nx - 1, mesh->Nz - 1, mesh->Nx + 1, mesh->Nz + 1};
+    int nx = g.nx, nz = g.nz, ncx = g.ncx, nxvz = g.nxvz, nzvx = g.nzvx;
+    rowctx x;
+    memset(&x, 0, sizeof(x));
+    x.S = Stokes; x.model = &model; x.assemble = Assemble;
+    x.celvol = mesh->dx * mesh->dz;
+    x.one_dx = 1.0 / mesh->dx; x.one_dz = 1.0 / mesh->dz;
+    x.one_dx_dx = 1.0 / mesh->dx / mesh->dx; x.one_dz_dz = 1.0 / mesh->dz / mesh->dz;
+    x.one_dx_dz = 1.0 / mesh->dx / mesh->dz;
+    x.stab = model.free_surf_stab > 0 ? 1 : 0;
+    x.om = model.free_surf_stab;
+    /* Picard: comp follows the model (only the Neumann rows use it; the inner
+     * rows force the deviatoric form).  Jacobian: comp = 1 throughout. */
+    int comp_builder = kind == KIND_JACOBIAN ? 1 : (model.compressible == 1 ? 1 : 0);
+
+    SA->neq = Stokes->neq_mom; SB->neq = Stokes->neq_mom;
+    SC->neq = Stokes->neq_cont; SC->neq_mom = Stokes->neq_mom;
+    SD->neq = Stokes->neq_cont; SD->neq_mom = Stokes->neq_mom;
+
+    /* right-hand sides (StokesAssemblyDecoupled.c:1640-1670) */
+    {
+        int inc = 0;
+        for (int l = 0; l < nzvx; l++)
+            for (int k = 0; k < nx; k++) {
+                int cc = k + l * nx, t = mesh->BCu.type[cc];
+                if (t != 0 && t != 30 && t != 11 && t != 13 && t != -12) SA->b[inc++] = mesh->roger_x[cc];
+            }
+        for (int l = 0; l < nz; l++)
+            for (int k = 0; k < nxvz; k++) {
+                int cc = k + l * nxvz, t = mesh->BCv.type[cc];
+                if (t != 0 && t != 30 && t != 11 && t != 13 && t != -12) SA->b[inc++] = mesh->roger_z[cc];
+            }
+        inc = 0;
+        for (int l = 0; l < g.ncz; l++)
+            for (int k = 0; k < ncx; k++) {
+                int cc = k + l * ncx, t = mesh->BCp.type[cc];
+                if (t != 31 && t != 0 && t != 30) SC->b[inc++] = mesh->rhs_p[cc];
+            }
+    }
+
+    int nth = orc_num_threads();
+    blockbuild bA, bB, bC;
+    double *bbcA = NULL, *bbcB = NULL, *bbcC = NULL;
+    if (Assemble == 1) {
+        bb_init(&bA, Stokes->neq_mom, 2 * nth);
+        bb_init(&bB, Stokes->neq_mom, 2 * nth);
+        bb_init(&bC, Stokes->neq_cont, nth);
+        bbcA = (double *)calloc((size_t)Stokes->neq_mom + 1, sizeof(double));
+        bbcB = (double *)calloc((size_t)Stokes->neq_mom + 1, sizeof(double));
+        bbcC = (double *)calloc((size_t)Stokes->neq_cont + 1, sizeof(double));
+    }
+
+    /* momentum rows: u nodes then v nodes, each thread a contiguous range */
+    x.u = u; x.v = v; x.p = p_corr;
+    x.bA = SA->b; x.FA = SA->F; x.bB = SB->b; x.comp = comp_builder;
+    for (int pass = 0; pass < 2; pass++) {
+        int N = pass == 0 ? nx * nzvx : nxvz * nz;
+#pragma omp parallel num_threads(nth)
+        {
+#ifdef _OPENMP
+            int ith = omp_get_thread_num();
+#else
+            int ith = 0;
+#endif
+            int chunk = N / nth, eps = N % nth;
+            int st = ith * chunk, en = st + chunk + (ith == nth - 1 ? eps : 0);
+            for (int node = st; node < en; node++) {
+                rowbuf ra, rb;
+                ra.n = 0; ra.bbc = 0.0; rb.n = 0; rb.bbc = 0.0;
+                int eqn, k, l, c1, c2, c3;
+                if (pass == 0) {
+                    c1 = node; k = c1 % nx; l = c1 / nx;
+                    c2 = k - 1 + (l - 1) * ncx; c3 = k + l * nxvz;
+                    int t = mesh->BCu.type[c1];
+                    if (!(t == -1 || t == 2 || t == -2)) continue;
+                    eqn = Stokes->eqn_u[c1];
+                    /* residual mode of the Neumann rows reads the assembled bbc */
+                    if (Assemble == 0 && t == 2) ra.bbc = SA->bbc[eqn];
+                    int inner = l > 0 && l < nzvx - 1 && k > 0 && k < nx - 1;
+                    if (kind == KIND_PICARD) {
+                        if (inner || (k == 0 && t == -2)) xmom_inner(&g, &x, c1, c2, c3, eqn, &ra, &rb);
+                    } else {
+                        if (inner || (k == 0 && t == -2)) xjac_row(&g, &x, c1, c2, c3, eqn, k, l, &ra, &rb);
+                    }
+                    if (k == 0 && t == 2) xmom_neumann(&g, &x, c1, c2, c3, eqn, 0, comp_builder, &ra, &rb);
+                    if (k == nx - 1 && t == 2) xmom_neumann(&g, &x, c1, c2, c3, eqn, 1, comp_builder, &ra, &rb);
+                } else {
+                    c3 = node; k = c3 % nxvz; l = c3 / nxvz;
+                    c1 = k + l * nx; c2 = k - 1 + (l - 1) * ncx;
+                    int t = mesh->BCv.type[c3];
+                    if (!(t == -1 || t == 2)) continue;
+                    eqn = Stokes->eqn_v[c3];
+                    if (Assemble == 0 && t == 2) ra.bbc = SA->bbc[eqn];
+                    int inner = k > 0 && k < nxvz - 1 && l > 0 && l < nz - 1;
+                    if (inner) {
+                        if (kind == KIND_PICARD) zmom_inner(&g, &x, c1, c2, c3, eqn, &ra, &rb);
+                        else zjac_row(&g, &x, c1, c2, c3, eqn, k, l, &ra, &rb);
+                    }
+                    if (l == 0 && t == 2) zmom_neumann(&g, &x, c1, c2, c3, eqn, 0, comp_builder, &ra, &rb);
+                    if (l == nz - 1 && t == 2) zmom_neumann(&g, &x, c1, c2, c3, eqn, 1, comp_builder, &ra, &rb);
+                }
+                if (Assemble == 1) {
+                    int fi = pass * nth + ith;
+                    bA.rowthr[eqn] = fi; bA.rowstart[eqn] = bA.fr[fi].n;
+                    bB.rowthr[eqn] = fi; bB.rowstart[eqn] = bB.fr[fi].n;
+                    frag_push(&bA.fr[fi], &ra);
+                    frag_push(&bB.fr[fi], &rb);
+                    bbcA[eqn] += ra.bbc;
+                    bbcB[eqn] += rb.bbc;
+                }
+            }
+        }
+    }
+
+    /* continuity rows (StokesAssemblyDecoupled.c:1861-1889, 4100-4124) */
+    x.p = p; x.bA = SC->b; x.FA = SC->F; x.bB = SD->b; x.FD = SD->F;
+    {
+        int N = ncx * g.ncz;
+#pragma omp parallel num_threads(nth)
+        {
+#ifdef _OPENMP
+            int ith = omp_get_thread_num();
+#else
+            int ith = 0;
+#endif
+            int chunk = N / nth, eps = N % nth;
+            int st = ith * chunk, en = st + chunk + (ith == nth - 1 ? eps : 0);
+            rowctx xc = x;
+            for (int c2 = st; c2 < en; c2++) {
+                int k = c2 % ncx, l = c2 / ncx;
+                int c1 = k + (l + 1) * nx, c3 = k + l * nxvz + 1;
+                if (mesh->BCp.type[c2] != -1) continue;
+                /* the Jacobian builder reads comp_cells at a vertex-family
+                 * index (StokesAssemblyDecoupled.c:4113); it only matters for
+                 * the residual branch */
+                xc.comp = kind == KIND_JACOBIAN ? mesh->comp_cells[c1] : mesh->comp_cells[c2];
+                int eqn = Stokes->eqn_p[c2] - Stokes->neq_mom;
+                rowbuf rc;
+                rc.n = 0; rc.bbc = 0.0;
+                continuity_row(&g, &xc, c1, c2, c3, eqn, &rc);
+                if (Assemble == 1) {
+                    bC.rowthr[eqn] = ith; bC.rowstart[eqn] = bC.fr[ith].n;
+                    frag_push(&bC.fr[ith], &rc);
+                    bbcC[eqn] += rc.bbc;
+                }
+            }
+        }
+    }
+
+    if (Assemble == 1) {
+        bb_merge(&bA, SA);
+        bb_merge(&bB, SB);
+        bb_merge(&bC, SC);
+        SA->bbc = bbcA; SB->bbc = bbcB; SC->bbc = bbcC;
+        /* D block: no coefficients (StokesAssemblyDecoupled.c:81-88) */
+        SD->Ic = (int *)calloc((size_t)SD->neq + 1, sizeof(int));
+        SD->J = NULL; SD->A = NULL; SD->nnz = 0;
+        SD->bbc = (double *)calloc((size_t)SD->neq + 1, sizeof(double));
+        for (int k = 0; k < SA->neq; k++) { SA->b[k] += SA->bbc[k]; SB->b[k] = SA->b[k]; }
+        for (int k = 0; k < SC->neq; k++) { SC->b[k] += SC->bbc[k]; SD->b[k] = SC->b[k]; }
+    }
+}
+
+void orc_BuildStokesOperatorDecoupled(grid *mesh, params model, int lev, double *p_corr, double *p,
+                                      double *u, double *v, SparseMat *Stokes, SparseMat *SA,
+                                      SparseMat *SB, SparseMat *SC, SparseMat *SD, int Assemble) {
+    (void)lev;
+    build_operator(KIND_PICARD, mesh, model, p_corr, p, u, v, Stokes, SA, SB, SC, SD, Assemble);
+}
+
+void orc_BuildJacobianOperatorDecoupled(grid *mesh, params model, int lev, double *p_corr, double *p,
+                                        double *u, double *v, SparseMat *Stokes, SparseMat *SA,
+                                        SparseMat *SB, SparseMat *SC, SparseMat *SD, int Assemble) {
+    (void)lev;
+    /* the Jacobian X rows receive p_in (and the periodic ones p_corr); only
+     * the residual branch reads it, through mesh->p_in */
+    build_operator(KIND_JACOBIAN, mesh, model, p_corr, p, u, v, Stokes, SA, SB, SC, SD, Assemble);
+}
+
+/* StokesRoutines.c:839 EvaluateStokesResidualDecoupled */
+void orc_EvaluateStokesResidualDecoupled(SparseMat *Stokes, SparseMat *SA, SparseMat *SB, SparseMat *SC,
+                                         SparseMat *SD, Nparams *Nmodel, grid *mesh, params model,
+                                         scale scaling, int quiet) {
+    int nx = model.Nx, nz = model.Nz, nzvx = nz + 1, nxvz = nx + 1, ncx = nx - 1, ncz = nz - 1;
+    int ndofx = 0, ndofz = 0, ndofp = 0;
+    double resx = 0.0, resz = 0.0, resp = 0.0;
+    if (model.aniso == 0) orc_BuildStokesOperatorDecoupled(mesh, model, 0, mesh->p_corr, mesh->p_in, mesh->u_in, mesh->v_in, Stokes, SA, SB, SC, SD, 0);
+    if (model.aniso == 1) orc_BuildJacobianOperatorDecoupled(mesh, model, 0, mesh->p_corr, mesh->p_in, mesh->u_in, mesh->v_in, Stokes, SA, SB, SC, SD, 0);
+#pragma omp parallel for reduction(+ : resx, ndofx)
+    for (int cc = 0; cc < nzvx * nx; cc++) {
+        int t = mesh->BCu.type[cc];
+        if (t != 0 && t != 30 && t != 11 && t != 13 && t != -12) {
+            int e = Stokes->eqn_u[cc];
+            ndofx++;
+            resx += SA->F[e] * SA->F[e];
+            mesh->ru[cc] = SA->F[e];
+            SA->F[e] *= SA->d[e];
+        }
+    }
+    Nmodel->resx = resx;
+#pragma omp parallel for reduction(+ : resz, ndofz)
+    for (int cc = 0; cc < nz * nxvz; cc++) {
+        int t = mesh->BCv.type[cc];
+        if (t != 0 && t != 30 && t != 11 && t != 13 && t != -12) {
+            int e = Stokes->eqn_v[cc];
+            ndofz++;
+            resz += SA->F[e] * SA->F[e];
+            mesh->rv[cc] = SA->F[e];
+            SA->F[e] *= SA->d[e];
+        }
+    }
+    Nmodel->resz = resz;
+#pragma omp parallel for reduction(+ : resp, ndofp)
+    for (int cc = 0; cc < ncz * ncx; cc++) {
+        int t = mesh->BCp.type[cc];
+        if (t != 0 && t != 30 && t != 31) {
+            int e = Stokes->eqn_p[cc] - Stokes->neq_mom;
+            ndofp++;
+            resp += SD->F[e] * SD->F[e];
+            mesh->rp[cc] = SD->F[e];
+            SD->F[e] *= SD->d[e];
+        }
+    }
+    Nmodel->resp = resp;
+    Nmodel->resx = sqrt(Nmodel->resx / ndofx);
+    Nmodel->resz = sqrt(Nmodel->resz / ndofz);
+    Nmodel->resp = sqrt(Nmodel->resp / ndofp);
+    if (Nmodel->nit == 0) { Nmodel->resx0 = Nmodel->resx; Nmodel->resz0 = Nmodel->resz; Nmodel->resp0 = Nmodel->resp; }
+    if (quiet == 0) {
+        printf("Fu abs. = %2.6e --- Fu rel. = %2.6e\n", Nmodel->resx, Nmodel->resx / Nmodel->resx0);
+        printf("Fv abs. = %2.6e --- Fv rel. = %2.6e\n", Nmodel->resz, Nmodel->resz / Nmodel->resz0);
+        printf("Fp abs. = %2.6e --- Fp rel. = %2.6e\n", Nmodel->resp, Nmodel->resp / Nmodel->resp0);
+    }
+    if (isnan(Nmodel->resx) || isnan(Nmodel->resz) || isnan(Nmodel->resp)) {
+        printf("Fu = %2.6e\n", Nmodel->resx * scaling.S * scaling.L);
+        printf("Fv = %2.6e\n", Nmodel->resz * scaling.S * scaling.L);
+        printf("Fp = %2.6e\n", Nmodel->resp * scaling.E * scaling.L * scaling.L);
+        printf("Solve went wrong - Nan residuals...\nExiting...\n");
+        exit(122);
+    }
+}
+
+/* StokesRoutines.c:137 UpdateNonLinearity */
+void orc_UpdateNonLinearity(grid *mesh, mat_prop materials, params *model, Nparams *Nmodel, scale scaling) {
+    orc_StrainRateComponents(mesh, scaling, model);
+    orc_NonNewtonianViscosityGrid(mesh, &materials, model, *Nmodel, &scaling);
+    orc_EvaluateRHS(mesh, *model, scaling, materials.rho[0]);
+    orc_RheologicalOperators(mesh, model, &scaling, 0);
+}
+
+void orc_free_block(SparseMat *M) {
+    free(M->Ic); free(M->J); free(M->A); free(M->bbc);
+    M->Ic = NULL; M->J = NULL; M->A = NULL; M->bbc = NULL;
+}
+
+void orc_salloc(SparseMat *M, int neq) {
+    M->d = (double *)malloc(sizeof(double) * ((size_t)neq + 1));
+    for (int i = 0; i < neq; i++) M->d[i] = 1.0;
+    M->b = (double *)calloc((size_t)neq + 1, sizeof(double));
+    M->x = (double *)calloc((size_t)neq + 1, sizeof(double));
+    M->F = (double *)calloc((size_t)neq + 1, sizeof(double));
+}
+
+void orc_sfree(SparseMat *M) {
+    free(M->d); free(M->b); free(M->x); free(M->F);
+    M->d = M->b = M->x = M->F = NULL;
+}
+
+void orc_newton_unit(grid *mesh, mat_prop *materials, params *model, Nparams *Nmodel, scale *scaling,
+                     SparseMat *Stokes, SparseMat *SA, SparseMat *SB, SparseMat *SC, SparseMat *SD,
+                     SparseMat *JA, SparseMat *JB, SparseMat *JC, SparseMat *JD) {
+    orc_StrainRateComponents(mesh, *scaling, model);
+    orc_NonNewtonianViscosityGrid(mesh, materials, model, *Nmodel, scaling);
+    orc_EvaluateRHS(mesh, *model, *scaling, materials->rho[0]);
+    orc_RheologicalOperators(mesh, model, scaling, 0);
+    orc_BuildStokesOperatorDecoupled(mesh, *model, 0, mesh->p_corr, mesh->p_in, mesh->u_in, mesh->v_in, Stokes, SA, SB, SC, SD, 1);
+    orc_RheologicalOperators(mesh, model, scaling, 1);
+    orc_BuildJacobianOperatorDecoupled(mesh, *model, 0, mesh->p_corr, mesh->p_in, mesh->u_in, mesh->v_in, Stokes, JA, JB, JC, JD, 1);
+    orc_RheologicalOperators(mesh, model, scaling, 0);
+    orc_EvaluateStokesResidualDecoupled(Stokes, SA, SB, SC, SD, Nmodel, mesh, *model, *scaling, 1);
+}

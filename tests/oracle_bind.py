@@ -1,0 +1,41 @@
+"""ctypes binding of oracle/liboracle.so, the CPU restatement of the
+Newton-iteration path (test infrastructure: checker and CPU baseline)."""
+import ctypes as C
+import os
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+_lib = None
+
+
+def load(mdz):
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = C.CDLL(LIB)
+        G, SM, NP, SC, PR, MP = (C.POINTER(mdz.grid), C.POINTER(mdz.SparseMat), mdz.Nparams, mdz.scale,
+                                 mdz.params, mdz.mat_prop)
+        dp = C.POINTER(C.c_double)
+        L.orc_NonNewtonianViscosityGrid.argtypes = [G, C.POINTER(MP), C.POINTER(PR), NP, C.POINTER(SC)]
+        L.orc_StrainRateComponents.argtypes = [G, SC, C.POINTER(PR)]
+        L.orc_RheologicalOperators.argtypes = [G, C.POINTER(PR), C.POINTER(SC), C.c_int]
+        L.orc_EvaluateRHS.argtypes = [G, PR, SC, C.c_double]
+        L.orc_InterpCentroidsToVerticesDouble.argtypes = [dp, dp, G, C.POINTER(PR)]
+        L.orc_EvalNumberOfEquations.argtypes = [G, SM]
+        L.orc_BuildStokesOperatorDecoupled.argtypes = [G, PR, C.c_int, dp, dp, dp, dp, SM, SM, SM, SM, SM, C.c_int]
+        L.orc_BuildJacobianOperatorDecoupled.argtypes = [G, PR, C.c_int, dp, dp, dp, dp, SM, SM, SM, SM, SM, C.c_int]
+        L.orc_EvaluateStokesResidualDecoupled.argtypes = [SM, SM, SM, SM, SM, C.POINTER(NP), G, PR, SC, C.c_int]
+        L.orc_UpdateNonLinearity.argtypes = [G, MP, C.POINTER(PR), C.POINTER(NP), SC]
+        L.orc_newton_unit.argtypes = [G, C.POINTER(MP), C.POINTER(PR), C.POINTER(NP), C.POINTER(SC)] + [SM] * 9
+        L.orc_free_block.argtypes = [SM]
+        L.orc_salloc.argtypes = [SM, C.c_int]
+        L.orc_sfree.argtypes = [SM]
+        L.orc_num_threads.restype = C.c_int
+        L.orc_Viscosity.restype = C.c_double
+        L.orc_Viscosity.argtypes = ([C.c_int] + [C.c_double] * 15 + [C.POINTER(MP), C.POINTER(PR), C.POINTER(SC)]
+                                    + [dp] * 19 + [C.c_double] * 4 + [dp] * 4 + [C.c_double] + [dp] * 9
+                                    + [C.c_double] * 2 + [dp] * 3)
+        L.orc_last_iterations.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        _lib = L
+    return _lib

@@ -1,0 +1,248 @@
+"""GPU parity: every drop-in entry point of libmdz_hot.so against the oracle
+on the same seeded inputs, and the device-resident Newton unit against the
+oracle's unit.  Integer work (equation numbers, CSR row pointers and column
+indices) must be bit-exact; FP64 values are compared with the tolerances
+below (elementwise relative with a floor at a fraction of the field's scale,
+CSR values relative to the largest entry of their row).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from helpers import blocks, numbering, oracle_rheology_inputs, oracle_update, relerr, row_relerr, salloc
+
+pytestmark = pytest.mark.gpu
+
+# tolerances
+TOL_ARITH = 1e-13      # strain rates, interpolation, right-hand sides, Picard D tensor
+TOL_RHEO = 1e-11       # outputs of the local iterations (FP64 transcendentals, converged to 1e-13)
+TOL_CSR = 1e-12        # CSR values, relative to the row's largest entry
+TOL_RES = 1e-10        # residual vectors, relative to 1e-6 of their largest entry
+FLOOR = 1e-8
+
+
+def _cases():
+    out = [("C1", "inclusion", (41, 41, 1)), ("litho", "lithosphere", (121, 61, 3))]
+    for fam in range(5):
+        for mat in range(2):
+            out.append(("edge%d_%d" % (fam, mat), "edgecase", (33, 25, 11 + fam, fam, mat)))
+    return out
+
+
+CASES = _cases()
+
+
+def make(mdz, kind, args):
+    return getattr(mdz.Model, kind)(*args)
+
+
+@pytest.fixture(params=CASES, ids=[c[0] for c in CASES])
+def case(request, mdz):
+    name, kind, args = request.param
+    m = make(mdz, kind, args)
+    yield m
+    m.free()
+
+
+def _cmp_fields(mo, mg, names, tol, floor=FLOOR):
+    worst = {}
+    for n in names:
+        e = relerr(mg.field(n), mo.field(n), floor)
+        worst[n] = e
+    bad = {k: v for k, v in worst.items() if not v <= tol}
+    print("max errors:", {k: "%.1e" % v for k, v in worst.items()})
+    assert not bad, "fields beyond %.0e: %s" % (tol, bad)
+
+
+STRAIN_OUT = ["div_u", "exxd", "ezzd", "exz", "exz_n", "exxd_s", "ezzd_s"]
+RHEO_OUT = ["div_u_s", "T_s", "P_s", "d0_s", "phi0_s", "eta_n", "eta_phys_n", "VE_n", "sxxd", "szzd", "sxz_n",
+            "eII_el", "eII_pl", "eII_pwl", "eII_exp", "eII_lin", "eII_gbs", "eII_cst", "d_n", "exx_el", "ezz_el",
+            "exx_diss", "ezz_diss", "detadexx_n", "detadezz_n", "detadgxz_n", "detadp_n", "ddivpdexx_n",
+            "ddivpdezz_n", "ddivpdgxz_n", "ddivpdp_n", "p_corr", "div_u_el", "div_u_pl", "div_u_r", "Wtot", "Wel",
+            "Wdiss", "X_n", "OverS_n", "VE_s", "sxz", "eta_phys_s", "eta_s", "exz_el", "exz_diss", "detadexx_s",
+            "detadezz_s", "detadgxz_s", "detadp_s", "X_s", "OverS_s"]
+D_OUT = ["D11_n", "D12_n", "D13_n", "D14_n", "D21_n", "D22_n", "D23_n", "D24_n", "D31_s", "D32_s", "D33_s", "D34_s"]
+RHS_OUT = ["roger_x", "roger_z", "gx", "gz", "rhs_p"]
+
+
+def test_strain_rates(case, mdz, orc, hot):
+    mo, mg = case, case.clone()
+    orc.orc_StrainRateComponents(mo.m.mesh, mo.m.scaling, C.byref(mo.m.model))
+    hot.StrainRateComponents(mg.m.mesh, mg.m.scaling, C.byref(mg.m.model))
+    assert hot.mdz_last_status() == 0, hot.mdz_last_error()
+    _cmp_fields(mo, mg, STRAIN_OUT, TOL_ARITH)
+    mg.free()
+
+
+def test_interp(case, mdz, orc, hot):
+    mo, mg = case, case.clone()
+    src = mo.field("T").copy()
+    dst_o = np.zeros(mo.Nx * mo.Nz)
+    dst_g = np.zeros(mo.Nx * mo.Nz)
+    dp = C.POINTER(C.c_double)
+    orc.orc_InterpCentroidsToVerticesDouble(src.ctypes.data_as(dp), dst_o.ctypes.data_as(dp), mo.m.mesh, C.byref(mo.m.model))
+    hot.InterpCentroidsToVerticesDouble(src.ctypes.data_as(dp), dst_g.ctypes.data_as(dp), mg.m.mesh, C.byref(mg.m.model))
+    assert np.array_equal(dst_o, dst_g)
+    mg.free()
+
+
+def test_rheology(case, mdz, orc, hot):
+    mo = case
+    oracle_rheology_inputs(orc, mo)
+    mg = mo.clone()
+    orc.orc_NonNewtonianViscosityGrid(mo.m.mesh, C.byref(mo.m.materials), C.byref(mo.m.model), mo.m.Nmodel, C.byref(mo.m.scaling))
+    hot.NonNewtonianViscosityGrid(mg.m.mesh, C.byref(mg.m.materials), C.byref(mg.m.model), mg.m.Nmodel, C.byref(mg.m.scaling))
+    assert hot.mdz_last_status() == 0, hot.mdz_last_error()
+    _cmp_fields(mo, mg, RHEO_OUT, TOL_RHEO)
+    mg.free()
+
+
+@pytest.mark.parametrize("jac", [0, 1])
+def test_dtensor(case, mdz, orc, hot, jac):
+    mo = case
+    oracle_update(orc, mo)
+    mg = mo.clone()
+    orc.orc_RheologicalOperators(mo.m.mesh, C.byref(mo.m.model), C.byref(mo.m.scaling), jac)
+    hot.RheologicalOperators(mg.m.mesh, C.byref(mg.m.model), C.byref(mg.m.scaling), jac)
+    assert hot.mdz_last_status() == 0, hot.mdz_last_error()
+    _cmp_fields(mo, mg, D_OUT, TOL_ARITH if jac == 0 else TOL_RHEO)
+    mg.free()
+
+
+def test_rhs(case, mdz, orc, hot):
+    mo = case
+    oracle_rheology_inputs(orc, mo)
+    orc.orc_NonNewtonianViscosityGrid(mo.m.mesh, C.byref(mo.m.materials), C.byref(mo.m.model), mo.m.Nmodel, C.byref(mo.m.scaling))
+    mg = mo.clone()
+    orc.orc_EvaluateRHS(mo.m.mesh, mo.m.model, mo.m.scaling, mo.m.materials.rho[0])
+    hot.EvaluateRHS(mg.m.mesh, mg.m.model, mg.m.scaling, mg.m.materials.rho[0])
+    assert hot.mdz_last_status() == 0, hot.mdz_last_error()
+    _cmp_fields(mo, mg, RHS_OUT, TOL_ARITH)
+    mg.free()
+
+
+def test_numbering(case, mdz, orc, hot):
+    mo = case
+    So = numbering(orc, mdz, mo)
+    Sg = mdz.SparseMat()
+    hot.EvalNumberOfEquations(mo.m.mesh, C.byref(Sg))
+    assert (So.neq, So.neq_mom, So.neq_cont) == (Sg.neq, Sg.neq_mom, Sg.neq_cont)
+    nu, nw, nc = mo.Nx * (mo.Nz + 1), (mo.Nx + 1) * mo.Nz, (mo.Nx - 1) * (mo.Nz - 1)
+    for f, n in (("eqn_u", nu), ("eqn_v", nw), ("eqn_p", nc)):
+        a = np.ctypeslib.as_array(getattr(So, f), shape=(n,))
+        b = np.ctypeslib.as_array(getattr(Sg, f), shape=(n,))
+        assert np.array_equal(a, b), f
+
+
+def _assemble(mdz, orc, hot, m_o, m_g, builder, S):
+    """Runs builder 'BuildStokes...' or 'BuildJacobian...' on both, returns block lists."""
+    bo = blocks(orc, mdz, S)
+    bg = blocks(orc, mdz, S)
+    mo, mg = m_o.m, m_g.m
+    dp = C.POINTER(C.c_double)
+    fo = getattr(orc, "orc_" + builder)
+    fg = getattr(hot, builder)
+    fo(mo.mesh, mo.model, 0, m_o.field("p_corr").ctypes.data_as(dp), m_o.field("p_in").ctypes.data_as(dp),
+       m_o.field("u_in").ctypes.data_as(dp), m_o.field("v_in").ctypes.data_as(dp), C.byref(S), *[C.byref(b) for b in bo], 1)
+    fg(mg.mesh, mg.model, 0, m_g.field("p_corr").ctypes.data_as(dp), m_g.field("p_in").ctypes.data_as(dp),
+       m_g.field("u_in").ctypes.data_as(dp), m_g.field("v_in").ctypes.data_as(dp), C.byref(S), *[C.byref(b) for b in bg], 1)
+    assert hot.mdz_last_status() == 0, hot.mdz_last_error()
+    return bo, bg
+
+
+def _cmp_blocks(mdz, bo, bg):
+    for i, (a, b) in enumerate(zip(bo, bg)):
+        assert (a.neq, a.nnz) == (b.neq, b.nnz), "block %d: neq/nnz %s vs %s" % (i, (a.neq, a.nnz), (b.neq, b.nnz))
+        Ia, Ja, Aa, ba_bbc, ba_b = mdz.csr_arrays(a)
+        Ib, Jb, Ab, bb_bbc, bb_b = mdz.csr_arrays(b)
+        assert np.array_equal(Ia, Ib), "block %d Ic" % i
+        assert np.array_equal(Ja, Jb), "block %d J" % i
+        e = row_relerr(Ia, Ab, Aa)
+        print("block %d nnz %d value err %.2e" % (i, a.nnz, e))
+        assert e <= TOL_CSR, "block %d values %.2e" % (i, e)
+        if a.neq:
+            eb = relerr(bb_b, ba_b, 1e-6)
+            ec = relerr(bb_bbc, ba_bbc, 1e-6)
+            print("   b err %.2e bbc err %.2e" % (eb, ec))
+            assert eb <= TOL_RES and ec <= TOL_RES
+
+
+@pytest.mark.parametrize("builder", ["BuildStokesOperatorDecoupled", "BuildJacobianOperatorDecoupled"])
+def test_assembly(case, mdz, orc, hot, builder):
+    mo = case
+    oracle_update(orc, mo)
+    if builder.startswith("BuildJacobian"):
+        orc.orc_RheologicalOperators(mo.m.mesh, C.byref(mo.m.model), C.byref(mo.m.scaling), 1)
+    mg = mo.clone()
+    S = numbering(orc, mdz, mo)
+    bo, bg = _assemble(mdz, orc, hot, mo, mg, builder, S)
+    _cmp_blocks(mdz, bo, bg)
+    mg.free()
+
+
+def test_residual(case, mdz, orc, hot):
+    mo = case
+    oracle_update(orc, mo)
+    mg = mo.clone()
+    S = numbering(orc, mdz, mo)
+    bo, bg = _assemble(mdz, orc, hot, mo, mg, "BuildStokesOperatorDecoupled", S)
+    No, Ng = mdz.Nparams(), mdz.Nparams()
+    orc.orc_EvaluateStokesResidualDecoupled(C.byref(S), *[C.byref(b) for b in bo], C.byref(No), mo.m.mesh, mo.m.model, mo.m.scaling, 1)
+    hot.EvaluateStokesResidualDecoupled(C.byref(S), *[C.byref(b) for b in bg], C.byref(Ng), mg.m.mesh, mg.m.model, mg.m.scaling, 1)
+    assert hot.mdz_last_status() == 0, hot.mdz_last_error()
+    for k in ("resx", "resz", "resp"):
+        a, b = getattr(Ng, k), getattr(No, k)
+        print(k, a, b, abs(a - b) / abs(b))
+        assert abs(a - b) <= 1e-12 * abs(b) + 1e-300
+    for i in (0, 2, 3):
+        n = bo[i].neq
+        Fa = np.ctypeslib.as_array(bo[i].F, shape=(n,))
+        Fb = np.ctypeslib.as_array(bg[i].F, shape=(n,))
+        e = relerr(Fb, Fa, 1e-6)
+        print("F block %d err %.2e" % (i, e))
+        assert e <= TOL_RES
+    _cmp_fields(mo, mg, ["ru", "rv", "rp"], TOL_RES, 1e-6)
+    mg.free()
+
+
+def test_newton_unit_device_resident(case, mdz, orc, hot):
+    """mdz_ctx_newton_unit (everything resident in HBM) against the oracle's
+    sequence; CSR patterns of all eight blocks bit-exact."""
+    mo = case
+    mg = mo.clone()
+    S = numbering(orc, mdz, mo)
+    bo = blocks(orc, mdz, S, 8)
+    orc.orc_newton_unit(mo.m.mesh, C.byref(mo.m.materials), C.byref(mo.m.model), C.byref(mo.m.Nmodel),
+                        C.byref(mo.m.scaling), C.byref(S), *[C.byref(b) for b in bo])
+    ctx = C.c_void_p()
+    mdz.check(hot.mdz_ctx_create(C.byref(ctx), mg.m.mesh, C.byref(mg.m.materials), C.byref(mg.m.model), C.byref(mg.m.scaling)))
+    try:
+        res = (C.c_double * 3)()
+        mdz.check(hot.mdz_ctx_newton_unit(ctx, res))
+        for k, v in zip(("resx", "resz", "resp"), res):
+            ref = getattr(mo.m.Nmodel, k)
+            assert abs(v - ref) <= 1e-12 * abs(ref), (k, v, ref)
+        for i in range(8):
+            neq, nnz = C.c_int(), C.c_int()
+            mdz.check(hot.mdz_ctx_block_info(ctx, i, C.byref(neq), C.byref(nnz)))
+            assert (neq.value, nnz.value) == (bo[i].neq, bo[i].nnz), i
+            Ic = np.zeros(neq.value + 1, np.int32)
+            J = np.zeros(max(nnz.value, 1), np.int32)
+            A = np.zeros(max(nnz.value, 1))
+            b = np.zeros(max(neq.value, 1))
+            ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+            mdz.check(hot.mdz_ctx_download_block(ctx, i, Ic.ctypes.data_as(ip), J.ctypes.data_as(ip),
+                                                 A.ctypes.data_as(dp), b.ctypes.data_as(dp), None))
+            Io, Jo, Ao, _, bo_b = mdz.csr_arrays(bo[i])
+            assert np.array_equal(Ic, Io), "Ic %d" % i
+            assert np.array_equal(J[:nnz.value], Jo), "J %d" % i
+            e = row_relerr(Io, A[:nnz.value], Ao)
+            print("unit block %d err %.2e" % (i, e))
+            assert e <= TOL_CSR
+        mdz.check(hot.mdz_ctx_download_grid(ctx, mg.m.mesh))
+        _cmp_fields(mo, mg, ["eta_n", "eta_s", "D11_n", "D33_s", "roger_x", "roger_z"], TOL_RHEO)
+        _cmp_fields(mo, mg, ["ru", "rv", "rp"], TOL_RES, 1e-6)
+    finally:
+        hot.mdz_ctx_destroy(ctx)
+        mg.free()
