@@ -45,10 +45,17 @@ def case(request, mdz):
     m.free()
 
 
+# derivative fields: sums of terms of both signs (tangent, plastic multiplier
+# derivatives); compared against 1e-6 of the field's scale
+DERIV = {"detadexx_n", "detadezz_n", "detadgxz_n", "detadp_n", "detadexx_s", "detadezz_s", "detadgxz_s",
+         "detadp_s", "ddivpdexx_n", "ddivpdezz_n", "ddivpdgxz_n", "ddivpdp_n", "D12_n", "D13_n", "D14_n", "D21_n",
+         "D23_n", "D24_n", "D31_s", "D32_s", "D34_s", "div_u_el", "div_u_pl", "Wel", "Wtot", "Wdiss"}
+
+
 def _cmp_fields(mo, mg, names, tol, floor=FLOOR):
     worst = {}
     for n in names:
-        e = relerr(mg.field(n), mo.field(n), floor)
+        e = relerr(mg.field(n), mo.field(n), max(floor, 1e-6) if n in DERIV else floor)
         worst[n] = e
     bad = {k: v for k, v in worst.items() if not v <= tol}
     print("max errors:", {k: "%.1e" % v for k, v in worst.items()})
