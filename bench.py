@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--ncx", type=int, default=2049)
     ap.add_argument("--ncz", type=int, default=1025)
     ap.add_argument("--mode", default="fp64", choices=["fp64", "mixed", "learned"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "generic"],
+                    help="grid rheology kernel: log-space where it applies (auto) or the generic one")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-units", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -89,6 +91,7 @@ def main():
 
     mdz = _pkg()
     hot = mdz.lib_hot()
+    mdz.check(hot.mdz_set_rheology_kernel(0 if args.kernel == "auto" else 1))
     m = mdz.Model.lithosphere(args.ncx, args.ncz, args.seed)
     ctx = C.c_void_p()
     mdz.check(hot.mdz_ctx_create(C.byref(ctx), m.m.mesh, C.byref(m.m.materials), C.byref(m.m.model),
@@ -142,7 +145,8 @@ def main():
         "dtype": "f64" if mode == 0 else "f32+f64",
         "data": "synthetic (seeded lithosphere model)",
         "config": {"workload": "lithosphere %dx%d cells, 3 phases VEP, one Newton-iteration unit" % (args.ncx, args.ncz),
-                   "rheology": args.mode, "cells": cells, "parallelism": "single GPU"},
+                   "rheology": args.mode, "rheology_kernel": args.kernel, "cells": cells,
+                   "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},

@@ -295,6 +295,13 @@ enum {
     MDZ_RHEO_LEARNED = 2    /* MIXED with the surrogate network's initial guess */
 };
 
+/* Grid rheology kernel (process-wide; no counterpart in the reference).
+ * 0 (default): the log-space kernel wherever it applies (eta_average = 0,
+ * isotropic, no grain-size evolution, no phase mixing, no reaction
+ * softening), the generic kernel otherwise.  1: always the generic kernel,
+ * a line-by-line restatement of Viscosity (RheologyDensity.c:44-707). */
+int mdz_set_rheology_kernel(int which);
+
 typedef struct mdz_ctx mdz_ctx;
 
 /* Creates a device mirror of *mesh (all fields the path reads or writes),
@@ -329,8 +336,11 @@ int  mdz_ctx_download_block(mdz_ctx *ctx, int which, int *Ic, int *J, double *A,
                             double *bbc);
 /* Residual vectors (F of the momentum and continuity blocks). */
 int  mdz_ctx_download_residual(mdz_ctx *ctx, double *Fmom, double *Fcont);
-/* Local-iteration counts accumulated since the last call (node x phase
- * evaluations): hist[0..63] = FP64 visco-elastic Newton iterations,
+/* Switches the collection of local-iteration histograms on (1) or off (0,
+ * the default: collection costs a few percent of the rheology kernels). */
+int  mdz_ctx_collect_histogram(mdz_ctx *ctx, int on);
+/* Local-iteration counts accumulated since the last call while collection
+ * was on (node x phase evaluations): hist[0..63] = FP64 visco-elastic Newton iterations,
  * hist[64..127] = plastic return-map iterations, hist[128..191] = FP32
  * pre-iterations (mixed / learned modes).  Resets the counters. */
 int  mdz_ctx_iteration_histogram(mdz_ctx *ctx, long long *hist192);

@@ -71,13 +71,6 @@ static void dropin_check(int rc) {
 /* ------------------------------------------------------------------ */
 /* field table                                                          */
 /* ------------------------------------------------------------------ */
-enum {
-#define MDZ_FENUM(name, fam, role) FI_##name,
-    MDZ_ALL_DOUBLE_FIELDS(MDZ_FENUM)
-#undef MDZ_FENUM
-    FI_COUNT
-};
-
 struct FieldRef {
     size_t host_off, dev_off;
     int fam, role;
@@ -110,6 +103,7 @@ struct mdz_ctx {
     DevGrid g;
     DevParams prm;
     mat_prop *d_mat;
+    void *d_pk;          /* per-phase constants of the log-space rheology */
     double *xg, *zg, *xvz, *zvx;
     DevEqn e;
     DevBlock blk[8];     /* 0..3 Picard A, B, C, D; 4..7 Jacobian A, B, C, D */
@@ -120,6 +114,7 @@ struct mdz_ctx {
     double *norm_partial, *norm_out;
     int *status;
     unsigned long long *hist;
+    int hist_on;         /* collect local-iteration histograms (off by default) */
     float *guess_n, *guess_s;
     double *tmp_c, *tmp_v;  /* staging for InterpCentroidsToVerticesDouble */
     params model;
@@ -228,6 +223,8 @@ static int ctx_alloc(mdz_ctx **out, int Nx, int Nz, int nph) {
     g.BCg = (int8_t *)(c->slab + o_tg);
     g.comp_cells = (int *)(c->slab + o_comp);
     g.phase_n = (double *)(c->slab + o_phn); g.phase_s = (double *)(c->slab + o_phs);
+    g.cstride = (long long)(align256(nc * sizeof(double)) / sizeof(double));
+    g.vstride = (long long)(align256(nv * sizeof(double)) / sizeof(double));
     c->xg = (double *)(c->slab + o_xg); c->zg = (double *)(c->slab + o_zg);
     c->xvz = (double *)(c->slab + o_xvz); c->zvx = (double *)(c->slab + o_zvx);
     c->e.eqn_u = (int *)(c->slab + o_eu); c->e.eqn_v = (int *)(c->slab + o_ev); c->e.eqn_p = (int *)(c->slab + o_ep);
@@ -245,6 +242,7 @@ static int ctx_alloc(mdz_ctx **out, int Nx, int Nz, int nph) {
     if (hipMalloc((void **)&c->status, 4 * sizeof(int)) != hipSuccess) return set_err(MDZ_ENOMEM, "status");
     if (hipMalloc((void **)&c->hist, 3 * MDZ_HIST_BINS * sizeof(unsigned long long)) != hipSuccess) return set_err(MDZ_ENOMEM, "hist");
     if (hipMalloc((void **)&c->d_mat, sizeof(mat_prop)) != hipSuccess) return set_err(MDZ_ENOMEM, "materials");
+    if (hipMalloc(&c->d_pk, mdz_phasek_bytes()) != hipSuccess) return set_err(MDZ_ENOMEM, "phase constants");
     hipMemset(c->status, 0, 4 * sizeof(int));
     hipMemset(c->hist, 0, 3 * MDZ_HIST_BINS * sizeof(unsigned long long));
     if (hipStreamCreate(&c->stream) != hipSuccess) return set_err(MDZ_EHIP, "stream");
@@ -257,7 +255,7 @@ static void ctx_release(mdz_ctx *c) {
     hipStreamSynchronize(c->stream);
     ctx_free_blocks(c);
     hipFree(c->slab); hipFree(c->scratch); hipFree(c->norm_partial); hipFree(c->norm_out);
-    hipFree(c->status); hipFree(c->hist); hipFree(c->d_mat); hipFree(c->guess_n); hipFree(c->guess_s);
+    hipFree(c->status); hipFree(c->hist); hipFree(c->d_mat); hipFree(c->d_pk); hipFree(c->guess_n); hipFree(c->guess_s);
     hipFree(c->d_net);
     hipStreamDestroy(c->stream);
     free(c);
@@ -321,6 +319,8 @@ static int up_coords(mdz_ctx *c, const grid *m) {
 static int up_materials(mdz_ctx *c, const mat_prop *mt) {
     c->materials = *mt;
     HIPCHK(hipMemcpyAsync(c->d_mat, &c->materials, sizeof(mat_prop), hipMemcpyHostToDevice, c->stream));
+    mdz_launch_phase_constants(c->d_mat, MDZ_MAX_PHASES, c->d_pk, c->stream);
+    HIPCHK(hipGetLastError());
     return MDZ_OK;
 }
 
@@ -482,9 +482,21 @@ extern "C" void InterpCentroidsToVerticesDouble(double *CentroidArray, double *V
         FI_div_u_r, FI_Wtot, FI_Wel, FI_Wdiss, FI_X_n, FI_OverS_n, FI_VE_s, FI_sxz, FI_eta_phys_s, FI_eta_s,         \
         FI_exz_el, FI_exz_diss, FI_detadexx_s, FI_detadezz_s, FI_detadgxz_s, FI_detadp_s, FI_X_s, FI_OverS_s
 
+/* kernel selection: 0 = log-space kernel where eligible, 1 = generic kernel */
+static int g_rheo_kernel = 0;
+extern "C" int mdz_set_rheology_kernel(int which) {
+    if (which < 0 || which > 1) return set_err(MDZ_EINVAL, "rheology kernel %d", which);
+    g_rheo_kernel = which;
+    return MDZ_OK;
+}
+
 static int rheology_pass(mdz_ctx *c, int use_guess) {
     const float *gn = use_guess ? c->guess_n : nullptr, *gs = use_guess ? c->guess_s : nullptr;
-    mdz_launch_rheology(c->g, c->prm, c->d_mat, gn, gs, c->hist, c->stream);
+    if (g_rheo_kernel == 0 && mdz_rheo_fast_eligible(c->prm, &c->materials))
+        mdz_launch_rheology_fast(c->g, c->prm, c->d_pk, c->materials.R, gn, gs, c->hist_on ? c->hist : nullptr,
+                                 c->stream);
+    else
+        mdz_launch_rheology(c->g, c->prm, c->d_mat, gn, gs, c->hist_on ? c->hist : nullptr, c->stream);
     return MDZ_OK;
 }
 
@@ -785,6 +797,9 @@ extern "C" void EvaluateStokesResidualDecoupled(SparseMat *Stokes, SparseMat *St
                           StokesB, StokesC, StokesD, 0);
     if (g_status && !g_exit_on_error) return;
     DevBlock *A = &c->blk[4 * kind], *D = &c->blk[4 * kind + 3];
+    /* ru/rv/rp are written on the degrees of freedom only; the other entries
+     * keep the caller's values */
+    dropin_check(up_list(c, mesh, {FI_ru, FI_rv, FI_rp}));
     /* residual scaling vectors d */
     hipMemcpyAsync(A->d, StokesA->d, (size_t)StokesA->neq * 8, hipMemcpyHostToDevice, c->stream);
     hipMemcpyAsync(D->d, StokesD->d, (size_t)StokesD->neq * 8, hipMemcpyHostToDevice, c->stream);
@@ -992,6 +1007,12 @@ extern "C" int mdz_ctx_download_residual(mdz_ctx *c, double *Fmom, double *Fcont
     if (Fmom) HIPCHK(hipMemcpyAsync(Fmom, c->blk[4 * kind].F, (size_t)c->e.neq_mom * 8, hipMemcpyDeviceToHost, c->stream));
     if (Fcont) HIPCHK(hipMemcpyAsync(Fcont, c->blk[4 * kind + 3].F, (size_t)c->e.neq_cont * 8, hipMemcpyDeviceToHost, c->stream));
     return sync(c);
+}
+
+extern "C" int mdz_ctx_collect_histogram(mdz_ctx *c, int on) {
+    if (!c) return set_err(MDZ_EINVAL, "NULL context");
+    c->hist_on = on != 0;
+    return MDZ_OK;
 }
 
 extern "C" int mdz_ctx_iteration_histogram(mdz_ctx *c, long long *hist) {

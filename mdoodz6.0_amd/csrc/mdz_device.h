@@ -15,10 +15,21 @@
 #include "../../include/mdoodz_hot.h"
 #include "mdz_fields.h"
 
+/* Field indices in list order: the slab holds the fields in this order, each
+ * array padded to a multiple of 256 bytes, so a run of same-family fields
+ * is a strided 2-D array (stride cstride / vstride doubles). */
+enum {
+#define MDZ_FENUM(name, fam, role) FI_##name,
+    MDZ_ALL_DOUBLE_FIELDS(MDZ_FENUM)
+#undef MDZ_FENUM
+    FI_COUNT
+};
+
 /* All double fields of the path, by name (pointers into the device slab). */
 struct DevGrid {
     int Nx, Nz, ncx, ncz, nxvz, nzvx, nph;
     double dx, dz;
+    long long cstride, vstride; /* padded centroid / vertex array lengths, in doubles */
 #define MDZ_DEVPTR(name, fam, role) double *name;
     MDZ_ALL_DOUBLE_FIELDS(MDZ_DEVPTR)
 #undef MDZ_DEVPTR
@@ -40,6 +51,31 @@ struct DevParams {
 
 /* Histogram bins of the local-iteration counters. */
 #define MDZ_HIST_BINS 64
+
+/* Per-block histogram of local-iteration counts: counted in LDS and flushed
+ * to the global 64-bit counters once per block (one global atomic per lane
+ * on a handful of addresses would serialise the whole kernel).  Parts:
+ * 0 = FP64 visco-elastic iterations, 1 = plastic iterations (0 = elastic),
+ * 2 = FP32 pre-iterations.  Collection is off when the global pointer is
+ * null; the begin/flush barriers are then skipped (uniform condition). */
+struct BlockHist {
+    unsigned int *lds;
+    __device__ __forceinline__ void add(int part, int bin) const {
+        if (lds) atomicAdd(&lds[part * MDZ_HIST_BINS + (bin < MDZ_HIST_BINS ? bin : MDZ_HIST_BINS - 1)], 1u);
+    }
+};
+__device__ __forceinline__ BlockHist hist_begin(unsigned int *lds, const unsigned long long *global) {
+    if (!global) return BlockHist{nullptr};
+    for (int i = threadIdx.x; i < 3 * MDZ_HIST_BINS; i += blockDim.x) lds[i] = 0u;
+    __syncthreads();
+    return BlockHist{lds};
+}
+__device__ __forceinline__ void hist_flush(const BlockHist &h, unsigned long long *global) {
+    if (!global) return;
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * MDZ_HIST_BINS; i += blockDim.x)
+        if (h.lds[i]) atomicAdd(&global[i], (unsigned long long)h.lds[i]);
+}
 
 /* Equation numbering and the row -> node maps built with the pattern. */
 struct DevEqn {

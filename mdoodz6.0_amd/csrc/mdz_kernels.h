@@ -10,6 +10,17 @@ void mdz_launch_strain_rates(const DevGrid &g, const DevParams &prm, hipStream_t
 void mdz_launch_interp(const DevGrid &g, const DevParams &prm, const double *src, double *dst, hipStream_t s);
 void mdz_launch_rheology(const DevGrid &g, const DevParams &prm, const mat_prop *mt, const float *guess_n,
                          const float *guess_s, unsigned long long *hist, hipStream_t s);
+/* div_u, T, P, d, phi from centroids to vertices (input of the vertex rheology) */
+void mdz_launch_interp5(const DevGrid &g, const DevParams &prm, hipStream_t s);
+
+/* log-space rheology for eta_average = 0, isotropic, no grain-size evolution
+ * or phase mixing (mdz_rheo_fast.hip); per-phase constants precomputed */
+#define MDZ_MAX_PHASES 20
+int mdz_rheo_fast_eligible(const DevParams &prm, const mat_prop *host_mt);
+size_t mdz_phasek_bytes(void);
+void mdz_launch_phase_constants(const mat_prop *d_mt, int nph, void *d_pk, hipStream_t s);
+void mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
+                              const float *guess_s, unsigned long long *hist, hipStream_t s);
 void mdz_launch_dtensor(const DevGrid &g, const DevParams &prm, int jac, int *status, hipStream_t s);
 void mdz_launch_rhs(const DevGrid &g, const DevParams &prm, const double *xg, const double *zvx,
                     const double *xvz, const double *zg, hipStream_t s);

@@ -1,0 +1,595 @@
+/*
+ * mdz_rheo_fast.hip -- gfx950 grid rheology for the common case: arithmetic
+ * phase averaging (eta_average = 0), isotropic, no grain-size evolution, no
+ * phase mixing, no reaction softening.  Same local constitutive update as
+ * RheologyDensity.c:44-707 (Viscosity) and the same phase accumulation as
+ * RheologyDensity.c:961-1415 (NonNewtonianViscosityGrid), organised for the
+ * GPU:
+ *
+ *  - Creep laws in log space.  The node-independent part of each prefactor
+ *    is folded once per phase into log B0 (k_phase_constants).  Per node and
+ *    phase: one log(Eii); per local iteration one log(Tii) shared by all
+ *    mechanisms and one exp per active mechanism.  Viscosity evaluates each
+ *    prefactor with pow/exp per factor and each mechanism with its own pow
+ *    per iteration; both forms agree to rounding (~1e-15 relative).
+ *  - The phase-averaged outputs are streamed to HBM as each phase finishes
+ *    (stored for the first active phase, added for the later ones in phase
+ *    order: the reference's summation order), so no accumulators stay live
+ *    across the local solve.
+ *  - Phase constants are read with wave-uniform indices (scalar loads).
+ *  - Mixed precision (MDZ_RHEO_MIXED / LEARNED): the visco-elastic Newton
+ *    iteration first runs in FP32 with the hardware log/exp from the
+ *    reference guess (or the network's guess) to |r/Eii| < 1e-6; FP64 Newton
+ *    steps then finish to the reference tolerance |r/Eii| < 1e-13.  The
+ *    tangent, the plastic return and every output stay FP64.
+ */
+#include "mdz_kernels.h"
+
+#define BLK 256
+#define ZEROC 273.15
+#define LN2 0.69314718055994530942
+#define LN3 1.09861228866810969140
+
+struct PhaseK {
+    int cst, pwl, expv, lin, gbs, tensile;
+    double eta_cst;
+    /* log B = lnB0 + (Ea + P Va) / (R T n) + (m / n) ln d + lnfr - (a / n) phi */
+    double lnB0_pwl, n_pwl, Ea_pwl, Va_pwl, mn_pwl, lnfr_pwl, an_pwl;
+    double lnB0_lin, n_lin, Ea_lin, Va_lin, m_lin, lnfr_lin, an_lin;
+    double lnB0_gbs, n_gbs, Ea_gbs, Va_gbs, mn_gbs, lnfr_gbs, an_gbs;
+    double Ea_exp, gamma, n_exp, ln_gS, ln_E;
+    double Slim, eta_vp0, n_vp, gs_ref, ln_gs_ref;
+};
+
+/* -(r / n) ln f, with pow(f, -0) = 1 for every f */
+__device__ __forceinline__ double lnpow_neg(double r_over_n, double f) {
+    return r_over_n == 0.0 ? 0.0 : -r_over_n * log(f);
+}
+
+__global__ void k_phase_constants(const mat_prop *__restrict__ mt, int nph, PhaseK *pk) {
+    int p = threadIdx.x;
+    if (p >= nph) return;
+    PhaseK k;
+    k.cst = mt->cstv[p] != 0;
+    k.pwl = mt->pwlv[p] != 0;
+    k.expv = mt->expv[p] != 0;
+    k.lin = mt->linv[p] != 0;
+    k.gbs = mt->gbsv[p] != 0;
+    k.tensile = mt->is_tensile[p] == 1;
+    k.eta_cst = mt->eta0[p];
+    double n = mt->npwl[p];
+    k.n_pwl = n; k.Ea_pwl = mt->Qpwl[p]; k.Va_pwl = mt->Vpwl[p];
+    k.lnB0_pwl = k.pwl ? log(mt->pref_pwl[p] * mt->Fpwl[p] * pow(mt->Apwl[p], -1.0 / n)) : 0.0;
+    k.mn_pwl = mt->mpwl[p] / n;
+    k.lnfr_pwl = k.pwl ? lnpow_neg(mt->rpwl[p] / n, mt->fpwl[p]) : 0.0;
+    k.an_pwl = mt->apwl[p] / n;
+    n = mt->nlin[p];
+    k.n_lin = n; k.Ea_lin = mt->Qlin[p]; k.Va_lin = mt->Vlin[p]; k.m_lin = mt->mlin[p];
+    k.lnB0_lin = k.lin ? log(mt->Flin[p] * pow(mt->Alin[p], -1.0 / n)) : 0.0;
+    k.lnfr_lin = k.lin ? lnpow_neg(mt->rlin[p] / n, mt->flin[p]) : 0.0;
+    k.an_lin = mt->alin[p] / n;
+    n = mt->ngbs[p];
+    k.n_gbs = n; k.Ea_gbs = mt->Qgbs[p]; k.Va_gbs = mt->Vgbs[p];
+    k.lnB0_gbs = k.gbs ? log(mt->Fgbs[p] * pow(mt->Agbs[p], -1.0 / n)) : 0.0;
+    k.mn_gbs = mt->mgbs[p] / n;
+    k.lnfr_gbs = k.gbs ? lnpow_neg(mt->rgbs[p] / n, mt->fgbs[p]) : 0.0;
+    k.an_gbs = mt->agbs[p] / n;
+    k.Ea_exp = mt->Qexp[p]; k.gamma = mt->Gexp[p]; k.n_exp = mt->nexp[p];
+    k.ln_gS = k.expv ? log(mt->Gexp[p] * mt->Sexp[p]) : 0.0;
+    k.ln_E = k.expv ? log(mt->Eexp[p]) : 0.0;
+    k.Slim = mt->Slim[p]; k.eta_vp0 = mt->eta_vp[p]; k.n_vp = mt->n_vp[p];
+    k.gs_ref = mt->gs_ref[p];
+    k.ln_gs_ref = log(mt->gs_ref[p]);
+    pk[p] = k;
+}
+
+/* The rheology outputs are two runs of consecutive fields in the slab
+ * (eta_n .. drhodp_n on centroids, VE_s .. OverS_s on vertices), so each
+ * output is addressed as run base + position * stride + node. */
+template <int FIRST, int LAST, int FAM>
+struct FieldRun {
+    static constexpr bool same_family() {
+        constexpr int fam[] = {
+#define MDZ_FFAM(name, fam, role) MDZ_FAM_##fam,
+            MDZ_ALL_DOUBLE_FIELDS(MDZ_FFAM)
+#undef MDZ_FFAM
+        };
+        for (int i = FIRST; i <= LAST; i++)
+            if (fam[i] != FAM) return false;
+        return true;
+    }
+};
+static_assert(FieldRun<FI_eta_n, FI_drhodp_n, MDZ_FAM_C>::same_family(), "centroid rheology outputs must be one run");
+static_assert(FieldRun<FI_VE_s, FI_OverS_s, MDZ_FAM_V>::same_family(), "vertex rheology outputs must be one run");
+
+/* Output stream of one node: fr * v stored (first active phase) or added. */
+struct Out {
+    double *base; /* first field of the run */
+    long long stride;
+    int c, first_fi;
+    double fr;
+    bool first;
+    __device__ __forceinline__ double *at(int fi) const { return base + (long long)(fi - first_fi) * stride + c; }
+    __device__ __forceinline__ void put(int fi, double v) const {
+        double x = fr * v;
+        double *f = at(fi);
+        if (first) *f = x;
+        else *f += x;
+    }
+    /* fr * 1 / v, the reference's form for the grain size */
+    __device__ __forceinline__ void put_inv(int fi, double v) const {
+        double x = fr * 1.0 / v;
+        double *f = at(fi);
+        if (first) *f = x;
+        else *f += x;
+    }
+};
+
+/* Node inputs; the friction and dilatancy angles enter as sines/cosines,
+ * evaluated once per node. */
+struct NodeIn {
+    double G, T, P, d, phi, X0, Exx, Ezz, Exz, Gxx, Gzz, Gxz, Txx0, Tzz0, Txz0, sin_dil, sin_fric, cos_fric, C, P0,
+        beta;
+};
+
+/* Effective visco-elastic strain rates for an isotropic node: the inverse
+ * anisotropy tensor of RheologyDensity.c:1061-1094 is diag(1/2, 1/2, 1). */
+__device__ __forceinline__ void ve_rates_iso(double exx, double ezz, double exz, double sxx0, double szz0, double sxz0,
+                                             double etae, NodeIn &in) {
+    in.Exx = exx + (0.5 * sxx0) / etae;
+    in.Ezz = ezz + (0.5 * szz0) / etae;
+    in.Exz = exz + sxz0 / 2.0 / etae;
+    double gxz = 2.0 * exz + sxz0 / etae;
+    in.Gxx = in.Exx;
+    in.Gzz = in.Ezz;
+    in.Gxz = gxz * 0.5;
+}
+
+/* Local update of one node and one phase (RheologyDensity.c:44-707 with
+ * gs = 0, constant_mix = 0); outputs go through o.  CELL selects the
+ * centroid output set, otherwise the vertex set. */
+template <bool CELL>
+__device__ __forceinline__ void local_update(const PhaseK &k, double R, const DevParams &prm, const NodeIn &in,
+                                             double *Xout, const Out &o, double exxd, double ezzd, double exzc,
+                                             double guess, const BlockHist &hist) {
+    const double dt = prm.dt;
+    int elastic = prm.iselastic;
+    double G = in.G, T = in.T, P = in.P, C = in.C, K = 1.0 / in.beta;
+    const double TmaxPeierls = (1200.0 + ZEROC) / prm.sT;
+    const double Exx = in.Exx, Ezz = in.Ezz, Exz = in.Exz;
+    double Eyy = -(Exx + Ezz), Gyy = -(in.Gxx + in.Gzz);
+    double Eii = sqrt(1.0 / 2.0 * (Exx * Exx + Ezz * Ezz + Eyy * Eyy) + Exz * Exz);
+    double Gii = sqrt(1.0 / 2.0 * (in.Gxx * in.Gxx + in.Gzz * in.Gzz + Gyy * Gyy) + in.Gxz * in.Gxz);
+    if (Eii * prm.sE < 1e-30) Eii = 1e-30 / prm.sE;
+    if (Gii * prm.sE < 1e-30) Gii = 1e-30 / prm.sE;
+    const double f_ani = Gii / Eii;
+    const int peierls = k.expv && T < TmaxPeierls;
+    if (prm.step == 0) elastic = 0;
+    if (elastic == 0) { G = 1e1; K = 1e1; } /* and dil = 0 (sin_dil set by the caller) */
+    if (T < ZEROC / prm.sT) T = ZEROC / prm.sT;
+    const double RT = R * T;
+    const double lnEii = log(Eii), lnf2 = log(2.0 * f_ani), lnd = log(in.d);
+
+    /* prefactors B and compliances C = (2 f B)^-n, in logs */
+    double lnB_pwl = 0, lnC_pwl = 0, lnB_lin = 0, lnC_lin = 0, lnB_gbs = 0, lnC_gbs = 0, lnB_exp = 0, lnC_exp = 0;
+    double nST = 0;
+    if (k.pwl) {
+        lnB_pwl = k.lnB0_pwl + (k.Ea_pwl + P * k.Va_pwl) / RT / k.n_pwl + (k.mn_pwl != 0.0 ? k.mn_pwl * lnd : 0.0)
+                + k.lnfr_pwl - k.an_pwl * in.phi;
+        lnC_pwl = -k.n_pwl * (lnf2 + lnB_pwl);
+    }
+    if (k.lin) {
+        lnB_lin = k.lnB0_lin + (k.Ea_lin + P * k.Va_lin) / RT / k.n_lin + k.lnfr_lin - k.an_lin * in.phi;
+        /* the diffusion rate carries d1^-m with d1 the reference grain size */
+        lnC_lin = -k.n_lin * (lnf2 + lnB_lin) - (k.m_lin != 0.0 ? k.m_lin * k.ln_gs_ref : 0.0);
+    }
+    if (k.gbs) {
+        lnB_gbs = k.lnB0_gbs + (k.Ea_gbs + P * k.Va_gbs) / RT / k.n_gbs + (k.mn_gbs != 0.0 ? k.mn_gbs * lnd : 0.0)
+                + k.lnfr_gbs - k.an_gbs * in.phi;
+        lnC_gbs = -k.n_gbs * (lnf2 + lnB_gbs);
+    }
+    if (peierls) {
+        double ST = k.Ea_exp / R / T * 2.0 * k.gamma * (1 - k.gamma);
+        nST = ST + k.n_exp;
+        double lnArr = -k.Ea_exp / R / T * (1.0 - k.gamma) * (1.0 - k.gamma);
+        double lnF = ((1.0 - nST) * LN2 - 0.5 * (nST + 1.0) * LN3) / nST;
+        lnB_exp = lnF + (ST / nST) * k.ln_gS - (k.ln_E + lnArr) / nST;
+        lnC_exp = -nST * (lnf2 + lnB_exp);
+    }
+
+    /* Drucker-Prager yield with the von Mises and tension cut-offs */
+    double cos_fric = in.cos_fric, sin_fric = in.sin_fric, sin_dil = in.sin_dil;
+    double Tyield = C * cos_fric + P * sin_fric;
+    if (k.Slim < Tyield) { sin_fric = 0.0; cos_fric = 1.0; sin_dil = 0.0; C = k.Slim; Tyield = k.Slim; }
+    int tens = 0;
+    {
+        double P_tens = -25e6 / prm.sS;
+        double sft = -C * cos_fric / P_tens;
+        if (C * cos_fric + P * sft < Tyield && k.tensile) { sin_fric = sft; Tyield = C * cos_fric + P * sft; tens = 1; }
+    }
+
+    /* initial guess: the smallest isolated viscosity */
+    const double eta_el = G * dt;
+    double eta_ve = 1.0e100 / prm.seta;
+    if (k.cst) eta_ve = fmin(eta_ve, k.eta_cst);
+    if (k.pwl) eta_ve = fmin(eta_ve, exp(lnB_pwl + (1.0 / k.n_pwl - 1.0) * lnEii));
+    if (elastic) eta_ve = fmin(eta_ve, eta_el);
+    if (peierls) eta_ve = fmin(eta_ve, exp(lnB_exp + (1.0 / nST - 1.0) * lnEii));
+    if (k.lin)
+        eta_ve = fmin(eta_ve, exp(lnB_lin + (1.0 / k.n_lin - 1.0) * lnEii + (k.m_lin != 0.0 ? k.m_lin / k.n_lin * lnd : 0.0)));
+    if (k.gbs) eta_ve = fmin(eta_ve, exp(lnB_gbs + (1.0 / k.n_gbs - 1.0) * lnEii));
+
+    /* FP32 pre-iterations */
+    int it32 = 0;
+    if (prm.rheo_mode != MDZ_RHEO_FP64) {
+        float e = guess > 0.0 ? (float)guess : (float)eta_ve;
+        const float Ef = (float)Eii, ff = (float)f_ani, eelf = (float)eta_el, ecst = (float)k.eta_cst;
+        const float lcp = (float)lnC_pwl, lce = (float)lnC_exp, lcl = (float)lnC_lin, lcg = (float)lnC_gbs;
+        const float np_ = (float)k.n_pwl, ne = (float)nST, nl = (float)k.n_lin, ng = (float)k.n_gbs;
+        for (it32 = 0; it32 < 20; it32++) {
+            float Tf = 2.0f * ff * e * Ef;
+            float lt = __logf(Tf);
+            float Ep = k.pwl ? __expf(lcp + np_ * lt) : 0.0f;
+            float Ee = peierls ? __expf(lce + ne * lt) : 0.0f;
+            float El = k.lin ? __expf(lcl + nl * lt) : 0.0f;
+            float Eg = k.gbs ? __expf(lcg + ng * lt) : 0.0f;
+            float Ec = k.cst ? Tf / ff / 2.0f / ecst : 0.0f;
+            float r = Ef - (elastic ? Tf / (2.0f * eelf) : 0.0f) - (Ep + Ee + El + Eg + Ec);
+            if (fabsf(r / Ef) < 1e-6f) break;
+            float df = 0.0f;
+            if (elastic) df -= Ef / eelf;
+            if (peierls) df -= Ee * ne / e;
+            if (k.lin) df -= El * nl / e;
+            if (k.pwl) df -= Ep * np_ / e;
+            if (k.cst) df -= Ef / ecst;
+            float en = e - r / df;
+            e = en > 0.0f ? en : 0.5f * e; /* stay where the residual is monotone */
+        }
+        if (e > 0.0f && e < 3.0e38f) eta_ve = (double)e;
+    } else if (guess > 0.0) {
+        eta_ve = guess;
+    }
+
+    /* FP64 visco-elastic Newton iteration (RheologyDensity.c:330-370) */
+    double Ep = 0, Ee = 0, El = 0, Eg = 0, Ec = 0, lT = 0;
+    int it;
+    for (it = 0; it < 20; it++) {
+        double Ti = 2.0 * f_ani * eta_ve * Eii;
+        lT = log(Ti);
+        if (k.cst) Ec = Ti / f_ani / 2.0 / k.eta_cst;
+        if (k.pwl) Ep = exp(lnC_pwl + k.n_pwl * lT);
+        if (k.gbs) Eg = exp(lnC_gbs + k.n_gbs * lT);
+        if (peierls) Ee = exp(lnC_exp + nST * lT);
+        if (k.lin) El = exp(lnC_lin + k.n_lin * lT);
+        double r = Eii - elastic * Ti / (2.0 * eta_el) - (Ep + Ee + El + Eg + Ec);
+        if (fabs(r / Eii) < 1.0e-13) break;
+        double df = 0.0;
+        if (elastic) df += -Eii / eta_el;
+        if (peierls) df += -Ee * nST / eta_ve;
+        if (k.lin) df += -El * k.n_lin / eta_ve;
+        if (k.pwl) df += -Ep * k.n_pwl / eta_ve;
+        if (k.cst) df += -Eii / k.eta_cst;
+        eta_ve -= r / df;
+    }
+    hist.add(0, it);
+    hist.add(2, it32);
+    /* stresses at the converged eta_ve */
+    double Tii = 2.0 * eta_ve * f_ani * Eii;
+    double Txx = in.Gxx * Tii / f_ani / Eii, Tzz = in.Gzz * Tii / f_ani / Eii, Txz = in.Gxz * Tii / f_ani / Eii;
+    /* creep rates at this Tii: the loop's values unless it ran out of
+     * iterations (then eta_ve moved after the last evaluation) */
+    double lTp = lT, Epp = Ep, Eep = Ee, Elp = El;
+    if (it == 20) {
+        lTp = log(Tii);
+        if (k.pwl) Epp = exp(lnC_pwl + k.n_pwl * lTp);
+        if (peierls) Eep = exp(lnC_exp + nST * lTp);
+        if (k.lin) Elp = exp(lnC_lin + k.n_lin * lTp);
+    }
+    /* inverse viscosities of the creep mechanisms */
+    const double inv_lin = k.lin ? exp(-(lnB_lin + (1.0 / k.n_lin - 1.0) * (lnC_lin + k.n_lin * lT)
+                                         + (k.m_lin != 0.0 ? k.m_lin / k.n_lin * k.ln_gs_ref : 0.0)))
+                                 : 0.0;
+    const double inv_exp = peierls ? exp(-(lnB_exp + (1.0 / nST - 1.0) * (lnC_exp + nST * lT))) : 0.0;
+    double inv_pwl = k.pwl ? exp(LN2 + lnC_pwl - (1.0 - k.n_pwl) * lTp) : 0.0;
+
+    /* tangent of the visco-elastic update (RheologyDensity.c:425-447) */
+    const double Ti3 = Tii * Tii * Tii;
+    double deta = 0.0;
+    if (k.pwl) deta -= Epp / Ti3 * (k.n_pwl - 1.0);
+    if (peierls) deta -= Eep / Ti3 * (nST - 1.0);
+    if (k.lin) deta -= Elp / Ti3 * (k.n_lin - 1.0);
+    const double e2 = eta_ve * eta_ve;
+    double dTxx = deta * (2.0 * Txx + Tzz) * e2, dTzz = deta * (2.0 * Tzz + Txx) * e2, dTxz = 2.0 * deta * Txz * e2;
+    double den = 1.0 - 2.0 * (dTxx * Exx + dTzz * Ezz + dTxz * Exz);
+    const double de_xx = dTxx * 2.0 * eta_ve / den, de_zz = dTzz * 2.0 * eta_ve / den, de_xz = dTxz * 2.0 * eta_ve / den;
+    double dummy = 0.0;
+    if (k.pwl) dummy += 2.0 * Epp / Tii * k.Va_pwl / RT;
+    if (k.lin) dummy += 2.0 * Elp / Tii * (k.m_lin != 0.0 ? exp(k.m_lin * (k.ln_gs_ref - lnd)) : 1.0) * k.Va_lin / RT;
+    const double de_p = e2 * dummy;
+
+    /* plastic return map (RheologyDensity.c:468-531) */
+    const double vp_exp = 1.0 / k.n_vp - 1.0;
+    double F_trial = Tii - Tyield, gdot = 0.0, eta_vp = 0.0, Pc = P, divp = 0.0;
+    double dQdtxx = 0, dQdtzz = 0, dQdtxz = 0;
+    if (tens == 1) {
+        eta_vp = k.eta_vp0 * (vp_exp == 0.0 ? 1.0 : pow(Eii, vp_exp));
+        double Pc_chk = -(C * cos_fric) / (-Tii / P + sin_fric);
+        sin_dil = (P * eta_ve + P * eta_vp - Pc_chk * eta_ve - Pc_chk * eta_vp)
+                / (K * dt * (C * cos_fric + Pc_chk * sin_fric - Tii));
+    }
+    /* the diffusion, Peierls and constant-viscosity parts of the strain-rate
+     * partition use the visco-elastic stresses (before the return map) */
+    const double Txx_ve = Txx, Tzz_ve = Tzz, Txz_ve = Txz;
+    const bool is_pl = F_trial > 1e-17;
+    if (is_pl) {
+        eta_vp = k.eta_vp0 * (vp_exp == 0.0 ? 1.0 : pow(Eii, vp_exp));
+        gdot = F_trial / (eta_ve + eta_vp + K * dt * sin_fric * sin_dil);
+        dQdtxx = Txx / 2.0 / Tii; dQdtzz = Tzz / 2.0 / Tii; dQdtxz = Txz / 1.0 / Tii;
+        const double F0 = F_trial;
+        int itp;
+        for (itp = 0; itp < 20; itp++) {
+            divp = gdot * sin_dil;
+            Pc = P + K * dt * divp;
+            eta_vp = k.eta_vp0 * (vp_exp == 0.0 ? 1.0 : pow(fabs(gdot), vp_exp));
+            Tyield = C * cos_fric + Pc * sin_fric + gdot * eta_vp;
+            double F = Tii - eta_ve * gdot - Tyield;
+            double res = fabs(F);
+            if (res < 1.0e-11 || res / F0 < 1.0e-11) break;
+            gdot -= F / (-eta_ve - eta_vp / k.n_vp - K * dt * sin_fric * sin_dil);
+        }
+        hist.add(1, itp + 1);
+        Txx = 2.0 * eta_ve * (Exx - gdot * dQdtxx);
+        Tzz = 2.0 * eta_ve * (Ezz - gdot * dQdtzz);
+        double Tyy = -(Txx + Tzz);
+        Txz = 2.0 * eta_ve * (Exz - gdot * dQdtxz / 2.0);
+        Tii = sqrt(0.5 * (Txx * Txx + Tzz * Tzz + Tyy * Tyy) + Txz * Txz);
+    } else {
+        hist.add(1, 0);
+    }
+    const double eta_vep = Tii / (2.0 * Eii);
+    if (is_pl) Tii = 2.0 * eta_vep * Eii;
+
+    /* dissipative viscosity: Peierls, dislocation, diffusion, constant,
+     * visco-plastic, in this order */
+    double inv_diss = 0.0;
+    if (peierls) inv_diss += inv_exp;
+    if (k.pwl) inv_diss += inv_pwl;
+    if (k.lin) inv_diss += inv_lin;
+    if (k.cst) inv_diss += 1.0 / k.eta_cst;
+
+    /* consistent tangent and plastic divergence derivatives */
+    double etaVE = eta_ve, dexx = de_xx, dezz = de_zz, dexz = de_xz, dp = de_p;
+    double ddxx = 0, ddzz = 0, ddxz = 0, ddp = 0;
+    if (is_pl) {
+        double dsx = 0, dsz = 0, dsxz = 0, dsp = 0;
+        if (tens == 1) {
+            const double dE[3] = {(2.0 * Exx + Ezz) / 2.0 / Eii, (2.0 * Ezz + Exx) / 2.0 / Eii, Exz / 1.0 / Eii};
+            const double dev[3] = {de_xx, de_zz, de_xz};
+            double a2 = 2 * Eii * eta_ve - P * sin_fric;
+            double bden = K * dt * a2 * (C * P * cos_fric * sin_fric + (C * cos_fric - 2 * Eii * eta_ve) * a2);
+            double q1 = C * P * cos_fric * sin_fric + a2 * a2;
+            double od[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                double dT = Eii * dev[i] + dE[i] * eta_ve;
+                od[i] = (2 * K * dt * sin_dil * dT * q1
+                         + P * (-C * cos_fric * dev[i] * a2 + 2 * C * cos_fric * (eta_ve + eta_vp) * dT + dev[i] * a2 * a2)) / bden;
+            }
+            dsx = od[0]; dsz = od[1]; dsxz = od[2];
+            dsp = (2 * C * Eii * cos_fric * (eta_ve + eta_vp) * (P * de_p - eta_ve) - C * P * cos_fric * de_p * a2
+                   + 2 * Eii * K * dt * sin_dil * (C * cos_fric * sin_fric * (P * de_p - eta_ve) + de_p * a2 * a2)
+                   + a2 * a2 * (P * de_p + eta_ve + eta_vp)) / bden;
+        }
+        const double gg = 1.0 / (eta_ve + eta_vp / k.n_vp + K * dt * sin_fric * sin_dil);
+        const double Kds = K * dt * sin_fric;
+        double lx = gg * (((2.0 * Exx + Ezz) * eta_ve / Eii + 2.0 * Eii * de_xx) - gdot * (de_xx + Kds * dsx));
+        double lz = gg * (((2.0 * Ezz + Exx) * eta_ve / Eii + 2.0 * Eii * de_zz) - gdot * (de_zz + Kds * dsz));
+        double lxz = gg * ((2.0 * Exz * eta_ve / Eii + 2.0 * Eii * de_xz) - gdot * (de_xz + Kds * dsxz));
+        double lp = gg * ((-sin_fric + 2.0 * Eii * de_p) - gdot * (de_p + Kds * dsp));
+        double vx = eta_vp / gdot * lx * vp_exp, vz = eta_vp / gdot * lz * vp_exp;
+        double vxz = eta_vp / gdot * lxz * vp_exp, vpp = eta_vp / gdot * lp * vp_exp;
+        double a = eta_vp + K * dt * sin_fric * sin_dil;
+        double E3 = 2.0 * (Eii * Eii * Eii);
+        dexx = -0.5 * (2.0 * Exx + Ezz) * Tyield / E3 + (a * lx + gdot * vx + gdot * Kds * dsx) / (2.0 * Eii);
+        dezz = -0.5 * (2.0 * Ezz + Exx) * Tyield / E3 + (a * lz + gdot * vz + gdot * Kds * dsz) / (2.0 * Eii);
+        dexz = -Exz * Tyield / E3 + (a * lxz + gdot * vxz + gdot * Kds * dsxz) / (2.0 * Eii);
+        dp = (sin_fric + a * lp + gdot * vpp + gdot * Kds * dsp) / (2.0 * Eii);
+        etaVE = eta_vep;
+        ddxx = sin_dil * lx + gdot * dsx;
+        ddzz = sin_dil * lz + gdot * dsz;
+        ddxz = sin_dil * lxz + gdot * dsxz;
+        ddp = sin_dil * lp + gdot * dsp + 0.0;
+        inv_diss += 1.0 / eta_vep;
+        /* the dislocation viscosity of the strain-rate partition is
+         * evaluated at the post-return stress */
+        if (k.pwl) inv_pwl = exp(LN2 + lnC_pwl - (1.0 - k.n_pwl) * log(Tii));
+    }
+    double eta = 1.0 / inv_diss;
+    const double VE = elastic ? eta_ve / eta_el : 0.0;
+    if (prm.step == 0) etaVE = eta;
+    eta = fmin(fmax(eta, prm.mineta), prm.maxeta);
+    etaVE = fmin(fmax(etaVE, prm.mineta), prm.maxeta);
+
+    /* strain-rate partition (RheologyDensity.c:640-690) */
+    const double ex_el = (double)elastic * (Txx - in.Txx0) / 2.0 / eta_el;
+    const double ez_el = (double)elastic * (Tzz - in.Tzz0) / 2.0 / eta_el;
+    const double exz_el = (double)elastic * (Txz - in.Txz0) / 2.0 / eta_el;
+    const double h_lin = 0.5 * inv_lin, h_pwl = 0.5 * inv_pwl, h_exp = 0.5 * inv_exp;
+    const double h_cst = k.cst ? 0.5 / k.eta_cst : 0.0;
+    const double ex_d = gdot * dQdtxx + Txx_ve * h_lin + Txx * h_pwl + Txx_ve * h_exp + 0.0 + Txx_ve * h_cst;
+    const double ez_d = gdot * dQdtzz + Tzz_ve * h_lin + Tzz * h_pwl + Tzz_ve * h_exp + 0.0 + Tzz_ve * h_cst;
+    const double exz_d = gdot * dQdtxz / 2.0 + Txz_ve * h_lin + Txz * h_pwl + Txz_ve * h_exp + 0.0 + Txz_ve * h_cst;
+
+    if (CELL) {
+        o.put(FI_eta_n, etaVE);
+        o.put(FI_eta_phys_n, eta);
+        o.put(FI_detadexx_n, dexx);
+        o.put(FI_detadezz_n, dezz);
+        o.put(FI_detadgxz_n, dexz / 2.0);
+        o.put(FI_detadp_n, dp);
+        o.put(FI_sxxd, Txx);
+        o.put(FI_szzd, Tzz);
+        o.put(FI_sxz_n, Txz);
+        o.put(FI_VE_n, VE);
+        o.put(FI_eII_el, (double)elastic * fabs(Tii - sqrt(in.Txx0 * in.Txx0 + in.Txz0 * in.Txz0)) / 2.0 / eta_el);
+        o.put(FI_eII_pl, gdot / 2.0);
+        o.put(FI_eII_pwl, Tii * h_pwl);
+        o.put(FI_eII_exp, Ee);
+        o.put(FI_eII_lin, El);
+        o.put(FI_eII_gbs, Eg);
+        o.put(FI_eII_cst, Ec);
+        o.put_inv(FI_d_n, k.gs_ref);
+        o.put(FI_exx_el, ex_el);
+        o.put(FI_exx_diss, ex_d);
+        o.put(FI_ezz_el, ez_el);
+        o.put(FI_ezz_diss, ez_d);
+        o.put(FI_Wtot, Txx * exxd + Tzz * ezzd + (Txx + Tzz) * (exxd + ezzd) + 2.0 * Txz * exzc);
+        o.put(FI_Wdiss, Txx * ex_d + Tzz * ez_d + (Txx + Tzz) * (ex_d + ez_d) + 2.0 * Txz * exz_d);
+        o.put(FI_Wel, Txx * ex_el + Tzz * ez_el + (Txx + Tzz) * (ex_el + ez_el) + 2.0 * Txz * exz_el);
+        o.put(FI_ddivpdexx_n, ddxx);
+        o.put(FI_ddivpdezz_n, ddzz);
+        o.put(FI_ddivpdgxz_n, ddxz / 2.0);
+        o.put(FI_ddivpdp_n, ddp);
+        o.put(FI_p_corr, Pc);
+        o.put(FI_div_u_el, -(Pc - in.P0) / (K * dt));
+        o.put(FI_div_u_pl, divp);
+        o.put(FI_div_u_r, 0.0);
+        if (Xout) { double x = o.fr * in.X0; if (o.first) Xout[o.c] = x; else Xout[o.c] += x; }
+        o.put(FI_OverS_n, is_pl ? eta_vp * gdot : 0.0);
+    } else {
+        o.put(FI_eta_s, etaVE);
+        o.put(FI_eta_phys_s, eta);
+        o.put(FI_detadexx_s, dexx);
+        o.put(FI_detadezz_s, dezz);
+        o.put(FI_detadgxz_s, dexz / 2.0);
+        o.put(FI_detadp_s, dp);
+        o.put(FI_sxz, Txz);
+        o.put(FI_VE_s, VE);
+        o.put(FI_exz_el, exz_el);
+        o.put(FI_exz_diss, exz_d);
+        o.put(FI_OverS_s, is_pl ? eta_vp * gdot : 0.0);
+        if (Xout) { double x = o.fr * in.X0; if (o.first) Xout[o.c] = x; else Xout[o.c] += x; }
+    }
+}
+
+/* sines and cosines of the node's friction and dilatancy angles; dil = 0
+ * when elasticity is off (RheologyDensity.c:160) */
+__device__ __forceinline__ void node_angles(const DevParams &prm, double fric, double dil, NodeIn &in) {
+    const int elastic = prm.step == 0 ? 0 : prm.iselastic;
+    in.sin_fric = sin(fric);
+    in.cos_fric = cos(fric);
+    in.sin_dil = elastic == 0 ? 0.0 : sin(dil);
+}
+
+__device__ __forceinline__ void rheo_cell(const DevGrid &g, const DevParams &prm, const PhaseK *__restrict__ pk,
+                                          double R, const float *__restrict__ guess, const BlockHist &hist, int c0,
+                                          int nc) {
+    const int t = g.BCp[c0];
+    const bool active = t != 30 && t != 31;
+    double *Xout = prm.UnsplitDiffReac == 0 ? g.X_n : nullptr;
+    bool any = false;
+    if (active) {
+        const double etae = prm.iselastic == 1 ? prm.dt * g.mu_n[c0] : 1.0;
+        const double exxd = g.exxd[c0], ezzd = g.ezzd[c0], exzc = g.exz_n[c0];
+        NodeIn in;
+        in.Txx0 = g.sxxd0[c0]; in.Tzz0 = g.szzd0[c0]; in.Txz0 = g.sxz0_n[c0];
+        ve_rates_iso(exxd, ezzd, exzc, in.Txx0, in.Tzz0, in.Txz0, etae, in);
+        in.G = g.mu_n[c0]; in.T = g.T[c0]; in.P = g.p_in[c0]; in.d = g.d0_n[c0]; in.phi = g.phi0_n[c0];
+        in.X0 = g.X0_n[c0]; in.C = g.C_n[c0]; in.P0 = g.p0_n[c0]; in.beta = g.bet_n[c0];
+        node_angles(prm, g.fric_n[c0], g.dil_n[c0], in);
+        for (int p = 0; p < prm.Nb_phases; p++) {
+            const double fr = g.phase_n[(size_t)p * nc + c0];
+            if (!(fabs(fr) > 1.0e-13)) continue;
+            const Out o{g.eta_n, g.cstride, c0, FI_eta_n, fr, !any};
+            const double gs = guess ? (double)guess[(size_t)p * nc + c0] : 0.0;
+            local_update<true>(pk[p], R, prm, in, Xout, o, exxd, ezzd, exzc, gs, hist);
+            any = true;
+        }
+    }
+    double *dn = g.eta_n + (long long)(FI_d_n - FI_eta_n) * g.cstride + c0;
+    if (!any) {
+        /* air cell, or no phase above the fraction threshold */
+        for (int i = 0; i <= FI_OverS_n - FI_eta_n; i++) g.eta_n[(long long)i * g.cstride + c0] = 0.0;
+        if (Xout) Xout[c0] = 0.0;
+        if (active) *dn = 1.0 / 0.0; /* 1 / (empty sum) */
+    } else {
+        *dn = 1.0 / *dn;
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_rheo_cells_fast(DevGrid g, DevParams prm, const PhaseK *__restrict__ pk,
+                                                         double R, const float *__restrict__ guess,
+                                                         unsigned long long *hist) {
+    __shared__ unsigned int lh[3 * MDZ_HIST_BINS];
+    const BlockHist bh = hist_begin(lh, hist);
+    const int c0 = blockIdx.x * BLK + threadIdx.x;
+    const int nc = g.ncx * g.ncz;
+    if (c0 < nc) rheo_cell(g, prm, pk, R, guess, bh, c0, nc);
+    hist_flush(bh, hist);
+}
+
+__device__ __forceinline__ void rheo_vertex(const DevGrid &g, const DevParams &prm, const PhaseK *__restrict__ pk,
+                                            double R, const float *__restrict__ guess, const BlockHist &hist, int c1,
+                                            int nv) {
+    double *Xout = prm.UnsplitDiffReac == 0 ? g.X_s : nullptr;
+    bool any = false;
+    if (g.BCg[c1] != 30) {
+        const double etae = prm.iselastic == 1 ? prm.dt * g.mu_s[c1] : 1.0;
+        NodeIn in;
+        in.Txx0 = g.sxxd0_s[c1]; in.Tzz0 = g.szzd0_s[c1]; in.Txz0 = g.sxz0[c1];
+        ve_rates_iso(g.exxd_s[c1], g.ezzd_s[c1], g.exz[c1], in.Txx0, in.Tzz0, in.Txz0, etae, in);
+        in.G = g.mu_s[c1]; in.T = g.T_s[c1]; in.P = g.P_s[c1]; in.d = g.d0_s[c1]; in.phi = g.phi0_s[c1];
+        in.X0 = g.X0_s[c1]; in.C = g.C_s[c1]; in.P0 = g.p0_s[c1]; in.beta = g.bet_s[c1];
+        node_angles(prm, g.fric_s[c1], g.dil_s[c1], in);
+        for (int p = 0; p < prm.Nb_phases; p++) {
+            const double fr = g.phase_s[(size_t)p * nv + c1];
+            if (!(fabs(fr) > 1.0e-13)) continue;
+            const Out o{g.VE_s, g.vstride, c1, FI_VE_s, fr, !any};
+            const double gs = guess ? (double)guess[(size_t)p * nv + c1] : 0.0;
+            local_update<false>(pk[p], R, prm, in, Xout, o, 0.0, 0.0, 0.0, gs, hist);
+            any = true;
+        }
+    }
+    if (!any) {
+        for (int i = 0; i <= FI_OverS_s - FI_VE_s; i++) g.VE_s[(long long)i * g.vstride + c1] = 0.0;
+        if (Xout) Xout[c1] = 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_rheo_vertices_fast(DevGrid g, DevParams prm, const PhaseK *__restrict__ pk,
+                                                            double R, const float *__restrict__ guess,
+                                                            unsigned long long *hist) {
+    __shared__ unsigned int lh[3 * MDZ_HIST_BINS];
+    const BlockHist bh = hist_begin(lh, hist);
+    const int c1 = blockIdx.x * BLK + threadIdx.x;
+    const int nv = g.Nx * g.Nz;
+    if (c1 < nv) rheo_vertex(g, prm, pk, R, guess, bh, c1, nv);
+    hist_flush(bh, hist);
+}
+
+int mdz_rheo_fast_eligible(const DevParams &prm, const mat_prop *host_mt) {
+    if (prm.eta_avg != 0 || prm.aniso != 0 || prm.Nb_phases > MDZ_MAX_PHASES || prm.VolChangeReac != 0) return 0;
+    for (int p = 0; p < prm.Nb_phases; p++) {
+        if (host_mt->gs[p] != 0 || host_mt->reac_soft[p] == 1) return 0;
+        if (prm.diffuse_X != 0 && host_mt->phase_mix[p] != 0) return 0;
+    }
+    return 1;
+}
+
+size_t mdz_phasek_bytes(void) { return MDZ_MAX_PHASES * sizeof(PhaseK); }
+
+void mdz_launch_phase_constants(const mat_prop *d_mt, int nph, void *d_pk, hipStream_t s) {
+    hipLaunchKernelGGL(k_phase_constants, dim3(1), dim3(64), 0, s, d_mt, nph < MDZ_MAX_PHASES ? nph : MDZ_MAX_PHASES,
+                       (PhaseK *)d_pk);
+}
+
+void mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
+                              const float *guess_s, unsigned long long *hist, hipStream_t s) {
+    int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
+    /* vertex values of div_u, T, P, d and phi first (same as the generic path) */
+    mdz_launch_interp5(g, prm, s);
+    hipLaunchKernelGGL(k_rheo_cells_fast, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, g, prm, (const PhaseK *)d_pk, R,
+                       guess_n, hist);
+    hipLaunchKernelGGL(k_rheo_vertices_fast, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm,
+                       (const PhaseK *)d_pk, R, guess_s, hist);
+}

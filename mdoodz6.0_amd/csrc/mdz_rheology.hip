@@ -145,15 +145,10 @@ __device__ __forceinline__ void ve_rates(const DevParams &prm, double nx, double
     Gxz = Exx * ani * d1 - Exx * ani * d1 + gxz * (ani * (d0 - 0.5) + 0.5);
 }
 
-__device__ __forceinline__ void hist_add(unsigned long long *hist, int bin) {
-    if (hist) atomicAdd(&hist[bin < MDZ_HIST_BINS ? bin : MDZ_HIST_BINS - 1], 1ULL);
-}
 
-__global__ void __launch_bounds__(BLK) k_rheo_cells(DevGrid g, DevParams prm, const mat_prop *__restrict__ mt,
-                                                    const float *__restrict__ guess_n, unsigned long long *hist) {
-    int c0 = blockIdx.x * BLK + threadIdx.x;
-    int nc = g.ncx * g.ncz;
-    if (c0 >= nc) return;
+__device__ __forceinline__ void rheo_cell_generic(const DevGrid &g, const DevParams &prm,
+                                                  const mat_prop *__restrict__ mt, const float *__restrict__ guess_n,
+                                                  const BlockHist &hist, int c0, int nc) {
     const int average = prm.eta_avg;
     double eta_n = 0, eta_phys_n = 0, VE_n = 0, sxxd = 0, szzd = 0, sxz_n = 0;
     double eII_el = 0, eII_pl = 0, eII_pwl = 0, eII_exp = 0, eII_lin = 0, eII_gbs = 0, eII_cst = 0, d_n = 0;
@@ -180,9 +175,9 @@ __global__ void __launch_bounds__(BLK) k_rheo_cells(DevGrid g, DevParams prm, co
             VOut o;
             mdz_viscosity(p, mu, T, P, d0, phi0, X0, Exx, Ezz, Exz, Gxx, Gzz, Gxz, s0xx, s0zz, s0xz, mt, prm, dil,
                           fric, C, P0, bet, gs, o);
-            hist_add(hist, o.it_ve & 0xff);
-            hist_add(hist ? hist + MDZ_HIST_BINS : nullptr, o.it_pl);
-            hist_add(hist ? hist + 2 * MDZ_HIST_BINS : nullptr, o.it_ve >> 8);
+            hist.add(0, o.it_ve & 0xff);
+            hist.add(1, o.it_pl);
+            hist.add(2, o.it_ve >> 8);
             if (average == 0) {
                 eta_n += fr * o.etaVE;
                 eta_phys_n += fr * o.eta;
@@ -269,11 +264,9 @@ __global__ void __launch_bounds__(BLK) k_rheo_cells(DevGrid g, DevParams prm, co
     g.OverS_n[c0] = OverS;
 }
 
-__global__ void __launch_bounds__(BLK) k_rheo_vertices(DevGrid g, DevParams prm, const mat_prop *__restrict__ mt,
-                                                       const float *__restrict__ guess_s, unsigned long long *hist) {
-    int c1 = blockIdx.x * BLK + threadIdx.x;
-    int nv = g.Nx * g.Nz;
-    if (c1 >= nv) return;
+__device__ __forceinline__ void rheo_vertex_generic(const DevGrid &g, const DevParams &prm,
+                                                    const mat_prop *__restrict__ mt, const float *__restrict__ guess_s,
+                                                    const BlockHist &hist, int c1, int nv) {
     const int average = prm.eta_avg;
     double VE_s = 0, sxz = 0, eta_phys_s = 0, eta_s = 0, exz_el = 0, exz_diss = 0;
     double dexx = 0, dezz = 0, dgxz = 0, dp = 0, X_s = 0, OverS = 0;
@@ -294,9 +287,9 @@ __global__ void __launch_bounds__(BLK) k_rheo_vertices(DevGrid g, DevParams prm,
             VOut o;
             mdz_viscosity(p, mu, T, P, d0, phi0, X0, Exx, Ezz, Exz, Gxx, Gzz, Gxz, s0xx, s0zz, s0xz, mt, prm, dil,
                           fric, C, P0, bet, gs, o);
-            hist_add(hist, o.it_ve & 0xff);
-            hist_add(hist ? hist + MDZ_HIST_BINS : nullptr, o.it_pl);
-            hist_add(hist ? hist + 2 * MDZ_HIST_BINS : nullptr, o.it_ve >> 8);
+            hist.add(0, o.it_ve & 0xff);
+            hist.add(1, o.it_pl);
+            hist.add(2, o.it_ve >> 8);
             if (average == 0) {
                 eta_s += fr * o.etaVE;
                 eta_phys_s += fr * o.eta;
@@ -345,6 +338,26 @@ __global__ void __launch_bounds__(BLK) k_rheo_vertices(DevGrid g, DevParams prm,
     g.detadexx_s[c1] = dexx; g.detadezz_s[c1] = dezz; g.detadgxz_s[c1] = dgxz; g.detadp_s[c1] = dp;
     if (prm.UnsplitDiffReac == 0) g.X_s[c1] = X_s;
     g.OverS_s[c1] = OverS;
+}
+
+__global__ void __launch_bounds__(BLK) k_rheo_cells(DevGrid g, DevParams prm, const mat_prop *__restrict__ mt,
+                                                    const float *__restrict__ guess_n, unsigned long long *hist) {
+    __shared__ unsigned int lh[3 * MDZ_HIST_BINS];
+    const BlockHist bh = hist_begin(lh, hist);
+    const int c0 = blockIdx.x * BLK + threadIdx.x;
+    const int nc = g.ncx * g.ncz;
+    if (c0 < nc) rheo_cell_generic(g, prm, mt, guess_n, bh, c0, nc);
+    hist_flush(bh, hist);
+}
+
+__global__ void __launch_bounds__(BLK) k_rheo_vertices(DevGrid g, DevParams prm, const mat_prop *__restrict__ mt,
+                                                       const float *__restrict__ guess_s, unsigned long long *hist) {
+    __shared__ unsigned int lh[3 * MDZ_HIST_BINS];
+    const BlockHist bh = hist_begin(lh, hist);
+    const int c1 = blockIdx.x * BLK + threadIdx.x;
+    const int nv = g.Nx * g.Nz;
+    if (c1 < nv) rheo_vertex_generic(g, prm, mt, guess_s, bh, c1, nv);
+    hist_flush(bh, hist);
 }
 
 /* ---------------------------------------------------------------- */
@@ -539,12 +552,17 @@ void mdz_launch_interp(const DevGrid &g, const DevParams &prm, const double *src
                        (const double *)nullptr, (double *)nullptr, (const double *)nullptr, (double *)nullptr);
 }
 
-void mdz_launch_rheology(const DevGrid &g, const DevParams &prm, const mat_prop *mt, const float *guess_n,
-                         const float *guess_s, unsigned long long *hist, hipStream_t s) {
-    int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
+void mdz_launch_interp5(const DevGrid &g, const DevParams &prm, hipStream_t s) {
+    int nv = g.Nx * g.Nz;
     hipLaunchKernelGGL(k_interp_c2v, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm.isperiodic_x, 5,
                        (const double *)g.div_u, g.div_u_s, (const double *)g.T, g.T_s, (const double *)g.p_in, g.P_s,
                        (const double *)g.d0_n, g.d0_s, (const double *)g.phi0_n, g.phi0_s);
+}
+
+void mdz_launch_rheology(const DevGrid &g, const DevParams &prm, const mat_prop *mt, const float *guess_n,
+                         const float *guess_s, unsigned long long *hist, hipStream_t s) {
+    int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
+    mdz_launch_interp5(g, prm, s);
     hipLaunchKernelGGL(k_rheo_cells, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, g, prm, mt, guess_n, hist);
     hipLaunchKernelGGL(k_rheo_vertices, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm, mt, guess_s, hist);
 }

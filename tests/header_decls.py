@@ -1,0 +1,62 @@
+"""Function names and struct layouts declared in a C header, read from its
+text (comments, preprocessor lines and struct/enum bodies skipped)."""
+import re
+
+
+def _strip(text):
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    text = "\n".join(l for l in text.splitlines() if not l.strip().startswith("#"))
+    return text.replace('extern "C" {', "")
+
+
+def function_names(path):
+    """Names of the functions a header declares (prototypes at file scope)."""
+    text = _strip(open(path).read())
+    depth, out = 0, []
+    for ch in text:
+        if ch == "{":
+            depth += 1
+            out.append(";")
+        elif ch == "}":
+            depth = max(depth - 1, 0)
+        elif depth == 0:
+            out.append(ch)
+    names = []
+    for st in "".join(out).split(";"):
+        st = " ".join(st.split())
+        if "(" not in st or st.startswith("typedef"):
+            continue
+        m = re.match(r"^[\w\s\*]+?\b(\w+)\s*\(", st)
+        if m:
+            names.append(m.group(1))
+    return names
+
+
+def struct_fields(text, name):
+    """[(type, field, array_len)] of `typedef struct ... { ... } name;` or
+    `struct name { ... };` in header text, in declaration order."""
+    text = _strip(text)
+    m = None
+    tags = re.findall(r"typedef\s+struct\s+(\w+)\s+" + name + r"\s*;", text) + ["_" + name, name]
+    for tag in tags:
+        m = re.search(r"struct\s+" + tag + r"\s*\{(.*?)\}\s*;", text, flags=re.S)
+        if m:
+            break
+    if not m:
+        raise KeyError(name)
+    fields = []
+    for decl in m.group(1).split(";"):
+        decl = " ".join(decl.split())
+        if not decl:
+            continue
+        dm = re.match(r"^((?:const\s+|unsigned\s+|struct\s+)*\w+)\s+(.*)$", decl)
+        base, rest = dm.group(1), dm.group(2)
+        for item in rest.split(","):
+            item = item.strip()
+            stars = item.count("*")
+            item = item.replace("*", "").strip()
+            am = re.match(r"^(\w+)\s*(?:\[(\w+)\])?$", item)
+            base_n = "double" if base == "DoodzFP" else base
+            fields.append((base_n + "*" * stars, am.group(1), am.group(2)))
+    return fields

@@ -1,0 +1,83 @@
+"""The drop-in boundary, checked without a GPU: the C-ABI library loads and
+exports every function the public header declares, the header's structs
+match the reference's header_MDOODZ.h field for field (golden list written
+by tests/golden/gen_reference_fixtures.py), the Python mirror of the structs
+has the C layout, the product path refuses to run without the HIP library,
+and the product library carries no oracle code."""
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+import header_decls
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mdoodz_hot.h")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+PKG = os.path.join(ROOT, "mdoodz6.0_amd")
+
+
+def test_header_declares_the_reference_entry_points():
+    names = set(header_decls.function_names(HEADER))
+    ref = {"Viscosity", "NonNewtonianViscosityGrid", "StrainRateComponents", "RheologicalOperators", "EvaluateRHS",
+           "UpdateNonLinearity", "InterpCentroidsToVerticesDouble", "EvalNumberOfEquations",
+           "BuildStokesOperatorDecoupled", "BuildJacobianOperatorDecoupled", "EvaluateStokesResidualDecoupled",
+           "AddCoeff2"}
+    assert ref <= names, ref - names
+
+
+def test_library_exports_every_declared_symbol(mdz):
+    lib = mdz.lib_hot()  # loading needs no GPU; nothing is called
+    missing = [n for n in header_decls.function_names(HEADER) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(mdz.HOT_SYMBOLS) == set(header_decls.function_names(HEADER))
+
+
+@pytest.mark.parametrize("struct", ["scale", "mat_prop", "BC", "BCT", "SparseMat", "params", "Nparams", "grid"])
+def test_abi_structs_match_reference_header(struct):
+    golden = json.load(open(os.path.join(GOLDEN, "abi_structs.json")))
+    ours = header_decls.struct_fields(open(HEADER).read(), struct)
+    assert [list(f) for f in ours] == golden[struct]
+
+
+@pytest.mark.parametrize("struct", ["scale", "mat_prop", "BC", "SparseMat", "params", "Nparams", "grid"])
+def test_python_mirror_has_c_layout(mdz, struct):
+    L = mdz.lib_model()
+    py = getattr(mdz, struct)
+    assert C.sizeof(py) == L.mdz_sizeof(struct.encode())
+    for name, _ in py._fields_:
+        assert getattr(py, name).offset == L.mdz_offsetof(struct.encode(), name.encode()), name
+
+
+def test_no_cpu_fallback(mdz, monkeypatch, tmp_path):
+    monkeypatch.setattr(mdz, "LIB_HOT", str(tmp_path / "absent.so"))
+    saved = mdz._libs.pop("hot", None)
+    try:
+        with pytest.raises(RuntimeError, match="no CPU fallback"):
+            mdz.lib_hot()
+    finally:
+        if saved is not None:
+            mdz._libs["hot"] = saved
+
+
+def _dynamic_symbols(path):
+    out = subprocess.run(["nm", "-D", path], check=True, capture_output=True, text=True).stdout
+    return [line.split()[-1] for line in out.splitlines() if line.strip()]
+
+
+def test_product_library_has_no_oracle_code():
+    so = os.path.join(PKG, "libmdz_hot.so")
+    syms = _dynamic_symbols(so)
+    assert not [s for s in syms if s.startswith("orc_")]
+    needed = subprocess.run(["readelf", "-d", so], check=True, capture_output=True, text=True).stdout
+    assert "liboracle" not in needed
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stdout
