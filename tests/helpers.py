@@ -71,3 +71,63 @@ def oracle_rheology_inputs(orc, m):
 def oracle_update(orc, m):
     mm = m.m
     orc.orc_UpdateNonLinearity(mm.mesh, mm.materials, C.byref(mm.model), C.byref(mm.Nmodel), mm.scaling)
+
+
+# ---------------------------------------------------------------------------
+# pointwise Viscosity on the oracle
+# ---------------------------------------------------------------------------
+_VOUT1 = ["txx", "tzz", "txz", "etaVE", "VE", "Eii_el", "Eii_pl", "Eii_pwl", "Eii_exp", "Eii_lin", "Eii_gbs",
+          "Eii_cst", "Exx_el", "Ezz_el", "Exz_el", "Exx_diss", "Ezz_diss", "Exz_diss", "d1"]
+_VOUT2 = ["detadexx", "detadezz", "detadexz", "detadp"]
+_VOUT3 = ["X1", "OverS", "ddivpdexx", "ddivpdezz", "ddivpdexz", "ddivpdp", "Pcorr", "drhodp", "rho"]
+_VOUT4 = ["div_el", "div_pl", "div_r"]
+
+
+def point_viscosity(orc, m, phase, G, T, P, Exx, Ezz, Exz, d=2e-3, phi=0.0, X0=0.0, Txx0=0.0, Tzz0=0.0, Txz0=0.0,
+                    dil=0.0, fric=0.5, C_coh=1e30, P0=0.0, beta=1e-3, Gxx=None, Gzz=None, Gxz=None):
+    """orc_Viscosity for one node and phase of model m (its materials, params
+    and scales); returns all outputs by name plus 'eta', 'it_ve', 'it_pl'.
+    Isotropic by default: G = E."""
+    if Gxx is None:
+        Gxx, Gzz, Gxz = Exx, Ezz, Exz
+    o = {n: C.c_double(0.0) for n in _VOUT1 + _VOUT2 + _VOUT3 + _VOUT4}
+    mm = m.m
+    eta = orc.orc_Viscosity(phase, G, T, P, d, phi, X0, Exx, Ezz, Exz, Gxx, Gzz, Gxz, Txx0, Tzz0, Txz0,
+                            C.byref(mm.materials), C.byref(mm.model), C.byref(mm.scaling),
+                            *[C.byref(o[n]) for n in _VOUT1], 0.0, dil, fric, C_coh,
+                            *[C.byref(o[n]) for n in _VOUT2], P0, *[C.byref(o[n]) for n in _VOUT3], beta, 0.0,
+                            *[C.byref(o[n]) for n in _VOUT4])
+    r = {n: v.value for n, v in o.items()}
+    r["eta"] = eta
+    ive, ipl = C.c_int(), C.c_int()
+    orc.orc_last_iterations(C.byref(ive), C.byref(ipl))
+    r["it_ve"], r["it_pl"] = ive.value, ipl.value
+    return r
+
+
+def set_single_law(m, phase, cst=0, pwl=0, lin=0, exp=0):
+    """Switches the creep mechanisms of one phase (flags as in mat_prop)."""
+    mt = m.m.materials
+    mt.cstv[phase], mt.pwlv[phase], mt.linv[phase], mt.expv[phase], mt.gbsv[phase] = cst, pwl, lin, exp, 0
+
+
+def csr_matrix(mdz, block, ncols):
+    import scipy.sparse as sp
+    Ic, J, A, _, _ = mdz.csr_arrays(block)
+    return sp.csr_matrix((A, J, Ic), shape=(len(Ic) - 1, ncols))
+
+
+def newton_unit(orc, mdz, m, S):
+    """The oracle's Newton-iteration unit on model m; returns the 8 blocks
+    (Picard A, B, C, D, Jacobian A, B, C, D)."""
+    bo = blocks(orc, mdz, S, 8)
+    mm = m.m
+    orc.orc_newton_unit(mm.mesh, C.byref(mm.materials), C.byref(mm.model), C.byref(mm.Nmodel), C.byref(mm.scaling),
+                        C.byref(S), *[C.byref(b) for b in bo])
+    return bo
+
+
+def free_blocks(orc, bl):
+    for b in bl:
+        orc.orc_free_block(C.byref(b))
+        orc.orc_sfree(C.byref(b))

@@ -1,0 +1,555 @@
+"""Pins of the oracle (the C/OpenMP restatement under oracle/) that do not
+restate its formulas: closed forms, invariants, finite differences, brute
+force, and the reference's own data (flow-law table, exported matrices).
+CPU only; every GPU parity test compares against this oracle.
+
+Rheology (RheologyDensity.c:44-707, :961-1415):
+  constant viscosity; power-law scaling in Eii, T and P; Maxwell
+  visco-elasticity; Drucker-Prager stress on the yield surface; the local
+  residual at the returned root; bisection of the visco-elastic residual;
+  the consistent tangent against central differences; the flow-law table.
+Kinematics (RheologyDensity.c:1902-1982, GridRoutines.c:106-196):
+  exact strain rates of linear velocity fields; exact interpolation of
+  constant and linear fields.
+Assembly and residual (StokesAssemblyDecoupled.c, StokesRoutines.c:839-1128):
+  uniform viscosity + pure shear gives F = 0; hydrostatic pressure balances
+  gravity; A annihilates constant velocities on interior rows; B = -C^T;
+  Picard A symmetric with pure-shear walls; Jacobian = Picard for a linear
+  rheology; Jacobian . dx against central differences of the residual; the
+  non-zero pattern of the reference's exported 50 x 50 Jacobian
+  (MATLAB_visualisation/Jacobian.mat) and its row counts; equation counts.
+"""
+import ctypes as C
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from helpers import (blocks, csr_matrix, free_blocks, newton_unit, numbering, point_viscosity, set_single_law)
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def tii(r):
+    txx, tzz, txz = r["txx"], r["tzz"], r["txz"]
+    tyy = -(txx + tzz)
+    return math.sqrt(0.5 * (txx * txx + tzz * tzz + tyy * tyy) + txz * txz)
+
+
+def eii(exx, ezz, exz):
+    eyy = -(exx + ezz)
+    return math.sqrt(0.5 * (exx * exx + ezz * ezz + eyy * eyy) + exz * exz)
+
+
+@pytest.fixture
+def inc(mdz):
+    m = mdz.Model.inclusion(16, 16, 1)
+    yield m
+    m.free()
+
+
+@pytest.fixture
+def litho(mdz):
+    m = mdz.Model.lithosphere(40, 20, 2)
+    yield m
+    m.free()
+
+
+# ---------------------------------------------------------------------------
+# rheology
+# ---------------------------------------------------------------------------
+def test_constant_viscosity_elastic_off(orc, inc):
+    """(i) constant viscosity with elasticity off (step 0): eta = eta0 and
+    tau = 2 eta0 E."""
+    inc.m.model.step = 0
+    eta0 = inc.m.materials.eta0[0]
+    E = (0.3, -0.2, 0.05)
+    r = point_viscosity(orc, inc, 0, 1.0, 1.0, 0.0, *E)
+    assert r["eta"] == pytest.approx(eta0, rel=1e-15)
+    assert r["etaVE"] == pytest.approx(eta0, rel=1e-15)
+    assert r["txx"] == pytest.approx(2 * eta0 * E[0], rel=1e-14)
+    assert r["txz"] == pytest.approx(2 * eta0 * E[2], rel=1e-14)
+
+
+def _power_law_phase(m, n=3.0, A=2.0, Q=0.0, V=0.0):
+    mt = m.m.materials
+    set_single_law(m, 0, pwl=1)
+    mt.npwl[0], mt.Apwl[0], mt.Qpwl[0], mt.Vpwl[0] = n, A, Q, V
+    mt.mpwl[0], mt.rpwl[0], mt.apwl[0], mt.fpwl[0], mt.Fpwl[0], mt.pref_pwl[0] = 0.0, 0.0, 0.0, 0.0, 1.0, 1.0
+
+
+def test_power_law_scalings(orc, inc):
+    """(ii) single power law, elasticity off: eta ~ Eii^(1/n-1),
+    eta(T1)/eta(T2) = exp(Q/(nR) (1/T1 - 1/T2)), eta(P1)/eta(P2) =
+    exp(V (P1-P2)/(nRT)); all strain rate is dislocation creep; and the
+    creep law in the invariant form Eii = A (Tii/2F)^n exp(-(Q+PV)/RT)
+    (here F = 1) holds at the root."""
+    n, A, Q, V = 3.0, 2.0, 3000.0, 700.0
+    _power_law_phase(inc, n, A, Q, V)
+    inc.m.model.iselastic = 0
+    R = inc.m.materials.R
+    T0, T1 = 400.0, 600.0  # above the 273.15 floor of the temperature
+    base = dict(G=1.0, T=T0, P=0.3, Exx=0.4, Ezz=-0.1, Exz=0.2)
+    r0 = point_viscosity(orc, inc, 0, **base)
+    E0 = eii(0.4, -0.1, 0.2)
+    r2 = point_viscosity(orc, inc, 0, **dict(base, Exx=0.8, Ezz=-0.2, Exz=0.4))
+    assert r2["eta"] / r0["eta"] == pytest.approx(2.0 ** (1 / n - 1), rel=1e-12)
+    rT = point_viscosity(orc, inc, 0, **dict(base, T=T1))
+    assert r0["eta"] / rT["eta"] == pytest.approx(math.exp((Q + 0.3 * V) / (n * R) * (1 / T0 - 1 / T1)), rel=1e-12)
+    assert r0["eta"] / rT["eta"] > 1.01
+    rP = point_viscosity(orc, inc, 0, **dict(base, P=0.9))
+    assert rP["eta"] / r0["eta"] == pytest.approx(math.exp(V * 0.6 / (n * R * T0)), rel=1e-12)
+    assert rP["eta"] / r0["eta"] > 1.01
+    assert r0["Eii_pwl"] == pytest.approx(E0, rel=1e-12)
+    assert E0 == pytest.approx(A * (tii(r0) / 2) ** n * math.exp(-(Q + 0.3 * V) / (R * T0)), rel=1e-12)
+
+
+def test_maxwell_visco_elasticity(orc, inc):
+    """(iii) linear Maxwell body, no old stress: eta_ve = (1/eta + 1/(G dt))^-1."""
+    G, dt, eta0 = 2.5, inc.m.model.dt, inc.m.materials.eta0[0]
+    r = point_viscosity(orc, inc, 0, G, 1.0, 0.0, 1e-3, -1e-3, 0.0)
+    assert r["it_pl"] == 0
+    assert r["etaVE"] == pytest.approx(1.0 / (1.0 / eta0 + 1.0 / (G * dt)), rel=1e-13)
+    assert r["VE"] == pytest.approx(r["etaVE"] / (G * dt), rel=1e-13)
+
+
+def test_drucker_prager_stress_on_yield(orc, inc):
+    """(iv) Drucker-Prager, eta_vp = 0, psi = 0: after the return map the
+    stress invariant sits on C cos(phi) + P sin(phi)."""
+    inc.m.materials.eta_vp[0] = 0.0
+    fric, coh, P = 0.5, 0.2, 0.4
+    r = point_viscosity(orc, inc, 0, 1.0, 1.0, P, 5e3, -5e3, 1e3, fric=fric, C_coh=coh)
+    assert r["it_pl"] > 0
+    assert tii(r) == pytest.approx(coh * math.cos(fric) + P * math.sin(fric), rel=1e-10)
+    assert r["Pcorr"] == pytest.approx(P, rel=1e-15)  # no dilatancy
+
+
+def _composite(m):
+    mt = m.m.materials
+    set_single_law(m, 0, cst=1, pwl=1, lin=1)
+    mt.npwl[0], mt.Apwl[0], mt.Qpwl[0], mt.Vpwl[0] = 3.5, 0.8, 0.0, 0.0
+    mt.mpwl[0], mt.rpwl[0], mt.apwl[0], mt.fpwl[0], mt.Fpwl[0], mt.pref_pwl[0] = 0.0, 0.0, 0.0, 0.0, 1.0, 1.0
+    mt.nlin[0], mt.Alin[0], mt.Qlin[0], mt.Vlin[0], mt.mlin[0] = 1.0, 0.3, 0.0, 0.0, 0.0
+    mt.rlin[0], mt.alin[0], mt.flin[0], mt.Flin[0] = 0.0, 0.0, 0.0, 1.0
+    mt.eta0[0] = 5.0
+
+
+def test_local_residual_at_root(orc, inc):
+    """(v) at the returned eta_ve the strain-rate partition closes:
+    Eii = Eii_el + Eii_cst + Eii_pwl + Eii_lin to the local tolerance."""
+    _composite(inc)
+    for E in (1e-2, 0.3, 7.0):
+        Exx, Ezz, Exz = E, -0.6 * E, 0.4 * E
+        r = point_viscosity(orc, inc, 0, 1.0, 1.0, 0.0, Exx, Ezz, Exz)
+        assert r["it_pl"] == 0
+        parts = r["Eii_el"] + r["Eii_cst"] + r["Eii_pwl"] + r["Eii_lin"]
+        assert parts == pytest.approx(eii(Exx, Ezz, Exz), rel=1e-12)
+
+
+def test_visco_elastic_root_by_bisection(orc, inc):
+    """(vii) brute force: bisection of Eii - Tii/(2 G dt) - Tii/(2 eta0)
+    - A (Tii/2)^n - Alin (Tii/2) over Tii, the creep laws in invariant form
+    (F = 1, no activation terms); the oracle's stress invariant is that root."""
+    _composite(inc)
+    G, dt, mt = 1.0, inc.m.model.dt, inc.m.materials
+    n, A, Al, eta0 = mt.npwl[0], mt.Apwl[0], mt.Alin[0], mt.eta0[0]
+    for E in (1e-2, 0.3, 7.0):
+        Exx, Ezz, Exz = E, -0.6 * E, 0.4 * E
+        Ei = eii(Exx, Ezz, Exz)
+        f = lambda t: Ei - t / (2 * G * dt) - t / (2 * eta0) - A * (t / 2) ** n - Al * (t / 2)
+        lo, hi = 0.0, 2 * G * dt * Ei
+        for _ in range(200):
+            mid = 0.5 * (lo + hi)
+            lo, hi = (mid, hi) if f(mid) > 0 else (lo, mid)
+        r = point_viscosity(orc, inc, 0, G, 1.0, 0.0, Exx, Ezz, Exz)
+        assert tii(r) == pytest.approx(0.5 * (lo + hi), rel=1e-12)
+
+
+def _fd_tangent(orc, m, phase, args, key_sets, h_rel=1e-4):
+    out = {}
+    for comp, keys in key_sets.items():
+        h = h_rel * max(abs(args[keys[0]]), 1e-3 * abs(args["Exx"]))
+        vals = []
+        for sg in (1, -1):
+            a = dict(args)
+            for k in keys:
+                a[k] = args[k] + sg * h
+            vals.append(point_viscosity(orc, m, phase, **a)["etaVE"])
+        out[comp] = (vals[0] - vals[1]) / (2 * h)
+    return out
+
+
+STRAIN_KEYS = {"detadexx": ("Exx", "Gxx"), "detadezz": ("Ezz", "Gzz"), "detadexz": ("Exz", "Gxz")}
+
+
+@pytest.mark.parametrize("layer,phase,E", [(18, 0, 1e-3), (18, 0, 0.1), (18, 0, 3.0), (16, 1, 0.1), (16, 1, 3.0),
+                                           (12, 2, 1e-3), (12, 2, 0.1), (5, 2, 3.0)])
+def test_tangent_against_finite_differences(orc, litho, layer, phase, E):
+    """(vi) d(eta)/d(Exx, Ezz, Exz) of the visco-elastic and visco-elasto-
+    plastic updates (RheologyDensity.c:425-447, :538-572) against central
+    differences of the oracle itself; d(eta)/dP where the creep laws carry
+    no activation volume (crust phases)."""
+    c = 20 + layer * (litho.Nx - 1)
+    args = dict(G=litho.field("mu_n")[c], T=litho.field("T")[c], P=litho.field("p_in")[c], d=litho.field("d0_n")[c],
+                Exx=E, Ezz=-0.7 * E, Exz=0.3 * E, Gxx=E, Gzz=-0.7 * E, Gxz=0.3 * E, Txx0=0.01, Tzz0=-0.01,
+                Txz0=0.003, fric=math.radians(30.0), C_coh=litho.field("C_n")[c])
+    r = point_viscosity(orc, litho, phase, **args)
+    keys = dict(STRAIN_KEYS)
+    if litho.m.materials.Vpwl[phase] == 0.0:
+        keys["detadp"] = ("P",)
+    fd = _fd_tangent(orc, litho, phase, args, keys)
+    for comp, v in fd.items():
+        assert r[comp] == pytest.approx(v, rel=1e-5, abs=1e-9 * abs(r["etaVE"])), comp
+
+
+def test_pressure_tangent_with_activation_volume_keeps_reference_form(orc, litho):
+    """With an activation volume the reference's d(eta)/dP
+    (RheologyDensity.c:436-443: eta^2 * 2 Tii^(n-1) C V/(RT)) is not the
+    derivative of the implicit visco-elastic root; the restatement keeps
+    that form (pinned here by its departure from the finite difference)."""
+    c = 20 + 12 * (litho.Nx - 1)
+    args = dict(G=litho.field("mu_n")[c], T=litho.field("T")[c], P=litho.field("p_in")[c], d=litho.field("d0_n")[c],
+                Exx=0.1, Ezz=-0.07, Exz=0.03, Txx0=0.01, Tzz0=-0.01, Txz0=0.003, fric=0.5, C_coh=1e10)
+    assert litho.m.materials.Vpwl[2] > 0.0
+    r = point_viscosity(orc, litho, 2, **args)
+    fd = _fd_tangent(orc, litho, 2, args, {"detadp": ("P",)})["detadp"]
+    assert r["detadp"] / fd > 1.5
+
+
+def test_flow_law_table_matches_reference(mdz, litho):
+    """(viii) the synthetic models' creep parameters, back in SI units, are
+    the FlowLaws.c table entries (golden data extracted from the reference
+    source); the invariant correction is the axial-compression factor
+    F = 2^(1/n) 3^((n-1)/2n) / 6."""
+    g = json.load(open(os.path.join(GOLDEN, "flow_laws.json")))
+    mt, s = litho.m.materials, litho.m.scaling
+    seen = 0
+    for k in range(litho.m.model.Nb_phases):
+        if mt.pwlv[k]:
+            ref = g["pwl"][str(mt.pwlv[k])]
+            n = mt.npwl[k]
+            assert n == ref["npwl"]
+            assert mt.Qpwl[k] * s.J == pytest.approx(ref["Qpwl"], rel=1e-14)
+            assert mt.Vpwl[k] * s.L ** 3 == pytest.approx(ref["Vpwl"], rel=1e-14, abs=1e-30)
+            assert mt.Apwl[k] * (s.S ** -n / s.t) == pytest.approx(ref["Apwl"], rel=1e-13)
+            assert mt.Fpwl[k] == pytest.approx(2 ** (1 / n) * 3 ** ((n - 1) / (2 * n)) / 6, rel=1e-15)
+            seen += 1
+        if mt.expv[k]:
+            ref = g["exp"][str(mt.expv[k])]
+            assert mt.Sexp[k] * s.S == pytest.approx(ref["Sexp"], rel=1e-14)
+            assert mt.Qexp[k] * s.J == pytest.approx(ref["Qexp"], rel=1e-14)
+            assert mt.Eexp[k] * s.E == pytest.approx(ref["Eexp"], rel=1e-14)
+            assert (mt.Gexp[k], mt.nexp[k], mt.qexp[k]) == (ref["Gexp"], ref["nexp"], ref["qexp"])
+            seen += 1
+    assert seen == 4  # wet quartzite, plagioclase, olivine dislocation + Peierls
+
+
+# ---------------------------------------------------------------------------
+# kinematics
+# ---------------------------------------------------------------------------
+def _coords(m):
+    g = m.m.mesh.contents
+    Nx, Nz = m.Nx, m.Nz
+    arr = lambda p, n: np.ctypeslib.as_array(p, shape=(n,)).copy()
+    return arr(g.xg_coord, Nx), arr(g.zg_coord, Nz), arr(g.xvz_coord, Nx + 1), arr(g.zvx_coord, Nz + 1)
+
+
+def test_strain_rates_of_linear_field(mdz, orc, inc):
+    """u = a x + b z, v = c x - a z: exx = a, ezz = -a, div = 0,
+    exz = (b + c)/2 at every interior node."""
+    a, b, c = 0.7, -0.3, 1.1
+    xg, zg, xvz, zvx = _coords(inc)
+    Nx, Nz = inc.Nx, inc.Nz
+    inc.field("u_in")[:] = (a * xg[None, :] + b * zvx[:, None]).ravel()
+    inc.field("v_in")[:] = (c * xvz[None, :] - a * zg[:, None]).ravel()
+    orc.orc_StrainRateComponents(inc.m.mesh, inc.m.scaling, C.byref(inc.m.model))
+    assert np.allclose(inc.field("exxd"), a, rtol=1e-12, atol=0)
+    assert np.allclose(inc.field("ezzd"), -a, rtol=1e-12, atol=0)
+    assert np.allclose(inc.field("div_u"), 0.0, atol=1e-13)
+    exz = inc.field("exz").reshape(Nz, Nx)
+    assert np.allclose(exz[1:-1, 1:-1], 0.5 * (b + c), rtol=1e-12)
+    exz_n = inc.field("exz_n").reshape(Nz - 1, Nx - 1)
+    assert np.allclose(exz_n[1:-1, 1:-1], 0.5 * (b + c), rtol=1e-12)
+
+
+def test_interpolation_of_constant_and_linear_fields(mdz, orc, inc):
+    """The 4-point mean reproduces constants everywhere and linear fields on
+    interior vertices (GridRoutines.c:106-196)."""
+    Nx, Nz = inc.Nx, inc.Nz
+    g = inc.m.mesh.contents
+    xc = np.ctypeslib.as_array(g.xc_coord, shape=(Nx - 1,))
+    zc = np.ctypeslib.as_array(g.zc_coord, shape=(Nz - 1,))
+    xg, zg, _, _ = _coords(inc)
+    dp = C.POINTER(C.c_double)
+    dst = np.zeros(Nx * Nz)
+    src = np.full((Nx - 1) * (Nz - 1), 3.25)
+    orc.orc_InterpCentroidsToVerticesDouble(src.ctypes.data_as(dp), dst.ctypes.data_as(dp), inc.m.mesh, C.byref(inc.m.model))
+    assert np.all(dst == 3.25)
+    src = (2.0 * xc[None, :] - 0.5 * zc[:, None]).ravel()
+    orc.orc_InterpCentroidsToVerticesDouble(src.ctypes.data_as(dp), dst.ctypes.data_as(dp), inc.m.mesh, C.byref(inc.m.model))
+    exact = 2.0 * xg[None, :] - 0.5 * zg[:, None]
+    assert np.allclose(dst.reshape(Nz, Nx)[1:-1, 1:-1], exact[1:-1, 1:-1], rtol=0, atol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# assembly and residual
+# ---------------------------------------------------------------------------
+def _unit(orc, mdz, m):
+    S = numbering(orc, mdz, m)
+    bl = newton_unit(orc, mdz, m, S)
+    return S, bl
+
+
+def _F(S, bl):
+    Fm = np.ctypeslib.as_array(bl[0].F, shape=(S.neq_mom,)).copy()
+    Fc = np.ctypeslib.as_array(bl[3].F, shape=(S.neq_cont,)).copy()
+    return Fm, Fc
+
+
+def _exact_pure_shear(m, eta=1.0):
+    """Uniform viscosity, no old stress, no gravity, exact pure shear."""
+    mt = m.m.materials
+    for k in range(m.m.model.Nb_phases):
+        mt.eta0[k] = eta
+    for f in ("sxxd0", "szzd0", "sxz0", "sxz0_n", "sxxd0_s", "szzd0_s", "p_in", "p0_n", "p0_s"):
+        m.field(f)[:] = 0.0
+    m.field("C_n")[:] = 1e30
+    m.field("C_s")[:] = 1e30
+    xg, zg, xvz, zvx = _coords(m)
+    E = m.m.model.EpsBG
+    m.field("u_in")[:] = np.tile(-xg * E, m.Nz + 1)
+    m.field("v_in")[:] = np.repeat(zg * E, m.Nx + 1)
+
+
+def test_uniform_viscosity_pure_shear_has_zero_residual(mdz, orc, inc):
+    _exact_pure_shear(inc)
+    S, bl = _unit(orc, mdz, inc)
+    Fm, Fc = _F(S, bl)
+    assert np.abs(Fm).max() < 1e-12 and np.abs(Fc).max() < 1e-12
+    assert inc.m.Nmodel.resx < 1e-13 and inc.m.Nmodel.resz < 1e-13
+    free_blocks(orc, bl)
+
+
+def test_hydrostatic_pressure_balances_gravity(mdz, orc, inc):
+    """u = v = 0, p = rho g z: the vertical momentum residual vanishes."""
+    inc.m.model.gz = -1.0
+    for f in ("sxxd0", "szzd0", "sxz0", "sxz0_n", "sxxd0_s", "szzd0_s", "u_in", "v_in"):
+        inc.field(f)[:] = 0.0
+    inc.field("rho_n")[:] = 2.0
+    inc.field("rho_s")[:] = 2.0
+    inc.field("C_n")[:] = 1e30
+    inc.field("C_s")[:] = 1e30
+    g = inc.m.mesh.contents
+    zc = np.ctypeslib.as_array(g.zc_coord, shape=(inc.Nz - 1,))
+    inc.field("p_in")[:] = np.repeat(2.0 * -1.0 * zc, inc.Nx - 1)
+    inc.field("p0_n")[:] = inc.field("p_in")
+    for t in ("BCu", "BCv"):
+        inc.tags(t)  # tags untouched: Dirichlet walls carry zero velocity
+    inc.field("BCu.val")[:] = 0.0
+    inc.field("BCv.val")[:] = 0.0
+    S, bl = _unit(orc, mdz, inc)
+    Fm, _ = _F(S, bl)
+    assert np.abs(Fm).max() < 1e-11 * 2.0 * inc.m.model.dx
+    free_blocks(orc, bl)
+
+
+def _mats(mdz, S, bl):
+    nm, nc = S.neq_mom, S.neq_cont
+    return {"A": csr_matrix(mdz, bl[0], nm), "B": csr_matrix(mdz, bl[1], nc), "C": csr_matrix(mdz, bl[2], nm),
+            "JA": csr_matrix(mdz, bl[4], nm), "JB": csr_matrix(mdz, bl[5], nc), "JC": csr_matrix(mdz, bl[6], nm)}
+
+
+MODELS = [("inclusion", ("inclusion", (20, 20, 1))), ("lithosphere", ("lithosphere", (30, 20, 2))),
+          ("simple_shear", ("edgecase", (20, 16, 3, 1, 0))), ("stress_we", ("edgecase", (20, 16, 4, 2, 1))),
+          ("free_surface", ("edgecase", (20, 16, 5, 3, 0))), ("neumann_top", ("edgecase", (20, 16, 6, 4, 1)))]
+
+
+@pytest.mark.parametrize("name,spec", MODELS, ids=[m[0] for m in MODELS])
+def test_gradient_is_minus_divergence_transpose(mdz, orc, name, spec):
+    m = getattr(mdz.Model, spec[0])(*spec[1])
+    S, bl = _unit(orc, mdz, m)
+    M = _mats(mdz, S, bl)
+    assert abs(M["B"] + M["C"].T).max() == 0.0
+    free_blocks(orc, bl)
+    m.free()
+
+
+def test_picard_operator_properties(mdz, orc):
+    """Pure-shear walls: A symmetric; A annihilates a constant velocity on
+    rows whose stencil touches no wall."""
+    m = mdz.Model.lithosphere(30, 20, 2)
+    S, bl = _unit(orc, mdz, m)
+    M = _mats(mdz, S, bl)
+    A = M["A"]
+    assert abs(A - A.T).max() <= 1e-15 * abs(A).max()
+    Nx, Nz = m.Nx, m.Nz
+    eu = np.ctypeslib.as_array(S.eqn_u, shape=(Nx * (Nz + 1),))
+    ones = np.zeros(S.neq_mom)
+    tu = m.tags("BCu")
+    ones[eu[(tu == -1)]] = 1.0
+    r = A @ ones
+    rows = [eu[k + l * Nx] for l in range(2, Nz - 1) for k in range(2, Nx - 2)]
+    assert np.abs(r[rows]).max() <= 1e-13 * abs(A).max()
+    free_blocks(orc, bl)
+    m.free()
+
+
+def test_jacobian_equals_picard_for_linear_rheology(mdz, orc, inc):
+    S, bl = _unit(orc, mdz, inc)
+    assert (inc.field("eII_pl") == 0).all()
+    M = _mats(mdz, S, bl)
+    assert abs(M["JA"] - M["A"]).max() <= 1e-15 * abs(M["A"]).max()
+    assert abs(M["JB"] - M["B"]).max() == 0.0
+    assert abs(M["JC"] - M["C"]).max() == 0.0
+    free_blocks(orc, bl)
+
+
+def _fd_check(mdz, orc, m, which, eps=1e-6, seed=0):
+    """J.dx and the central difference of the residual F(u, v, p)."""
+    S = numbering(orc, mdz, m)
+    nm, nc = S.neq_mom, S.neq_cont
+    bl = newton_unit(orc, mdz, m, S)
+    M = _mats(mdz, S, bl)
+    J = sp.bmat([[M["JA"], M["JB"]], [M["JC"], None]], format="csr")
+    free_blocks(orc, bl)
+    Nx, Nz = m.Nx, m.Nz
+    eu = np.ctypeslib.as_array(S.eqn_u, shape=(Nx * (Nz + 1),))
+    ev = np.ctypeslib.as_array(S.eqn_v, shape=((Nx + 1) * Nz,))
+    ep = np.ctypeslib.as_array(S.eqn_p, shape=((Nx - 1) * (Nz - 1),))
+    tu, tv, tp = m.tags("BCu"), m.tags("BCv"), m.tags("BCp")
+    mu, mv, mp = (tu == -1) | (tu == 2), (tv == -1) | (tv == 2), tp == -1
+    rng = np.random.default_rng(seed)
+    x0 = {k: m.field(k).copy() for k in ("u_in", "v_in", "p_in")}
+    d = {k: np.zeros_like(v) for k, v in x0.items()}
+    if "u" in which:
+        d["u_in"][mu] = rng.standard_normal(mu.sum()) * np.abs(x0["u_in"]).max()
+    if "v" in which:
+        d["v_in"][mv] = rng.standard_normal(mv.sum()) * np.abs(x0["v_in"]).max()
+    if "p" in which:
+        d["p_in"][mp] = rng.standard_normal(mp.sum()) * max(np.abs(x0["p_in"]).max(), 1.0)
+    dx = np.zeros(nm + nc)
+    dx[eu[mu]] = d["u_in"][mu]
+    dx[ev[mv]] = d["v_in"][mv]
+    dx[ep[mp]] = d["p_in"][mp]
+    Fs = []
+    for sg in (1, -1):
+        for k in x0:
+            m.field(k)[:] = x0[k] + sg * eps * d[k]
+        b2 = newton_unit(orc, mdz, m, S)
+        Fs.append(np.concatenate(_F(S, b2)))
+        free_blocks(orc, b2)
+    for k in x0:
+        m.field(k)[:] = x0[k]
+    fd = (Fs[0] - Fs[1]) / (2 * eps)
+    interior = np.zeros(nm + nc, bool)
+    for c in np.flatnonzero(mu):
+        k, l = c % Nx, c // Nx
+        interior[eu[c]] = 2 <= k <= Nx - 3 and 3 <= l <= Nz - 3
+    for c in np.flatnonzero(mv):
+        k, l = c % (Nx + 1), c // (Nx + 1)
+        interior[ev[c]] = 3 <= k <= Nx - 3 and 2 <= l <= Nz - 3
+    interior[nm:] = True
+    return fd, J @ dx, interior
+
+
+@pytest.mark.parametrize("which", ["u", "v", "p", "uvp"])
+def test_jacobian_is_derivative_of_residual_power_law(mdz, orc, which):
+    """Visco-elastic power law (n = 3, two phases, pure-shear walls): every
+    row of J.dx matches the central difference of F."""
+    m = mdz.Model.inclusion(20, 20, 1)
+    mt = m.m.materials
+    for k in range(2):
+        set_single_law(m, k, pwl=1)
+        mt.npwl[k], mt.Apwl[k], mt.Qpwl[k], mt.Vpwl[k] = 3.0, (1.0 if k == 0 else 100.0), 0.0, 0.0
+        mt.mpwl[k], mt.rpwl[k], mt.apwl[k], mt.fpwl[k], mt.Fpwl[k], mt.pref_pwl[k] = 0.0, 0.0, 0.0, 0.0, 1.0, 1.0
+    m.field("C_n")[:] = 1e30
+    m.field("C_s")[:] = 1e30
+    fd, jd, _ = _fd_check(mdz, orc, m, which)
+    assert np.abs(fd - jd).max() <= 1e-8 * np.abs(fd).max()
+    m.free()
+
+
+@pytest.mark.parametrize("which", ["u", "v"])
+def test_jacobian_is_derivative_of_residual_lithosphere(mdz, orc, which):
+    """Three-phase lithosphere (power laws, Peierls, elasticity, no yielding):
+    J.dx against the central difference on rows whose stencil stays two
+    nodes off the walls."""
+    m = mdz.Model.lithosphere(40, 20, 2, -1.0, 1e15, -1.0)
+    fd, jd, interior = _fd_check(mdz, orc, m, which)
+    err = np.abs(fd - jd)[interior]
+    assert err.max() <= 1e-7 * np.abs(fd).max()
+    m.free()
+
+
+def test_jacobian_pattern_of_reference_matrix(mdz, orc):
+    """MATLAB_visualisation/Jacobian.mat holds the Jacobian blocks of a
+    50 x 50 cell pure-shear model, zeros dropped.  On the same grid and
+    walls, row by row: the reference's entries are in our pattern, our
+    non-zero entries for a linear rheology are in the reference's pattern,
+    and so are our non-zero entries for a non-linear rheology except where
+    the reference's own values vanished; explicit zeros are kept (row counts
+    21/6/4 at most, as in the reference; Picard 9/2/4)."""
+    pins = np.load(os.path.join(GOLDEN, "matrix_pins.npz"))
+    m = mdz.Model.inclusion(50, 50, 1)
+    S, bl = _unit(orc, mdz, m)
+    assert (S.neq_mom, S.neq_cont) == (4900, 2500)
+    Ic, J, A, _, _ = mdz.csr_arrays(bl[4])
+    fi, fj = pins["jac_MA_indptr"], pins["jac_MA_indices"]
+    for r in range(S.neq_mom):
+        ours = J[Ic[r]:Ic[r + 1]]
+        nonzero = set(ours[A[Ic[r]:Ic[r + 1]] != 0.0].tolist())
+        ref = set(fj[fi[r]:fi[r + 1]].tolist())
+        assert nonzero <= ref <= set(ours.tolist()), r
+    for blk, key, cap in ((bl[4], "MA", 21), (bl[5], "MB", 6), (bl[6], "MC", 4)):
+        Ic, _, _, _, _ = mdz.csr_arrays(blk)
+        cnt = np.diff(Ic)
+        assert cnt.max() == cap == pins["jac_%s_rownnz" % key].max()
+        assert (cnt >= pins["jac_%s_rownnz" % key]).all()
+    for blk, key, cap in ((bl[0], "MA", 9), (bl[1], "MB", 2), (bl[2], "MC", 4)):
+        Ic, _, _, _, _ = mdz.csr_arrays(blk)
+        assert np.diff(Ic).max() == cap == pins["pic_%s_rownnz" % key].max()
+    free_blocks(orc, bl)
+    m.free()
+    # non-linear rheology on the same grid: the reference pattern is in ours
+    # and most of our rows carry exactly the reference's non-zeros
+    m = mdz.Model.edgecase(50, 50, 3, 0, 1)
+    S, bl = _unit(orc, mdz, m)
+    Ic, J, A, _, _ = mdz.csr_arrays(bl[4])
+    same = 0
+    for r in range(S.neq_mom):
+        ours = J[Ic[r]:Ic[r + 1]]
+        ref = set(fj[fi[r]:fi[r + 1]].tolist())
+        assert ref <= set(ours.tolist()), r
+        same += set(ours[A[Ic[r]:Ic[r + 1]] != 0.0].tolist()) == ref
+    assert same >= 0.9 * S.neq_mom
+    free_blocks(orc, bl)
+    m.free()
+
+
+def test_equation_counts_of_the_benchmark_grid(mdz, orc):
+    """101 x 101 cells with pure-shear walls: 10 100 u + 10 100 v momentum
+    equations and 10 201 continuity equations."""
+    m = mdz.Model.inclusion(101, 101, 1)
+    S = numbering(orc, mdz, m)
+    assert (S.neq_mom, S.neq_cont) == (20200, 10201)
+    m.free()
+
+
+def test_add_coeff2_folds_dirichlet_neighbours(hot):
+    """AddCoeff2 (StokesAssemblyCoupled.c:47-57): types 0, 31, 11, 13 go to
+    the right-hand side, the rest is appended; zero coefficients are kept."""
+    J = (C.c_int * 8)()
+    A = (C.c_double * 8)()
+    rhs = (C.c_double * 2)(1.0, 0.0)
+    n = C.c_int(0)
+    hot.AddCoeff2.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(C.c_int),
+                              C.c_double, C.c_int, C.c_double, C.POINTER(C.c_double)]
+    for jeq, coeff, typ, val in ((5, 2.0, 0, 3.0), (6, 0.0, -1, 9.0), (7, 1.5, 31, 2.0), (8, -1.0, 2, 0.0),
+                                 (9, 4.0, 11, 0.5), (10, 1.0, 13, 1.0), (11, 3.0, -2, 0.0)):
+        hot.AddCoeff2(J, A, 0, jeq, C.byref(n), coeff, typ, val, rhs)
+    assert n.value == 3
+    assert list(J[:3]) == [6, 8, 11] and list(A[:3]) == [0.0, -1.0, 3.0]
+    assert rhs[0] == pytest.approx(1.0 - 6.0 - 3.0 - 2.0 - 1.0)
