@@ -901,16 +901,34 @@ __device__ __forceinline__ double mom_rhs(const DevAsm &a, int eqn) {
     return node < nu ? a.g.roger_x[node] : a.g.roger_z[node - nu];
 }
 
-/* pattern: per-row counts */
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+/* pattern: per-row counts; the largest row length of each block goes to
+ * cnt[neq] (one atomic per wave) */
 __global__ void __launch_bounds__(BLK) k_count_mom(int kind, DevAsm a, int *cntA, int *cntB) {
-    int eqn = blockIdx.x * BLK + threadIdx.x;
-    if (eqn >= a.e.neq_mom) return;
-    Tags T = make_tags(a.g);
-    Sink<SINK_COUNT> ra{0, nullptr, nullptr, 0.0}, rb{0, nullptr, nullptr, 0.0};
-    double b = 0.0, F = 0.0;
-    momentum_row<true, SINK_COUNT>(kind, a, T, eqn, ra, rb, b, 0.0, F);
-    cntA[eqn] = ra.n;
-    cntB[eqn] = rb.n;
+    const int eqn = blockIdx.x * BLK + threadIdx.x;
+    const int neq = a.e.neq_mom;
+    int na = 0, nb = 0;
+    if (eqn < neq) {
+        Tags T = make_tags(a.g);
+        Sink<SINK_COUNT> ra{0, nullptr, nullptr, 0.0}, rb{0, nullptr, nullptr, 0.0};
+        double b = 0.0, F = 0.0;
+        momentum_row<true, SINK_COUNT>(kind, a, T, eqn, ra, rb, b, 0.0, F);
+        na = ra.n;
+        nb = rb.n;
+        cntA[eqn] = na;
+        cntB[eqn] = nb;
+    }
+    na = wave_max(na);
+    nb = wave_max(nb);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&cntA[neq], na);
+        atomicMax(&cntB[neq], nb);
+    }
 }
 
 __device__ __forceinline__ void cont_geom(const DevAsm &a, int eqn, int &c1, int &c2, int &c3) {
@@ -921,14 +939,20 @@ __device__ __forceinline__ void cont_geom(const DevAsm &a, int eqn, int &c1, int
 }
 
 __global__ void __launch_bounds__(BLK) k_count_cont(DevAsm a, int *cntC) {
-    int eqn = blockIdx.x * BLK + threadIdx.x;
-    if (eqn >= a.e.neq_cont) return;
-    int c1, c2, c3;
-    cont_geom(a, eqn, c1, c2, c3);
-    Sink<SINK_COUNT> rc{0, nullptr, nullptr, 0.0};
-    double b = 0.0, F = 0.0;
-    continuity_row<true, SINK_COUNT>(a, 0, c1, c2, c3, rc, b, F);
-    cntC[eqn] = rc.n;
+    const int eqn = blockIdx.x * BLK + threadIdx.x;
+    const int neq = a.e.neq_cont;
+    int nc = 0;
+    if (eqn < neq) {
+        int c1, c2, c3;
+        cont_geom(a, eqn, c1, c2, c3);
+        Sink<SINK_COUNT> rc{0, nullptr, nullptr, 0.0};
+        double b = 0.0, F = 0.0;
+        continuity_row<true, SINK_COUNT>(a, 0, c1, c2, c3, rc, b, F);
+        nc = rc.n;
+        cntC[eqn] = nc;
+    }
+    nc = wave_max(nc);
+    if ((threadIdx.x & 63) == 0) atomicMax(&cntC[neq], nc);
 }
 
 __global__ void __launch_bounds__(BLK) k_pattern_mom(int kind, DevAsm a, DevBlock A, DevBlock B) {
@@ -950,14 +974,45 @@ __global__ void __launch_bounds__(BLK) k_pattern_cont(DevAsm a, DevBlock C) {
     continuity_row<true, SINK_PATTERN>(a, 0, c1, c2, c3, rc, b, F);
 }
 
-/* values (Assemble = 1): A/B values, bbc, final b = rhs*celvol (+/- Neumann) + bbc */
-__global__ void __launch_bounds__(BLK) k_values_mom(int kind, DevAsm a, DevBlock A, DevBlock B) {
+/* values (Assemble = 1): A/B values, bbc, final b = rhs*celvol (+/- Neumann) + bbc.
+ * Rows of one block are consecutive, so their values are one contiguous
+ * range of A (and of B): each thread writes its row into LDS and the block
+ * copies the range out with coalesced stores.  Needs every row length
+ * within the LDS slot (CAPA, CAPB); KIND 0 = Picard, 1 = Jacobian. */
+template <int KIND, int VBLK, int CAPA, int CAPB>
+__global__ void __launch_bounds__(VBLK) k_values_mom(DevAsm a, DevBlock A, DevBlock B) {
+    __shared__ double sA[VBLK * CAPA];
+    __shared__ double sB[VBLK * CAPB];
+    const int r0 = blockIdx.x * VBLK;
+    const int r1 = min(r0 + VBLK, a.e.neq_mom);
+    const int eqn = r0 + threadIdx.x;
+    const int a0 = A.Ic[r0], b0 = B.Ic[r0];
+    if (eqn < r1) {
+        Tags T = make_tags(a.g);
+        Sink<SINK_VALUES> ra{0, nullptr, sA + (A.Ic[eqn] - a0), 0.0}, rb{0, nullptr, sB + (B.Ic[eqn] - b0), 0.0};
+        double b = mom_rhs(a, eqn), F = 0.0;
+        momentum_row<true, SINK_VALUES>(KIND, a, T, eqn, ra, rb, b, 0.0, F);
+        A.bbc[eqn] = ra.bbc;
+        B.bbc[eqn] = rb.bbc;
+        double bf = b + ra.bbc;
+        A.b[eqn] = bf;
+        B.b[eqn] = bf;
+    }
+    __syncthreads();
+    const int na = A.Ic[r1] - a0, nb = B.Ic[r1] - b0;
+    for (int i = threadIdx.x; i < na; i += VBLK) A.A[a0 + i] = sA[i];
+    for (int i = threadIdx.x; i < nb; i += VBLK) B.A[b0 + i] = sB[i];
+}
+
+/* the same without staging, for patterns with longer rows */
+template <int KIND>
+__global__ void __launch_bounds__(BLK) k_values_mom_direct(DevAsm a, DevBlock A, DevBlock B) {
     int eqn = blockIdx.x * BLK + threadIdx.x;
     if (eqn >= a.e.neq_mom) return;
     Tags T = make_tags(a.g);
     Sink<SINK_VALUES> ra{0, nullptr, A.A + A.Ic[eqn], 0.0}, rb{0, nullptr, B.A + B.Ic[eqn], 0.0};
     double b = mom_rhs(a, eqn), F = 0.0;
-    momentum_row<true, SINK_VALUES>(kind, a, T, eqn, ra, rb, b, 0.0, F);
+    momentum_row<true, SINK_VALUES>(KIND, a, T, eqn, ra, rb, b, 0.0, F);
     A.bbc[eqn] = ra.bbc;
     B.bbc[eqn] = rb.bbc;
     double bf = b + ra.bbc;
@@ -982,13 +1037,14 @@ __global__ void __launch_bounds__(BLK) k_values_cont(int kind, DevAsm a, DevBloc
 }
 
 /* residual (Assemble = 0): b = rhs (Neumann rows adjusted), F */
-__global__ void __launch_bounds__(BLK) k_residual_mom(int kind, DevAsm a, DevBlock A) {
+template <int KIND>
+__global__ void __launch_bounds__(BLK) k_residual_mom(DevAsm a, DevBlock A) {
     int eqn = blockIdx.x * BLK + threadIdx.x;
     if (eqn >= a.e.neq_mom) return;
     Tags T = make_tags(a.g);
     Sink<SINK_COUNT> ra{0, nullptr, nullptr, 0.0}, rb{0, nullptr, nullptr, 0.0};
     double b = mom_rhs(a, eqn), F = A.F[eqn];
-    momentum_row<false, SINK_COUNT>(kind, a, T, eqn, ra, rb, b, A.bbc[eqn], F);
+    momentum_row<false, SINK_COUNT>(KIND, a, T, eqn, ra, rb, b, A.bbc[eqn], F);
     A.b[eqn] = b;
     A.F[eqn] = F;
 }
@@ -1164,9 +1220,21 @@ int mdz_build_pattern(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBl
                       size_t scratch_len, hipStream_t s) {
     int nm = a.e.neq_mom, ncnt = a.e.neq_cont;
     int gm = mdz_div_up(nm > 0 ? nm : 1, BLK), gc = mdz_div_up(ncnt > 0 ? ncnt : 1, BLK);
-    /* counts go into Ic[0..neq-1], scanned in place; Ic[neq] = nnz */
+    /* counts go into Ic[0..neq-1], scanned in place; the largest row length
+     * is collected in Ic[neq] (later overwritten with nnz) */
+    if (hipMemsetAsync(A->Ic + nm, 0, sizeof(int), s) != hipSuccess) return MDZ_EHIP;
+    if (hipMemsetAsync(B->Ic + nm, 0, sizeof(int), s) != hipSuccess) return MDZ_EHIP;
+    if (C && hipMemsetAsync(C->Ic + ncnt, 0, sizeof(int), s) != hipSuccess) return MDZ_EHIP;
     hipLaunchKernelGGL(k_count_mom, dim3(gm), dim3(BLK), 0, s, kind, a, A->Ic, B->Ic);
     if (C) hipLaunchKernelGGL(k_count_cont, dim3(gc), dim3(BLK), 0, s, a, C->Ic);
+    int mx[3] = {0, 0, 0};
+    if (hipMemcpyAsync(&mx[0], A->Ic + nm, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return MDZ_EHIP;
+    if (hipMemcpyAsync(&mx[1], B->Ic + nm, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return MDZ_EHIP;
+    if (C && hipMemcpyAsync(&mx[2], C->Ic + ncnt, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return MDZ_EHIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return MDZ_EHIP;
+    A->maxrow = mx[0];
+    B->maxrow = mx[1];
+    if (C) C->maxrow = mx[2];
     int tot = 0, rc;
     if ((rc = mdz_scan_exclusive(A->Ic, nm, scratch, scratch_len, s, &tot))) return rc;
     A->nnz = tot;
@@ -1186,15 +1254,38 @@ void mdz_fill_pattern(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBl
     if (C && ncnt > 0) hipLaunchKernelGGL(k_pattern_cont, dim3(mdz_div_up(ncnt, BLK)), dim3(BLK), 0, s, a, *C);
 }
 
+/* LDS slots of the staged value kernels: the reference's row budgets
+ * (StokesAssemblyDecoupled.c:1632-1635, :3887-3890) */
+#define PIC_CAPA 9
+#define PIC_CAPB 2
+#define JAC_CAPA 21
+#define JAC_CAPB 6
+#define PIC_VBLK 256
+#define JAC_VBLK 128
+
 void mdz_launch_values(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, hipStream_t s) {
     int nm = a.e.neq_mom, ncnt = a.e.neq_cont;
-    if (nm > 0) hipLaunchKernelGGL(k_values_mom, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, kind, a, *A, *B);
+    if (nm > 0) {
+        if (kind == 0 && A->maxrow <= PIC_CAPA && B->maxrow <= PIC_CAPB)
+            hipLaunchKernelGGL((k_values_mom<0, PIC_VBLK, PIC_CAPA, PIC_CAPB>), dim3(mdz_div_up(nm, PIC_VBLK)),
+                               dim3(PIC_VBLK), 0, s, a, *A, *B);
+        else if (kind == 1 && A->maxrow <= JAC_CAPA && B->maxrow <= JAC_CAPB)
+            hipLaunchKernelGGL((k_values_mom<1, JAC_VBLK, JAC_CAPA, JAC_CAPB>), dim3(mdz_div_up(nm, JAC_VBLK)),
+                               dim3(JAC_VBLK), 0, s, a, *A, *B);
+        else if (kind == 0)
+            hipLaunchKernelGGL(k_values_mom_direct<0>, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, a, *A, *B);
+        else
+            hipLaunchKernelGGL(k_values_mom_direct<1>, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, a, *A, *B);
+    }
     if (ncnt > 0) hipLaunchKernelGGL(k_values_cont, dim3(mdz_div_up(ncnt, BLK)), dim3(BLK), 0, s, kind, a, *C, *D);
 }
 
 void mdz_launch_residual(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, DevBlock *D, hipStream_t s) {
     int nm = a.e.neq_mom, ncnt = a.e.neq_cont;
-    if (nm > 0) hipLaunchKernelGGL(k_residual_mom, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, kind, a, *A);
+    if (nm > 0) {
+        if (kind == 0) hipLaunchKernelGGL(k_residual_mom<0>, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, a, *A);
+        else hipLaunchKernelGGL(k_residual_mom<1>, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, a, *A);
+    }
     if (ncnt > 0) hipLaunchKernelGGL(k_residual_cont, dim3(mdz_div_up(ncnt, BLK)), dim3(BLK), 0, s, kind, a, *C, *D);
 }
 

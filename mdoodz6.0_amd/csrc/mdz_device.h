@@ -88,6 +88,7 @@ struct DevEqn {
 /* One CSR block on the device. */
 struct DevBlock {
     int neq, nnz;
+    int maxrow; /* largest row length of the pattern */
     int *Ic, *J;
     double *A, *b, *bbc, *F, *d;
 };
