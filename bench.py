@@ -28,6 +28,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 # element read or written once; FP64 = 8 B, tags 1 B, equation numbers 4 B)
 STAGES = ["strain_rates", "rheology", "rhs", "dtensor_picard", "picard_values", "dtensor_jacobian",
           "jacobian_values", "dtensor_picard_2", "residual"]
+# kernels of each stage (names as in the rocprofv3 kernel trace)
+STAGE_KERNELS = {"strain_rates": ["k_strain_cells", "k_strain_vertices", "k_exz_n", "k_interp_c2v (2 fields)"],
+                 "rheology": ["k_interp_c2v (5 fields)", "k_rheo_cells_fast", "k_rheo_vertices_fast"],
+                 "rhs": ["k_rhs_u", "k_rhs_v", "k_rhs_p"],
+                 "dtensor_picard": ["k_dtensor_cells (J=0)", "k_dtensor_vertices (J=0)"],
+                 "picard_values": ["k_values_mom<0, 256, 9, 2>", "k_values_cont"],
+                 "dtensor_jacobian": ["k_dtensor_cells (J=1)", "k_dtensor_vertices (J=1)"],
+                 "jacobian_values": ["k_values_mom<1, 128, 21, 6>", "k_values_cont"],
+                 "dtensor_picard_2": ["k_dtensor_cells (J=0)", "k_dtensor_vertices (J=0)"],
+                 "residual": ["k_residual_mom<0>", "k_residual_cont", "k_norms", "k_norms_final"]}
 BYTES_PER_CELL = {"strain_rates": 76, "rheology": 810, "rhs": 160, "dtensor_picard": 114, "picard_values": 340,
                   "dtensor_jacobian": 330, "jacobian_values": 668, "dtensor_picard_2": 114, "residual": 204}
 HBM_PEAK_GBS = 8000.0
@@ -39,6 +49,23 @@ def _pkg():
     sys.modules["mdoodz_amd"] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+def traffic_from_profile(stage, ncx, ncz):
+    """HBM bytes per launch of the stage's kernels from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy probe), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_lithosphere_%dx%d.md" % (ncx, ncz))
+    if not os.path.exists(path):
+        return None
+    total, found = 0.0, set()
+    for line in open(path):
+        if line.startswith("| `"):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            name = cells[0].strip("`")
+            if name in STAGE_KERNELS[stage] and name not in found:
+                total += float(cells[-1])
+                found.add(name)
+    return total if found == set(STAGE_KERNELS[stage]) else None
 
 
 def cpu_baseline(mdz, ncx, ncz, seed, units):
@@ -83,6 +110,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-units", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-mb", type=int, default=1024, help="size of the HBM copy probe (MiB)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path is built and measured on a single MI355X; "
@@ -111,15 +139,22 @@ def main():
     ms_per_step = ms.value
     value = cells / (ms_per_step * 1e-3)
 
+    probe = C.c_double()
+    mdz.check(hot.mdz_copy_probe(args.probe_mb << 20, 10, C.byref(probe)))
+
     st = (C.c_double * 10)()
     mdz.check(hot.mdz_ctx_stage_times(ctx, 3, st))
     stage_ms = {s: st[i] for i, s in enumerate(STAGES)}
     dom = max(STAGES, key=lambda s: stage_ms[s])
     dom_bytes = BYTES_PER_CELL[dom] * cells
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_cell": BYTES_PER_CELL[dom], "stage_ms": round(stage_ms[dom], 4)}
+    traffic = traffic_from_profile(dom, args.ncx, args.ncz)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(traffic) if traffic is not None else None,
+                "stage": dom, "kernels": STAGE_KERNELS[dom], "algorithmic_bytes": dom_bytes,
+                "bytes_per_cell": BYTES_PER_CELL[dom], "launch_ms": round(stage_ms[dom], 4),
+                "probe_GBs": round(probe.value, 1), "frac_of_probe": round(achieved / probe.value, 4)}
     whole = 2816.0 * cells / (ms_per_step * 1e-3) / 1e9
     hot.mdz_ctx_destroy(ctx)
     m.free()

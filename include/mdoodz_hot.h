@@ -257,6 +257,14 @@ void BuildJacobianOperatorDecoupled(grid *mesh, params model, int lev, double *p
 void EvaluateStokesResidualDecoupled(SparseMat *Stokes, SparseMat *StokesA, SparseMat *StokesB,
                                      SparseMat *StokesC, SparseMat *StokesD, Nparams *Nmodel,
                                      grid *mesh, params model, scale scaling, int quiet);
+/* StokesRoutines.c:1232 ExtractDiagonalScale (header_MDOODZ.h:577): d of
+ * the momentum rows = diagonal of A (copied to B), d of the continuity rows
+ * = diagonal of D (copied to C). */
+void ExtractDiagonalScale(SparseMat *StokesA, SparseMat *StokesB, SparseMat *StokesC, SparseMat *StokesD);
+/* StokesRoutines.c:1276 ScaleMatrix (header_MDOODZ.h:578): values times
+ * d_i d_j, b and F times d_i. */
+void ScaleMatrix(SparseMat *StokesA, SparseMat *StokesB, SparseMat *StokesC, SparseMat *StokesD);
+
 /* header_MDOODZ.h:477 -> StokesAssemblyCoupled.c:47.  Host bookkeeping
  * helper of the assembly (append a coefficient or fold a Dirichlet
  * neighbour into the right-hand side); exported because the reference
@@ -338,6 +346,10 @@ int  mdz_ctx_download_block(mdz_ctx *ctx, int which, int *Ic, int *J, double *A,
 int  mdz_ctx_download_residual(mdz_ctx *ctx, double *Fmom, double *Fcont);
 /* Switches the collection of local-iteration histograms on (1) or off (0,
  * the default: collection costs a few percent of the rheology kernels). */
+/* ExtractDiagonalScale + ScaleMatrix on the device-resident Picard (kind 0)
+ * or Jacobian (kind 1) blocks, after mdz_ctx_newton_unit
+ * (Main_DOODZ.c:801-820 with diag_scaling = 1). */
+int  mdz_ctx_diagonal_scaling(mdz_ctx *ctx, int kind);
 int  mdz_ctx_collect_histogram(mdz_ctx *ctx, int on);
 /* Local-iteration counts accumulated since the last call while collection
  * was on (node x phase evaluations): hist[0..63] = FP64 visco-elastic Newton iterations,
@@ -351,6 +363,10 @@ int  mdz_ctx_iteration_histogram(mdz_ctx *ctx, long long *hist192);
 int  mdz_ctx_stage_times(mdz_ctx *ctx, int reps, double *ms10);
 /* Mean milliseconds per unit over `reps` back-to-back units. */
 int  mdz_ctx_time_units(mdz_ctx *ctx, int reps, double *ms_per_unit);
+/* Achievable HBM bandwidth: `reps` streaming copies of `bytes` (read +
+ * write counted), GB/s.  Also the known-size kernel for scaling memory
+ * counters (kernel name k_copy_probe). */
+int  mdz_copy_probe(size_t bytes, int reps, double *gbs);
 /* The HIP stream (hipStream_t) the context launches on. */
 void *mdz_ctx_stream(mdz_ctx *ctx);
 

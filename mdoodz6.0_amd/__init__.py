@@ -122,12 +122,12 @@ def lib_model():
 HOT_SYMBOLS = ["Viscosity", "NonNewtonianViscosityGrid", "StrainRateComponents", "RheologicalOperators",
                "EvaluateRHS", "UpdateNonLinearity", "InterpCentroidsToVerticesDouble", "EvalNumberOfEquations",
                "BuildStokesOperatorDecoupled", "BuildJacobianOperatorDecoupled", "EvaluateStokesResidualDecoupled",
-               "AddCoeff2", "mdz_last_error", "mdz_set_exit_on_error", "mdz_last_status", "mdz_set_rheology_kernel", "mdz_ctx_create",
+               "ExtractDiagonalScale", "ScaleMatrix", "AddCoeff2", "mdz_last_error", "mdz_set_exit_on_error", "mdz_last_status", "mdz_set_rheology_kernel", "mdz_ctx_create",
                "mdz_ctx_destroy", "mdz_ctx_upload_step_fields", "mdz_ctx_upload_solution",
                "mdz_ctx_download_solution", "mdz_ctx_download_grid", "mdz_ctx_set_rheology_mode",
                "mdz_ctx_set_network", "mdz_ctx_newton_unit", "mdz_ctx_block_info", "mdz_ctx_download_block",
-               "mdz_ctx_download_residual", "mdz_ctx_collect_histogram", "mdz_ctx_iteration_histogram", "mdz_ctx_stream",
-               "mdz_ctx_stage_times", "mdz_ctx_time_units"]
+               "mdz_ctx_download_residual", "mdz_ctx_diagonal_scaling", "mdz_ctx_collect_histogram", "mdz_ctx_iteration_histogram", "mdz_ctx_stream",
+               "mdz_ctx_stage_times", "mdz_ctx_time_units", "mdz_copy_probe"]
 
 
 def lib_hot():
@@ -163,8 +163,12 @@ def lib_hot():
         L.mdz_ctx_set_network.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_size_t]
         L.mdz_ctx_iteration_histogram.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
         L.mdz_ctx_collect_histogram.argtypes = [C.c_void_p, C.c_int]
+        L.mdz_ctx_diagonal_scaling.argtypes = [C.c_void_p, C.c_int]
+        L.ExtractDiagonalScale.argtypes = [SM, SM, SM, SM]
+        L.ScaleMatrix.argtypes = [SM, SM, SM, SM]
         L.mdz_ctx_stage_times.argtypes = [C.c_void_p, C.c_int, dp]
         L.mdz_ctx_time_units.argtypes = [C.c_void_p, C.c_int, dp]
+        L.mdz_copy_probe.argtypes = [C.c_size_t, C.c_int, dp]
         L.mdz_ctx_stream.restype = C.c_void_p
         L.mdz_ctx_stream.argtypes = [C.c_void_p]
         L.mdz_set_rheology_kernel.argtypes = [C.c_int]

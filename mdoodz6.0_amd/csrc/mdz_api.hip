@@ -814,6 +814,90 @@ extern "C" void EvaluateStokesResidualDecoupled(SparseMat *Stokes, SparseMat *St
     finish_norms(Nmodel, out6, scaling, quiet);
 }
 
+/* ---- diagonal scaling drop-ins (host CSR in, host CSR out) ---- */
+struct HostBlocks {
+    DevBlock b[4];
+    int rows[4];
+};
+
+static void tmp_blocks_free(HostBlocks &h) {
+    for (int q = 0; q < 4; q++) {
+        hipFree(h.b[q].Ic); hipFree(h.b[q].J); hipFree(h.b[q].A);
+        hipFree(h.b[q].b); hipFree(h.b[q].F); hipFree(h.b[q].d);
+    }
+}
+
+/* Device copies of the four blocks: rows of D are those of C, as in the
+ * reference's loops. */
+static int tmp_blocks_upload(SparseMat *M[4], HostBlocks &h, hipStream_t s) {
+    memset(&h, 0, sizeof(h));
+    h.rows[0] = M[0]->neq; h.rows[1] = M[1]->neq; h.rows[2] = M[2]->neq; h.rows[3] = M[2]->neq;
+    for (int q = 0; q < 4; q++) {
+        int n = h.rows[q];
+        if (n <= 0) continue;
+        if (!M[q]->Ic || !M[q]->d || !M[q]->b || !M[q]->F) return set_err(MDZ_EINVAL, "block %d: missing arrays", q);
+        int nnz = M[q]->Ic[n];
+        DevBlock &b = h.b[q];
+        b.neq = n; b.nnz = nnz;
+        HIPCHK(hipMalloc((void **)&b.Ic, ((size_t)n + 1) * sizeof(int)));
+        HIPCHK(hipMalloc((void **)&b.d, (size_t)n * 8));
+        HIPCHK(hipMalloc((void **)&b.b, (size_t)n * 8));
+        HIPCHK(hipMalloc((void **)&b.F, (size_t)n * 8));
+        HIPCHK(hipMemcpyAsync(b.Ic, M[q]->Ic, ((size_t)n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(b.d, M[q]->d, (size_t)n * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(b.b, M[q]->b, (size_t)n * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(b.F, M[q]->F, (size_t)n * 8, hipMemcpyHostToDevice, s));
+        if (nnz > 0) {
+            HIPCHK(hipMalloc((void **)&b.J, (size_t)nnz * sizeof(int)));
+            HIPCHK(hipMalloc((void **)&b.A, (size_t)nnz * 8));
+            HIPCHK(hipMemcpyAsync(b.J, M[q]->J, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(b.A, M[q]->A, (size_t)nnz * 8, hipMemcpyHostToDevice, s));
+        }
+    }
+    return MDZ_OK;
+}
+
+static int diag_dropin(SparseMat *SA, SparseMat *SB, SparseMat *SC, SparseMat *SD, int scale) {
+    if (!SA || !SB || !SC || !SD) return set_err(MDZ_EINVAL, "NULL block");
+    SparseMat *M[4] = {SA, SB, SC, SD};
+    hipStream_t s;
+    HIPCHK(hipStreamCreate(&s));
+    HostBlocks h;
+    int rc = tmp_blocks_upload(M, h, s);
+    if (rc == MDZ_OK) {
+        if (scale) mdz_launch_scale_matrix(&h.b[0], &h.b[1], &h.b[2], &h.b[3], s);
+        else mdz_launch_extract_diagonal(&h.b[0], &h.b[1], &h.b[2], &h.b[3], s);
+        for (int q = 0; q < 4 && rc == MDZ_OK; q++) {
+            int n = h.rows[q];
+            if (n <= 0) continue;
+            if (!scale) {
+                if (hipMemcpyAsync(M[q]->d, h.b[q].d, (size_t)n * 8, hipMemcpyDeviceToHost, s) != hipSuccess) rc = MDZ_EHIP;
+                continue;
+            }
+            if (hipMemcpyAsync(M[q]->b, h.b[q].b, (size_t)n * 8, hipMemcpyDeviceToHost, s) != hipSuccess) rc = MDZ_EHIP;
+            if (hipMemcpyAsync(M[q]->F, h.b[q].F, (size_t)n * 8, hipMemcpyDeviceToHost, s) != hipSuccess) rc = MDZ_EHIP;
+            if (h.b[q].nnz > 0 &&
+                hipMemcpyAsync(M[q]->A, h.b[q].A, (size_t)h.b[q].nnz * 8, hipMemcpyDeviceToHost, s) != hipSuccess)
+                rc = MDZ_EHIP;
+        }
+        if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) rc = MDZ_EHIP;
+        if (rc) set_err(rc, "diagonal scaling: HIP error");
+    }
+    tmp_blocks_free(h);
+    hipStreamDestroy(s);
+    return rc;
+}
+
+extern "C" void ExtractDiagonalScale(SparseMat *StokesA, SparseMat *StokesB, SparseMat *StokesC, SparseMat *StokesD) {
+    g_status = 0;
+    dropin_check(diag_dropin(StokesA, StokesB, StokesC, StokesD, 0));
+}
+
+extern "C" void ScaleMatrix(SparseMat *StokesA, SparseMat *StokesB, SparseMat *StokesC, SparseMat *StokesD) {
+    g_status = 0;
+    dropin_check(diag_dropin(StokesA, StokesB, StokesC, StokesD, 1));
+}
+
 /* StokesAssemblyCoupled.c:47 -- append a coefficient, or fold a Dirichlet /
  * surface-pressure / non-conforming / matching-Neumann neighbour into the
  * right-hand side.  Host bookkeeping helper of the assembly interface. */
@@ -1009,6 +1093,16 @@ extern "C" int mdz_ctx_download_residual(mdz_ctx *c, double *Fmom, double *Fcont
     return sync(c);
 }
 
+extern "C" int mdz_ctx_diagonal_scaling(mdz_ctx *c, int kind) {
+    if (!c || kind < 0 || kind > 1) return set_err(MDZ_EINVAL, "diagonal scaling kind %d", kind);
+    if (!c->pattern_ready[kind]) return set_err(MDZ_EINVAL, "diagonal scaling before the blocks are assembled");
+    DevBlock *b = &c->blk[4 * kind];
+    mdz_launch_extract_diagonal(&b[0], &b[1], &b[2], &b[3], c->stream);
+    mdz_launch_scale_matrix(&b[0], &b[1], &b[2], &b[3], c->stream);
+    HIPCHK(hipGetLastError());
+    return MDZ_OK;
+}
+
 extern "C" int mdz_ctx_collect_histogram(mdz_ctx *c, int on) {
     if (!c) return set_err(MDZ_EINVAL, "NULL context");
     c->hist_on = on != 0;
@@ -1086,6 +1180,42 @@ extern "C" int mdz_ctx_stage_times(mdz_ctx *c, int reps, double *ms) {
 
 /* Times `reps` back-to-back Newton units (no host synchronisation inside)
  * between two events; returns the mean milliseconds per unit. */
+/* Streaming copy of n doubles (8-byte loads and stores, grid-stride): the
+ * reference kernel for achievable HBM bandwidth and for scaling the
+ * FETCH_SIZE / WRITE_SIZE counters. */
+__global__ void __launch_bounds__(256) k_copy_probe(const double *__restrict__ src, double *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+extern "C" int mdz_copy_probe(size_t bytes, int reps, double *gbs) {
+    if (bytes < 1024 || reps < 1 || !gbs) return set_err(MDZ_EINVAL, "copy probe");
+    size_t n = bytes / sizeof(double);
+    double *a = nullptr, *b = nullptr;
+    hipStream_t s;
+    HIPCHK(hipStreamCreate(&s));
+    if (hipMalloc((void **)&a, n * sizeof(double)) != hipSuccess) return set_err(MDZ_ENOMEM, "copy probe");
+    if (hipMalloc((void **)&b, n * sizeof(double)) != hipSuccess) { hipFree(a); return set_err(MDZ_ENOMEM, "copy probe"); }
+    HIPCHK(hipMemsetAsync(a, 0, n * sizeof(double), s));
+    int blocks = 256 * 8 * 4; /* 8 XCDs x 32 CUs x 4 blocks */
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, s, (const double *)a, b, n);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, s, (const double *)a, b, n);
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *gbs = 2.0 * (double)(n * sizeof(double)) * reps / (t * 1e-3) / 1e9;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    HIPCHK(hipStreamDestroy(s));
+    return MDZ_OK;
+}
+
 extern "C" int mdz_ctx_time_units(mdz_ctx *c, int reps, double *ms_per_unit) {
     if (!c || reps < 1 || !ms_per_unit) return set_err(MDZ_EINVAL, "time units");
     hipEvent_t e0, e1;

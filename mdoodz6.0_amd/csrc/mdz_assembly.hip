@@ -1290,6 +1290,55 @@ void mdz_launch_residual(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, De
 }
 
 /* ------------------------------------------------------------------ */
+/* diagonal scaling (StokesRoutines.c:1232-1342)                        */
+/* ------------------------------------------------------------------ */
+/* X.d[i] = last diagonal entry of row i of X (kept when there is none);
+ * Y.d[i] = X.d[i] */
+__global__ void __launch_bounds__(BLK) k_extract_diag(DevBlock X, DevBlock Y, int nrows) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= nrows) return;
+    double d = X.d[i];
+    for (int k = X.Ic[i]; k < X.Ic[i + 1]; k++)
+        if (X.J[k] == i) d = X.A[k];
+    X.d[i] = d;
+    Y.d[i] = d;
+}
+
+/* row i: b, F scaled by M.d[i]; values by dr[i] * dc[j] */
+__global__ void __launch_bounds__(BLK) k_scale_rows(DevBlock M, int nrows, const double *__restrict__ dr,
+                                                    const double *__restrict__ dc) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= nrows) return;
+    const double di = M.d[i];
+    M.b[i] *= di;
+    M.F[i] *= di;
+    const double ri = dr[i];
+    for (int k = M.Ic[i]; k < M.Ic[i + 1]; k++) M.A[k] *= ri * dc[M.J[k]];
+}
+
+void mdz_launch_extract_diagonal(DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, hipStream_t s) {
+    int nm = A->neq, nc = C->neq;
+    if (nm > 0) hipLaunchKernelGGL(k_extract_diag, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, *A, *B, nm);
+    if (nc > 0) hipLaunchKernelGGL(k_extract_diag, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, *D, *C, nc);
+}
+
+void mdz_launch_scale_matrix(DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, hipStream_t s) {
+    int nm = A->neq, nc = C->neq;
+    if (nm > 0) {
+        hipLaunchKernelGGL(k_scale_rows, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, *A, nm, (const double *)A->d,
+                           (const double *)A->d);
+        hipLaunchKernelGGL(k_scale_rows, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, *B, nm, (const double *)A->d,
+                           (const double *)D->d);
+    }
+    if (nc > 0) {
+        hipLaunchKernelGGL(k_scale_rows, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, *C, nc, (const double *)D->d,
+                           (const double *)A->d);
+        hipLaunchKernelGGL(k_scale_rows, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, *D, nc, (const double *)D->d,
+                           (const double *)D->d);
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* norms                                                                */
 /* ------------------------------------------------------------------ */
 #define NRM_BLK 256

@@ -52,5 +52,10 @@ void mdz_launch_residual(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, De
 size_t mdz_norm_partials(const DevGrid &g);
 void mdz_launch_norms(const DevAsm &a, DevBlock *A, DevBlock *D, double *partial, double *out6, hipStream_t s);
 
+/* diagonal scaling: d from the diagonals of A and D (ExtractDiagonalScale),
+ * then A_ij d_i d_j with b, F scaled by d (ScaleMatrix) */
+void mdz_launch_extract_diagonal(DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, hipStream_t s);
+void mdz_launch_scale_matrix(DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, hipStream_t s);
+
 /* device-wide exclusive scan of n ints (in place), returns the total on the host (synchronises) */
 int mdz_scan_exclusive(int *data, int n, int *scratch, size_t scratch_len, hipStream_t s, int *total);

@@ -52,6 +52,8 @@ void orc_EvaluateStokesResidualDecoupled(SparseMat *Stokes, SparseMat *StokesA, 
                                          grid *mesh, params model, scale scaling, int quiet);
 void orc_UpdateNonLinearity(grid *mesh, mat_prop materials, params *model, Nparams *Nmodel,
                             scale scaling);
+void orc_ExtractDiagonalScale(SparseMat *StokesA, SparseMat *StokesB, SparseMat *StokesC, SparseMat *StokesD);
+void orc_ScaleMatrix(SparseMat *StokesA, SparseMat *StokesB, SparseMat *StokesC, SparseMat *StokesD);
 
 /* One Newton-iteration unit (the measured sequence), all on the CPU:
  * StrainRateComponents -> NonNewtonianViscosityGrid -> EvaluateRHS ->

@@ -1242,3 +1242,41 @@ void orc_newton_unit(grid *mesh, mat_prop *materials, params *model, Nparams *Nm
     orc_RheologicalOperators(mesh, model, scaling, 0);
     orc_EvaluateStokesResidualDecoupled(Stokes, SA, SB, SC, SD, Nmodel, mesh, *model, *scaling, 1);
 }
+
+/* StokesRoutines.c:1232 ExtractDiagonalScale: d of the momentum rows = the
+ * diagonal entry of A (the last one in the row if a row holds several),
+ * copied to B; d of the continuity rows = the diagonal of D, copied to C.
+ * Rows without a diagonal entry keep their d. */
+void orc_ExtractDiagonalScale(SparseMat *SA, SparseMat *SB, SparseMat *SC, SparseMat *SD) {
+#pragma omp parallel for
+    for (int i = 0; i < SA->neq; i++) {
+        for (int k = SA->Ic[i]; k < SA->Ic[i + 1]; k++)
+            if (SA->J[k] == i) SA->d[i] = SA->A[k];
+        SB->d[i] = SA->d[i];
+    }
+#pragma omp parallel for
+    for (int i = 0; i < SD->neq; i++) {
+        for (int k = SD->Ic[i]; k < SD->Ic[i + 1]; k++)
+            if (SD->J[k] == i) SD->d[i] = SD->A[k];
+        SC->d[i] = SD->d[i];
+    }
+}
+
+/* row i of M: b, F scaled by M->d[i]; values by dr[i] * dc[j] */
+static void scale_rows(SparseMat *M, int nrows, const double *dr, const double *dc) {
+#pragma omp parallel for
+    for (int i = 0; i < nrows; i++) {
+        M->b[i] *= M->d[i];
+        M->F[i] *= M->d[i];
+        for (int k = M->Ic[i]; k < M->Ic[i + 1]; k++) M->A[k] *= dr[i] * dc[M->J[k]];
+    }
+}
+
+/* StokesRoutines.c:1276 ScaleMatrix: A_ij d_i d_j with the momentum d for
+ * u/v rows and columns and the continuity d for p rows and columns. */
+void orc_ScaleMatrix(SparseMat *SA, SparseMat *SB, SparseMat *SC, SparseMat *SD) {
+    scale_rows(SA, SA->neq, SA->d, SA->d);
+    scale_rows(SB, SB->neq, SA->d, SD->d);
+    scale_rows(SC, SC->neq, SD->d, SA->d);
+    scale_rows(SD, SC->neq, SD->d, SD->d);
+}

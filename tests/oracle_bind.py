@@ -29,6 +29,8 @@ def load(mdz):
         L.orc_UpdateNonLinearity.argtypes = [G, MP, C.POINTER(PR), C.POINTER(NP), SC]
         L.orc_newton_unit.argtypes = [G, C.POINTER(MP), C.POINTER(PR), C.POINTER(NP), C.POINTER(SC)] + [SM] * 9
         L.orc_free_block.argtypes = [SM]
+        L.orc_ExtractDiagonalScale.argtypes = [SM, SM, SM, SM]
+        L.orc_ScaleMatrix.argtypes = [SM, SM, SM, SM]
         L.orc_salloc.argtypes = [SM, C.c_int]
         L.orc_sfree.argtypes = [SM]
         L.orc_num_threads.restype = C.c_int

@@ -553,3 +553,32 @@ def test_add_coeff2_folds_dirichlet_neighbours(hot):
     assert n.value == 3
     assert list(J[:3]) == [6, 8, 11] and list(A[:3]) == [0.0, -1.0, 3.0]
     assert rhs[0] == pytest.approx(1.0 - 6.0 - 3.0 - 2.0 - 1.0)
+
+
+def test_diagonal_scaling(mdz, orc):
+    """ExtractDiagonalScale + ScaleMatrix (StokesRoutines.c:1232-1342): d is
+    the diagonal of A (and of D, empty here: d stays 1); the scaled blocks
+    are diag(dr) M diag(dc) (scipy), b and F are scaled by the row d."""
+    m = mdz.Model.lithosphere(30, 20, 2)
+    S, bl = _unit(orc, mdz, m)
+    nm, nc = S.neq_mom, S.neq_cont
+    before = _mats(mdz, S, bl)
+    b0 = np.ctypeslib.as_array(bl[0].b, shape=(nm,)).copy()
+    F0 = np.ctypeslib.as_array(bl[0].F, shape=(nm,)).copy()
+    pic = [C.byref(b) for b in bl[:4]]
+    orc.orc_ExtractDiagonalScale(*pic)
+    dA = np.ctypeslib.as_array(bl[0].d, shape=(nm,)).copy()
+    dD = np.ctypeslib.as_array(bl[3].d, shape=(nc,)).copy()
+    assert np.array_equal(dA, before["A"].diagonal())
+    assert np.array_equal(np.ctypeslib.as_array(bl[1].d, shape=(nm,)), dA)
+    assert (dD == 1.0).all()
+    orc.orc_ScaleMatrix(*pic)
+    after = _mats(mdz, S, bl)
+    DA, DD = sp.diags(dA), sp.diags(dD)
+    assert abs(after["A"] - DA @ before["A"] @ DA).max() <= 1e-15 * abs(after["A"]).max()
+    assert abs(after["B"] - DA @ before["B"] @ DD).max() <= 1e-15 * abs(after["B"]).max()
+    assert abs(after["C"] - DD @ before["C"] @ DA).max() <= 1e-15 * abs(after["C"]).max()
+    assert np.array_equal(np.ctypeslib.as_array(bl[0].b, shape=(nm,)), b0 * dA)
+    assert np.array_equal(np.ctypeslib.as_array(bl[0].F, shape=(nm,)), F0 * dA)
+    free_blocks(orc, bl)
+    m.free()

@@ -303,3 +303,79 @@ def test_iteration_histogram(mdz, orc, hot):
     finally:
         hot.mdz_ctx_destroy(ctx)
         m.free()
+
+
+def _block_copy(mdz, orc, src, S, rows):
+    """Deep copy of an oracle-built SparseMat (CSR + b/F/d)."""
+    dst = mdz.SparseMat()
+    orc.orc_salloc(C.byref(dst), rows)
+    dst.neq, dst.nnz = src.neq, src.nnz
+    n = rows
+    Ic = (C.c_int * (n + 1))(*np.ctypeslib.as_array(src.Ic, shape=(n + 1,)))
+    dst.Ic = C.cast(Ic, C.POINTER(C.c_int))
+    nnz = int(Ic[n])
+    if nnz:
+        J = (C.c_int * nnz)(*np.ctypeslib.as_array(src.J, shape=(nnz,)))
+        A = (C.c_double * nnz)(*np.ctypeslib.as_array(src.A, shape=(nnz,)))
+        dst.J, dst.A = C.cast(J, C.POINTER(C.c_int)), C.cast(A, C.POINTER(C.c_double))
+        dst._keep = (Ic, J, A)
+    else:
+        dst._keep = (Ic,)
+    for f in ("b", "F", "d"):
+        np.ctypeslib.as_array(getattr(dst, f), shape=(n,))[:] = np.ctypeslib.as_array(getattr(src, f), shape=(n,))
+    return dst
+
+
+def test_diagonal_scaling_dropins(mdz, orc, hot):
+    """ExtractDiagonalScale / ScaleMatrix drop-ins against the oracle on the
+    Jacobian blocks of the lithosphere model: bit-exact (same products)."""
+    m = mdz.Model.lithosphere(48, 24, 4)
+    S = numbering(orc, mdz, m)
+    bo = blocks(orc, mdz, S, 8)
+    orc.orc_newton_unit(m.m.mesh, C.byref(m.m.materials), C.byref(m.m.model), C.byref(m.m.Nmodel),
+                        C.byref(m.m.scaling), C.byref(S), *[C.byref(b) for b in bo])
+    rows = [S.neq_mom, S.neq_mom, S.neq_cont, S.neq_cont]
+    jo = bo[4:]
+    jg = [_block_copy(mdz, orc, jo[q], S, rows[q]) for q in range(4)]
+    orc.orc_ExtractDiagonalScale(*[C.byref(b) for b in jo])
+    hot.ExtractDiagonalScale(*[C.byref(b) for b in jg])
+    assert hot.mdz_last_status() == 0, hot.mdz_last_error()
+    for q in range(4):
+        assert np.array_equal(np.ctypeslib.as_array(jg[q].d, shape=(rows[q],)),
+                              np.ctypeslib.as_array(jo[q].d, shape=(rows[q],))), q
+    orc.orc_ScaleMatrix(*[C.byref(b) for b in jo])
+    hot.ScaleMatrix(*[C.byref(b) for b in jg])
+    assert hot.mdz_last_status() == 0, hot.mdz_last_error()
+    for q in range(4):
+        n, nnz = rows[q], jo[q].nnz
+        for f in ("b", "F"):
+            assert np.array_equal(np.ctypeslib.as_array(getattr(jg[q], f), shape=(n,)),
+                                  np.ctypeslib.as_array(getattr(jo[q], f), shape=(n,))), (q, f)
+        if nnz:
+            assert np.array_equal(np.ctypeslib.as_array(jg[q].A, shape=(nnz,)),
+                                  np.ctypeslib.as_array(jo[q].A, shape=(nnz,))), q
+    # device-resident: same on the context's Jacobian blocks
+    mg = mdz.Model.lithosphere(48, 24, 4)
+    ctx = C.c_void_p()
+    mdz.check(hot.mdz_ctx_create(C.byref(ctx), mg.m.mesh, C.byref(mg.m.materials), C.byref(mg.m.model), C.byref(mg.m.scaling)))
+    try:
+        mdz.check(hot.mdz_ctx_newton_unit(ctx, None))
+        mdz.check(hot.mdz_ctx_diagonal_scaling(ctx, 1))
+        neq, nnz = C.c_int(), C.c_int()
+        mdz.check(hot.mdz_ctx_block_info(ctx, 4, C.byref(neq), C.byref(nnz)))
+        Ic = np.zeros(neq.value + 1, np.int32)
+        J = np.zeros(nnz.value, np.int32)
+        A = np.zeros(nnz.value)
+        b = np.zeros(neq.value)
+        ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+        mdz.check(hot.mdz_ctx_download_block(ctx, 4, Ic.ctypes.data_as(ip), J.ctypes.data_as(ip),
+                                             A.ctypes.data_as(dp), b.ctypes.data_as(dp), None))
+        Io, Jo, Ao, _, _ = mdz.csr_arrays(jo[0])
+        assert np.array_equal(J, Jo)
+        e = row_relerr(Io, A, Ao)
+        print("device-resident scaled JA err %.2e" % e)
+        assert e <= TOL_CSR
+    finally:
+        hot.mdz_ctx_destroy(ctx)
+        mg.free()
+    m.free()
