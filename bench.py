@@ -68,6 +68,42 @@ def traffic_from_profile(stage, ncx, ncz):
     return total if found == set(STAGE_KERNELS[stage]) else None
 
 
+def train_surrogate(mdz, hot, m, samples, epochs, batch, lr, seed):
+    """Training data from the FP64 reference rheology on the GPU, then Adam
+    epochs with the BF16 matrix-core kernels.  Returns (weights, info)."""
+    net = C.c_void_p()
+    mdz.check(hot.mdz_mlp_create(C.byref(net), C.byref(m.m.materials), C.byref(m.m.model), C.byref(m.m.scaling)))
+    try:
+        t0 = time.perf_counter()
+        mdz.check(hot.mdz_mlp_generate(net, samples, seed))
+        t1 = time.perf_counter()
+        mdz.check(hot.mdz_mlp_init(net, seed + 1))
+        hist = (C.c_double * epochs)()
+        mdz.check(hot.mdz_mlp_train(net, epochs, batch, lr, seed + 2, hist))
+        t2 = time.perf_counter()
+        w = (C.c_float * mdz.MLP_NPARAMS)()
+        mdz.check(hot.mdz_mlp_get_weights(net, w, mdz.MLP_NPARAMS))
+    finally:
+        hot.mdz_mlp_destroy(net)
+    info = {"samples": samples, "epochs": epochs, "batch": batch, "lr": lr, "generate_s": round(t1 - t0, 3),
+            "train_s": round(t2 - t1, 3), "final_mse": hist[epochs - 1]}
+    return w, info
+
+
+def iteration_stats(mdz, hot, ctx):
+    """Mean local iterations of one extra (untimed) unit: FP64, plastic, FP32."""
+    mdz.check(hot.mdz_ctx_collect_histogram(ctx, 1))
+    mdz.check(hot.mdz_ctx_newton_unit(ctx, None))
+    h = (C.c_longlong * 192)()
+    mdz.check(hot.mdz_ctx_iteration_histogram(ctx, h))
+    mdz.check(hot.mdz_ctx_collect_histogram(ctx, 0))
+    out = {}
+    for name, a in (("fp64", 0), ("plastic", 64), ("fp32", 128)):
+        cnt = sum(h[a:a + 64])
+        out[name] = round(sum(i * h[a + i] for i in range(64)) / cnt, 3) if cnt else 0.0
+    return out
+
+
 def cpu_baseline(mdz, ncx, ncz, seed, units):
     """The oracle (C/OpenMP restatement) on the host cores: full-size units,
     one warm-up then the median of `units`."""
@@ -111,6 +147,8 @@ def main():
     ap.add_argument("--cpu-units", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-mb", type=int, default=1024, help="size of the HBM copy probe (MiB)")
+    ap.add_argument("--train-samples", type=int, default=1 << 24, help="surrogate training samples (learned mode)")
+    ap.add_argument("--train-epochs", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path is built and measured on a single MI355X; "
@@ -125,6 +163,10 @@ def main():
     mdz.check(hot.mdz_ctx_create(C.byref(ctx), m.m.mesh, C.byref(m.m.materials), C.byref(m.m.model),
                                  C.byref(m.m.scaling)))
     mode = {"fp64": 0, "mixed": 1, "learned": 2}[args.mode]
+    training = None
+    if mode == 2:
+        w, training = train_surrogate(mdz, hot, m, args.train_samples, args.train_epochs, 65536, 1e-3, args.seed)
+        mdz.check(hot.mdz_ctx_set_network(ctx, w, mdz.MLP_NPARAMS))
     mdz.check(hot.mdz_ctx_set_rheology_mode(ctx, mode))
     cells = args.ncx * args.ncz
 
@@ -141,6 +183,15 @@ def main():
 
     probe = C.c_double()
     mdz.check(hot.mdz_copy_probe(args.probe_mb << 20, 10, C.byref(probe)))
+
+    iters = iteration_stats(mdz, hot, ctx)
+    plain_ms = None
+    if mode != 0:
+        mdz.check(hot.mdz_ctx_set_rheology_mode(ctx, 0))
+        pm = C.c_double()
+        mdz.check(hot.mdz_ctx_time_units(ctx, args.steps, C.byref(pm)))
+        plain_ms = pm.value
+        mdz.check(hot.mdz_ctx_set_rheology_mode(ctx, mode))
 
     st = (C.c_double * 10)()
     mdz.check(hot.mdz_ctx_stage_times(ctx, 3, st))
@@ -188,7 +239,13 @@ def main():
         "whole_unit_algorithmic_GBs": round(whole, 1),
         "host_wall_ms_per_step": round(wall, 4),
         "residual_norms": list(res),
+        "local_iterations_mean": iters,
     }
+    if plain_ms is not None:
+        line["plain_fp64_ms_per_step"] = round(plain_ms, 4)
+        line["speedup_vs_plain_fp64"] = round(plain_ms / ms_per_step, 4)
+    if training is not None:
+        line["surrogate_training"] = training
     print(json.dumps(line))
 
 

@@ -326,7 +326,8 @@ int  mdz_ctx_download_solution(mdz_ctx *ctx, double *u_in, double *v_in, double 
 /* Copy every output field of the path back into *mesh. */
 int  mdz_ctx_download_grid(mdz_ctx *ctx, grid *mesh);
 int  mdz_ctx_set_rheology_mode(mdz_ctx *ctx, int mode);
-/* Loads surrogate-network weights (layout documented in mdz_mlp_*). */
+/* Loads the surrogate network used by MDZ_RHEO_LEARNED (a parameter blob of
+ * MDZ_MLP_NPARAMS floats, see below; models of at most three phases). */
 int  mdz_ctx_set_network(mdz_ctx *ctx, const float *weights, size_t n_floats);
 
 /* One Newton-iteration unit, all device-resident, on the context's stream:
@@ -369,6 +370,43 @@ int  mdz_ctx_time_units(mdz_ctx *ctx, int reps, double *ms_per_unit);
 int  mdz_copy_probe(size_t bytes, int reps, double *gbs);
 /* The HIP stream (hipStream_t) the context launches on. */
 void *mdz_ctx_stream(mdz_ctx *ctx);
+
+/* ---------------------------------------------------------------------- */
+/* Surrogate network for the initial guess of the local iteration (no     */
+/* counterpart in the reference): MLP 9 -> 64 -> 64 -> 64 -> 2, SiLU, BF16 */
+/* matrix-core operands with FP32 accumulation.                           */
+/* Features: (log10 Eii + 15)/3, (T - 1123)/850 [K], (P - 5)/5 [GPa],      */
+/* (tau0 - 250)/250 [MPa], log10 dt - 11 [s], one-hot phase 0..2,          */
+/* (log10 tau_y - 1.5)/1.5 [MPa] with tau_y = C cos(phi) + P sin(phi).     */
+/* Outputs: 4 log10(eta_ve / eta_ref) with eta_ref the reference's guess  */
+/* (smallest isolated viscosity), (log10 max(gdot, 1e-22) + 16)/6.         */
+/* Blob: W1[64][9] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[2][64]    */
+/* b4[2] (row-major [out][in]).                                            */
+/* ---------------------------------------------------------------------- */
+#define MDZ_MLP_NPARAMS 9090
+typedef struct mdz_mlp mdz_mlp;
+/* materials/params/scale: the three-phase model whose flow laws the
+ * training samples use */
+int  mdz_mlp_create(mdz_mlp **out, const mat_prop *materials, const params *model, const scale *scaling);
+void mdz_mlp_destroy(mdz_mlp *net);
+int  mdz_mlp_init(mdz_mlp *net, unsigned long long seed);                   /* He-normal weights */
+int  mdz_mlp_set_weights(mdz_mlp *net, const float *w, size_t n_floats);
+int  mdz_mlp_get_weights(mdz_mlp *net, float *w, size_t n_floats);
+/* `count` samples drawn by `seed` (log10 Eii U[-18,-12], T U[273,1973] K,
+ * P U[0,10] GPa, tau0 U[0,500] MPa, dt log-U[1e10,1e12] s, phase U{0,1,2},
+ * C U[5,50] MPa, phi U[10,35] deg), labelled by the FP64 reference rheology
+ * on the GPU; kept in device memory */
+int  mdz_mlp_generate(mdz_mlp *net, long long count, unsigned long long seed);
+int  mdz_mlp_upload_samples(mdz_mlp *net, long long count, const float *X9, const float *Y2);
+int  mdz_mlp_download_samples(mdz_mlp *net, long long first, long long count, float *X9, float *Y2);
+/* gradient of sum((y - t)^2) / (2 count) over samples [first, first+count),
+ * count a multiple of 64; loss = sum of squared errors */
+int  mdz_mlp_gradient(mdz_mlp *net, long long first, int count, float *grad, double *loss);
+int  mdz_mlp_adam_step(mdz_mlp *net, const float *grad, float lr);         /* beta 0.9/0.999, eps 1e-8 */
+/* Adam epochs over the stored samples; loss_per_epoch[e] = mean squared
+ * error per output */
+int  mdz_mlp_train(mdz_mlp *net, int epochs, int batch, float lr, unsigned long long seed, double *loss_per_epoch);
+int  mdz_mlp_predict(mdz_mlp *net, long long first, long long count, float *Y2);
 
 #ifdef __cplusplus
 }

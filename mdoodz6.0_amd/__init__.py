@@ -127,7 +127,11 @@ HOT_SYMBOLS = ["Viscosity", "NonNewtonianViscosityGrid", "StrainRateComponents",
                "mdz_ctx_download_solution", "mdz_ctx_download_grid", "mdz_ctx_set_rheology_mode",
                "mdz_ctx_set_network", "mdz_ctx_newton_unit", "mdz_ctx_block_info", "mdz_ctx_download_block",
                "mdz_ctx_download_residual", "mdz_ctx_diagonal_scaling", "mdz_ctx_collect_histogram", "mdz_ctx_iteration_histogram", "mdz_ctx_stream",
-               "mdz_ctx_stage_times", "mdz_ctx_time_units", "mdz_copy_probe"]
+               "mdz_ctx_stage_times", "mdz_ctx_time_units", "mdz_copy_probe", "mdz_mlp_create",
+               "mdz_mlp_destroy", "mdz_mlp_init", "mdz_mlp_set_weights", "mdz_mlp_get_weights", "mdz_mlp_generate",
+               "mdz_mlp_upload_samples", "mdz_mlp_download_samples", "mdz_mlp_gradient", "mdz_mlp_adam_step",
+               "mdz_mlp_train", "mdz_mlp_predict"]
+MLP_NPARAMS = 9090
 
 
 def lib_hot():
@@ -169,6 +173,20 @@ def lib_hot():
         L.mdz_ctx_stage_times.argtypes = [C.c_void_p, C.c_int, dp]
         L.mdz_ctx_time_units.argtypes = [C.c_void_p, C.c_int, dp]
         L.mdz_copy_probe.argtypes = [C.c_size_t, C.c_int, dp]
+        fp, vp, ll = C.POINTER(C.c_float), C.c_void_p, C.c_longlong
+        L.mdz_mlp_create.argtypes = [C.POINTER(vp), C.POINTER(MP), C.POINTER(PR), C.POINTER(SC)]
+        L.mdz_mlp_destroy.argtypes = [vp]
+        L.mdz_mlp_destroy.restype = None
+        L.mdz_mlp_init.argtypes = [vp, C.c_ulonglong]
+        L.mdz_mlp_set_weights.argtypes = [vp, fp, C.c_size_t]
+        L.mdz_mlp_get_weights.argtypes = [vp, fp, C.c_size_t]
+        L.mdz_mlp_generate.argtypes = [vp, ll, C.c_ulonglong]
+        L.mdz_mlp_upload_samples.argtypes = [vp, ll, fp, fp]
+        L.mdz_mlp_download_samples.argtypes = [vp, ll, ll, fp, fp]
+        L.mdz_mlp_gradient.argtypes = [vp, ll, C.c_int, fp, dp]
+        L.mdz_mlp_adam_step.argtypes = [vp, fp, C.c_float]
+        L.mdz_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, dp]
+        L.mdz_mlp_predict.argtypes = [vp, ll, ll, fp]
         L.mdz_ctx_stream.restype = C.c_void_p
         L.mdz_ctx_stream.argtypes = [C.c_void_p]
         L.mdz_set_rheology_kernel.argtypes = [C.c_int]

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "mdz_kernels.h"
+#include "mdz_mlp.h"
 #include "mdz_viscosity.h"
 
 /* ------------------------------------------------------------------ */
@@ -43,6 +44,9 @@ static int set_err(int code, const char *fmt, ...) {
     g_status = code;
     return code;
 }
+
+/* error reporting for the other translation units of the library */
+int mdz_error(int code, const char *msg) { return set_err(code, "%s", msg); }
 
 #define HIPCHK(x)                                                                                  \
     do {                                                                                           \
@@ -120,16 +124,18 @@ struct mdz_ctx {
     params model;
     mat_prop materials;
     scale scaling;
-    /* surrogate network (set by mdz_ctx_set_network) */
+    /* surrogate network (set by mdz_ctx_set_network): blob and kernel layout */
     float *d_net;
     size_t net_floats;
+    unsigned short *net_wp;
+    float *net_bp;
 };
 
 static void fill_params(mdz_ctx *c, const params *m, const scale *s) {
     DevParams &p = c->prm;
     p.dt = m->dt; p.mineta = m->mineta; p.maxeta = m->maxeta; p.gx = m->gx; p.gz = m->gz;
     p.free_surf_stab = m->free_surf_stab; p.compressible = m->compressible;
-    p.sT = s->T; p.sE = s->E; p.seta = s->eta; p.sS = s->S; p.sL = s->L;
+    p.sT = s->T; p.sE = s->E; p.seta = s->eta; p.sS = s->S; p.sL = s->L; p.st = s->t;
     p.step = m->step; p.iselastic = m->iselastic; p.eta_avg = m->eta_avg; p.aniso = m->aniso;
     p.aniso_fstrain = m->aniso_fstrain; p.isperiodic_x = m->isperiodic_x; p.UnsplitDiffReac = m->UnsplitDiffReac;
     p.diffuse_X = m->diffuse_X; p.diffuse_avg = m->diffuse_avg; p.polar = m->polar;
@@ -256,7 +262,7 @@ static void ctx_release(mdz_ctx *c) {
     ctx_free_blocks(c);
     hipFree(c->slab); hipFree(c->scratch); hipFree(c->norm_partial); hipFree(c->norm_out);
     hipFree(c->status); hipFree(c->hist); hipFree(c->d_mat); hipFree(c->d_pk); hipFree(c->guess_n); hipFree(c->guess_s);
-    hipFree(c->d_net);
+    hipFree(c->d_net); hipFree(c->net_wp); hipFree(c->net_bp);
     hipStreamDestroy(c->stream);
     free(c);
 }
@@ -491,6 +497,12 @@ extern "C" int mdz_set_rheology_kernel(int which) {
 }
 
 static int rheology_pass(mdz_ctx *c, int use_guess) {
+    /* vertex values of div_u, T, P, d and phi (NonNewtonianViscosityGrid :979-983) */
+    mdz_launch_interp5(c->g, c->prm, c->stream);
+    if (use_guess) {
+        if (!c->net_wp || !c->guess_n || !c->guess_s) return set_err(MDZ_EINVAL, "learned guess without a network");
+        mdz_launch_mlp_guess(c->g, c->prm, DevMLPW{c->net_wp, c->net_bp}, c->guess_n, c->guess_s, c->stream);
+    }
     const float *gn = use_guess ? c->guess_n : nullptr, *gs = use_guess ? c->guess_s : nullptr;
     if (g_rheo_kernel == 0 && mdz_rheo_fast_eligible(c->prm, &c->materials))
         mdz_launch_rheology_fast(c->g, c->prm, c->d_pk, c->materials.R, gn, gs, c->hist_on ? c->hist : nullptr,
@@ -921,7 +933,7 @@ __global__ void k_viscosity_point(ViscArgs a, const mat_prop *mt, DevParams prm,
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     VGuess gs;
     gs.mode = MDZ_RHEO_FP64;
-    gs.eta_ve0 = 0.0;
+    gs.factor = 0.0;
     VOut o;
     mdz_viscosity(a.phase, a.G, a.T, a.P, a.d, a.phi, a.X0, a.Exx, a.Ezz, a.Exz, a.Gxx, a.Gzz, a.Gxz, a.Txx0, a.Tzz0,
                   a.Txz0, mt, prm, a.dil, a.fric, a.C, a.P0, a.beta, gs, o);
@@ -1117,11 +1129,22 @@ extern "C" int mdz_ctx_iteration_histogram(mdz_ctx *c, long long *hist) {
 }
 
 extern "C" int mdz_ctx_set_network(mdz_ctx *c, const float *w, size_t n) {
-    if (!c || !w || !n) return set_err(MDZ_EINVAL, "network weights");
-    hipFree(c->d_net);
-    c->d_net = nullptr;
-    HIPCHK(hipMalloc((void **)&c->d_net, n * sizeof(float)));
-    HIPCHK(hipMemcpy(c->d_net, w, n * sizeof(float), hipMemcpyHostToDevice));
+    if (!c || !w) return set_err(MDZ_EINVAL, "network weights");
+    if (n != MLP_NPARAMS) return set_err(MDZ_EINVAL, "network weights: %zu floats, expected %d", n, MLP_NPARAMS);
+    if (c->nph > 3 || c->prm.Nb_phases > 3) return set_err(MDZ_EUNSUPPORTED, "the network encodes three phases");
+    if (!c->d_net) {
+        HIPCHK(hipMalloc((void **)&c->d_net, n * sizeof(float)));
+        HIPCHK(hipMalloc((void **)&c->net_wp, MLP_WP_TOTAL * sizeof(unsigned short)));
+        HIPCHK(hipMalloc((void **)&c->net_bp, MLP_BP_TOTAL * sizeof(float)));
+    }
+    if (!c->guess_n) {
+        HIPCHK(hipMalloc((void **)&c->guess_n, (size_t)c->nph * (c->Nx - 1) * (c->Nz - 1) * sizeof(float)));
+        HIPCHK(hipMalloc((void **)&c->guess_s, (size_t)c->nph * c->Nx * c->Nz * sizeof(float)));
+    }
+    HIPCHK(hipMemcpyAsync(c->d_net, w, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    mdz_mlp_pack(c->d_net, c->net_wp, c->net_bp, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipGetLastError());
     c->net_floats = n;
     return MDZ_OK;
 }

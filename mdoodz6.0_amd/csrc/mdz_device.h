@@ -43,11 +43,16 @@ struct DevGrid {
 /* Scalar model parameters the kernels read (from params and scale). */
 struct DevParams {
     double dt, mineta, maxeta, gx, gz, free_surf_stab, compressible;
-    double sT, sE, seta, sS, sL;
+    double sT, sE, seta, sS, sL, st;
     int step, iselastic, eta_avg, aniso, aniso_fstrain, isperiodic_x, UnsplitDiffReac;
     int diffuse_X, diffuse_avg, polar, VolChangeReac, adiab_heat, Nb_phases, Nx, Nz;
     int rheo_mode; /* MDZ_RHEO_* */
 };
+
+/* Stop of the FP32 stage of the mixed-precision local solve, |r/Eii| (the
+ * FP32 evaluation of C Tii^n = exp(log C + n log Tii) carries ~1e-6 relative
+ * error for large exponents); the FP64 polish then goes to 1e-13. */
+#define MDZ_F32_TOL 1e-5f
 
 /* Histogram bins of the local-iteration counters. */
 #define MDZ_HIST_BINS 64

@@ -5,9 +5,13 @@
 
 #include "mdz_device.h"
 
+/* records an error for mdz_last_error() / mdz_last_status(); returns code */
+int mdz_error(int code, const char *msg);
+
 /* rheology half (mdz_rheology.hip) */
 void mdz_launch_strain_rates(const DevGrid &g, const DevParams &prm, hipStream_t s);
 void mdz_launch_interp(const DevGrid &g, const DevParams &prm, const double *src, double *dst, hipStream_t s);
+/* grid rheology (the vertex fields of mdz_launch_interp5 must be in place) */
 void mdz_launch_rheology(const DevGrid &g, const DevParams &prm, const mat_prop *mt, const float *guess_n,
                          const float *guess_s, unsigned long long *hist, hipStream_t s);
 /* div_u, T, P, d, phi from centroids to vertices (input of the vertex rheology) */

@@ -1,0 +1,809 @@
+/*
+ * mdz_mlp.hip -- the surrogate network of the local visco-elastic iteration
+ * on the matrix cores of gfx950: training-data generation with the FP64
+ * reference rheology (Viscosity, RheologyDensity.c:44-707), forward /
+ * backward / Adam kernels (BF16 operands of v_mfma_f32_16x16x16_bf16, FP32
+ * accumulation), and the fused grid inference that writes the initial guess
+ * of eta_ve for the rheology kernels.
+ *
+ * Tile convention of the 16x16x16 BF16 matrix instruction, lane l = 16 q + r
+ * (r = l mod 16, q = l / 16):
+ *   A (M x K):  lane holds A[r][4q .. 4q+3]
+ *   B (K x N):  lane holds B[4q .. 4q+3][r]
+ *   D (M x N):  lane holds D[4q .. 4q+3][r]
+ * A layer's output tile in D layout (neurons x samples) is therefore, after
+ * rounding to BF16, exactly the B operand of the next layer's K step: the
+ * whole forward pass of 16 samples stays in registers.  One wave handles 16
+ * samples; a training block of 4 waves handles 64, and the weight gradients
+ * (K = samples) are formed from the block's activations staged in LDS.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "mdz_kernels.h"
+#include "mdz_mlp.h"
+#include "mdz_viscosity.h"
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+#define TBLK 256 /* training / inference block: 4 waves x 16 samples */
+#define TSAMP 64
+
+/* LDS copies of the weights and activations use padded rows (16 -> 20 and
+ * 64 -> 68 BF16 entries) so that the 16 rows one operand load touches fall
+ * into different banks */
+#define LS16 20
+#define LS64 68
+#define LDS_W1 0
+#define LDS_W2 (LDS_W1 + 64 * LS16)
+#define LDS_W3 (LDS_W2 + 64 * LS64)
+#define LDS_W4 (LDS_W3 + 64 * LS64)
+#define LDS_FWD (LDS_W4 + 16 * LS64) /* end of the forward weights */
+#define LDS_W2T LDS_FWD
+#define LDS_W3T (LDS_W2T + 64 * LS64)
+#define LDS_W4T (LDS_W3T + 64 * LS64)
+#define LDS_ALL (LDS_W4T + 64 * LS16)
+#define AS LS64 /* row stride of the staged activations [rows][64 samples] */
+
+/* FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32) */
+__device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
+__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float(((unsigned)b) << 16); }
+
+__device__ __forceinline__ bf16x4 pack4(f32x4 v) {
+    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+    bf4 b;
+    b[0] = (__bf16)v[0];
+    b[1] = (__bf16)v[1];
+    b[2] = (__bf16)v[2];
+    b[3] = (__bf16)v[3];
+    return __builtin_bit_cast(bf16x4, b);
+}
+
+__device__ __forceinline__ bf16x4 lds4(const unsigned short *p) { return *(const bf16x4 *)p; }
+
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float sigmoidf_(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoidf_(z); }
+__device__ __forceinline__ float dsilu(float z) {
+    float s = sigmoidf_(z);
+    return s * (1.0f + z * (1.0f - s));
+}
+
+__device__ __forceinline__ f32x4 silu4(f32x4 z) {
+    f32x4 h;
+    h[0] = silu(z[0]); h[1] = silu(z[1]); h[2] = silu(z[2]); h[3] = silu(z[3]);
+    return h;
+}
+
+/* z[mt] = W[16mt .. 16mt+15][:] * B (+ bias), NK K-steps of 16 inputs; W
+ * row-major with leading dimension ldw (in LDS); NM output tiles. */
+template <int NM, int NK, bool BIAS = true>
+__device__ __forceinline__ void dense(const unsigned short *W, int ldw, const bf16x4 *b, const float *bias, f32x4 *z) {
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+#pragma unroll
+    for (int mt = 0; mt < NM; mt++) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < NK; ks++) acc = mfma(lds4(W + (16 * mt + r) * ldw + 16 * ks + 4 * q), b[ks], acc);
+#pragma unroll
+        for (int i = 0; i < 4; i++) z[mt][i] = BIAS ? acc[i] + bias[16 * mt + 4 * q + i] : acc[i];
+    }
+}
+
+/* forward pass of one wave's 16 samples (weights in the padded LDS layout);
+ * x = the lane's 4 features (BF16).
+ * Returns the pre-activations of the hidden layers and the two outputs in
+ * lanes q == 0 (y[0], y[1]). */
+__device__ __forceinline__ void mlp_forward(const unsigned short *W, const float *B, bf16x4 x, f32x4 *z1, f32x4 *z2,
+                                            f32x4 *z3, f32x4 &y) {
+    bf16x4 h[4];
+    dense<4, 1>(W + LDS_W1, LS16, &x, B, z1);
+#pragma unroll
+    for (int t = 0; t < 4; t++) h[t] = pack4(silu4(z1[t]));
+    dense<4, 4>(W + LDS_W2, LS64, h, B + 64, z2);
+#pragma unroll
+    for (int t = 0; t < 4; t++) h[t] = pack4(silu4(z2[t]));
+    dense<4, 4>(W + LDS_W3, LS64, h, B + 128, z3);
+#pragma unroll
+    for (int t = 0; t < 4; t++) h[t] = pack4(silu4(z3[t]));
+    dense<1, 4>(W + LDS_W4, LS64, h, B + 192, &y);
+}
+
+/* ------------------------------------------------------------------ */
+/* features                                                             */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ void mlp_features(double Eii_s, double T_K, double P_Pa, double tau0_Pa, double dt_s,
+                                             int phase, double tauy_Pa, float *f) {
+    f[0] = (float)((log10(fmax(Eii_s, 1e-30)) + 15.0) / 3.0);
+    f[1] = (float)((T_K - 1123.0) / 850.0);
+    f[2] = (float)((P_Pa * 1e-9 - 5.0) / 5.0);
+    f[3] = (float)((tau0_Pa * 1e-6 - 250.0) / 250.0);
+    f[4] = (float)(log10(fmax(dt_s, 1e-30)) - 11.0);
+    f[5] = phase == 0 ? 1.0f : 0.0f;
+    f[6] = phase == 1 ? 1.0f : 0.0f;
+    f[7] = phase == 2 ? 1.0f : 0.0f;
+    f[8] = (float)((log10(fmax(tauy_Pa * 1e-6, 1e-3)) - 1.5) / 1.5);
+}
+
+/* the lane's 4 features 4q .. 4q+3 (9 .. 15 are zero) */
+__device__ __forceinline__ bf16x4 lane_features(const float *f) {
+    const int q = (threadIdx.x & 63) >> 4;
+    bf16x4 x;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int k = 4 * q + j;
+        x[j] = (short)f2bf(k < MLP_IN ? f[k] : 0.0f);
+    }
+    return x;
+}
+
+/* ------------------------------------------------------------------ */
+/* parameter layout                                                     */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ void pack_param(int p, float w, unsigned short *wp, float *bp) {
+    unsigned short b = f2bf(w);
+    if (p < MLP_OFF_B1) {
+        int o = p / MLP_IN, i = p % MLP_IN;
+        wp[MLP_WP_W1 + o * 16 + i] = b;
+    } else if (p < MLP_OFF_W2) {
+        bp[p - MLP_OFF_B1] = w;
+    } else if (p < MLP_OFF_B2) {
+        int o = (p - MLP_OFF_W2) / 64, i = (p - MLP_OFF_W2) % 64;
+        wp[MLP_WP_W2 + o * 64 + i] = b;
+        wp[MLP_WP_W2T + i * 64 + o] = b;
+    } else if (p < MLP_OFF_W3) {
+        bp[64 + p - MLP_OFF_B2] = w;
+    } else if (p < MLP_OFF_B3) {
+        int o = (p - MLP_OFF_W3) / 64, i = (p - MLP_OFF_W3) % 64;
+        wp[MLP_WP_W3 + o * 64 + i] = b;
+        wp[MLP_WP_W3T + i * 64 + o] = b;
+    } else if (p < MLP_OFF_W4) {
+        bp[128 + p - MLP_OFF_B3] = w;
+    } else if (p < MLP_OFF_B4) {
+        int o = (p - MLP_OFF_W4) / 64, i = (p - MLP_OFF_W4) % 64;
+        wp[MLP_WP_W4 + o * 64 + i] = b;
+        wp[MLP_WP_W4T + i * 16 + o] = b;
+    } else {
+        bp[192 + p - MLP_OFF_B4] = w;
+    }
+}
+
+/* zero the padded buffers, then scatter the blob (one block of 256 threads
+ * per part; launched as two kernels) */
+__global__ void k_mlp_zero(unsigned short *wp, float *bp) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < MLP_WP_TOTAL; i += gridDim.x * 256) wp[i] = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < MLP_BP_TOTAL; i += gridDim.x * 256) bp[i] = 0.0f;
+}
+__global__ void k_mlp_scatter(const float *__restrict__ blob, unsigned short *wp, float *bp) {
+    int p = blockIdx.x * 256 + threadIdx.x;
+    if (p < MLP_NPARAMS) pack_param(p, blob[p], wp, bp);
+}
+
+void mdz_mlp_pack(const float *blob, unsigned short *wp, float *bp, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_zero, dim3(16), dim3(256), 0, s, wp, bp);
+    hipLaunchKernelGGL(k_mlp_scatter, dim3(mdz_div_up(MLP_NPARAMS, 256)), dim3(256), 0, s, blob, wp, bp);
+}
+
+/* dense kernel-layout index -> padded LDS index */
+__device__ __forceinline__ int lds_index(int i) {
+    if (i < MLP_WP_W2) return LDS_W1 + (i / 16) * LS16 + i % 16;
+    if (i < MLP_WP_W3) { i -= MLP_WP_W2; return LDS_W2 + (i / 64) * LS64 + i % 64; }
+    if (i < MLP_WP_W4) { i -= MLP_WP_W3; return LDS_W3 + (i / 64) * LS64 + i % 64; }
+    if (i < MLP_WP_W2T) { i -= MLP_WP_W4; return LDS_W4 + (i / 64) * LS64 + i % 64; }
+    if (i < MLP_WP_W3T) { i -= MLP_WP_W2T; return LDS_W2T + (i / 64) * LS64 + i % 64; }
+    if (i < MLP_WP_W4T) { i -= MLP_WP_W3T; return LDS_W3T + (i / 64) * LS64 + i % 64; }
+    i -= MLP_WP_W4T;
+    return LDS_W4T + (i / 16) * LS16 + i % 16;
+}
+
+/* weights into LDS: the forward ones (nw = MLP_WP_W2T) or all (MLP_WP_TOTAL),
+ * two entries per thread and step */
+__device__ __forceinline__ void load_weights(DevMLPW w, unsigned short *sW, float *sB, int nw) {
+    const unsigned *src = (const unsigned *)w.wp;
+    for (int i = threadIdx.x; i < nw / 2; i += blockDim.x) {
+        const unsigned v = src[i];
+        const int d = lds_index(2 * i); /* rows have even lengths: 2i and 2i+1 share a row */
+        *(unsigned *)(sW + d) = v;
+    }
+    for (int i = threadIdx.x; i < MLP_BP_TOTAL; i += blockDim.x) sB[i] = w.bp[i];
+}
+
+/* ------------------------------------------------------------------ */
+/* training step: forward + backward of 64 samples per block            */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ long long sample_index(long long j, long long n, unsigned long long mul,
+                                                  unsigned long long add) {
+    return (long long)((((unsigned long long)j % (unsigned long long)n) * mul + add) % (unsigned long long)n);
+}
+
+/* partial[block][p]: this block's gradient of sum((y - t)^2) / (2 * batch);
+ * loss[block]: this block's sum of squared errors */
+__global__ void __launch_bounds__(TBLK) k_mlp_train_step(const float *__restrict__ X, const float *__restrict__ Y,
+                                                         long long n, long long first, unsigned long long mul,
+                                                         unsigned long long add, float inv_batch, DevMLPW w,
+                                                         float *__restrict__ partial, float *__restrict__ loss) {
+    __shared__ __attribute__((aligned(16))) unsigned short sW[LDS_ALL];
+    __shared__ float sB[MLP_BP_TOTAL];
+    __shared__ __attribute__((aligned(16))) unsigned short Xs[16 * AS];
+    __shared__ __attribute__((aligned(16))) unsigned short H1s[64 * AS], H2s[64 * AS], H3s[64 * AS];
+    __shared__ __attribute__((aligned(16))) unsigned short D1s[64 * AS], D2s[64 * AS], D3s[64 * AS], D4s[16 * AS];
+    __shared__ float sLoss[4];
+    load_weights(w, sW, sB, MLP_WP_TOTAL);
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4, wv = threadIdx.x >> 6;
+    const int s = 16 * wv + r; /* sample within the block */
+    const long long idx = sample_index(first + (long long)blockIdx.x * TSAMP + s, n, mul, add);
+    /* features and targets */
+    float f[MLP_IN];
+#pragma unroll
+    for (int k = 0; k < MLP_IN; k++) f[k] = X[idx * MLP_IN + k];
+    const bf16x4 x = lane_features(f);
+#pragma unroll
+    for (int j = 0; j < 4; j++) Xs[(4 * q + j) * AS + s] = (unsigned short)x[j];
+    float t0 = 0.0f, t1 = 0.0f;
+    if (q == 0) {
+        t0 = Y[idx * 2];
+        t1 = Y[idx * 2 + 1];
+    }
+    __syncthreads();
+
+    /* forward */
+    f32x4 z1[4], z2[4], z3[4], y;
+    mlp_forward(sW, sB, x, z1, z2, z3, y);
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = 16 * mt + 4 * q + i;
+            H1s[row * AS + s] = f2bf(silu(z1[mt][i]));
+            H2s[row * AS + s] = f2bf(silu(z2[mt][i]));
+            H3s[row * AS + s] = f2bf(silu(z3[mt][i]));
+        }
+    /* loss and output gradient (rows 0, 1 in lanes q == 0) */
+    f32x4 dz4 = {0.0f, 0.0f, 0.0f, 0.0f};
+    float se = 0.0f;
+    if (q == 0) {
+        float e0 = y[0] - t0, e1 = y[1] - t1;
+        se = e0 * e0 + e1 * e1;
+        dz4[0] = e0 * inv_batch;
+        dz4[1] = e1 * inv_batch;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) D4s[(4 * q + i) * AS + s] = f2bf(dz4[i]);
+    for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o, 64);
+    if (l == 0) sLoss[wv] = se;
+
+    /* backward through the hidden layers */
+    f32x4 d[4];
+    bf16x4 bd[4];
+    bd[0] = pack4(dz4);
+    dense<4, 1, false>(sW + LDS_W4T, LS16, bd, nullptr, d);
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) d[mt][i] *= dsilu(z3[mt][i]);
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        bd[mt] = pack4(d[mt]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) D3s[(16 * mt + 4 * q + i) * AS + s] = (unsigned short)bd[mt][i];
+    }
+    dense<4, 4, false>(sW + LDS_W3T, LS64, bd, nullptr, d);
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) d[mt][i] *= dsilu(z2[mt][i]);
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        bd[mt] = pack4(d[mt]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) D2s[(16 * mt + 4 * q + i) * AS + s] = (unsigned short)bd[mt][i];
+    }
+    dense<4, 4, false>(sW + LDS_W2T, LS64, bd, nullptr, d);
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) D1s[(16 * mt + 4 * q + i) * AS + s] = f2bf(d[mt][i] * dsilu(z1[mt][i]));
+    __syncthreads();
+
+    /* weight gradients, K = the block's 64 samples; wave wv owns output rows
+     * 16 wv .. 16 wv + 15 of W1..W3 and input columns 16 wv .. of W4 */
+    float *P = partial + (size_t)blockIdx.x * MLP_NPARAMS;
+    {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+            acc = mfma(lds4(D1s + (16 * wv + r) * AS + 16 * ks + 4 * q), lds4(Xs + r * AS + 16 * ks + 4 * q), acc);
+        if (r < MLP_IN)
+#pragma unroll
+            for (int i = 0; i < 4; i++) P[MLP_OFF_W1 + (16 * wv + 4 * q + i) * MLP_IN + r] = acc[i];
+    }
+#pragma unroll
+    for (int layer = 0; layer < 2; layer++) {
+        const unsigned short *Dz = layer == 0 ? D2s : D3s;
+        const unsigned short *Hin = layer == 0 ? H1s : H2s;
+        const int off = layer == 0 ? MLP_OFF_W2 : MLP_OFF_W3;
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++)
+                acc = mfma(lds4(Dz + (16 * wv + r) * AS + 16 * ks + 4 * q),
+                           lds4(Hin + (16 * nt + r) * AS + 16 * ks + 4 * q), acc);
+#pragma unroll
+            for (int i = 0; i < 4; i++) P[off + (16 * wv + 4 * q + i) * 64 + 16 * nt + r] = acc[i];
+        }
+    }
+    {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+            acc = mfma(lds4(D4s + r * AS + 16 * ks + 4 * q), lds4(H3s + (16 * wv + r) * AS + 16 * ks + 4 * q), acc);
+        if (q == 0) {
+            P[MLP_OFF_W4 + 0 * 64 + 16 * wv + r] = acc[0];
+            P[MLP_OFF_W4 + 1 * 64 + 16 * wv + r] = acc[1];
+        }
+    }
+    /* bias gradients: row sums of the staged output gradients */
+    for (int o = threadIdx.x; o < 3 * 64 + MLP_OUT; o += TBLK) {
+        const unsigned short *row = o < 64 ? D1s + o * AS : o < 128 ? D2s + (o - 64) * AS
+                                  : o < 192 ? D3s + (o - 128) * AS : D4s + (o - 192) * AS;
+        float acc = 0.0f;
+        for (int j = 0; j < TSAMP; j++) acc += bf2f(row[j]);
+        const int dst = o < 64 ? MLP_OFF_B1 + o : o < 128 ? MLP_OFF_B2 + o - 64 : o < 192 ? MLP_OFF_B3 + o - 128
+                                                                                           : MLP_OFF_B4 + o - 192;
+        P[dst] = acc;
+    }
+    if (threadIdx.x == 0) loss[blockIdx.x] = sLoss[0] + sLoss[1] + sLoss[2] + sLoss[3];
+}
+
+/* g[p] = sum over blocks of partial[block][p] (fixed order); loss_acc +=
+ * sum of the block losses */
+__global__ void k_mlp_reduce(const float *__restrict__ partial, int nblocks, float *__restrict__ g,
+                             const float *__restrict__ loss, double *loss_acc) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p < MLP_NPARAMS) {
+        float acc = 0.0f;
+        for (int b = 0; b < nblocks; b++) acc += partial[(size_t)b * MLP_NPARAMS + p];
+        g[p] = acc;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && loss_acc) {
+        double s = 0.0;
+        for (int b = 0; b < nblocks; b++) s += loss[b];
+        *loss_acc += s;
+    }
+}
+
+/* Adam (Kingma & Ba 2015) on the FP32 master weights; refreshes the BF16
+ * kernel copies */
+__global__ void k_mlp_adam(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
+                           const float *__restrict__ g, float lr, float b1, float b2, float eps, float c1, float c2,
+                           unsigned short *wp, float *bp) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= MLP_NPARAMS) return;
+    const float gp = g[p];
+    const float mp = b1 * m[p] + (1.0f - b1) * gp;
+    const float vp = b2 * v[p] + (1.0f - b2) * gp * gp;
+    m[p] = mp;
+    v[p] = vp;
+    const float wn = w[p] - lr * (mp / c1) / (sqrtf(vp / c2) + eps);
+    w[p] = wn;
+    pack_param(p, wn, wp, bp);
+}
+
+/* forward pass over stored samples (outputs Y[n][2]) */
+__global__ void __launch_bounds__(TBLK) k_mlp_predict(const float *__restrict__ X, long long first, long long count,
+                                                      DevMLPW w, float *__restrict__ Yout) {
+    __shared__ __attribute__((aligned(16))) unsigned short sW[LDS_FWD];
+    __shared__ float sB[MLP_BP_TOTAL];
+    load_weights(w, sW, sB, MLP_WP_W2T);
+    __syncthreads();
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4, wv = threadIdx.x >> 6;
+    const long long j = (long long)blockIdx.x * TSAMP + 16 * wv + r;
+    float f[MLP_IN];
+#pragma unroll
+    for (int k = 0; k < MLP_IN; k++) f[k] = j < count ? X[(first + j) * MLP_IN + k] : 0.0f;
+    f32x4 z1[4], z2[4], z3[4], y;
+    mlp_forward(sW, sB, lane_features(f), z1, z2, z3, y);
+    if (q == 0 && j < count) {
+        Yout[j * 2] = y[0];
+        Yout[j * 2 + 1] = y[1];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* training data: the FP64 reference rheology on sampled inputs         */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ unsigned long long splitmix(unsigned long long x) {
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ double urand(unsigned long long seed, long long i, int k) {
+    return (double)(splitmix(seed ^ splitmix((unsigned long long)i * 16ULL + (unsigned long long)k)) >> 11) *
+           (1.0 / 9007199254740992.0);
+}
+
+__global__ void k_mlp_generate(long long first, long long count, unsigned long long seed, const mat_prop *mt,
+                               DevParams prm, float *__restrict__ X, float *__restrict__ Y) {
+    const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= count) return;
+    const long long i = first + j;
+    /* SI draws */
+    const double lE = -18.0 + 6.0 * urand(seed, i, 0);
+    const double T_K = 273.0 + 1700.0 * urand(seed, i, 1);
+    const double P_Pa = 10.0e9 * urand(seed, i, 2);
+    const double tau0 = 500.0e6 * urand(seed, i, 3);
+    const double dt_s = pow(10.0, 10.0 + 2.0 * urand(seed, i, 4));
+    int phase = (int)(3.0 * urand(seed, i, 5));
+    phase = phase > 2 ? 2 : phase;
+    const double coh = 5.0e6 + 45.0e6 * urand(seed, i, 6);
+    const double fric = (10.0 + 25.0 * urand(seed, i, 7)) * M_PI / 180.0;
+    /* nondimensional local update (pure shear; the old stress enters the
+     * effective strain rate already) */
+    DevParams p = prm;
+    p.dt = dt_s / prm.st;
+    p.rheo_mode = MDZ_RHEO_FP64;
+    const double E = pow(10.0, lE) / prm.sE;
+    const double P = P_Pa / prm.sS, T = T_K / prm.sT, t0 = tau0 / prm.sS, C = coh / prm.sS;
+    VGuess gs;
+    gs.mode = MDZ_RHEO_FP64;
+    gs.factor = 0.0;
+    VOut o;
+    mdz_viscosity(phase, mt->mu[phase], T, P, mt->gs_ref[phase], 0.0, 0.0, E, -E, 0.0, E, -E, 0.0, t0, -t0, 0.0, mt, p,
+                  0.0, fric, C, P, mt->bet[phase], gs, o);
+    float f[MLP_IN];
+    mlp_features(pow(10.0, lE), T_K, P_Pa, tau0, dt_s, phase, coh * cos(fric) + P_Pa * sin(fric), f);
+#pragma unroll
+    for (int k = 0; k < MLP_IN; k++) X[j * MLP_IN + k] = f[k];
+    Y[j * 2] = (float)(4.0 * log10(o.eta_ve / o.eta_up));
+    Y[j * 2 + 1] = (float)((log10(fmax(o.gdot * prm.sE, 1e-22)) + 16.0) / 6.0);
+}
+
+/* ------------------------------------------------------------------ */
+/* grid inference: initial guess of eta_ve                              */
+/* ------------------------------------------------------------------ */
+/* dominant phase among the first three with a fraction above 1e-13, or -1 */
+__device__ __forceinline__ int dominant_phase(const double *frac, size_t stride, int node, int nph) {
+    int best = -1;
+    double bf = 1.0e-13;
+    for (int p = 0; p < nph; p++) {
+        double fr = frac[(size_t)p * stride + node];
+        if (fabs(fr) > bf) { bf = fabs(fr); best = p; }
+    }
+    return best <= 2 ? best : -1;
+}
+
+template <bool CELL>
+__global__ void __launch_bounds__(TBLK) k_mlp_guess(DevGrid g, DevParams prm, DevMLPW w, float *__restrict__ guess) {
+    __shared__ __attribute__((aligned(16))) unsigned short sW[LDS_FWD];
+    __shared__ float sB[MLP_BP_TOTAL];
+    load_weights(w, sW, sB, MLP_WP_W2T);
+    __syncthreads();
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4, wv = threadIdx.x >> 6;
+    const int n = CELL ? g.ncx * g.ncz : g.Nx * g.Nz;
+    const int ntiles = (n + TSAMP - 1) / TSAMP;
+    /* grid-stride over tiles of 64 nodes: the weights are staged once per block */
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int node = tile * TSAMP + 16 * wv + r;
+    const bool inside = node < n;
+    int ph = -1;
+    float f[MLP_IN] = {0.0f};
+    if (inside) {
+        const bool active = CELL ? (g.BCp[node] != 30 && g.BCp[node] != 31) : g.BCg[node] != 30;
+        if (active) ph = dominant_phase(CELL ? g.phase_n : g.phase_s, n, node, min(prm.Nb_phases, g.nph));
+        if (ph >= 0) {
+            /* features in FP32 (they are rounded to BF16 for the network) */
+            const float mu = (float)(CELL ? g.mu_n[node] : g.mu_s[node]);
+            const float etae = prm.iselastic == 1 ? (float)prm.dt * mu : 1.0f;
+            const float sxx0 = (float)(CELL ? g.sxxd0[node] : g.sxxd0_s[node]);
+            const float szz0 = (float)(CELL ? g.szzd0[node] : g.szzd0_s[node]);
+            const float sxz0 = (float)(CELL ? g.sxz0_n[node] : g.sxz0[node]);
+            const float Exx = (float)(CELL ? g.exxd[node] : g.exxd_s[node]) + 0.5f * sxx0 / etae;
+            const float Ezz = (float)(CELL ? g.ezzd[node] : g.ezzd_s[node]) + 0.5f * szz0 / etae;
+            const float Exz = (float)(CELL ? g.exz_n[node] : g.exz[node]) + 0.5f * sxz0 / etae;
+            const float Eyy = -(Exx + Ezz);
+            const float Eii = sqrtf(0.5f * (Exx * Exx + Ezz * Ezz + Eyy * Eyy) + Exz * Exz);
+            const float sT = (float)prm.sT, sS = (float)prm.sS;
+            const float T = fmaxf((float)(CELL ? g.T[node] : g.T_s[node]) * sT, 273.15f);
+            const float P = (float)(CELL ? g.p_in[node] : g.P_s[node]) * sS;
+            const float C = (float)(CELL ? g.C_n[node] : g.C_s[node]) * sS;
+            const float fr = (float)(CELL ? g.fric_n[node] : g.fric_s[node]);
+            const float L10 = 0.43429448f; /* log10(e) */
+            const float tauy = fmaxf(C * __cosf(fr) + P * __sinf(fr), 1.0e3f);
+            f[0] = (__logf(fmaxf(Eii * (float)prm.sE, 1e-30f)) * L10 + 15.0f) / 3.0f;
+            f[1] = (T - 1123.0f) / 850.0f;
+            f[2] = (P * 1e-9f - 5.0f) / 5.0f;
+            f[3] = (sqrtf(sxx0 * sxx0 + sxz0 * sxz0) * sS * 1e-6f - 250.0f) / 250.0f;
+            f[4] = __logf((float)(prm.dt * prm.st)) * L10 - 11.0f;
+            f[5] = ph == 0 ? 1.0f : 0.0f;
+            f[6] = ph == 1 ? 1.0f : 0.0f;
+            f[7] = ph == 2 ? 1.0f : 0.0f;
+            f[8] = (__logf(tauy * 1e-6f) * L10 - 1.5f) / 1.5f;
+            /* outside the sampled domain the reference guess is kept */
+            const float lim = 1.02f;
+            if (fabsf(f[0]) > lim || fabsf(f[1]) > lim || fabsf(f[2]) > lim || fabsf(f[4]) > lim) ph = -1;
+        }
+    }
+    /* every lane of the wave takes part in the matrix instructions */
+    f32x4 z1[4], z2[4], z3[4], y;
+    mlp_forward(sW, sB, lane_features(f), z1, z2, z3, y);
+    if (q == 0 && inside) {
+        /* factor on the reference's guess, kept on the upper side of the
+         * root: the strain-rate sum is convex and increasing in log(eta), so
+         * Newton converges monotonically from above and overshoots from below;
+         * the shift (+0.02 in log10, three times the network's held-out RMS
+         * error) and the cap at 1 (the root lies below the smallest isolated
+         * viscosity) keep the start on that side */
+        float fac = 0.0f;
+        if (ph >= 0) fac = fminf(fmaxf(__expf((0.25f * y[0] + 0.02f) * 2.30258509f), 1.0e-3f), 1.0f);
+        for (int p = 0; p < prm.Nb_phases && p < g.nph; p++) guess[(size_t)p * n + node] = p == ph ? fac : 0.0f;
+    }
+    }
+}
+
+void mdz_launch_mlp_guess(const DevGrid &g, const DevParams &prm, DevMLPW w, float *guess_n, float *guess_s,
+                          hipStream_t s) {
+    const int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
+    const int cap = 2048; /* 8 blocks per CU */
+    const int bc = mdz_div_up(nc, TSAMP), bv = mdz_div_up(nv, TSAMP);
+    hipLaunchKernelGGL(k_mlp_guess<true>, dim3(bc < cap ? bc : cap), dim3(TBLK), 0, s, g, prm, w, guess_n);
+    hipLaunchKernelGGL(k_mlp_guess<false>, dim3(bv < cap ? bv : cap), dim3(TBLK), 0, s, g, prm, w, guess_s);
+}
+
+/* ------------------------------------------------------------------ */
+/* host API                                                             */
+/* ------------------------------------------------------------------ */
+struct mdz_mlp {
+    hipStream_t s;
+    float *w, *m, *v, *g;           /* master weights, Adam moments, gradient */
+    unsigned short *wp;
+    float *bp;
+    float *partial, *loss;          /* per-block gradient / loss */
+    int partial_blocks;
+    double *loss_acc;
+    float *X, *Y;                   /* dataset */
+    long long n, cap;
+    mat_prop *d_mat;
+    DevParams prm;
+    int t;                          /* Adam step */
+};
+
+#define MCHK(x)                                                                                   \
+    do {                                                                                          \
+        if ((x) != hipSuccess) return mdz_error(MDZ_EHIP, "surrogate network: " #x " failed");    \
+    } while (0)
+
+extern "C" int mdz_mlp_create(mdz_mlp **out, const mat_prop *materials, const params *model, const scale *scaling) {
+    if (!out || !materials || !model || !scaling) return mdz_error(MDZ_EINVAL, "mdz_mlp_create: NULL argument");
+    if (materials->Nb_phases < 3 && model->Nb_phases < 3)
+        return mdz_error(MDZ_EINVAL, "mdz_mlp_create: the network encodes three phases");
+    mdz_mlp *n = (mdz_mlp *)calloc(1, sizeof(mdz_mlp));
+    if (!n) return mdz_error(MDZ_ENOMEM, "mdz_mlp_create");
+    *out = n;
+    MCHK(hipStreamCreate(&n->s));
+    MCHK(hipMalloc((void **)&n->w, MLP_NPARAMS * 4));
+    MCHK(hipMalloc((void **)&n->m, MLP_NPARAMS * 4));
+    MCHK(hipMalloc((void **)&n->v, MLP_NPARAMS * 4));
+    MCHK(hipMalloc((void **)&n->g, MLP_NPARAMS * 4));
+    MCHK(hipMalloc((void **)&n->wp, MLP_WP_TOTAL * 2));
+    MCHK(hipMalloc((void **)&n->bp, MLP_BP_TOTAL * 4));
+    MCHK(hipMalloc((void **)&n->loss_acc, 8));
+    MCHK(hipMalloc((void **)&n->d_mat, sizeof(mat_prop)));
+    MCHK(hipMemcpy(n->d_mat, materials, sizeof(mat_prop), hipMemcpyHostToDevice));
+    DevParams &p = n->prm;
+    memset(&p, 0, sizeof(p));
+    p.dt = model->dt; p.mineta = model->mineta; p.maxeta = model->maxeta;
+    p.sT = scaling->T; p.sE = scaling->E; p.seta = scaling->eta; p.sS = scaling->S; p.sL = scaling->L; p.st = scaling->t;
+    p.step = 1; p.iselastic = 1; p.Nb_phases = 3;
+    return MDZ_OK;
+}
+
+extern "C" void mdz_mlp_destroy(mdz_mlp *n) {
+    if (!n) return;
+    hipStreamSynchronize(n->s);
+    hipFree(n->w); hipFree(n->m); hipFree(n->v); hipFree(n->g); hipFree(n->wp); hipFree(n->bp);
+    hipFree(n->partial); hipFree(n->loss); hipFree(n->loss_acc); hipFree(n->X); hipFree(n->Y); hipFree(n->d_mat);
+    hipStreamDestroy(n->s);
+    free(n);
+}
+
+static int mlp_upload_weights(mdz_mlp *n, const float *w) {
+    MCHK(hipMemcpyAsync(n->w, w, MLP_NPARAMS * 4, hipMemcpyHostToDevice, n->s));
+    MCHK(hipMemsetAsync(n->m, 0, MLP_NPARAMS * 4, n->s));
+    MCHK(hipMemsetAsync(n->v, 0, MLP_NPARAMS * 4, n->s));
+    mdz_mlp_pack(n->w, n->wp, n->bp, n->s);
+    MCHK(hipStreamSynchronize(n->s));
+    n->t = 0;
+    return MDZ_OK;
+}
+
+/* He-normal weights (std sqrt(2 / fan_in)), zero biases, from a splitmix64
+ * stream of the seed (Box-Muller). */
+extern "C" int mdz_mlp_init(mdz_mlp *n, unsigned long long seed) {
+    if (!n) return mdz_error(MDZ_EINVAL, "NULL network");
+    std::vector<float> w(MLP_NPARAMS, 0.0f);
+    unsigned long long st = seed;
+    auto next = [&]() {
+        st += 0x9e3779b97f4a7c15ULL;
+        unsigned long long x = st;
+        x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+        x ^= x >> 31;
+        return ((x >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    };
+    auto normal = [&]() { return sqrt(-2.0 * log(next())) * cos(2.0 * M_PI * next()); };
+    const int offs[4] = {MLP_OFF_W1, MLP_OFF_W2, MLP_OFF_W3, MLP_OFF_W4};
+    const int fan[4] = {MLP_IN, MLP_H, MLP_H, MLP_H}, cnt[4] = {MLP_H * MLP_IN, MLP_H * MLP_H, MLP_H * MLP_H, MLP_OUT * MLP_H};
+    for (int L = 0; L < 4; L++)
+        for (int i = 0; i < cnt[L]; i++) w[offs[L] + i] = (float)(normal() * sqrt(2.0 / fan[L]));
+    return mlp_upload_weights(n, w.data());
+}
+
+extern "C" int mdz_mlp_set_weights(mdz_mlp *n, const float *w, size_t n_floats) {
+    if (!n || !w || n_floats != MLP_NPARAMS) return mdz_error(MDZ_EINVAL, "mdz_mlp_set_weights: size");
+    return mlp_upload_weights(n, w);
+}
+
+extern "C" int mdz_mlp_get_weights(mdz_mlp *n, float *w, size_t n_floats) {
+    if (!n || !w || n_floats != MLP_NPARAMS) return mdz_error(MDZ_EINVAL, "mdz_mlp_get_weights: size");
+    MCHK(hipMemcpyAsync(w, n->w, MLP_NPARAMS * 4, hipMemcpyDeviceToHost, n->s));
+    MCHK(hipStreamSynchronize(n->s));
+    return MDZ_OK;
+}
+
+static int mlp_reserve(mdz_mlp *n, long long count) {
+    if (count <= n->cap) return MDZ_OK;
+    hipFree(n->X);
+    hipFree(n->Y);
+    n->X = n->Y = nullptr;
+    MCHK(hipMalloc((void **)&n->X, (size_t)count * MLP_IN * 4));
+    MCHK(hipMalloc((void **)&n->Y, (size_t)count * 2 * 4));
+    n->cap = count;
+    return MDZ_OK;
+}
+
+extern "C" int mdz_mlp_generate(mdz_mlp *n, long long count, unsigned long long seed) {
+    if (!n || count < TSAMP) return mdz_error(MDZ_EINVAL, "mdz_mlp_generate: count");
+    int rc = mlp_reserve(n, count);
+    if (rc) return rc;
+    const long long chunk = 1LL << 22;
+    for (long long f = 0; f < count; f += chunk) {
+        long long c = count - f < chunk ? count - f : chunk;
+        hipLaunchKernelGGL(k_mlp_generate, dim3((unsigned)((c + 255) / 256)), dim3(256), 0, n->s, f, c, seed,
+                           (const mat_prop *)n->d_mat, n->prm, n->X + f * MLP_IN, n->Y + f * 2);
+    }
+    MCHK(hipGetLastError());
+    MCHK(hipStreamSynchronize(n->s));
+    n->n = count;
+    return MDZ_OK;
+}
+
+extern "C" int mdz_mlp_upload_samples(mdz_mlp *n, long long count, const float *X, const float *Y) {
+    if (!n || !X || !Y || count < 1) return mdz_error(MDZ_EINVAL, "mdz_mlp_upload_samples");
+    int rc = mlp_reserve(n, count);
+    if (rc) return rc;
+    MCHK(hipMemcpyAsync(n->X, X, (size_t)count * MLP_IN * 4, hipMemcpyHostToDevice, n->s));
+    MCHK(hipMemcpyAsync(n->Y, Y, (size_t)count * 2 * 4, hipMemcpyHostToDevice, n->s));
+    MCHK(hipStreamSynchronize(n->s));
+    n->n = count;
+    return MDZ_OK;
+}
+
+extern "C" int mdz_mlp_download_samples(mdz_mlp *n, long long first, long long count, float *X, float *Y) {
+    if (!n || first < 0 || count < 0 || first + count > n->n) return mdz_error(MDZ_EINVAL, "mdz_mlp_download_samples");
+    if (X) MCHK(hipMemcpyAsync(X, n->X + first * MLP_IN, (size_t)count * MLP_IN * 4, hipMemcpyDeviceToHost, n->s));
+    if (Y) MCHK(hipMemcpyAsync(Y, n->Y + first * 2, (size_t)count * 2 * 4, hipMemcpyDeviceToHost, n->s));
+    MCHK(hipStreamSynchronize(n->s));
+    return MDZ_OK;
+}
+
+/* gradient of sum((y-t)^2) / (2 * count) over the stored samples [first, first + count) */
+static int mlp_gradient(mdz_mlp *n, long long first, int count, unsigned long long mul, unsigned long long add,
+                        double *loss_acc) {
+    if (count < TSAMP || count % TSAMP) return mdz_error(MDZ_EINVAL, "batch must be a multiple of 64");
+    const int nb = count / TSAMP;
+    if (nb > n->partial_blocks) {
+        hipFree(n->partial);
+        hipFree(n->loss);
+        n->partial = nullptr;
+        n->loss = nullptr;
+        MCHK(hipMalloc((void **)&n->partial, (size_t)nb * MLP_NPARAMS * 4));
+        MCHK(hipMalloc((void **)&n->loss, (size_t)nb * 4));
+        n->partial_blocks = nb;
+    }
+    DevMLPW w{n->wp, n->bp};
+    hipLaunchKernelGGL(k_mlp_train_step, dim3(nb), dim3(TBLK), 0, n->s, (const float *)n->X, (const float *)n->Y, n->n,
+                       first, mul, add, 1.0f / (float)count, w, n->partial, n->loss);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(mdz_div_up(MLP_NPARAMS, 256)), dim3(256), 0, n->s, (const float *)n->partial,
+                       nb, n->g, (const float *)n->loss, loss_acc);
+    MCHK(hipGetLastError());
+    return MDZ_OK;
+}
+
+extern "C" int mdz_mlp_gradient(mdz_mlp *n, long long first, int count, float *grad, double *loss) {
+    if (!n || first < 0 || first + count > n->n) return mdz_error(MDZ_EINVAL, "mdz_mlp_gradient: range");
+    MCHK(hipMemsetAsync(n->loss_acc, 0, 8, n->s));
+    int rc = mlp_gradient(n, first, count, 1ULL, 0ULL, n->loss_acc);
+    if (rc) return rc;
+    if (grad) MCHK(hipMemcpyAsync(grad, n->g, MLP_NPARAMS * 4, hipMemcpyDeviceToHost, n->s));
+    if (loss) MCHK(hipMemcpyAsync(loss, n->loss_acc, 8, hipMemcpyDeviceToHost, n->s));
+    MCHK(hipStreamSynchronize(n->s));
+    return MDZ_OK;
+}
+
+static int mlp_adam(mdz_mlp *n, float lr) {
+    n->t++;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.0f - powf(b1, (float)n->t), c2 = 1.0f - powf(b2, (float)n->t);
+    hipLaunchKernelGGL(k_mlp_adam, dim3(mdz_div_up(MLP_NPARAMS, 256)), dim3(256), 0, n->s, n->w, n->m, n->v,
+                       (const float *)n->g, lr, b1, b2, eps, c1, c2, n->wp, n->bp);
+    MCHK(hipGetLastError());
+    return MDZ_OK;
+}
+
+/* one Adam step with a caller-supplied gradient */
+extern "C" int mdz_mlp_adam_step(mdz_mlp *n, const float *grad, float lr) {
+    if (!n || !grad) return mdz_error(MDZ_EINVAL, "mdz_mlp_adam_step");
+    MCHK(hipMemcpyAsync(n->g, grad, MLP_NPARAMS * 4, hipMemcpyHostToDevice, n->s));
+    int rc = mlp_adam(n, lr);
+    if (rc) return rc;
+    MCHK(hipStreamSynchronize(n->s));
+    return MDZ_OK;
+}
+
+/* epochs over the stored samples in batches (a multiple of 64); each epoch
+ * visits the samples in the order i -> (i * a + c) mod n with a random odd
+ * a coprime to n; loss_per_epoch[e] = mean squared error per output */
+extern "C" int mdz_mlp_train(mdz_mlp *n, int epochs, int batch, float lr, unsigned long long seed,
+                             double *loss_per_epoch) {
+    if (!n || epochs < 1 || batch < TSAMP || batch % TSAMP || n->n < batch)
+        return mdz_error(MDZ_EINVAL, "mdz_mlp_train: arguments");
+    const long long steps = n->n / batch;
+    unsigned long long st = seed;
+    for (int e = 0; e < epochs; e++) {
+        st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+        unsigned long long mul = (st >> 17) | 1ULL;
+        while (true) { /* coprime to n */
+            unsigned long long a = mul % (unsigned long long)n->n, b = (unsigned long long)n->n;
+            while (b) { unsigned long long t = a % b; a = b; b = t; }
+            if (a == 1) break;
+            mul += 2;
+        }
+        mul %= (unsigned long long)n->n;
+        const unsigned long long add = (st >> 7) % (unsigned long long)n->n;
+        MCHK(hipMemsetAsync(n->loss_acc, 0, 8, n->s));
+        for (long long k = 0; k < steps; k++) {
+            int rc = mlp_gradient(n, k * batch, batch, mul, add, n->loss_acc);
+            if (rc) return rc;
+            rc = mlp_adam(n, lr);
+            if (rc) return rc;
+        }
+        double acc = 0.0;
+        MCHK(hipMemcpyAsync(&acc, n->loss_acc, 8, hipMemcpyDeviceToHost, n->s));
+        MCHK(hipStreamSynchronize(n->s));
+        if (loss_per_epoch) loss_per_epoch[e] = acc / (double)(steps * batch * 2);
+    }
+    return MDZ_OK;
+}
+
+extern "C" int mdz_mlp_predict(mdz_mlp *n, long long first, long long count, float *Yout) {
+    if (!n || !Yout || first < 0 || count < 1 || first + count > n->n) return mdz_error(MDZ_EINVAL, "mdz_mlp_predict");
+    float *d = nullptr;
+    MCHK(hipMalloc((void **)&d, (size_t)count * 2 * 4));
+    DevMLPW w{n->wp, n->bp};
+    hipLaunchKernelGGL(k_mlp_predict, dim3((unsigned)((count + TSAMP - 1) / TSAMP)), dim3(TBLK), 0, n->s,
+                       (const float *)n->X, first, count, w, d);
+    hipError_t e = hipMemcpyAsync(Yout, d, (size_t)count * 2 * 4, hipMemcpyDeviceToHost, n->s);
+    hipError_t e2 = hipStreamSynchronize(n->s);
+    hipFree(d);
+    if (e != hipSuccess || e2 != hipSuccess) return mdz_error(MDZ_EHIP, "mdz_mlp_predict");
+    return MDZ_OK;
+}

@@ -18,9 +18,9 @@
  *    across the local solve.
  *  - Phase constants are read with wave-uniform indices (scalar loads).
  *  - Mixed precision (MDZ_RHEO_MIXED / LEARNED): the visco-elastic Newton
- *    iteration first runs in FP32 with the hardware log/exp from the
- *    reference guess (or the network's guess) to |r/Eii| < 1e-6; FP64 Newton
- *    steps then finish to the reference tolerance |r/Eii| < 1e-13.  The
+ *    iteration first runs in FP32 (Newton in log eta, hardware log/exp) from
+ *    the reference guess (or the network's guess) to |r/Eii| < 1e-5; FP64
+ *    Newton steps then finish to the reference tolerance |r/Eii| < 1e-13.  The
  *    tangent, the plastic return and every output stay FP64.
  */
 #include "mdz_kernels.h"
@@ -222,7 +222,7 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
     /* FP32 pre-iterations */
     int it32 = 0;
     if (prm.rheo_mode != MDZ_RHEO_FP64) {
-        float e = guess > 0.0 ? (float)guess : (float)eta_ve;
+        float e = guess > 0.0 ? (float)(guess * eta_ve) : (float)eta_ve;
         const float Ef = (float)Eii, ff = (float)f_ani, eelf = (float)eta_el, ecst = (float)k.eta_cst;
         const float lcp = (float)lnC_pwl, lce = (float)lnC_exp, lcl = (float)lnC_lin, lcg = (float)lnC_gbs;
         const float np_ = (float)k.n_pwl, ne = (float)nST, nl = (float)k.n_lin, ng = (float)k.n_gbs;
@@ -235,19 +235,22 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
             float Eg = k.gbs ? __expf(lcg + ng * lt) : 0.0f;
             float Ec = k.cst ? Tf / ff / 2.0f / ecst : 0.0f;
             float r = Ef - (elastic ? Tf / (2.0f * eelf) : 0.0f) - (Ep + Ee + El + Eg + Ec);
-            if (fabsf(r / Ef) < 1e-6f) break;
+            if (fabsf(r / Ef) < MDZ_F32_TOL) break;
             float df = 0.0f;
             if (elastic) df -= Ef / eelf;
             if (peierls) df -= Ee * ne / e;
             if (k.lin) df -= El * nl / e;
             if (k.pwl) df -= Ep * np_ / e;
             if (k.cst) df -= Ef / ecst;
-            float en = e - r / df;
-            e = en > 0.0f ? en : 0.5f * e; /* stay where the residual is monotone */
+            /* Newton step in log(eta): the strain-rate sum is convex and
+             * increasing in log(eta), so the iteration converges from either
+             * side without leaving eta > 0 */
+            const float ds = fminf(fmaxf(-r / (e * df), -10.0f), 10.0f);
+            e *= __expf(ds);
         }
         if (e > 0.0f && e < 3.0e38f) eta_ve = (double)e;
     } else if (guess > 0.0) {
-        eta_ve = guess;
+        eta_ve *= guess;
     }
 
     /* FP64 visco-elastic Newton iteration (RheologyDensity.c:330-370) */
@@ -586,8 +589,6 @@ void mdz_launch_phase_constants(const mat_prop *d_mt, int nph, void *d_pk, hipSt
 void mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
                               const float *guess_s, unsigned long long *hist, hipStream_t s) {
     int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
-    /* vertex values of div_u, T, P, d and phi first (same as the generic path) */
-    mdz_launch_interp5(g, prm, s);
     hipLaunchKernelGGL(k_rheo_cells_fast, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, g, prm, (const PhaseK *)d_pk, R,
                        guess_n, hist);
     hipLaunchKernelGGL(k_rheo_vertices_fast, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm,

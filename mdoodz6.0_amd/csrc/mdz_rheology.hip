@@ -171,7 +171,7 @@ __device__ __forceinline__ void rheo_cell_generic(const DevGrid &g, const DevPar
             if (!(fabs(fr) > 1.0e-13)) continue;
             VGuess gs;
             gs.mode = prm.rheo_mode;
-            gs.eta_ve0 = guess_n ? (double)guess_n[(size_t)p * nc + c0] : 0.0;
+            gs.factor = guess_n ? (double)guess_n[(size_t)p * nc + c0] : 0.0;
             VOut o;
             mdz_viscosity(p, mu, T, P, d0, phi0, X0, Exx, Ezz, Exz, Gxx, Gzz, Gxz, s0xx, s0zz, s0xz, mt, prm, dil,
                           fric, C, P0, bet, gs, o);
@@ -283,7 +283,7 @@ __device__ __forceinline__ void rheo_vertex_generic(const DevGrid &g, const DevP
             if (!(fabs(fr) > 1.0e-13)) continue;
             VGuess gs;
             gs.mode = prm.rheo_mode;
-            gs.eta_ve0 = guess_s ? (double)guess_s[(size_t)p * nv + c1] : 0.0;
+            gs.factor = guess_s ? (double)guess_s[(size_t)p * nv + c1] : 0.0;
             VOut o;
             mdz_viscosity(p, mu, T, P, d0, phi0, X0, Exx, Ezz, Exz, Gxx, Gzz, Gxz, s0xx, s0zz, s0xz, mt, prm, dil,
                           fric, C, P0, bet, gs, o);
@@ -562,7 +562,6 @@ void mdz_launch_interp5(const DevGrid &g, const DevParams &prm, hipStream_t s) {
 void mdz_launch_rheology(const DevGrid &g, const DevParams &prm, const mat_prop *mt, const float *guess_n,
                          const float *guess_s, unsigned long long *hist, hipStream_t s) {
     int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
-    mdz_launch_interp5(g, prm, s);
     hipLaunchKernelGGL(k_rheo_cells, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, g, prm, mt, guess_n, hist);
     hipLaunchKernelGGL(k_rheo_vertices, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm, mt, guess_s, hist);
 }

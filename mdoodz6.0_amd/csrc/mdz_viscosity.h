@@ -22,19 +22,24 @@ struct VOut {
     double detadexx, detadezz, detadexz, detadp, X1, OverS;
     double ddivpdexx, ddivpdezz, ddivpdexz, ddivpdp, Pcorr, drhodp, rho;
     double div_el, div_pl, div_r;
+    double eta_ve, gdot; /* root of the visco-elastic iteration, plastic multiplier */
+    double eta_up;       /* the reference's initial guess: smallest isolated viscosity */
     int it_ve, it_pl;
 };
 
 /* Optional initial guess and arithmetic of the VE iteration. */
 struct VGuess {
     int mode;        /* MDZ_RHEO_FP64 / MIXED / LEARNED */
-    double eta_ve0;  /* > 0: initial guess for eta_ve (learned mode) */
+    double factor;   /* > 0: initial eta_ve = factor x the reference's guess (learned mode) */
 };
 
 __device__ __forceinline__ double mdz_min(double a, double b) { return a <= b ? a : b; }
 
-/* FP32 Newton on the VE residual from eta0 until |r/Eii| < 1e-6 or 20
- * iterations; returns the FP32 iterate (only used as a start for FP64). */
+/* FP32 stage of the mixed-precision local solve: Newton on the VE residual
+ * in log(eta) from eta0 until |r/Eii| < MDZ_F32_TOL (the FP32 evaluation of
+ * C Tii^n = exp(log C + n log Tii) carries ~1e-6 relative error for large
+ * exponents, so a tighter stop would not be reached) or 20 iterations;
+ * returns the FP32 iterate (only used as a start for the FP64 polish). */
 __device__ inline float ve_newton_f32(float eta0, float Eii, float f_ani, float eta_el, float eta_cst,
                                       float lC_pwl, float n_pwl, float lC_exp, float nST, float lC_lin, float n_lin,
                                       float lC_gbs, float n_gbs, int elastic, int constant,
@@ -51,16 +56,16 @@ __device__ inline float ve_newton_f32(float eta0, float Eii, float f_ani, float 
         float Eexp = peierls ? __expf(lC_exp + nST * lT) : 0.0f;
         float Elin = diffusion ? __expf(lC_lin + n_lin * lT) : 0.0f;
         float r = Eii - (elastic ? Tii / (2.0f * eta_el) : 0.0f) - (Epwl + Eexp + Elin + Egbs + Ecst);
-        if (fabsf(r / Eii) < 1e-6f) break;
+        if (fabsf(r / Eii) < MDZ_F32_TOL) break;
         float df = 0.0f;
         if (elastic) df += -Eii / eta_el;
         if (peierls) df += -Eexp * nST / eta;
         if (diffusion) df += -Elin * n_lin / eta;
         if (dislocation) df += -Epwl * n_pwl / eta;
         if (constant) df += -Eii / eta_cst;
-        float en = eta - r / df;
-        /* stay in the bracket where the residual is monotone */
-        eta = en > 0.0f ? en : 0.5f * eta;
+        /* Newton step in log(eta) (converges from either side, eta > 0) */
+        const float ds = fminf(fmaxf(-r / (eta * df), -10.0f), 10.0f);
+        eta *= __expf(ds);
     }
     *its = it;
     return eta;
@@ -119,7 +124,7 @@ __device__ inline void mdz_viscosity(int phase, double G, double T, double P, do
 
     o.Eii_exp = 0; o.Eii_lin = 0; o.Eii_pl = 0; o.Eii_pwl = 0; o.Eii_el = 0; o.Eii_gbs = 0; o.Eii_cst = 0;
     o.Exx_el = 0; o.Ezz_el = 0; o.Exz_el = 0;
-    o.OverS = 0; o.it_pl = 0;
+    o.OverS = 0; o.it_pl = 0; o.gdot = 0.0;
 
     double Eyy = -(Exx + Ezz), Gyy = -(Gxx + Gzz);
     Eii = sqrt(1.0 / 2.0 * (Exx * Exx + Ezz * Ezz + Eyy * Eyy) + Exz * Exz);
@@ -201,12 +206,13 @@ __device__ inline void mdz_viscosity(int phase, double G, double T, double P, do
     if (diffusion == 1) eta_up = mdz_min(eta_up, eta_lin);
     if (gbs == 1) eta_up = mdz_min(eta_up, eta_gbs);
     double eta_ve = eta_up;
+    o.eta_up = eta_up;
 
     /* reduced-precision pre-iterations: same residual in FP32 with the
      * exponents applied as exp(n log Tii); grain-size evolution stays FP64 */
     int it32 = 0;
     if (guess.mode != MDZ_RHEO_FP64 && gs == 0) {
-        double e0 = (guess.eta_ve0 > 0.0) ? guess.eta_ve0 : eta_ve;
+        double e0 = (guess.factor > 0.0) ? guess.factor * eta_ve : eta_ve;
         float lcp = dislocation ? (float)log(C_pwl) : 0.0f;
         float lce = peierls ? (float)log(C_exp) : 0.0f;
         float lcl = diffusion ? (float)(log(C_lin) - m_lin * log(d1)) : 0.0f;
@@ -217,8 +223,8 @@ __device__ inline void mdz_viscosity(int phase, double G, double T, double P, do
                                  peierls, diffusion, gbs, &it32);
         /* a finite, positive FP32 iterate replaces the reference guess */
         if (ef > 0.0f && ef < 3.0e38f) eta_ve = (double)ef;
-    } else if (guess.eta_ve0 > 0.0) {
-        eta_ve = guess.eta_ve0;
+    } else if (guess.factor > 0.0) {
+        eta_ve *= guess.factor;
     }
 
     double Eii_pwl = 0, Eii_gbs = 0, Eii_exp = 0, Eii_lin = 0, Eii_cst = 0;
@@ -244,6 +250,7 @@ __device__ inline void mdz_viscosity(int phase, double G, double T, double P, do
         eta_ve -= r_eta_ve / dfdeta;
     }
     o.it_ve = it + (it32 << 8);
+    o.eta_ve = eta_ve;
     o.Eii_pwl = Eii_pwl; o.Eii_gbs = Eii_gbs; o.Eii_exp = Eii_exp; o.Eii_lin = Eii_lin; o.Eii_cst = Eii_cst;
 
     Tii = 2.0 * eta_ve * f_ani * Eii;
@@ -305,6 +312,7 @@ __device__ inline void mdz_viscosity(int phase, double G, double T, double P, do
             gdot -= F_trial / dFdgdot;
         }
         o.it_pl = it + 1;
+        o.gdot = gdot;
         Txx = 2.0 * eta_ve * (Exx - gdot * dQdtxx);
         Tzz = 2.0 * eta_ve * (Ezz - gdot * dQdtzz);
         Tyy = -(Txx + Tzz);

@@ -31,6 +31,9 @@ double orc_Viscosity(int phase, double G, double T, double P, double d, double p
                      double *div_pl, double *div_r);
 /* Local iteration counts of the last orc_Viscosity call (VE, plastic). */
 void orc_last_iterations(int *it_ve, int *it_pl);
+/* Initial guess (smallest isolated viscosity) and root of the last
+ * visco-elastic local solve of orc_Viscosity. */
+void orc_last_local_solve(double *eta_up, double *eta_ve);
 
 void orc_InterpCentroidsToVerticesDouble(double *cen, double *ver, grid *mesh, params *model);
 void orc_StrainRateComponents(grid *mesh, scale scaling, params *model);

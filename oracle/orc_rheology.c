@@ -26,8 +26,10 @@
 #define MINV(a, b) ((a) <= (b) ? (a) : (b))
 
 static __thread int g_it_ve, g_it_pl;
+static __thread double g_eta_up, g_eta_ve;
 
 void orc_last_iterations(int *it_ve, int *it_pl) { *it_ve = g_it_ve; *it_pl = g_it_pl; }
+void orc_last_local_solve(double *eta_up, double *eta_ve) { *eta_up = g_eta_up; *eta_ve = g_eta_ve; }
 
 int orc_num_threads(void) {
 #ifdef _OPENMP
@@ -234,6 +236,8 @@ double orc_Viscosity(int phase, double G, double T, double P, double d, double p
         eta_ve -= r_eta_ve / dfdeta;
     }
     g_it_ve = it;
+    g_eta_up = eta_up;
+    g_eta_ve = eta_ve;
     (void)res0;
     /* :389 -- the check can only trigger on it == nitmax-1, as in the reference */
     if (it == nitmax - 1 && res > tol) { printf("Visco-Elastic iterations failed!\n"); exit(0); }

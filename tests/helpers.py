@@ -102,6 +102,9 @@ def point_viscosity(orc, m, phase, G, T, P, Exx, Ezz, Exz, d=2e-3, phi=0.0, X0=0
     ive, ipl = C.c_int(), C.c_int()
     orc.orc_last_iterations(C.byref(ive), C.byref(ipl))
     r["it_ve"], r["it_pl"] = ive.value, ipl.value
+    up, ve = C.c_double(), C.c_double()
+    orc.orc_last_local_solve(C.byref(up), C.byref(ve))
+    r["eta_up"], r["eta_ve"] = up.value, ve.value
     return r
 
 

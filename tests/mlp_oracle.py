@@ -1,0 +1,99 @@
+"""CPU FP64 restatement of the surrogate network (MLP 9 -> 64 -> 64 -> 64 -> 2,
+SiLU) used to check the GPU kernels: forward, gradient of
+sum((y - t)^2) / (2 n), and Adam (Kingma & Ba 2015).
+
+With bf16=True the operands of every matrix product are rounded to BF16
+(round to nearest even) at the points where the GPU kernels round them:
+inputs, weights, hidden activations, and the back-propagated output
+gradients; products and sums stay FP64.  With bf16=False it is the plain
+FP64 network (pinned against central differences in test_cpu_mlp.py).
+Test infrastructure only.
+"""
+import numpy as np
+
+NIN, NH, NOUT = 9, 64, 2
+OFF = {}
+_o = 0
+for name, size in (("W1", NH * NIN), ("b1", NH), ("W2", NH * NH), ("b2", NH), ("W3", NH * NH), ("b3", NH),
+                   ("W4", NOUT * NH), ("b4", NOUT)):
+    OFF[name] = (_o, size)
+    _o += size
+NPARAMS = _o
+SHAPES = {"W1": (NH, NIN), "b1": (NH,), "W2": (NH, NH), "b2": (NH,), "W3": (NH, NH), "b3": (NH,), "W4": (NOUT, NH),
+          "b4": (NOUT,)}
+
+
+def bf16(x):
+    """Round to the nearest BF16 (ties to even), returned as float64."""
+    x32 = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    u = x32.view(np.uint32).astype(np.uint64)
+    nan = np.isnan(x32)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    out = u.astype(np.uint32).view(np.float32).astype(np.float64)
+    out[nan] = np.nan
+    return out
+
+
+def unpack(blob):
+    blob = np.asarray(blob, dtype=np.float64)
+    return {k: blob[o:o + n].reshape(SHAPES[k]) for k, (o, n) in OFF.items()}
+
+
+def pack(p):
+    out = np.zeros(NPARAMS)
+    for k, (o, n) in OFF.items():
+        out[o:o + n] = np.asarray(p[k]).ravel()
+    return out
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def dsilu(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
+def forward(blob, X, bf=False):
+    """X: (n, 9) -> (y (n, 2), cache)."""
+    p = unpack(blob)
+    r = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
+    x = r(X).T                                   # (9, n)
+    W = {k: r(p[k]) for k in ("W1", "W2", "W3", "W4")}
+    z1 = W["W1"] @ x + p["b1"][:, None]
+    h1 = r(silu(z1))
+    z2 = W["W2"] @ h1 + p["b2"][:, None]
+    h2 = r(silu(z2))
+    z3 = W["W3"] @ h2 + p["b3"][:, None]
+    h3 = r(silu(z3))
+    y = W["W4"] @ h3 + p["b4"][:, None]
+    return y.T, dict(x=x, W=W, z1=z1, z2=z2, z3=z3, h1=h1, h2=h2, h3=h3, y=y)
+
+
+def gradient(blob, X, Y, bf=False):
+    """Gradient of sum((y - t)^2) / (2 n) and the sum of squared errors."""
+    r = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
+    n = X.shape[0]
+    _, c = forward(blob, X, bf)
+    e = c["y"] - np.asarray(Y, dtype=np.float64).T
+    dz4 = r(e / n)
+    W = c["W"]
+    dz3 = r((W["W4"].T @ dz4) * dsilu(c["z3"]))
+    dz2 = r((W["W3"].T @ dz3) * dsilu(c["z2"]))
+    dz1 = r((W["W2"].T @ dz2) * dsilu(c["z1"]))
+    g = {"W1": dz1 @ c["x"].T, "b1": dz1.sum(1), "W2": dz2 @ c["h1"].T, "b2": dz2.sum(1),
+         "W3": dz3 @ c["h2"].T, "b3": dz3.sum(1), "W4": dz4 @ c["h3"].T, "b4": dz4.sum(1)}
+    return pack(g), float((e ** 2).sum())
+
+
+def loss(blob, X, Y):
+    y, _ = forward(blob, X, False)
+    return float(((y - Y) ** 2).sum()) / (2.0 * X.shape[0])
+
+
+def adam(w, m, v, g, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    w = w - lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
+    return w, m, v

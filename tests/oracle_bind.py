@@ -39,5 +39,6 @@ def load(mdz):
                                     + [dp] * 19 + [C.c_double] * 4 + [dp] * 4 + [C.c_double] + [dp] * 9
                                     + [C.c_double] * 2 + [dp] * 3)
         L.orc_last_iterations.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_last_local_solve.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib = L
     return _lib

@@ -1,0 +1,48 @@
+"""Pins of the CPU network oracle (tests/mlp_oracle.py): its FP64 gradient
+against central differences of its loss, the BF16 rounding against exact
+cases, and Adam against the closed form of the first step."""
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+
+
+def _blob(seed):
+    rng = np.random.default_rng(seed)
+    p = {}
+    for k, shp in mo.SHAPES.items():
+        fan = shp[1] if len(shp) == 2 else 1
+        p[k] = rng.standard_normal(shp) * (np.sqrt(2.0 / fan) if len(shp) == 2 else 0.1)
+    return mo.pack(p)
+
+
+def test_gradient_against_central_differences():
+    rng = np.random.default_rng(1)
+    X = rng.uniform(-1, 1, (32, 9))
+    Y = rng.uniform(-1, 1, (32, 2))
+    w = _blob(2)
+    g, _ = mo.gradient(w, X, Y, bf=False)
+    idx = rng.choice(mo.NPARAMS, 200, replace=False)
+    h = 1e-6
+    for i in idx:
+        e = np.zeros(mo.NPARAMS)
+        e[i] = h
+        fd = (mo.loss(w + e, X, Y) - mo.loss(w - e, X, Y)) / (2 * h)
+        assert g[i] == pytest.approx(fd, rel=1e-5, abs=1e-9), i
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -2.5, 0.0, 3.0e38])
+    r = mo.bf16(x)
+    assert r[0] == 1.0
+    assert r[1] == 1.0                      # halfway: ties to even (mantissa 0)
+    assert r[2] == 1.0 + 2 ** -7            # above halfway: rounds up
+    assert r[3] == -2.5 and r[4] == 0.0
+    assert abs(r[5] - 3.0e38) / 3.0e38 < 2 ** -8
+
+
+def test_adam_first_step_moves_each_weight_by_lr():
+    """First Adam step: m/(1-b1) = g, v/(1-b2) = g^2, so w moves by lr*sign(g)."""
+    g = np.array([0.3, -2.0, 1e-3, 5.0])
+    w, m, v = mo.adam(np.zeros(4), np.zeros(4), np.zeros(4), g, 1, 1e-2)
+    assert np.allclose(w, -1e-2 * np.sign(g), rtol=1e-5)

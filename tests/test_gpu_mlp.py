@@ -1,0 +1,186 @@
+"""GPU tests of the surrogate network: training data against the oracle's
+Viscosity, the BF16 matrix-core forward / backward / Adam kernels against the
+CPU network oracle (tests/mlp_oracle.py, FP64 with the kernels' BF16
+rounding points), training, and the learned initial guess in the
+device-resident Newton unit.
+
+Tolerances: forward outputs 5e-4 absolute for 99.9 % of the samples (FP32
+accumulation order; outputs are O(1)) and 1e-2 for all (a hidden activation
+can round to the neighbouring BF16 value);
+gradients 5e-3 relative in norm per parameter group against the BF16-rounded
+oracle, 5e-2 against the plain FP64 gradient; Adam 1e-6 relative."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+from helpers import point_viscosity
+
+pytestmark = pytest.mark.gpu
+
+FP = C.POINTER(C.c_float)
+
+
+@pytest.fixture(scope="module")
+def litho(mdz):
+    m = mdz.Model.lithosphere(64, 32, 7)
+    yield m
+    m.free()
+
+
+@pytest.fixture
+def net(mdz, hot, litho):
+    h = C.c_void_p()
+    mdz.check(hot.mdz_mlp_create(C.byref(h), C.byref(litho.m.materials), C.byref(litho.m.model),
+                                 C.byref(litho.m.scaling)))
+    yield h
+    hot.mdz_mlp_destroy(h)
+
+
+def _samples(mdz, hot, net, n, first=0):
+    X = np.zeros((n, 9), np.float32)
+    Y = np.zeros((n, 2), np.float32)
+    mdz.check(hot.mdz_mlp_download_samples(net, first, n, X.ctypes.data_as(FP), Y.ctypes.data_as(FP)))
+    return X, Y
+
+
+def _weights(mdz, hot, net):
+    w = np.zeros(mo.NPARAMS, np.float32)
+    mdz.check(hot.mdz_mlp_get_weights(net, w.ctypes.data_as(FP), mo.NPARAMS))
+    return w
+
+
+def test_training_labels_are_the_reference_root(mdz, orc, hot, net, litho):
+    """Label y0 = 4 log10(eta_ve / eta_ref) from the oracle's Viscosity at the
+    decoded inputs (root and initial guess of its visco-elastic solve)."""
+    mdz.check(hot.mdz_mlp_generate(net, 4096, 11))
+    X, Y = _samples(mdz, hot, net, 4096)
+    s = litho.m.scaling
+    mt = litho.m.materials
+    dt0 = litho.m.model.dt
+    try:
+        for i in range(0, 4096, 64):
+            f = X[i].astype(np.float64)
+            E = 10 ** (3 * f[0] - 15) / s.E
+            T = (850 * f[1] + 1123) / s.T
+            P = (5 * f[2] + 5) * 1e9 / s.S
+            tau0 = (250 * f[3] + 250) * 1e6 / s.S
+            litho.m.model.dt = 10 ** (f[4] + 11) / s.t
+            ph = int(np.argmax(f[5:8]))
+            r = point_viscosity(orc, litho, ph, mt.mu[ph], T, P, E, -E, 0.0, d=mt.gs_ref[ph], Txx0=tau0, Tzz0=-tau0,
+                                fric=0.5, C_coh=1e30, P0=P, beta=mt.bet[ph])
+            # features are stored in FP32: inputs carry ~1e-7 relative rounding
+            assert Y[i, 0] == pytest.approx(4 * np.log10(r["eta_ve"] / r["eta_up"]), abs=1e-4), i
+            assert Y[i, 0] <= 1e-6  # the root lies below the smallest isolated viscosity
+    finally:
+        litho.m.model.dt = dt0
+
+
+def test_forward_and_gradient_against_oracle(mdz, hot, net):
+    mdz.check(hot.mdz_mlp_generate(net, 4096, 5))
+    mdz.check(hot.mdz_mlp_init(net, 3))
+    X, Y = _samples(mdz, hot, net, 4096)
+    w = _weights(mdz, hot, net).astype(np.float64)
+    yg = np.zeros((4096, 2), np.float32)
+    mdz.check(hot.mdz_mlp_predict(net, 0, 4096, yg.ctypes.data_as(FP)))
+    yo, _ = mo.forward(w, X, bf=True)
+    err = np.abs(yg - yo)
+    print("forward max abs err %.2e, 99.9th percentile %.2e" % (err.max(), np.quantile(err, 0.999)))
+    assert np.quantile(err, 0.999) <= 5e-4  # FP32 accumulation order
+    assert err.max() <= 1e-2                # hidden activations one BF16 ulp apart
+    g = np.zeros(mo.NPARAMS, np.float32)
+    loss = C.c_double()
+    mdz.check(hot.mdz_mlp_gradient(net, 0, 4096, g.ctypes.data_as(FP), C.byref(loss)))
+    go, lo = mo.gradient(w, X, Y, bf=True)
+    gx, _ = mo.gradient(w, X, Y, bf=False)
+    assert loss.value == pytest.approx(lo, rel=1e-3)
+    for k, (o, n) in mo.OFF.items():
+        a, b, c = g[o:o + n], go[o:o + n], gx[o:o + n]
+        e_bf = np.linalg.norm(a - b) / np.linalg.norm(b)
+        e_64 = np.linalg.norm(a - c) / np.linalg.norm(c)
+        print("%s: rel err vs BF16 oracle %.2e, vs FP64 %.2e" % (k, e_bf, e_64))
+        assert e_bf <= 5e-3, k
+        assert e_64 <= 5e-2, k
+
+
+def test_adam_step_against_oracle(mdz, hot, net):
+    mdz.check(hot.mdz_mlp_init(net, 4))
+    w0 = _weights(mdz, hot, net).astype(np.float64)
+    rng = np.random.default_rng(0)
+    g1 = rng.standard_normal(mo.NPARAMS).astype(np.float32)
+    g2 = rng.standard_normal(mo.NPARAMS).astype(np.float32)
+    mdz.check(hot.mdz_mlp_adam_step(net, g1.ctypes.data_as(FP), 1e-3))
+    mdz.check(hot.mdz_mlp_adam_step(net, g2.ctypes.data_as(FP), 1e-3))
+    w, m, v = mo.adam(w0, 0.0, 0.0, g1.astype(np.float64), 1, 1e-3)
+    w, m, v = mo.adam(w, m, v, g2.astype(np.float64), 2, 1e-3)
+    wg = _weights(mdz, hot, net)
+    assert np.abs(wg - w).max() <= 1e-6 * np.abs(w).max() + 1e-7
+
+
+def test_training_reduces_the_loss(mdz, hot, net):
+    mdz.check(hot.mdz_mlp_generate(net, 1 << 17, 9))
+    mdz.check(hot.mdz_mlp_init(net, 1))
+    hist = (C.c_double * 4)()
+    mdz.check(hot.mdz_mlp_train(net, 4, 4096, 1e-3, 2, hist))
+    print("loss per epoch", list(hist))
+    assert hist[3] < hist[0] and hist[3] < 0.5 * hist[0]
+
+
+def test_learned_guess_in_the_newton_unit(mdz, orc, hot, net, litho):
+    """Learned mode: the network's correction of the reference's guess
+    starts the FP32 iteration and FP64 finishes to the reference tolerance;
+    fields stay within the mixed-precision bounds; the gain over the method
+    without the network (mixed mode from the reference's guess) is the
+    change of the mean FP32 iteration count, printed."""
+    from helpers import numbering, blocks
+    # the training of the benchmark configuration: 2^24 samples, 20 epochs,
+    # batch 65536, Adam lr 1e-3 (about 2 s on the GPU)
+    mdz.check(hot.mdz_mlp_generate(net, 1 << 24, 21))
+    mdz.check(hot.mdz_mlp_init(net, 5))
+    hist = (C.c_double * 20)()
+    mdz.check(hot.mdz_mlp_train(net, 20, 65536, 1e-3, 3, hist))
+    print("loss per epoch", list(hist))
+    w = _weights(mdz, hot, net)
+    mo_ = litho.clone()
+    S = numbering(orc, mdz, mo_)
+    bo = blocks(orc, mdz, S, 8)
+    orc.orc_newton_unit(mo_.m.mesh, C.byref(mo_.m.materials), C.byref(mo_.m.model), C.byref(mo_.m.Nmodel),
+                        C.byref(mo_.m.scaling), C.byref(S), *[C.byref(b) for b in bo])
+    counts = {}
+    for mode in (1, 2):
+        mg = litho.clone()
+        ctx = C.c_void_p()
+        mdz.check(hot.mdz_ctx_create(C.byref(ctx), mg.m.mesh, C.byref(mg.m.materials), C.byref(mg.m.model),
+                                     C.byref(mg.m.scaling)))
+        try:
+            mdz.check(hot.mdz_ctx_set_network(ctx, w.ctypes.data_as(FP), mo.NPARAMS))
+            mdz.check(hot.mdz_ctx_set_rheology_mode(ctx, mode))
+            mdz.check(hot.mdz_ctx_collect_histogram(ctx, 1))
+            res = (C.c_double * 3)()
+            mdz.check(hot.mdz_ctx_newton_unit(ctx, res))
+            h = (C.c_longlong * 192)()
+            mdz.check(hot.mdz_ctx_iteration_histogram(ctx, h))
+            it = np.arange(64)
+            counts[mode] = (float((it * np.array(h[128:192])).sum() / sum(h[128:192])),
+                            float((it * np.array(h[0:64])).sum() / sum(h[0:64])))
+            for k, v in zip(("resx", "resz", "resp"), res):
+                ref = getattr(mo_.m.Nmodel, k)
+                assert abs(v - ref) <= 1e-10 * abs(ref), (mode, k)
+            mdz.check(hot.mdz_ctx_download_grid(ctx, mg.m.mesh))
+            from helpers import relerr
+            for f in ("eta_n", "eta_s", "sxxd", "sxz", "detadexx_n"):
+                e = relerr(mg.field(f), mo_.field(f), 1e-6 if f.startswith("detad") else 1e-8)
+                assert e <= 1e-10, (mode, f, e)
+        finally:
+            hot.mdz_ctx_destroy(ctx)
+            mg.free()
+    print("mean FP32 / FP64 local iterations: reference guess %.2f / %.2f, learned %.2f / %.2f"
+          % (counts[1] + counts[2]))
+    # the reference's guess (smallest isolated viscosity) is already within
+    # ~1e-3 of the root on this model: the network's correction can only trim
+    # the FP32 iterations a little, and must not cost any
+    assert counts[2][0] <= 1.05 * counts[1][0]
+    for b in bo:
+        orc.orc_free_block(C.byref(b))
+    mo_.free()

@@ -19,6 +19,7 @@ TOL_ARITH = 1e-13      # strain rates, interpolation, right-hand sides, Picard D
 TOL_RHEO = 1e-11       # outputs of the local iterations (FP64 transcendentals, converged to 1e-13)
 TOL_CSR = 1e-12        # CSR values, relative to the row's largest entry
 TOL_RES = 1e-10        # residual vectors, relative to 1e-6 of their largest entry
+TOL_MIXED = 1e-10      # fields of the mixed-precision path (FP32 start, FP64 polish to 1e-13)
 FLOOR = 1e-8
 
 
@@ -259,7 +260,11 @@ def test_newton_unit_device_resident(case, mdz, orc, hot, mode):
             print("unit block %d err %.2e" % (i, e))
             assert e <= TOL_CSR
         mdz.check(hot.mdz_ctx_download_grid(ctx, mg.m.mesh))
-        _cmp_fields(mo, mg, ["eta_n", "eta_s", "D11_n", "D33_s", "roger_x", "roger_z"], TOL_RHEO)
+        # mixed mode: the FP64 polish reaches the reference's local tolerance
+        # from another start, so eta differs within that tolerance (~1e-13
+        # of the residual), amplified by the cancellation in the RHS terms
+        _cmp_fields(mo, mg, ["eta_n", "eta_s", "D11_n", "D33_s", "roger_x", "roger_z"],
+                    TOL_RHEO if mode == 0 else TOL_MIXED)
         # F = K(eta) x - b is a small difference of large terms, and eta
         # carries the local-iteration tolerance (1e-13): entries are compared
         # against 1e-3 of the largest residual entry
