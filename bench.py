@@ -104,14 +104,14 @@ def iteration_stats(mdz, hot, ctx):
     return out
 
 
-def cpu_baseline(mdz, ncx, ncz, seed, units):
+def cpu_baseline(mdz, ncx, ncz, seed, units, model_args=()):
     """The oracle (C/OpenMP restatement) on the host cores: full-size units,
     one warm-up then the median of `units`."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind
 
     orc = oracle_bind.load(mdz)
-    m = mdz.Model.lithosphere(ncx, ncz, seed)
+    m = mdz.Model.lithosphere(ncx, ncz, seed, *model_args)
     S = mdz.SparseMat()
     orc.orc_EvalNumberOfEquations(m.m.mesh, C.byref(S))
     blk = [mdz.SparseMat() for _ in range(8)]
@@ -133,6 +133,135 @@ def cpu_baseline(mdz, ncx, ncz, seed, units):
     return ncx * ncz / t, orc.orc_num_threads(), t
 
 
+def ensemble_params(n, seed):
+    """Seeded draws of the ensemble: nx U{257..2049} cells, nz = nx / 2,
+    friction U[10, 35] deg, cohesion U[5, 50] MPa, extension rate
+    log-U[1e-16, 1e-14] 1/s."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nx = int(rng.integers(257, 2050))
+        out.append((nx, nx // 2, seed * 1000 + i, float(rng.uniform(10.0, 35.0)), float(rng.uniform(5e6, 50e6)),
+                    float(10 ** rng.uniform(-16.0, -14.0))))
+    return out
+
+
+def run_ensemble(args, mdz, hot):
+    """Ensemble of independent lithosphere models, one captured graph launch
+    per Newton iteration (every model through one unit), autotuned shapes."""
+    import numpy as np
+
+    plist = ensemble_params(args.models, args.seed)
+    mode = {"fp64": 0, "mixed": 1, "learned": 2}[args.mode]
+    w, training = None, None
+    ctxs, cells = [], []
+    t0 = time.perf_counter()
+    for i, p in enumerate(plist):
+        m = mdz.Model.lithosphere(*p)
+        if mode == 2 and w is None:
+            w, training = train_surrogate(mdz, hot, m, args.train_samples, args.train_epochs, 65536, 1e-3, args.seed)
+        c = C.c_void_p()
+        mdz.check(hot.mdz_ctx_create(C.byref(c), m.m.mesh, C.byref(m.m.materials), C.byref(m.m.model),
+                                     C.byref(m.m.scaling)))
+        m.free()
+        if w is not None:
+            mdz.check(hot.mdz_ctx_set_network(c, w, mdz.MLP_NPARAMS))
+        mdz.check(hot.mdz_ctx_set_rheology_mode(c, mode))
+        ctxs.append(c)
+        cells.append(p[0] * p[1])
+        print("model %d/%d: %dx%d ready" % (i + 1, len(plist), p[0], p[1]), file=sys.stderr, flush=True)
+    setup_s = time.perf_counter() - t0
+    total = sum(cells)
+    arr = (C.c_void_p * len(ctxs))(*[c.value for c in ctxs])
+    ens = C.c_void_p()
+    mdz.check(hot.mdz_ensemble_create(C.byref(ens), arr, len(ctxs)))
+    shapes = (C.c_int * (3 * len(ctxs)))()
+    br = C.c_int()
+    t0 = time.perf_counter()
+    mdz.check(hot.mdz_ensemble_autotune(ens, 2, shapes, C.byref(br)))
+    tune_s = time.perf_counter() - t0
+    print("autotuned: %d chains, %.1f s" % (br.value, tune_s), file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        mdz.check(hot.mdz_ensemble_launch(ens))
+    res = np.zeros((len(ctxs), 3))
+    mdz.check(hot.mdz_ensemble_residuals(ens, res.ctypes.data_as(C.POINTER(C.c_double))))
+    ms = C.c_double()
+    mdz.check(hot.mdz_ensemble_time(ens, args.steps, C.byref(ms)))
+    value = total / (ms.value * 1e-3)
+    plain_ms, err = None, None
+    if mode != 0:
+        for c in ctxs:
+            mdz.check(hot.mdz_ctx_set_rheology_mode(c, 0))
+        pm = C.c_double()
+        mdz.check(hot.mdz_ensemble_time(ens, args.steps, C.byref(pm)))
+        plain_ms = pm.value
+        ref = np.zeros((len(ctxs), 3))
+        mdz.check(hot.mdz_ensemble_residuals(ens, ref.ctypes.data_as(C.POINTER(C.c_double))))
+        e = np.abs(res - ref) / np.abs(ref)
+        err = {"max": float(e.max()), "median": float(np.median(e.max(axis=1))),
+               "per_model_max": [float(x) for x in e.max(axis=1)]}
+    sh = np.array(list(shapes)).reshape(-1, 3)
+    kinds = {}
+    for row in sh:
+        k = "%d/%d/%d" % tuple(row)
+        kinds[k] = kinds.get(k, 0) + 1
+    hot.mdz_ensemble_destroy(ens)
+    for c in ctxs:
+        hot.mdz_ctx_destroy(c)
+    whole = 2816.0 * total / (ms.value * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        # bounded sample: full units of the smallest models of the ensemble
+        order = sorted(range(len(plist)), key=lambda i: cells[i])
+        done_cells, done_t, used = 0, 0.0, []
+        for i in order:
+            if done_t > 10.0:
+                break
+            v, cores, t = cpu_baseline(mdz, plist[i][0], plist[i][1], plist[i][2], 1, model_args=plist[i][3:])
+            done_cells += cells[i]
+            done_t += t
+            used.append(i)
+        cpu = {"value": round(done_cells / done_t, 1), "unit": "cell-updates/s", "cores": cores, "kind": "port",
+               "sample": "one Newton unit (after 1 warm-up) of each of the %d smallest models of the ensemble, "
+                         "%d cells, %.2f s in total" % (len(used), done_cells, done_t)}
+
+    line = {
+        "metric": "Newton-iter cell-updates/s (rheol+resid+assembly)",
+        "value": round(value, 1),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms.value, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if mode == 0 else "f32+f64",
+        "data": "synthetic (seeded lithosphere models)",
+        "config": {"workload": "ensemble of %d lithosphere models, nx U{257..2049} x nz = nx/2 cells, friction "
+                               "U[10,35] deg, cohesion U[5,50] MPa, rate log-U[1e-16,1e-14] 1/s; one graph launch "
+                               "= one Newton-iteration unit of every model" % len(plist),
+                   "rheology": args.mode, "models": len(plist), "cells": total, "graph_chains": br.value,
+                   "shapes_rheo/picard/jacobian": kinds, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(whole, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(whole / HBM_PEAK_GBS, 4), "traffic": None, "stage": "whole unit",
+                     "bytes_per_cell": 2816},
+        "cpu_baseline": cpu,
+        "setup_s": round(setup_s, 1),
+        "autotune_s": round(tune_s, 1),
+    }
+    if plain_ms is not None:
+        line["plain_fp64_ms_per_step"] = round(plain_ms, 4)
+        line["speedup_vs_plain_fp64"] = round(plain_ms / ms.value, 4)
+        line["residual_rel_err_vs_fp64"] = err
+    if training is not None:
+        line["surrogate_training"] = training
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +278,9 @@ def main():
     ap.add_argument("--probe-mb", type=int, default=1024, help="size of the HBM copy probe (MiB)")
     ap.add_argument("--train-samples", type=int, default=1 << 24, help="surrogate training samples (learned mode)")
     ap.add_argument("--train-epochs", type=int, default=20)
+    ap.add_argument("--workload", default="single", choices=["single", "ensemble"],
+                    help="one model of --ncx x --ncz cells, or an ensemble of --models seeded models")
+    ap.add_argument("--models", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path is built and measured on a single MI355X; "
@@ -158,6 +290,9 @@ def main():
     mdz = _pkg()
     hot = mdz.lib_hot()
     mdz.check(hot.mdz_set_rheology_kernel(0 if args.kernel == "auto" else 1))
+    if args.workload == "ensemble":
+        run_ensemble(args, mdz, hot)
+        return
     m = mdz.Model.lithosphere(args.ncx, args.ncz, args.seed)
     ctx = C.c_void_p()
     mdz.check(hot.mdz_ctx_create(C.byref(ctx), m.m.mesh, C.byref(m.m.materials), C.byref(m.m.model),

@@ -345,12 +345,12 @@ int  mdz_ctx_download_block(mdz_ctx *ctx, int which, int *Ic, int *J, double *A,
                             double *bbc);
 /* Residual vectors (F of the momentum and continuity blocks). */
 int  mdz_ctx_download_residual(mdz_ctx *ctx, double *Fmom, double *Fcont);
-/* Switches the collection of local-iteration histograms on (1) or off (0,
- * the default: collection costs a few percent of the rheology kernels). */
 /* ExtractDiagonalScale + ScaleMatrix on the device-resident Picard (kind 0)
  * or Jacobian (kind 1) blocks, after mdz_ctx_newton_unit
  * (Main_DOODZ.c:801-820 with diag_scaling = 1). */
 int  mdz_ctx_diagonal_scaling(mdz_ctx *ctx, int kind);
+/* Switches the collection of local-iteration histograms on (1) or off (0,
+ * the default: collection costs a few percent of the rheology kernels). */
 int  mdz_ctx_collect_histogram(mdz_ctx *ctx, int on);
 /* Local-iteration counts accumulated since the last call while collection
  * was on (node x phase evaluations): hist[0..63] = FP64 visco-elastic Newton iterations,
@@ -370,6 +370,43 @@ int  mdz_ctx_time_units(mdz_ctx *ctx, int reps, double *ms_per_unit);
 int  mdz_copy_probe(size_t bytes, int reps, double *gbs);
 /* The HIP stream (hipStream_t) the context launches on. */
 void *mdz_ctx_stream(mdz_ctx *ctx);
+/* Block shapes of the tunable kernels of a context (no counterpart in the
+ * reference): threads per block of the rheology kernels (64, 128, 256),
+ * rows per block of the LDS-staged Picard (0 = unstaged, 64, 128, 256) and
+ * Jacobian (0, 64, 128) value kernels.  Results do not depend on them. */
+int  mdz_ctx_set_launch_shape(mdz_ctx *ctx, int rheo, int pic, int jac);
+int  mdz_ctx_get_launch_shape(mdz_ctx *ctx, int *shape3);
+
+/* ---------------------------------------------------------------------- */
+/* Ensembles of independent models (no counterpart in the reference: the  */
+/* reference runs one model per process).  One graph launch takes every    */
+/* model of the ensemble through one Newton-iteration unit.                */
+/* ---------------------------------------------------------------------- */
+typedef struct mdz_ensemble mdz_ensemble;
+/* The contexts stay owned by the caller and must outlive the ensemble;
+ * their pending work is synchronised here.  Between mdz_ensemble_launch
+ * and mdz_ensemble_synchronize no call may touch the contexts. */
+int  mdz_ensemble_create(mdz_ensemble **out, mdz_ctx *const *ctx, int n);
+void mdz_ensemble_destroy(mdz_ensemble *ens);
+/* Number of concurrent chains of the graph (1..8, default 4); models are
+ * spread over the chains by cell count, largest first. */
+int  mdz_ensemble_set_branches(mdz_ensemble *ens, int branches);
+/* Enqueues one Newton unit of every model as one graph launch (captured
+ * on first use and again after any context setting changed; the captured
+ * graph is kept and relaunched).  Asynchronous. */
+int  mdz_ensemble_launch(mdz_ensemble *ens);
+int  mdz_ensemble_synchronize(mdz_ensemble *ens);
+/* resx, resz, resp of every model after the last launch (n * 3 doubles;
+ * synchronises). */
+int  mdz_ensemble_residuals(mdz_ensemble *ens, double *res3n);
+/* Mean milliseconds per graph launch over `reps` launches. */
+int  mdz_ensemble_time(mdz_ensemble *ens, int reps, double *ms_per_launch);
+/* Autotuner: per size class (cell counts within a power of two) the block
+ * shape of each tunable kernel is timed on the class's largest model and
+ * applied to the class, then the chain count (1, 2, 4, 8) is timed on the
+ * whole graph.  shapes3n (optional, n * 3) and branches (optional) receive
+ * the choice. */
+int  mdz_ensemble_autotune(mdz_ensemble *ens, int reps, int *shapes3n, int *branches);
 
 /* ---------------------------------------------------------------------- */
 /* Surrogate network for the initial guess of the local iteration (no     */

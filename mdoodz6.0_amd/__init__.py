@@ -130,7 +130,9 @@ HOT_SYMBOLS = ["Viscosity", "NonNewtonianViscosityGrid", "StrainRateComponents",
                "mdz_ctx_stage_times", "mdz_ctx_time_units", "mdz_copy_probe", "mdz_mlp_create",
                "mdz_mlp_destroy", "mdz_mlp_init", "mdz_mlp_set_weights", "mdz_mlp_get_weights", "mdz_mlp_generate",
                "mdz_mlp_upload_samples", "mdz_mlp_download_samples", "mdz_mlp_gradient", "mdz_mlp_adam_step",
-               "mdz_mlp_train", "mdz_mlp_predict"]
+               "mdz_mlp_train", "mdz_mlp_predict", "mdz_ctx_set_launch_shape", "mdz_ctx_get_launch_shape",
+               "mdz_ensemble_create", "mdz_ensemble_destroy", "mdz_ensemble_set_branches", "mdz_ensemble_launch",
+               "mdz_ensemble_synchronize", "mdz_ensemble_residuals", "mdz_ensemble_time", "mdz_ensemble_autotune"]
 MLP_NPARAMS = 9090
 
 
@@ -190,6 +192,18 @@ def lib_hot():
         L.mdz_ctx_stream.restype = C.c_void_p
         L.mdz_ctx_stream.argtypes = [C.c_void_p]
         L.mdz_set_rheology_kernel.argtypes = [C.c_int]
+        ip = C.POINTER(C.c_int)
+        L.mdz_ctx_set_launch_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.mdz_ctx_get_launch_shape.argtypes = [vp, ip]
+        L.mdz_ensemble_create.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_int]
+        L.mdz_ensemble_destroy.argtypes = [vp]
+        L.mdz_ensemble_destroy.restype = None
+        L.mdz_ensemble_set_branches.argtypes = [vp, C.c_int]
+        L.mdz_ensemble_launch.argtypes = [vp]
+        L.mdz_ensemble_synchronize.argtypes = [vp]
+        L.mdz_ensemble_residuals.argtypes = [vp, dp]
+        L.mdz_ensemble_time.argtypes = [vp, C.c_int, dp]
+        L.mdz_ensemble_autotune.argtypes = [vp, C.c_int, ip, ip]
         L.mdz_set_exit_on_error(0)
         _libs["hot"] = L
     return _libs["hot"]

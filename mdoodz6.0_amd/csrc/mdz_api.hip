@@ -18,6 +18,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -129,6 +130,8 @@ struct mdz_ctx {
     size_t net_floats;
     unsigned short *net_wp;
     float *net_bp;
+    LaunchShape shape;   /* block shapes of the tunable kernels (set by the ensemble autotuner) */
+    unsigned version;    /* bumped by every setter that changes what a unit launches */
 };
 
 static void fill_params(mdz_ctx *c, const params *m, const scale *s) {
@@ -190,6 +193,7 @@ static int ctx_alloc(mdz_ctx **out, int Nx, int Nz, int nph) {
     if (Nx < 4 || Nz < 4) return set_err(MDZ_EINVAL, "grid too small: %d x %d", Nx, Nz);
     mdz_ctx *c = (mdz_ctx *)calloc(1, sizeof(mdz_ctx));
     if (!c) return set_err(MDZ_ENOMEM, "host allocation");
+    c->shape = MDZ_SHAPE_DEFAULT;
     c->Nx = Nx; c->Nz = Nz; c->nph = nph;
     size_t nc = fam_len(MDZ_FAM_C, Nx, Nz), nv = fam_len(MDZ_FAM_V, Nx, Nz);
     size_t nu = fam_len(MDZ_FAM_U, Nx, Nz), nw = fam_len(MDZ_FAM_W, Nx, Nz);
@@ -496,19 +500,18 @@ extern "C" int mdz_set_rheology_kernel(int which) {
     return MDZ_OK;
 }
 
-static int rheology_pass(mdz_ctx *c, int use_guess) {
+static int rheology_pass(mdz_ctx *c, int use_guess, hipStream_t s) {
     /* vertex values of div_u, T, P, d and phi (NonNewtonianViscosityGrid :979-983) */
-    mdz_launch_interp5(c->g, c->prm, c->stream);
+    mdz_launch_interp5(c->g, c->prm, s);
     if (use_guess) {
         if (!c->net_wp || !c->guess_n || !c->guess_s) return set_err(MDZ_EINVAL, "learned guess without a network");
-        mdz_launch_mlp_guess(c->g, c->prm, DevMLPW{c->net_wp, c->net_bp}, c->guess_n, c->guess_s, c->stream);
+        mdz_launch_mlp_guess(c->g, c->prm, DevMLPW{c->net_wp, c->net_bp}, c->guess_n, c->guess_s, s);
     }
     const float *gn = use_guess ? c->guess_n : nullptr, *gs = use_guess ? c->guess_s : nullptr;
     if (g_rheo_kernel == 0 && mdz_rheo_fast_eligible(c->prm, &c->materials))
-        mdz_launch_rheology_fast(c->g, c->prm, c->d_pk, c->materials.R, gn, gs, c->hist_on ? c->hist : nullptr,
-                                 c->stream);
-    else
-        mdz_launch_rheology(c->g, c->prm, c->d_mat, gn, gs, c->hist_on ? c->hist : nullptr, c->stream);
+        return mdz_launch_rheology_fast(c->g, c->prm, c->d_pk, c->materials.R, gn, gs, c->hist_on ? c->hist : nullptr,
+                                        c->shape.rheo, s);
+    mdz_launch_rheology(c->g, c->prm, c->d_mat, gn, gs, c->hist_on ? c->hist : nullptr, s);
     return MDZ_OK;
 }
 
@@ -525,7 +528,7 @@ extern "C" void NonNewtonianViscosityGrid(grid *mesh, mat_prop *materials, param
     dropin_check(up_phases(c, mesh, model->Nb_phases));
     dropin_check(up_materials(c, materials));
     dropin_check(up_list(c, mesh, {RHEO_IN}));
-    rheology_pass(c, 0);
+    dropin_check(rheology_pass(c, 0, c->stream));
     dropin_check(down_list(c, mesh, {RHEO_OUT}));
     dropin_check(sync(c));
 }
@@ -600,7 +603,7 @@ extern "C" void UpdateNonLinearity(grid *mesh, markers *particles, markers *topo
     dropin_check(up_list(c, mesh, {RHS_IN}));
     dropin_check(up_list(c, mesh, {DT_IN}));
     mdz_launch_strain_rates(c->g, c->prm, c->stream);
-    rheology_pass(c, 0);
+    dropin_check(rheology_pass(c, 0, c->stream));
     mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, c->stream);
     mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
     dropin_check(down_list(c, mesh, {STRAIN_OUT}));
@@ -734,7 +737,7 @@ static void build_operator_dropin(int kind, grid *mesh, params model, double *p_
         rc = build_pattern(c, kind);
         if (rc) { dropin_check(rc); return; }
         a = make_asm(c);
-        mdz_launch_values(kind, a, A, B, C, D, c->stream);
+        mdz_launch_values(kind, a, A, B, C, D, kind == 0 ? c->shape.pic : c->shape.jac, c->stream);
         dropin_check(download_block(c, A, SA, 1));
         dropin_check(download_block(c, B, SB, 1));
         dropin_check(download_block(c, C, SC, 1));
@@ -1047,33 +1050,49 @@ extern "C" int mdz_ctx_set_rheology_mode(mdz_ctx *c, int mode) {
     if (!c || mode < MDZ_RHEO_FP64 || mode > MDZ_RHEO_LEARNED) return set_err(MDZ_EINVAL, "rheology mode %d", mode);
     if (mode == MDZ_RHEO_LEARNED && !c->d_net) return set_err(MDZ_EINVAL, "learned mode needs mdz_ctx_set_network first");
     c->prm.rheo_mode = mode;
+    c->version++;
+    return MDZ_OK;
+}
+
+/* One Newton-iteration unit of a context, enqueued on stream s with no
+ * allocation, memset, host synchronisation or copy: safe to capture into a
+ * graph.  The norms stay in c->norm_out (six doubles). */
+static int enqueue_unit(mdz_ctx *c, hipStream_t s) {
+    DevAsm a = make_asm(c);
+    mdz_launch_strain_rates(c->g, c->prm, s);
+    int rc = rheology_pass(c, c->prm.rheo_mode == MDZ_RHEO_LEARNED, s);
+    if (rc) return rc;
+    mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, s);
+    mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
+    mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, s);
+    mdz_launch_dtensor(c->g, c->prm, 1, c->status, s);
+    mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s);
+    mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
+    int kind = c->model.aniso == 1 ? 1 : 0;
+    mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], s);
+    mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, s);
+    return MDZ_OK;
+}
+
+/* resx, resz, resp from the six sums of the norm kernels */
+static int norms_to_res(const double *out6, double *res3) {
+    res3[0] = sqrt(out6[0] / out6[3]);
+    res3[1] = sqrt(out6[1] / out6[4]);
+    res3[2] = sqrt(out6[2] / out6[5]);
+    if (isnan(res3[0]) || isnan(res3[1]) || isnan(res3[2])) return set_err(MDZ_ENAN, "NaN residual");
     return MDZ_OK;
 }
 
 extern "C" int mdz_ctx_newton_unit(mdz_ctx *c, double *res3) {
     if (!c) return set_err(MDZ_EINVAL, "NULL context");
     g_status = 0;
-    DevAsm a = make_asm(c);
-    mdz_launch_strain_rates(c->g, c->prm, c->stream);
-    rheology_pass(c, c->prm.rheo_mode == MDZ_RHEO_LEARNED);
-    mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, c->stream);
-    mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
-    mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->stream);
-    mdz_launch_dtensor(c->g, c->prm, 1, c->status, c->stream);
-    mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->stream);
-    mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
-    int kind = c->model.aniso == 1 ? 1 : 0;
-    mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], c->stream);
-    mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, c->stream);
+    int rc = enqueue_unit(c, c->stream);
+    if (rc) return rc;
     if (res3) {
         double out6[6];
         HIPCHK(hipMemcpyAsync(out6, c->norm_out, 6 * 8, hipMemcpyDeviceToHost, c->stream));
-        int rc = sync(c);
-        if (rc) return rc;
-        res3[0] = sqrt(out6[0] / out6[3]);
-        res3[1] = sqrt(out6[1] / out6[4]);
-        res3[2] = sqrt(out6[2] / out6[5]);
-        if (isnan(res3[0]) || isnan(res3[1]) || isnan(res3[2])) return set_err(MDZ_ENAN, "NaN residual");
+        if ((rc = sync(c))) return rc;
+        if ((rc = norms_to_res(out6, res3))) return rc;
     }
     HIPCHK(hipGetLastError());
     return MDZ_OK;
@@ -1118,6 +1137,7 @@ extern "C" int mdz_ctx_diagonal_scaling(mdz_ctx *c, int kind) {
 extern "C" int mdz_ctx_collect_histogram(mdz_ctx *c, int on) {
     if (!c) return set_err(MDZ_EINVAL, "NULL context");
     c->hist_on = on != 0;
+    c->version++;
     return MDZ_OK;
 }
 
@@ -1146,6 +1166,7 @@ extern "C" int mdz_ctx_set_network(mdz_ctx *c, const float *w, size_t n) {
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipGetLastError());
     c->net_floats = n;
+    c->version++;
     return MDZ_OK;
 }
 
@@ -1168,17 +1189,18 @@ extern "C" int mdz_ctx_stage_times(mdz_ctx *c, int reps, double *ms) {
         HIPCHK(hipEventRecord(ev[0], c->stream));
         mdz_launch_strain_rates(c->g, c->prm, c->stream);
         HIPCHK(hipEventRecord(ev[1], c->stream));
-        rheology_pass(c, c->prm.rheo_mode == MDZ_RHEO_LEARNED);
+        int rc = rheology_pass(c, c->prm.rheo_mode == MDZ_RHEO_LEARNED, c->stream);
+        if (rc) return rc;
         HIPCHK(hipEventRecord(ev[2], c->stream));
         mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, c->stream);
         HIPCHK(hipEventRecord(ev[3], c->stream));
         mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
         HIPCHK(hipEventRecord(ev[4], c->stream));
-        mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->stream);
+        mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, c->stream);
         HIPCHK(hipEventRecord(ev[5], c->stream));
         mdz_launch_dtensor(c->g, c->prm, 1, c->status, c->stream);
         HIPCHK(hipEventRecord(ev[6], c->stream));
-        mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->stream);
+        mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, c->stream);
         HIPCHK(hipEventRecord(ev[7], c->stream));
         mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
         HIPCHK(hipEventRecord(ev[8], c->stream));
@@ -1256,5 +1278,343 @@ extern "C" int mdz_ctx_time_units(mdz_ctx *c, int reps, double *ms_per_unit) {
     *ms_per_unit = t / reps;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    return MDZ_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* ensembles: many independent models, one graph launch per Newton      */
+/* iteration                                                            */
+/* ------------------------------------------------------------------ */
+extern "C" int mdz_ctx_set_launch_shape(mdz_ctx *c, int rheo, int pic, int jac) {
+    if (!c) return set_err(MDZ_EINVAL, "NULL context");
+    LaunchShape sh{rheo, pic, jac};
+    if (!mdz_shape_valid(sh))
+        return set_err(MDZ_EINVAL, "launch shape (%d, %d, %d): rheology 64/128/256, Picard 0/64/128/256, Jacobian 0/64/128",
+                       rheo, pic, jac);
+    c->shape = sh;
+    c->version++;
+    return MDZ_OK;
+}
+
+extern "C" int mdz_ctx_get_launch_shape(mdz_ctx *c, int *shape3) {
+    if (!c || !shape3) return set_err(MDZ_EINVAL, "NULL argument");
+    shape3[0] = c->shape.rheo;
+    shape3[1] = c->shape.pic;
+    shape3[2] = c->shape.jac;
+    return MDZ_OK;
+}
+
+#define MDZ_MAX_BRANCHES 8
+
+struct mdz_ensemble {
+    int n;
+    std::vector<mdz_ctx *> ctx;
+    int branches;                 /* concurrent chains of the captured graph */
+    hipStream_t st[MDZ_MAX_BRANCHES];
+    hipEvent_t fork, join[MDZ_MAX_BRANCHES];
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    std::vector<unsigned> captured_version;
+    int captured_kernel;          /* rheology kernel selection at capture */
+    double **d_src;               /* per model: its norm_out */
+    double *d_res;                /* gathered norm sums, 6 per model */
+    std::vector<int> branch_of;   /* model -> chain */
+};
+
+/* norm sums of every model into one array (one copy to the host after the graph) */
+__global__ void k_gather_norms(const double *const *__restrict__ src, double *__restrict__ dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 6 * n) dst[i] = src[i / 6][i % 6];
+}
+
+static size_t ctx_cells(const mdz_ctx *c) { return (size_t)(c->Nx - 1) * (c->Nz - 1); }
+
+/* Chains balanced by cell count: largest model first onto the least loaded
+ * chain (longest-processing-time rule). */
+static void assign_branches(mdz_ensemble *e) {
+    std::vector<int> order(e->n);
+    for (int i = 0; i < e->n; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [e](int a, int b) {
+        size_t ca = ctx_cells(e->ctx[a]), cb = ctx_cells(e->ctx[b]);
+        return ca != cb ? ca > cb : a < b;
+    });
+    std::vector<double> load(e->branches, 0.0);
+    e->branch_of.assign(e->n, 0);
+    for (int m : order) {
+        int best = 0;
+        for (int b = 1; b < e->branches; b++)
+            if (load[b] < load[best]) best = b;
+        e->branch_of[m] = best;
+        load[best] += (double)ctx_cells(e->ctx[m]);
+    }
+}
+
+static void drop_graph(mdz_ensemble *e) {
+    if (e->exec) hipGraphExecDestroy(e->exec);
+    if (e->graph) hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
+/* Captures one Newton unit of every model: a fork into `branches` chains,
+ * the units of each chain in order of decreasing size, a join, and the
+ * gather of the norm sums.  Nothing captured allocates, clears memory with
+ * a memset node or synchronises with the host. */
+static int capture(mdz_ensemble *e) {
+    drop_graph(e);
+    assign_branches(e);
+    hipStream_t s0 = e->st[0];
+    HIPCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+    int rc = MDZ_OK;
+    hipError_t he = hipEventRecord(e->fork, s0);
+    for (int b = 1; b < e->branches && he == hipSuccess; b++) he = hipStreamWaitEvent(e->st[b], e->fork, 0);
+    std::vector<int> order(e->n);
+    for (int i = 0; i < e->n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [e](int a, int b) { return ctx_cells(e->ctx[a]) > ctx_cells(e->ctx[b]); });
+    for (int k = 0; k < e->n && he == hipSuccess && rc == MDZ_OK; k++) {
+        int m = order[k];
+        rc = enqueue_unit(e->ctx[m], e->st[e->branch_of[m]]);
+    }
+    for (int b = 1; b < e->branches && he == hipSuccess; b++) {
+        he = hipEventRecord(e->join[b], e->st[b]);
+        if (he == hipSuccess) he = hipStreamWaitEvent(s0, e->join[b], 0);
+    }
+    if (he == hipSuccess)
+        hipLaunchKernelGGL(k_gather_norms, dim3(mdz_div_up(6 * e->n, 256)), dim3(256), 0, s0,
+                           (const double *const *)e->d_src, e->d_res, e->n);
+    hipGraph_t gr = nullptr;
+    hipError_t he2 = hipStreamEndCapture(s0, &gr);
+    if (rc) { if (gr) hipGraphDestroy(gr); return rc; }
+    if (he != hipSuccess) { if (gr) hipGraphDestroy(gr); return set_err(MDZ_EHIP, "ensemble capture: %s", hipGetErrorString(he)); }
+    if (he2 != hipSuccess) return set_err(MDZ_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(he2));
+    e->graph = gr;
+    HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    e->captured_version.resize(e->n);
+    for (int i = 0; i < e->n; i++) e->captured_version[i] = e->ctx[i]->version;
+    e->captured_kernel = g_rheo_kernel;
+    return MDZ_OK;
+}
+
+static bool graph_current(const mdz_ensemble *e) {
+    if (!e->exec || e->captured_kernel != g_rheo_kernel) return false;
+    for (int i = 0; i < e->n; i++)
+        if (e->captured_version[i] != e->ctx[i]->version) return false;
+    return true;
+}
+
+extern "C" void mdz_ensemble_destroy(mdz_ensemble *e) {
+    if (!e) return;
+    if (e->st[0]) hipStreamSynchronize(e->st[0]);
+    drop_graph(e);
+    for (int b = 0; b < MDZ_MAX_BRANCHES; b++) {
+        if (e->st[b]) hipStreamDestroy(e->st[b]);
+        if (e->join[b]) hipEventDestroy(e->join[b]);
+    }
+    if (e->fork) hipEventDestroy(e->fork);
+    hipFree(e->d_src);
+    hipFree(e->d_res);
+    delete e;
+}
+
+extern "C" int mdz_ensemble_create(mdz_ensemble **out, mdz_ctx *const *ctx, int n) {
+    g_status = 0;
+    if (!out || !ctx || n < 1) return set_err(MDZ_EINVAL, "ensemble of %d contexts", n);
+    for (int i = 0; i < n; i++) {
+        if (!ctx[i]) return set_err(MDZ_EINVAL, "ensemble: context %d is NULL", i);
+        for (int j = 0; j < i; j++)
+            if (ctx[j] == ctx[i]) return set_err(MDZ_EINVAL, "ensemble: context %d repeats context %d", i, j);
+    }
+    mdz_ensemble *e = new mdz_ensemble();
+    e->n = n;
+    e->ctx.assign(ctx, ctx + n);
+    e->branches = 4;
+    int rc = MDZ_OK;
+    for (int b = 0; b < MDZ_MAX_BRANCHES && !rc; b++) {
+        if (hipStreamCreate(&e->st[b]) != hipSuccess || hipEventCreateWithFlags(&e->join[b], hipEventDisableTiming) != hipSuccess)
+            rc = set_err(MDZ_EHIP, "ensemble streams");
+    }
+    if (!rc && hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess) rc = set_err(MDZ_EHIP, "ensemble event");
+    if (!rc && (hipMalloc((void **)&e->d_src, n * sizeof(double *)) != hipSuccess ||
+                hipMalloc((void **)&e->d_res, 6 * (size_t)n * sizeof(double)) != hipSuccess))
+        rc = set_err(MDZ_ENOMEM, "ensemble buffers");
+    if (!rc) {
+        std::vector<double *> src(n);
+        for (int i = 0; i < n; i++) {
+            src[i] = ctx[i]->norm_out;
+            if (hipStreamSynchronize(ctx[i]->stream) != hipSuccess) rc = set_err(MDZ_EHIP, "context %d stream", i);
+        }
+        if (!rc && hipMemcpy(e->d_src, src.data(), n * sizeof(double *), hipMemcpyHostToDevice) != hipSuccess)
+            rc = set_err(MDZ_EHIP, "ensemble pointer table");
+    }
+    if (rc) { mdz_ensemble_destroy(e); return rc; }
+    *out = e;
+    return MDZ_OK;
+}
+
+extern "C" int mdz_ensemble_set_branches(mdz_ensemble *e, int branches) {
+    if (!e || branches < 1 || branches > MDZ_MAX_BRANCHES)
+        return set_err(MDZ_EINVAL, "ensemble branches %d (1..%d)", branches, MDZ_MAX_BRANCHES);
+    if (e->branches != branches) {
+        e->branches = branches;
+        drop_graph(e);
+    }
+    return MDZ_OK;
+}
+
+extern "C" int mdz_ensemble_launch(mdz_ensemble *e) {
+    if (!e) return set_err(MDZ_EINVAL, "NULL ensemble");
+    g_status = 0;
+    if (!graph_current(e)) {
+        int rc = capture(e);
+        if (rc) return rc;
+    }
+    HIPCHK(hipGraphLaunch(e->exec, e->st[0]));
+    return MDZ_OK;
+}
+
+extern "C" int mdz_ensemble_synchronize(mdz_ensemble *e) {
+    if (!e) return set_err(MDZ_EINVAL, "NULL ensemble");
+    HIPCHK(hipStreamSynchronize(e->st[0]));
+    HIPCHK(hipGetLastError());
+    return MDZ_OK;
+}
+
+extern "C" int mdz_ensemble_residuals(mdz_ensemble *e, double *res) {
+    if (!e || !res) return set_err(MDZ_EINVAL, "NULL argument");
+    std::vector<double> h(6 * (size_t)e->n);
+    HIPCHK(hipMemcpyAsync(h.data(), e->d_res, h.size() * sizeof(double), hipMemcpyDeviceToHost, e->st[0]));
+    HIPCHK(hipStreamSynchronize(e->st[0]));
+    int rc = MDZ_OK;
+    for (int i = 0; i < e->n; i++) {
+        int r = norms_to_res(&h[6 * (size_t)i], res + 3 * (size_t)i);
+        if (r && !rc) rc = set_err(MDZ_ENAN, "NaN residual in model %d", i);
+    }
+    return rc;
+}
+
+extern "C" int mdz_ensemble_time(mdz_ensemble *e, int reps, double *ms_per_launch) {
+    if (!e || reps < 1 || !ms_per_launch) return set_err(MDZ_EINVAL, "ensemble time");
+    int rc = mdz_ensemble_launch(e); /* captures if needed; warm-up */
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, e->st[0]));
+    for (int r = 0; r < reps && !rc; r++) rc = mdz_ensemble_launch(e);
+    HIPCHK(hipEventRecord(e1, e->st[0]));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    *ms_per_launch = t / reps;
+    return MDZ_OK;
+}
+
+/* ---- autotuner ---- */
+/* Mean milliseconds of one tunable stage of context c on stream s, over
+ * `reps` launches after one untimed launch: 0 = rheology (interpolation,
+ * guess and local iterations), 1 = Picard values, 2 = Jacobian values (the
+ * D tensor of its kind is put in place first, and the tensor of the
+ * residual restored after). */
+static int time_stage(mdz_ctx *c, int stage, hipStream_t s, int reps, hipEvent_t e0, hipEvent_t e1, float *ms) {
+    DevAsm a = make_asm(c);
+    if (stage == 2) mdz_launch_dtensor(c->g, c->prm, 1, c->status, s);
+    float t = 0;
+    for (int r = 0; r <= reps; r++) {
+        if (r == 1) HIPCHK(hipEventRecord(e0, s));
+        if (stage == 0) {
+            int rc = rheology_pass(c, c->prm.rheo_mode == MDZ_RHEO_LEARNED, s);
+            if (rc) return rc;
+        } else if (stage == 1) {
+            mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, s);
+        } else {
+            mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s);
+        }
+    }
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    if (stage == 2) mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
+    HIPCHK(hipGetLastError());
+    *ms = t / reps;
+    return MDZ_OK;
+}
+
+/* Picks, for every size class of the ensemble (models whose cell counts
+ * share a power of two), the block shape of each tunable stage by timing
+ * the candidates on the largest model of the class, applies it to the
+ * class, then picks the number of graph chains (1, 2, 4, 8) by timing the
+ * captured ensemble.  report (optional, n * 3 ints) receives the chosen
+ * shape of each model; *branches (optional) the chosen chain count. */
+extern "C" int mdz_ensemble_autotune(mdz_ensemble *e, int reps, int *report, int *branches) {
+    if (!e || reps < 1) return set_err(MDZ_EINVAL, "autotune");
+    g_status = 0;
+    hipStream_t s = e->st[0];
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    static const int cand_rheo[] = {64, 128, 256}, cand_pic[] = {0, 64, 128, 256}, cand_jac[] = {0, 64, 128};
+    const int *cand[3] = {cand_rheo, cand_pic, cand_jac};
+    const int ncand[3] = {3, 4, 3};
+    int rc = MDZ_OK;
+    std::vector<int> done(e->n, 0);
+    for (int i = 0; i < e->n && !rc; i++) {
+        if (done[i]) continue;
+        int cls = 0;
+        for (size_t v = ctx_cells(e->ctx[i]); v > 1; v >>= 1) cls++;
+        int rep = i;
+        std::vector<int> members;
+        for (int j = i; j < e->n; j++) {
+            int cj = 0;
+            for (size_t v = ctx_cells(e->ctx[j]); v > 1; v >>= 1) cj++;
+            if (cj != cls || done[j]) continue;
+            members.push_back(j);
+            if (ctx_cells(e->ctx[j]) > ctx_cells(e->ctx[rep])) rep = j;
+        }
+        mdz_ctx *c = e->ctx[rep];
+        LaunchShape best = c->shape;
+        for (int stage = 0; stage < 3 && !rc; stage++) {
+            float tbest = 1e30f;
+            int vbest = 0;
+            for (int k = 0; k < ncand[stage] && !rc; k++) {
+                LaunchShape sh = best;
+                (stage == 0 ? sh.rheo : stage == 1 ? sh.pic : sh.jac) = cand[stage][k];
+                c->shape = sh;
+                float t = 0;
+                rc = time_stage(c, stage, s, reps, e0, e1, &t);
+                if (!rc && t < tbest) { tbest = t; vbest = cand[stage][k]; }
+            }
+            (stage == 0 ? best.rheo : stage == 1 ? best.pic : best.jac) = vbest;
+        }
+        for (int j : members) {
+            e->ctx[j]->shape = best;
+            e->ctx[j]->version++;
+            done[j] = 1;
+        }
+    }
+    int bbest = e->branches;
+    if (!rc) {
+        double tb = 1e30;
+        for (int b = 1; b <= MDZ_MAX_BRANCHES && !rc; b *= 2) {
+            if ((rc = mdz_ensemble_set_branches(e, b))) break;
+            double t = 0;
+            rc = mdz_ensemble_time(e, reps, &t);
+            if (!rc && t < tb) { tb = t; bbest = b; }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    if ((rc = mdz_ensemble_set_branches(e, bbest))) return rc;
+    if (report)
+        for (int i = 0; i < e->n; i++) {
+            report[3 * i] = e->ctx[i]->shape.rheo;
+            report[3 * i + 1] = e->ctx[i]->shape.pic;
+            report[3 * i + 2] = e->ctx[i]->shape.jac;
+        }
+    if (branches) *branches = bbest;
     return MDZ_OK;
 }

@@ -1260,18 +1260,31 @@ void mdz_fill_pattern(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBl
 #define PIC_CAPB 2
 #define JAC_CAPA 21
 #define JAC_CAPB 6
-#define PIC_VBLK 256
-#define JAC_VBLK 128
 
-void mdz_launch_values(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, hipStream_t s) {
+template <int KIND, int VBLK, int CAPA, int CAPB>
+static void launch_staged(const DevAsm &a, DevBlock *A, DevBlock *B, hipStream_t s) {
+    hipLaunchKernelGGL((k_values_mom<KIND, VBLK, CAPA, CAPB>), dim3(mdz_div_up(a.e.neq_mom, VBLK)), dim3(VBLK), 0, s,
+                       a, *A, *B);
+}
+
+int mdz_shape_valid(const LaunchShape &sh) {
+    return (sh.rheo == 64 || sh.rheo == 128 || sh.rheo == 256) &&
+           (sh.pic == 0 || sh.pic == 64 || sh.pic == 128 || sh.pic == 256) &&
+           (sh.jac == 0 || sh.jac == 64 || sh.jac == 128);
+}
+
+void mdz_launch_values(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, int vblk,
+                       hipStream_t s) {
     int nm = a.e.neq_mom, ncnt = a.e.neq_cont;
     if (nm > 0) {
-        if (kind == 0 && A->maxrow <= PIC_CAPA && B->maxrow <= PIC_CAPB)
-            hipLaunchKernelGGL((k_values_mom<0, PIC_VBLK, PIC_CAPA, PIC_CAPB>), dim3(mdz_div_up(nm, PIC_VBLK)),
-                               dim3(PIC_VBLK), 0, s, a, *A, *B);
-        else if (kind == 1 && A->maxrow <= JAC_CAPA && B->maxrow <= JAC_CAPB)
-            hipLaunchKernelGGL((k_values_mom<1, JAC_VBLK, JAC_CAPA, JAC_CAPB>), dim3(mdz_div_up(nm, JAC_VBLK)),
-                               dim3(JAC_VBLK), 0, s, a, *A, *B);
+        const bool fits = kind == 0 ? A->maxrow <= PIC_CAPA && B->maxrow <= PIC_CAPB
+                                    : A->maxrow <= JAC_CAPA && B->maxrow <= JAC_CAPB;
+        const int v = fits ? vblk : 0;
+        if (kind == 0 && v == 64) launch_staged<0, 64, PIC_CAPA, PIC_CAPB>(a, A, B, s);
+        else if (kind == 0 && v == 128) launch_staged<0, 128, PIC_CAPA, PIC_CAPB>(a, A, B, s);
+        else if (kind == 0 && v == 256) launch_staged<0, 256, PIC_CAPA, PIC_CAPB>(a, A, B, s);
+        else if (kind == 1 && v == 64) launch_staged<1, 64, JAC_CAPA, JAC_CAPB>(a, A, B, s);
+        else if (kind == 1 && v == 128) launch_staged<1, 128, JAC_CAPA, JAC_CAPB>(a, A, B, s);
         else if (kind == 0)
             hipLaunchKernelGGL(k_values_mom_direct<0>, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, a, *A, *B);
         else

@@ -23,8 +23,9 @@ void mdz_launch_interp5(const DevGrid &g, const DevParams &prm, hipStream_t s);
 int mdz_rheo_fast_eligible(const DevParams &prm, const mat_prop *host_mt);
 size_t mdz_phasek_bytes(void);
 void mdz_launch_phase_constants(const mat_prop *d_mt, int nph, void *d_pk, hipStream_t s);
-void mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
-                              const float *guess_s, unsigned long long *hist, hipStream_t s);
+/* nt: threads per block, 64, 128 or 256 (anything else is MDZ_EINVAL) */
+int mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
+                             const float *guess_s, unsigned long long *hist, int nt, hipStream_t s);
 void mdz_launch_dtensor(const DevGrid &g, const DevParams &prm, int jac, int *status, hipStream_t s);
 void mdz_launch_rhs(const DevGrid &g, const DevParams &prm, const double *xg, const double *zvx,
                     const double *xvz, const double *zg, hipStream_t s);
@@ -45,8 +46,21 @@ int mdz_build_pattern(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBl
                       size_t scratch_len, hipStream_t s);
 /* second half of the pattern build (after J is allocated to the nnz counts) */
 void mdz_fill_pattern(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, hipStream_t s);
-/* values of the assembled blocks (Assemble = 1) */
-void mdz_launch_values(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, hipStream_t s);
+/* Launch shape of the tunable kernels of one model: threads per block of
+ * the log-space rheology kernels (64, 128, 256) and rows per block of the
+ * LDS-staged momentum value kernels, Picard (64, 128, 256) and Jacobian
+ * (64, 128); 0 selects the unstaged value kernel.  Every shape computes the
+ * same values bit for bit: the per-row arithmetic does not depend on it. */
+struct LaunchShape {
+    int rheo, pic, jac;
+};
+#define MDZ_SHAPE_DEFAULT LaunchShape{256, 256, 128}
+int mdz_shape_valid(const LaunchShape &sh);
+/* values of the assembled blocks (Assemble = 1); vblk = rows per block of
+ * the staged momentum kernel (0 = unstaged; a pattern whose rows do not fit
+ * the LDS slots always takes the unstaged kernel) */
+void mdz_launch_values(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, int vblk,
+                       hipStream_t s);
 /* residual rows (Assemble = 0) through the Picard (kind 0) or Jacobian (kind 1) stencils;
  * A/C carry b, F, bbc; D carries F and d */
 void mdz_launch_residual(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, DevBlock *D, hipStream_t s);

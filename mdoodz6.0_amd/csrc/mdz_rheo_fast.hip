@@ -25,7 +25,6 @@
  */
 #include "mdz_kernels.h"
 
-#define BLK 256
 #define ZEROC 273.15
 #define LN2 0.69314718055994530942
 #define LN3 1.09861228866810969140
@@ -520,12 +519,13 @@ __device__ __forceinline__ void rheo_cell(const DevGrid &g, const DevParams &prm
     }
 }
 
-__global__ void __launch_bounds__(BLK) k_rheo_cells_fast(DevGrid g, DevParams prm, const PhaseK *__restrict__ pk,
-                                                         double R, const float *__restrict__ guess,
-                                                         unsigned long long *hist) {
+template <int NT>
+__global__ void __launch_bounds__(NT) k_rheo_cells_fast(DevGrid g, DevParams prm, const PhaseK *__restrict__ pk,
+                                                        double R, const float *__restrict__ guess,
+                                                        unsigned long long *hist) {
     __shared__ unsigned int lh[3 * MDZ_HIST_BINS];
     const BlockHist bh = hist_begin(lh, hist);
-    const int c0 = blockIdx.x * BLK + threadIdx.x;
+    const int c0 = blockIdx.x * NT + threadIdx.x;
     const int nc = g.ncx * g.ncz;
     if (c0 < nc) rheo_cell(g, prm, pk, R, guess, bh, c0, nc);
     hist_flush(bh, hist);
@@ -559,12 +559,13 @@ __device__ __forceinline__ void rheo_vertex(const DevGrid &g, const DevParams &p
     }
 }
 
-__global__ void __launch_bounds__(BLK) k_rheo_vertices_fast(DevGrid g, DevParams prm, const PhaseK *__restrict__ pk,
-                                                            double R, const float *__restrict__ guess,
-                                                            unsigned long long *hist) {
+template <int NT>
+__global__ void __launch_bounds__(NT) k_rheo_vertices_fast(DevGrid g, DevParams prm, const PhaseK *__restrict__ pk,
+                                                           double R, const float *__restrict__ guess,
+                                                           unsigned long long *hist) {
     __shared__ unsigned int lh[3 * MDZ_HIST_BINS];
     const BlockHist bh = hist_begin(lh, hist);
-    const int c1 = blockIdx.x * BLK + threadIdx.x;
+    const int c1 = blockIdx.x * NT + threadIdx.x;
     const int nv = g.Nx * g.Nz;
     if (c1 < nv) rheo_vertex(g, prm, pk, R, guess, bh, c1, nv);
     hist_flush(bh, hist);
@@ -586,11 +587,22 @@ void mdz_launch_phase_constants(const mat_prop *d_mt, int nph, void *d_pk, hipSt
                        (PhaseK *)d_pk);
 }
 
-void mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
-                              const float *guess_s, unsigned long long *hist, hipStream_t s) {
+template <int NT>
+static void launch_fast(const DevGrid &g, const DevParams &prm, const PhaseK *pk, double R, const float *guess_n,
+                        const float *guess_s, unsigned long long *hist, hipStream_t s) {
     int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
-    hipLaunchKernelGGL(k_rheo_cells_fast, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, g, prm, (const PhaseK *)d_pk, R,
-                       guess_n, hist);
-    hipLaunchKernelGGL(k_rheo_vertices_fast, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm,
-                       (const PhaseK *)d_pk, R, guess_s, hist);
+    hipLaunchKernelGGL(k_rheo_cells_fast<NT>, dim3(mdz_div_up(nc, NT)), dim3(NT), 0, s, g, prm, pk, R, guess_n, hist);
+    hipLaunchKernelGGL(k_rheo_vertices_fast<NT>, dim3(mdz_div_up(nv, NT)), dim3(NT), 0, s, g, prm, pk, R, guess_s,
+                       hist);
+}
+
+int mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
+                             const float *guess_s, unsigned long long *hist, int nt, hipStream_t s) {
+    const PhaseK *pk = (const PhaseK *)d_pk;
+    switch (nt) {
+    case 64: launch_fast<64>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
+    case 128: launch_fast<128>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
+    case 256: launch_fast<256>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
+    default: return mdz_error(MDZ_EINVAL, "rheology block size: 64, 128 or 256 threads");
+    }
 }

@@ -81,3 +81,21 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stdout
+
+
+def test_argument_errors_before_any_device_call(mdz):
+    """Entry points reject bad arguments with MDZ_EINVAL before touching the
+    device (these calls run without a GPU)."""
+    lib = mdz.lib_hot()
+    EINVAL = -1
+    e = C.c_void_p()
+    assert lib.mdz_ensemble_create(C.byref(e), None, 0) == EINVAL
+    arr = (C.c_void_p * 2)(None, None)
+    assert lib.mdz_ensemble_create(C.byref(e), arr, 2) == EINVAL
+    assert b"NULL" in lib.mdz_last_error()
+    assert lib.mdz_ensemble_set_branches(None, 2) == EINVAL
+    assert lib.mdz_ensemble_launch(None) == EINVAL
+    assert lib.mdz_ensemble_autotune(None, 1, None, None) == EINVAL
+    assert lib.mdz_ctx_set_launch_shape(None, 256, 256, 128) == EINVAL
+    assert lib.mdz_ctx_newton_unit(None, None) == EINVAL
+    assert lib.mdz_set_rheology_kernel(2) == EINVAL
