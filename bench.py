@@ -28,16 +28,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 # element read or written once; FP64 = 8 B, tags 1 B, equation numbers 4 B)
 STAGES = ["strain_rates", "rheology", "rhs", "dtensor_picard", "picard_values", "dtensor_jacobian",
           "jacobian_values", "dtensor_picard_2", "residual"]
-# kernels of each stage (names as in the rocprofv3 kernel trace)
-STAGE_KERNELS = {"strain_rates": ["k_strain_cells", "k_strain_vertices", "k_exz_n", "k_interp_c2v (2 fields)"],
-                 "rheology": ["k_interp_c2v (5 fields)", "k_rheo_cells_fast", "k_rheo_vertices_fast"],
-                 "rhs": ["k_rhs_u", "k_rhs_v", "k_rhs_p"],
-                 "dtensor_picard": ["k_dtensor_cells (J=0)", "k_dtensor_vertices (J=0)"],
-                 "picard_values": ["k_values_mom<0, 256, 9, 2>", "k_values_cont"],
-                 "dtensor_jacobian": ["k_dtensor_cells (J=1)", "k_dtensor_vertices (J=1)"],
-                 "jacobian_values": ["k_values_mom<1, 128, 21, 6>", "k_values_cont"],
-                 "dtensor_picard_2": ["k_dtensor_cells (J=0)", "k_dtensor_vertices (J=0)"],
-                 "residual": ["k_residual_mom<0>", "k_residual_cont", "k_norms", "k_norms_final"]}
+# kernels of each stage (names as in the rocprofv3 kernel trace) for the
+# launch shape (rheology threads, Picard rows, Jacobian rows per block)
+def stage_kernels(shape, learned=False):
+    rh, pic, jac = shape
+    vp = "k_values_mom<0, %d, 9, 2>" % pic if pic else "k_values_mom_direct<0>"
+    vj = "k_values_mom<1, %d, 21, 6>" % jac if jac else "k_values_mom_direct<1>"
+    rheo = ["k_interp_c2v (5 fields)", "k_rheo_cells_fast<%d>" % rh, "k_rheo_vertices_fast<%d>" % rh]
+    if learned:
+        rheo += ["k_mlp_guess<true>", "k_mlp_guess<false>"]
+    return {"strain_rates": ["k_strain_cells", "k_strain_vertices", "k_exz_n", "k_interp_c2v (2 fields)"],
+            "rheology": rheo,
+            "rhs": ["k_rhs_u", "k_rhs_v", "k_rhs_p"],
+            "dtensor_picard": ["k_dtensor_cells (J=0)", "k_dtensor_vertices (J=0)"],
+            "picard_values": [vp, "k_values_cont"],
+            "dtensor_jacobian": ["k_dtensor_cells (J=1)", "k_dtensor_vertices (J=1)"],
+            "jacobian_values": [vj, "k_values_cont"],
+            "dtensor_picard_2": ["k_dtensor_cells (J=0)", "k_dtensor_vertices (J=0)"],
+            "residual": ["k_residual_mom<0>", "k_residual_cont", "k_norms", "k_norms_final"]}
+
+
 BYTES_PER_CELL = {"strain_rates": 76, "rheology": 810, "rhs": 160, "dtensor_picard": 114, "picard_values": 340,
                   "dtensor_jacobian": 330, "jacobian_values": 668, "dtensor_picard_2": 114, "residual": 204}
 HBM_PEAK_GBS = 8000.0
@@ -51,10 +61,10 @@ def _pkg():
     return mod
 
 
-def traffic_from_profile(stage, ncx, ncz):
-    """HBM bytes per launch of the stage's kernels from the committed counter
+def traffic_from_profile(kernels, ncx, ncz, mode):
+    """HBM bytes per launch of a stage's kernels from the committed counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy probe), or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_lithosphere_%dx%d.md" % (ncx, ncz))
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_lithosphere_%dx%d_%s.md" % (ncx, ncz, mode))
     if not os.path.exists(path):
         return None
     total, found = 0.0, set()
@@ -62,10 +72,10 @@ def traffic_from_profile(stage, ncx, ncz):
         if line.startswith("| `"):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             name = cells[0].strip("`")
-            if name in STAGE_KERNELS[stage] and name not in found:
+            if name in kernels and name not in found:
                 total += float(cells[-1])
                 found.add(name)
-    return total if found == set(STAGE_KERNELS[stage]) else None
+    return total if found == set(kernels) else None
 
 
 def train_surrogate(mdz, hot, m, samples, epochs, batch, lr, seed):
@@ -281,6 +291,7 @@ def main():
     ap.add_argument("--workload", default="single", choices=["single", "ensemble"],
                     help="one model of --ncx x --ncz cells, or an ensemble of --models seeded models")
     ap.add_argument("--models", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true", help="default block shapes instead of autotuned ones")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path is built and measured on a single MI355X; "
@@ -304,6 +315,14 @@ def main():
         mdz.check(hot.mdz_ctx_set_network(ctx, w, mdz.MLP_NPARAMS))
     mdz.check(hot.mdz_ctx_set_rheology_mode(ctx, mode))
     cells = args.ncx * args.ncz
+    shape = (C.c_int * 3)()
+    if not args.no_autotune:
+        # block shapes picked by the autotuner (an ensemble of this one model)
+        ens = C.c_void_p()
+        mdz.check(hot.mdz_ensemble_create(C.byref(ens), (C.c_void_p * 1)(ctx.value), 1))
+        mdz.check(hot.mdz_ensemble_autotune(ens, 3, shape, None))
+        hot.mdz_ensemble_destroy(ens)
+    mdz.check(hot.mdz_ctx_get_launch_shape(ctx, shape))
 
     for _ in range(args.warmup):
         mdz.check(hot.mdz_ctx_newton_unit(ctx, None))
@@ -334,11 +353,12 @@ def main():
     dom = max(STAGES, key=lambda s: stage_ms[s])
     dom_bytes = BYTES_PER_CELL[dom] * cells
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
-    traffic = traffic_from_profile(dom, args.ncx, args.ncz)
+    kern = stage_kernels(tuple(shape), mode == 2)
+    traffic = traffic_from_profile(kern[dom], args.ncx, args.ncz, args.mode)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": round(traffic) if traffic is not None else None,
-                "stage": dom, "kernels": STAGE_KERNELS[dom], "algorithmic_bytes": dom_bytes,
+                "stage": dom, "kernels": kern[dom], "algorithmic_bytes": dom_bytes,
                 "bytes_per_cell": BYTES_PER_CELL[dom], "launch_ms": round(stage_ms[dom], 4),
                 "probe_GBs": round(probe.value, 1), "frac_of_probe": round(achieved / probe.value, 4)}
     whole = 2816.0 * cells / (ms_per_step * 1e-3) / 1e9
@@ -367,6 +387,7 @@ def main():
         "data": "synthetic (seeded lithosphere model)",
         "config": {"workload": "lithosphere %dx%d cells, 3 phases VEP, one Newton-iteration unit" % (args.ncx, args.ncz),
                    "rheology": args.mode, "rheology_kernel": args.kernel, "cells": cells,
+                   "shape_rheo/picard/jacobian": "%d/%d/%d" % tuple(shape),
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
