@@ -29,30 +29,48 @@
 #define BLK 256
 #define OUTSIDE 99
 
+/* Emission of one row: COUNT counts the entries, PATTERN writes the column
+ * indices, VALUES the coefficients.  A neighbour of Dirichlet type (0, 31,
+ * 11, 13) is folded into the row's bbc instead of appended.  In VALUES mode
+ * the choice is made without a branch: a folded coefficient is stored to
+ * the lane's own dump slot, so every append is one store and one select. */
 template <int MODE>
 struct Sink {
     int n;
     int *J;
     double *A;
     double bbc;
-    __device__ __forceinline__ void add(int jeq, double coeff, int type, double val) {
-        if (type == 0 || type == 31 || type == 11 || type == 13) {
-            if (MODE == SINK_VALUES) bbc -= coeff * val;
-        } else {
+    double *dump; /* VALUES: per-lane slot for folded coefficients */
+    /* val(): the neighbour's boundary value, read only when it is folded */
+    template <class V>
+    __device__ __forceinline__ void add(int jeq, double coeff, int type, V val) {
+        const bool fold = type == 0 || type == 31 || type == 11 || type == 13;
+        if (MODE == SINK_VALUES) {
+            if (fold) bbc -= coeff * val();
+            *(fold ? dump : A + n) = coeff;
+            n += fold ? 0 : 1;
+        } else if (!fold) {
             if (MODE == SINK_PATTERN) J[n] = jeq;
-            if (MODE == SINK_VALUES) A[n] = coeff;
             n++;
         }
     }
 };
 
+/* Tag reads with the reference's out-of-array reads mapped to OUTSIDE; the
+ * index is clamped instead of branched around, so the load is always in
+ * bounds and the lanes stay converged. */
 struct Tags {
     const int8_t *u, *v, *p, *gv;
     int nu, nw, nc, nv;
-    __device__ __forceinline__ int U(int i) const { return (i >= 0 && i < nu) ? u[i] : OUTSIDE; }
-    __device__ __forceinline__ int V(int i) const { return (i >= 0 && i < nw) ? v[i] : OUTSIDE; }
-    __device__ __forceinline__ int P(int i) const { return (i >= 0 && i < nc) ? p[i] : OUTSIDE; }
-    __device__ __forceinline__ int G(int i) const { return (i >= 0 && i < nv) ? gv[i] : OUTSIDE; }
+    __device__ __forceinline__ static int rd(const int8_t *t, int i, int n) {
+        const bool in = (unsigned)i < (unsigned)n;
+        const int x = t[in ? i : 0];
+        return in ? x : OUTSIDE;
+    }
+    __device__ __forceinline__ int U(int i) const { return rd(u, i, nu); }
+    __device__ __forceinline__ int V(int i) const { return rd(v, i, nw); }
+    __device__ __forceinline__ int P(int i) const { return rd(p, i, nc); }
+    __device__ __forceinline__ int G(int i) const { return rd(gv, i, nv); }
 };
 
 __device__ __forceinline__ Tags make_tags(const DevGrid &g) {
@@ -74,10 +92,10 @@ __device__ void continuity_row(const DevAsm &a, int comp, int c1, int c2, int c3
     double vS = tS != 13 ? -a.one_dz * 1e0 : 0.0, vN = tN != 13 ? a.one_dz * 1e0 : 0.0;
     if (ASM) {
         b *= a.celvol;
-        if (tW != 13) rc.add(a.e.eqn_u[c1], uW * a.celvol, tW, g.BCu_val[c1]);
-        if (tE != 13) rc.add(a.e.eqn_u[c1 + 1], uE * a.celvol, tE, g.BCu_val[c1 + 1]);
-        if (tS != 13) rc.add(a.e.eqn_v[c3], vS * a.celvol, tS, g.BCv_val[c3]);
-        if (tN != 13) rc.add(a.e.eqn_v[c3 + nxvz], vN * a.celvol, tN, g.BCv_val[c3 + nxvz]);
+        if (tW != 13) rc.add(a.e.eqn_u[c1], uW * a.celvol, tW, [&] { return g.BCu_val[c1]; });
+        if (tE != 13) rc.add(a.e.eqn_u[c1 + 1], uE * a.celvol, tE, [&] { return g.BCu_val[c1 + 1]; });
+        if (tS != 13) rc.add(a.e.eqn_v[c3], vS * a.celvol, tS, [&] { return g.BCv_val[c3]; });
+        if (tN != 13) rc.add(a.e.eqn_v[c3 + nxvz], vN * a.celvol, tN, [&] { return g.BCv_val[c3 + nxvz]; });
     } else {
         double pc = comp == 0 ? 0.0 : g.bet_n[c2] / a.dt;
         double f = pc * a.p_cont[c2] + uW * a.u[c1] + uE * a.u[c1 + 1] + vS * a.v[c3] + vN * a.v[c3 + nxvz];
@@ -127,18 +145,18 @@ __device__ void xmom_inner(const DevAsm &a, int c1, int c2, int c3, int eqn, Sin
     double cv = a.celvol;
     if (ASM) {
         b *= cv;
-        if (tS != 30) ra.add(a.e.eqn_u[iVxS], uS * cv, tS, (tS != 11 ? 1.0 : 2.0) * g.BCu_val[iVxS]);
-        if (g.BCu[iVxW] != 30) ra.add(a.e.eqn_u[iVxW], uW * cv, g.BCu[iVxW], g.BCu_val[iVxW]);
-        ra.add(eqn, uC * cv, g.BCu[iVxC], g.BCu_val[iVxC]);
-        if (g.BCu[iVxE] != 30) ra.add(a.e.eqn_u[iVxE], uE * cv, g.BCu[iVxE], g.BCu_val[iVxE]);
-        if (g.BCu[c1 + nx] != 30) ra.add(a.e.eqn_u[iVxN], uN * cv, tN, (tN != 11 ? 1.0 : 2.0) * g.BCu_val[iVxN]);
-        if (g.BCv[iVzSW] != 30) ra.add(a.e.eqn_v[iVzSW], vSW * cv, g.BCv[iVzSW], g.BCv_val[iVzSW]);
-        if (g.BCv[iVzSE] != 30) ra.add(a.e.eqn_v[iVzSE], vSE * cv, g.BCv[iVzSE], g.BCv_val[iVzSE]);
-        if (g.BCv[iVzNW] != 30) ra.add(a.e.eqn_v[iVzNW], vNW * cv, g.BCv[iVzNW], g.BCv_val[iVzNW]);
-        if (g.BCv[iVzNE] != 30) ra.add(a.e.eqn_v[iVzNE], vNE * cv, g.BCv[iVzNE], g.BCv_val[iVzNE]);
+        if (tS != 30) ra.add(a.e.eqn_u[iVxS], uS * cv, tS, [&] { return (tS != 11 ? 1.0 : 2.0) * g.BCu_val[iVxS]; });
+        if (g.BCu[iVxW] != 30) ra.add(a.e.eqn_u[iVxW], uW * cv, g.BCu[iVxW], [&] { return g.BCu_val[iVxW]; });
+        ra.add(eqn, uC * cv, g.BCu[iVxC], [&] { return g.BCu_val[iVxC]; });
+        if (g.BCu[iVxE] != 30) ra.add(a.e.eqn_u[iVxE], uE * cv, g.BCu[iVxE], [&] { return g.BCu_val[iVxE]; });
+        if (g.BCu[c1 + nx] != 30) ra.add(a.e.eqn_u[iVxN], uN * cv, tN, [&] { return (tN != 11 ? 1.0 : 2.0) * g.BCu_val[iVxN]; });
+        if (g.BCv[iVzSW] != 30) ra.add(a.e.eqn_v[iVzSW], vSW * cv, g.BCv[iVzSW], [&] { return g.BCv_val[iVzSW]; });
+        if (g.BCv[iVzSE] != 30) ra.add(a.e.eqn_v[iVzSE], vSE * cv, g.BCv[iVzSE], [&] { return g.BCv_val[iVzSE]; });
+        if (g.BCv[iVzNW] != 30) ra.add(a.e.eqn_v[iVzNW], vNW * cv, g.BCv[iVzNW], [&] { return g.BCv_val[iVzNW]; });
+        if (g.BCv[iVzNE] != 30) ra.add(a.e.eqn_v[iVzNE], vNE * cv, g.BCv[iVzNE], [&] { return g.BCv_val[iVzNE]; });
         if (pgrad) {
-            rb.add(a.e.eqn_p[iPrW] - a.e.neq_mom, pW * cv, g.BCp[iPrW], g.BCp_val[iPrW]);
-            rb.add(a.e.eqn_p[iPrE] - a.e.neq_mom, pE * cv, g.BCp[iPrE], g.BCp_val[iPrE]);
+            rb.add(a.e.eqn_p[iPrW] - a.e.neq_mom, pW * cv, g.BCp[iPrW], [&] { return g.BCp_val[iPrW]; });
+            rb.add(a.e.eqn_p[iPrE] - a.e.neq_mom, pE * cv, g.BCp[iPrE], [&] { return g.BCp_val[iPrE]; });
         }
     } else {
         const double *u = a.u, *v = a.v, *p = a.p_mom;
@@ -215,20 +233,20 @@ __device__ void xmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, i
     double cv = a.celvol;
     if (ASM) {
         b *= cv;
-        if (tS != 30) ra.add(a.e.eqn_u[c1 - nx], uS * cv, tS, (tS != 11 ? 1.0 : 2.0) * g.BCu_val[c1 - nx]);
-        if (east && g.BCu[c1 - 1] != 30) ra.add(a.e.eqn_u[c1 - 1], uW * cv, g.BCu[c1 - 1], g.BCu_val[c1 - 1]);
-        ra.add(eqn, uC * cv, g.BCu[c1], g.BCu_val[c1]);
-        if (!east && g.BCu[c1 + 1] != 30) ra.add(a.e.eqn_u[c1 + 1], uE * cv, g.BCu[c1 + 1], g.BCu_val[c1 + 1]);
-        if (tN != 30) ra.add(a.e.eqn_u[c1 + nx], uN * cv, tN, (tN != 11 ? 1.0 : 2.0) * g.BCu_val[c1 + nx]);
+        if (tS != 30) ra.add(a.e.eqn_u[c1 - nx], uS * cv, tS, [&] { return (tS != 11 ? 1.0 : 2.0) * g.BCu_val[c1 - nx]; });
+        if (east && g.BCu[c1 - 1] != 30) ra.add(a.e.eqn_u[c1 - 1], uW * cv, g.BCu[c1 - 1], [&] { return g.BCu_val[c1 - 1]; });
+        ra.add(eqn, uC * cv, g.BCu[c1], [&] { return g.BCu_val[c1]; });
+        if (!east && g.BCu[c1 + 1] != 30) ra.add(a.e.eqn_u[c1 + 1], uE * cv, g.BCu[c1 + 1], [&] { return g.BCu_val[c1 + 1]; });
+        if (tN != 30) ra.add(a.e.eqn_u[c1 + nx], uN * cv, tN, [&] { return (tN != 11 ? 1.0 : 2.0) * g.BCu_val[c1 + nx]; });
         if (vs) {
-            ra.add(a.e.eqn_v[c3 - nxvz], vSW * cv, g.BCv[c3 - nxvz], g.BCv_val[c3 - nxvz]);
-            ra.add(a.e.eqn_v[c3 - nxvz + 1], vSE * cv, g.BCv[c3 - nxvz + 1], g.BCv_val[c3 - nxvz + 1]);
+            ra.add(a.e.eqn_v[c3 - nxvz], vSW * cv, g.BCv[c3 - nxvz], [&] { return g.BCv_val[c3 - nxvz]; });
+            ra.add(a.e.eqn_v[c3 - nxvz + 1], vSE * cv, g.BCv[c3 - nxvz + 1], [&] { return g.BCv_val[c3 - nxvz + 1]; });
         }
         if (vn) {
-            ra.add(a.e.eqn_v[c3], vNW * cv, g.BCv[c3], g.BCv_val[c3]);
-            ra.add(a.e.eqn_v[c3 + 1], vNE * cv, g.BCv[c3 + 1], g.BCv_val[c3 + 1]);
+            ra.add(a.e.eqn_v[c3], vNW * cv, g.BCv[c3], [&] { return g.BCv_val[c3]; });
+            ra.add(a.e.eqn_v[c3 + 1], vNE * cv, g.BCv[c3 + 1], [&] { return g.BCv_val[c3 + 1]; });
         }
-        if (g.BCp[ip] != 30) rb.add(a.e.eqn_p[ip] - a.e.neq_mom, pcoef * cv, g.BCp[ip], g.BCp_val[ip]);
+        if (g.BCp[ip] != 30) rb.add(a.e.eqn_p[ip] - a.e.neq_mom, pcoef * cv, g.BCp[ip], [&] { return g.BCp_val[ip]; });
     } else {
         const double *u = a.u, *v = a.v, *p = a.p_mom;
         double f = -(b + bbc_in / cv);
@@ -296,20 +314,20 @@ __device__ void zmom_inner(const DevAsm &a, int c1, int c2, int c3, int eqn, Sin
     double cv = a.celvol;
     if (ASM) {
         b *= cv;
-        if (g.BCu[c1 + nx] != 30) ra.add(a.e.eqn_u[c1 + nx], uNE * cv, g.BCu[c1 + nx], g.BCu_val[c1 + nx]);
-        if (g.BCu[c1] != 30) ra.add(a.e.eqn_u[c1], uSE * cv, g.BCu[c1], g.BCu_val[c1]);
-        if (g.BCu[c1 + nx - 1] != 30) ra.add(a.e.eqn_u[c1 + nx - 1], uNW * cv, g.BCu[c1 + nx - 1], g.BCu_val[c1 + nx - 1]);
-        if (g.BCu[c1 - 1] != 30) ra.add(a.e.eqn_u[c1 - 1], uSW * cv, g.BCu[c1 - 1], g.BCu_val[c1 - 1]);
-        if (tVS != 30) ra.add(a.e.eqn_v[c3 - nxvz], vS * cv, tVS, g.BCv_val[c3 - nxvz]);
-        if (tW != 30 && tW != -12) ra.add(a.e.eqn_v[c3 - 1], vW * cv, tW, (tW != 11 ? 1.0 : 2.0) * g.BCv_val[c3 - 1]);
-        if (tW == -12) ra.add(a.e.eqn_v[c3 + nxvz - 3], vW * cv, g.BCv[c3 + nxvz - 3], g.BCv_val[c3 + nxvz - 3]);
-        ra.add(eqn, vC * cv, g.BCv[c3], g.BCv_val[c3]);
-        if (tE != 30 && tE != -12) ra.add(a.e.eqn_v[c3 + 1], vE * cv, tE, (tE != 11 ? 1.0 : 2.0) * g.BCv_val[c3 + 1]);
-        if (tE == -12) ra.add(a.e.eqn_v[c3 - nxvz + 3], vE * cv, g.BCv[c3 - nxvz + 3], g.BCv_val[c3 - nxvz + 3]);
-        if (tVN != 30) ra.add(a.e.eqn_v[c3 + nxvz], vN * cv, tVN, g.BCv_val[c3 + nxvz]);
+        if (g.BCu[c1 + nx] != 30) ra.add(a.e.eqn_u[c1 + nx], uNE * cv, g.BCu[c1 + nx], [&] { return g.BCu_val[c1 + nx]; });
+        if (g.BCu[c1] != 30) ra.add(a.e.eqn_u[c1], uSE * cv, g.BCu[c1], [&] { return g.BCu_val[c1]; });
+        if (g.BCu[c1 + nx - 1] != 30) ra.add(a.e.eqn_u[c1 + nx - 1], uNW * cv, g.BCu[c1 + nx - 1], [&] { return g.BCu_val[c1 + nx - 1]; });
+        if (g.BCu[c1 - 1] != 30) ra.add(a.e.eqn_u[c1 - 1], uSW * cv, g.BCu[c1 - 1], [&] { return g.BCu_val[c1 - 1]; });
+        if (tVS != 30) ra.add(a.e.eqn_v[c3 - nxvz], vS * cv, tVS, [&] { return g.BCv_val[c3 - nxvz]; });
+        if (tW != 30 && tW != -12) ra.add(a.e.eqn_v[c3 - 1], vW * cv, tW, [&] { return (tW != 11 ? 1.0 : 2.0) * g.BCv_val[c3 - 1]; });
+        if (tW == -12) ra.add(a.e.eqn_v[c3 + nxvz - 3], vW * cv, g.BCv[c3 + nxvz - 3], [&] { return g.BCv_val[c3 + nxvz - 3]; });
+        ra.add(eqn, vC * cv, g.BCv[c3], [&] { return g.BCv_val[c3]; });
+        if (tE != 30 && tE != -12) ra.add(a.e.eqn_v[c3 + 1], vE * cv, tE, [&] { return (tE != 11 ? 1.0 : 2.0) * g.BCv_val[c3 + 1]; });
+        if (tE == -12) ra.add(a.e.eqn_v[c3 - nxvz + 3], vE * cv, g.BCv[c3 - nxvz + 3], [&] { return g.BCv_val[c3 - nxvz + 3]; });
+        if (tVN != 30) ra.add(a.e.eqn_v[c3 + nxvz], vN * cv, tVN, [&] { return g.BCv_val[c3 + nxvz]; });
         if (pgrad) {
-            rb.add(a.e.eqn_p[c2] - a.e.neq_mom, pS * cv, g.BCp[c2], g.BCp_val[c2]);
-            rb.add(a.e.eqn_p[c2 + ncx] - a.e.neq_mom, pN * cv, g.BCp[c2 + ncx], g.BCp_val[c2 + ncx]);
+            rb.add(a.e.eqn_p[c2] - a.e.neq_mom, pS * cv, g.BCp[c2], [&] { return g.BCp_val[c2]; });
+            rb.add(a.e.eqn_p[c2 + ncx] - a.e.neq_mom, pN * cv, g.BCp[c2 + ncx], [&] { return g.BCp_val[c2 + ncx]; });
         }
     } else {
         const double *u = a.u, *v = a.v, *p = a.p_mom;
@@ -393,22 +411,22 @@ __device__ void zmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, i
     if (ASM) {
         b *= cv;
         if (us) {
-            ra.add(a.e.eqn_u[c1 - 1], uSW * cv, g.BCu[c1 - 1], g.BCu_val[c1 - 1]);
-            ra.add(a.e.eqn_u[c1], uSE * cv, g.BCu[c1], g.BCu_val[c1]);
+            ra.add(a.e.eqn_u[c1 - 1], uSW * cv, g.BCu[c1 - 1], [&] { return g.BCu_val[c1 - 1]; });
+            ra.add(a.e.eqn_u[c1], uSE * cv, g.BCu[c1], [&] { return g.BCu_val[c1]; });
         }
         if (un) {
-            ra.add(a.e.eqn_u[c1 + nx - 1], uNW * cv, g.BCu[c1 + nx - 1], g.BCu_val[c1 + nx - 1]);
-            ra.add(a.e.eqn_u[c1 + nx], uNE * cv, g.BCu[c1 + nx], g.BCu_val[c1 + nx]);
+            ra.add(a.e.eqn_u[c1 + nx - 1], uNW * cv, g.BCu[c1 + nx - 1], [&] { return g.BCu_val[c1 + nx - 1]; });
+            ra.add(a.e.eqn_u[c1 + nx], uNE * cv, g.BCu[c1 + nx], [&] { return g.BCu_val[c1 + nx]; });
         }
-        if (north && g.BCv[c3 - nxvz] != 30) ra.add(a.e.eqn_v[c3 - nxvz], vS * cv, g.BCv[c3 - nxvz], g.BCv_val[c3 - nxvz]);
-        if (tW != 30) ra.add(a.e.eqn_v[c3 - 1], vW * cv, tW, (tW != 11 ? 1.0 : 2.0) * g.BCv_val[c3 - 1]);
-        ra.add(eqn, vC * cv, g.BCv[c3], g.BCv_val[c3]);
-        if (tE != 30) ra.add(a.e.eqn_v[c3 + 1], vE * cv, tE, (tE != 11 ? 1.0 : 2.0) * g.BCv_val[c3 + 1]);
-        if (!north && g.BCv[c3 + nxvz] != 30) ra.add(a.e.eqn_v[c3 + nxvz], vN * cv, g.BCv[c3 + nxvz], g.BCv_val[c3 + nxvz]);
+        if (north && g.BCv[c3 - nxvz] != 30) ra.add(a.e.eqn_v[c3 - nxvz], vS * cv, g.BCv[c3 - nxvz], [&] { return g.BCv_val[c3 - nxvz]; });
+        if (tW != 30) ra.add(a.e.eqn_v[c3 - 1], vW * cv, tW, [&] { return (tW != 11 ? 1.0 : 2.0) * g.BCv_val[c3 - 1]; });
+        ra.add(eqn, vC * cv, g.BCv[c3], [&] { return g.BCv_val[c3]; });
+        if (tE != 30) ra.add(a.e.eqn_v[c3 + 1], vE * cv, tE, [&] { return (tE != 11 ? 1.0 : 2.0) * g.BCv_val[c3 + 1]; });
+        if (!north && g.BCv[c3 + nxvz] != 30) ra.add(a.e.eqn_v[c3 + nxvz], vN * cv, g.BCv[c3 + nxvz], [&] { return g.BCv_val[c3 + nxvz]; });
         if (!north) {
-            if (g.BCp[c2 + ncx] != 30) rb.add(a.e.eqn_p[c2 + ncx] - a.e.neq_mom, pN * cv, g.BCp[c2 + ncx], g.BCp_val[c2 + ncx]);
+            if (g.BCp[c2 + ncx] != 30) rb.add(a.e.eqn_p[c2 + ncx] - a.e.neq_mom, pN * cv, g.BCp[c2 + ncx], [&] { return g.BCp_val[c2 + ncx]; });
         } else {
-            if (g.BCp[c2] != 30) rb.add(a.e.eqn_p[c2] - a.e.neq_mom, pS * cv, g.BCp[c2], g.BCp_val[c2]);
+            if (g.BCp[c2] != 30) rb.add(a.e.eqn_p[c2] - a.e.neq_mom, pS * cv, g.BCp[c2], [&] { return g.BCp_val[c2]; });
         }
     } else {
         const double *u = a.u, *v = a.v, *p = a.p_mom;
@@ -444,19 +462,27 @@ __device__ void zmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, i
  * around vz(k, l); p slots [b+1][a] for centroid (k-1+a, l-1+b). */
 struct XAcc { double u[3][3]; double v[4][4]; double p[3][2]; };
 
-__device__ __forceinline__ void xa_exx(XAcc &c, int a, int b, double w, double cmp, double dx, double dz) {
-    double f = w * (1.0 - cmp / 3.0) / dx, h = w * cmp / 3.0 / dz;
+/* stencil factors of the Jacobian rows, one division each per row:
+ * (1 - cmp/3)/dx, (1 - cmp/3)/dz, (cmp/3)/dx, (cmp/3)/dz, 1/dx, 1/dz */
+struct JK { double e1x, e1z, e3x, e3z, idx, idz; };
+__device__ __forceinline__ JK make_jk(double dx, double dz, double cmp) {
+    return JK{(1.0 - cmp / 3.0) / dx, (1.0 - cmp / 3.0) / dz, cmp / 3.0 / dx, cmp / 3.0 / dz, 1.0 / dx, 1.0 / dz};
+}
+
+__device__ __forceinline__ void xa_exx(XAcc &c, int a, int b, double w, const JK &K) {
+    double f = w * K.e1x, h = w * K.e3z;
     c.u[b + 1][a + 1] += f; c.u[b + 1][a] -= f;
     c.v[b + 2][a + 1] -= h; c.v[b + 1][a + 1] += h;
 }
-__device__ __forceinline__ void xa_ezz(XAcc &c, int a, int b, double w, double cmp, double dx, double dz) {
-    double f = w * (1.0 - cmp / 3.0) / dz, h = w * cmp / 3.0 / dx;
+__device__ __forceinline__ void xa_ezz(XAcc &c, int a, int b, double w, const JK &K) {
+    double f = w * K.e1z, h = w * K.e3x;
     c.v[b + 2][a + 1] += f; c.v[b + 1][a + 1] -= f;
     c.u[b + 1][a + 1] -= h; c.u[b + 1][a] += h;
 }
-__device__ __forceinline__ void xa_gxz(XAcc &c, int a, int b, double w, double dx, double dz) {
-    c.u[b + 2][a + 1] += w / dz; c.u[b + 1][a + 1] -= w / dz;
-    c.v[b + 2][a + 2] += w / dx; c.v[b + 2][a + 1] -= w / dx;
+__device__ __forceinline__ void xa_gxz(XAcc &c, int a, int b, double w, const JK &K) {
+    const double wz = w * K.idz, wx = w * K.idx;
+    c.u[b + 2][a + 1] += wz; c.u[b + 1][a + 1] -= wz;
+    c.v[b + 2][a + 2] += wx; c.v[b + 2][a + 1] -= wx;
 }
 
 template <bool ASM, int MODE>
@@ -515,7 +541,7 @@ __device__ void xjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
     double vSSW = 0, vSSE = 0, vSWW = 0, vSW = 0, vSE = 0, vSEE = 0, vNWW = 0, vNW = 0, vNE = 0, vNEE = 0, vNNW = 0, vNNE = 0;
     double pSW = 0, pSE = 0, pW = 0, pE = 0, pNW = 0, pNE = 0;
     if (MODE == SINK_VALUES) {
-        double dx = g.dx, dz = g.dz, cmp = 1.0;
+        const JK K = make_jk(g.dx, g.dz, 1.0);
         double D11E = g.D11_n[iPrE], D12E = g.D12_n[iPrE], D13E = g.D13_n[iPrE], D14E = g.D14_n[iPrE];
         double D11W = g.D11_n[iPrW], D12W = g.D12_n[iPrW], D13W = g.D13_n[iPrW], D14W = g.D14_n[iPrW];
         double D31N = g.D31_s[ixyN], D32N = g.D32_s[ixyN], D33N = g.D33_s[ixyN], D34N = g.D34_s[ixyN];
@@ -535,23 +561,23 @@ __device__ void xjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
         /* normal stress at centroids W (aa=0) and E (aa=1) */
 #pragma unroll
         for (int aa = 0; aa < 2; aa++) {
-            double f = aa == 0 ? inW / dx : -inE / dx;
+            double f = aa == 0 ? inW * K.idx : -inE * K.idx;
             double D1 = aa == 0 ? D11W : D11E, D2 = aa == 0 ? D12W : D12E, D3 = aa == 0 ? D13W : D13E, D4 = aa == 0 ? D14W : D14E;
             double w = aa == 0 ? wW : wE;
-            xa_exx(c, aa, 0, f * D1, cmp, dx, dz);
-            xa_ezz(c, aa, 0, f * D2, cmp, dx, dz);
+            xa_exx(c, aa, 0, f * D1, K);
+            xa_ezz(c, aa, 0, f * D2, K);
             double fSo = aa == 0 ? inSWv : inSEv, fNo = aa == 0 ? inNWv : inNEv;
-            xa_gxz(c, 0, -1, f * D3 * w * inS, dx, dz);
-            xa_gxz(c, 0, 0, f * D3 * w * inN, dx, dz);
-            xa_gxz(c, aa == 0 ? -1 : 1, -1, f * D3 * w * fSo, dx, dz);
-            xa_gxz(c, aa == 0 ? -1 : 1, 0, f * D3 * w * fNo, dx, dz);
+            xa_gxz(c, 0, -1, f * D3 * w * inS, K);
+            xa_gxz(c, 0, 0, f * D3 * w * inN, K);
+            xa_gxz(c, aa == 0 ? -1 : 1, -1, f * D3 * w * fSo, K);
+            xa_gxz(c, aa == 0 ? -1 : 1, 0, f * D3 * w * fNo, K);
             c.p[1][aa] += f * D4;
         }
         /* shear stress at vertices S (bb=0) and N (bb=1) */
 #pragma unroll
         for (int bb = 0; bb < 2; bb++) {
             int bv = bb == 0 ? -1 : 0;
-            double f = bb == 0 ? inS / dz : -inN / dz;
+            double f = bb == 0 ? inS * K.idz : -inN * K.idz;
             double D1 = bb == 0 ? D31S : D31N, D2 = bb == 0 ? D32S : D32N, D3 = bb == 0 ? D33S : D33N, D4 = bb == 0 ? D34S : D34N;
             double w = bb == 0 ? wS : wN;
             int cb = bb == 0 ? -1 : 1;
@@ -561,14 +587,14 @@ __device__ void xjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 double wq = w * fl[q];
-                xa_exx(c, ca[q], cr[q], f * D1 * wq, cmp, dx, dz);
-                xa_ezz(c, ca[q], cr[q], f * D2 * wq, cmp, dx, dz);
+                xa_exx(c, ca[q], cr[q], f * D1 * wq, K);
+                xa_ezz(c, ca[q], cr[q], f * D2 * wq, K);
                 c.p[cr[q] + 1][ca[q]] += f * D4 * wq;
             }
-            xa_gxz(c, 0, bv, f * D3, dx, dz);
+            xa_gxz(c, 0, bv, f * D3, K);
         }
-        c.p[1][0] += -inW / dx;
-        c.p[1][1] += inE / dx;
+        c.p[1][0] += -inW * K.idx;
+        c.p[1][1] += inE * K.idx;
         uSW = c.u[0][0]; uS = c.u[0][1]; uSE = c.u[0][2]; uW = c.u[1][0]; uC = c.u[1][1]; uE = c.u[1][2];
         uNW = c.u[2][0]; uN = c.u[2][1]; uNE = c.u[2][2];
         vSSW = c.v[0][1]; vSSE = c.v[0][2]; vSWW = c.v[1][0]; vSW = c.v[1][1]; vSE = c.v[1][2]; vSEE = c.v[1][3];
@@ -596,18 +622,18 @@ __device__ void xjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
         const int *eu = a.e.eqn_u, *ev = a.e.eqn_v, *ep = a.e.eqn_p;
         const double *uv = g.BCu_val, *vv = g.BCv_val, *pv = g.BCp_val;
 #define UADD11(idx, coef) \
-    if (g.BCu[idx] != 30) ra.add(eu[idx], (coef)*cv, g.BCu[idx], (g.BCu[idx] != 11 ? 1.0 : 2.0) * uv[idx]);
+    if (g.BCu[idx] != 30) ra.add(eu[idx], (coef)*cv, g.BCu[idx], [&] { return (g.BCu[idx] != 11 ? 1.0 : 2.0) * uv[idx]; });
         UADD11(iVxSW, uSW)
         UADD11(iVxS, uS)
         UADD11(iVxSE, uSE)
-        if (g.BCu[iVxW] != 30) ra.add(eu[iVxW], uW * cv, g.BCu[iVxW], uv[iVxW]);
-        ra.add(eqn, uC * cv, g.BCu[iVxC], uv[iVxC]);
-        if (g.BCu[iVxE] != 30) ra.add(eu[iVxE], uE * cv, g.BCu[iVxE], uv[iVxE]);
+        if (g.BCu[iVxW] != 30) ra.add(eu[iVxW], uW * cv, g.BCu[iVxW], [&] { return uv[iVxW]; });
+        ra.add(eqn, uC * cv, g.BCu[iVxC], [&] { return uv[iVxC]; });
+        if (g.BCu[iVxE] != 30) ra.add(eu[iVxE], uE * cv, g.BCu[iVxE], [&] { return uv[iVxE]; });
         UADD11(iVxNW, uNW)
         UADD11(iVxN, uN)
         UADD11(iVxNE, uNE)
 #undef UADD11
-#define VADD(idx, coef) ra.add(ev[idx], (coef)*cv, g.BCv[idx], vv[idx])
+#define VADD(idx, coef) ra.add(ev[idx], (coef)*cv, g.BCv[idx], [&] { return vv[idx]; })
         if (l > 1 && g.BCv[iVzSSW] != 30) VADD(iVzSSW, vSSW);
         if (l > 1 && g.BCv[iVzSSE] != 30) VADD(iVzSSE, vSSE);
         if ((k > 1) || ((k == 0 || k == 1) && g.BCv[iVzSWW] == -1))
@@ -625,7 +651,7 @@ __device__ void xjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
         if (l < nz - 1 && g.BCv[iVzNNW] != 30) VADD(iVzNNW, vNNW);
         if (l < nz - 1 && g.BCv[iVzNNE] != 30) VADD(iVzNNE, vNNE);
 #undef VADD
-#define PADD(idx, coef) rb.add(ep[idx] - a.e.neq_mom, (coef)*cv, g.BCp[idx], pv[idx])
+#define PADD(idx, coef) rb.add(ep[idx] - a.e.neq_mom, (coef)*cv, g.BCp[idx], [&] { return pv[idx]; })
         if (l > 1) {
             if (g.BCp[iPrSW] != 30) PADD(iPrSW, pSW);
             if (g.BCp[iPrSE] != 30) PADD(iPrSE, pSE);
@@ -654,19 +680,20 @@ __device__ void xjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
  * around u(k, l); p slots [b][a+1] for centroid (k-1+a, l-1+b). */
 struct ZAcc { double v[3][3]; double u[4][4]; double p[2][3]; };
 
-__device__ __forceinline__ void za_exx(ZAcc &c, int a, int b, double w, double cmp, double dx, double dz) {
-    double f = w * (1.0 - cmp / 3.0) / dx, h = w * cmp / 3.0 / dz;
+__device__ __forceinline__ void za_exx(ZAcc &c, int a, int b, double w, const JK &K) {
+    double f = w * K.e1x, h = w * K.e3z;
     c.u[b + 1][a + 2] += f; c.u[b + 1][a + 1] -= f;
     c.v[b + 1][a + 1] -= h; c.v[b][a + 1] += h;
 }
-__device__ __forceinline__ void za_ezz(ZAcc &c, int a, int b, double w, double cmp, double dx, double dz) {
-    double f = w * (1.0 - cmp / 3.0) / dz, h = w * cmp / 3.0 / dx;
+__device__ __forceinline__ void za_ezz(ZAcc &c, int a, int b, double w, const JK &K) {
+    double f = w * K.e1z, h = w * K.e3x;
     c.v[b + 1][a + 1] += f; c.v[b][a + 1] -= f;
     c.u[b + 1][a + 2] -= h; c.u[b + 1][a + 1] += h;
 }
-__device__ __forceinline__ void za_gxz(ZAcc &c, int a, int b, double w, double dx, double dz) {
-    c.u[b + 2][a + 1] += w / dz; c.u[b + 1][a + 1] -= w / dz;
-    c.v[b + 1][a + 1] += w / dx; c.v[b + 1][a] -= w / dx;
+__device__ __forceinline__ void za_gxz(ZAcc &c, int a, int b, double w, const JK &K) {
+    const double wz = w * K.idz, wx = w * K.idx;
+    c.u[b + 2][a + 1] += wz; c.u[b + 1][a + 1] -= wz;
+    c.v[b + 1][a + 1] += wx; c.v[b + 1][a] -= wx;
 }
 
 template <bool ASM, int MODE>
@@ -724,7 +751,7 @@ __device__ void zjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
     double uSSW = 0, uSSE = 0, uSWW = 0, uSW = 0, uSE = 0, uSEE = 0, uNWW = 0, uNW = 0, uNE = 0, uNEE = 0, uNNW = 0, uNNE = 0;
     double pSW = 0, pS = 0, pSE = 0, pNW = 0, pN = 0, pNE = 0;
     if (MODE == SINK_VALUES) {
-        double dx = g.dx, dz = g.dz, cmp = 1.0;
+        const JK K = make_jk(g.dx, g.dz, 1.0);
         double D31W = g.D31_s[ixyW], D32W = g.D32_s[ixyW], D33W = g.D33_s[ixyW], D34W = g.D34_s[ixyW];
         double D31E = g.D31_s[ixyE], D32E = g.D32_s[ixyE], D33E = g.D33_s[ixyE], D34E = g.D34_s[ixyE];
         double D21S = g.D21_n[iPrS], D22S = g.D22_n[iPrS], D23S = g.D23_n[iPrS], D24S = g.D24_n[iPrS];
@@ -745,23 +772,23 @@ __device__ void zjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
         /* normal stress at centroids S (bb=0) and N (bb=1) */
 #pragma unroll
         for (int bb = 0; bb < 2; bb++) {
-            double f = bb == 0 ? inS / dz : -inN / dz;
+            double f = bb == 0 ? inS * K.idz : -inN * K.idz;
             double D1 = bb == 0 ? D21S : D21N, D2 = bb == 0 ? D22S : D22N, D3 = bb == 0 ? D23S : D23N, D4 = bb == 0 ? D24S : D24N;
             double w = bb == 0 ? wS : wN;
-            za_exx(c, 0, bb, f * D1, cmp, dx, dz);
-            za_ezz(c, 0, bb, f * D2, cmp, dx, dz);
+            za_exx(c, 0, bb, f * D1, K);
+            za_ezz(c, 0, bb, f * D2, K);
             int vb = bb == 0 ? -1 : 1;
             double fo_w = bb == 0 ? inSWv : inNWv, fo_e = bb == 0 ? inSEv : inNEv;
-            za_gxz(c, 0, 0, f * D3 * w * inW, dx, dz);
-            za_gxz(c, 1, 0, f * D3 * w * inE, dx, dz);
-            za_gxz(c, 0, vb, f * D3 * w * fo_w, dx, dz);
-            za_gxz(c, 1, vb, f * D3 * w * fo_e, dx, dz);
+            za_gxz(c, 0, 0, f * D3 * w * inW, K);
+            za_gxz(c, 1, 0, f * D3 * w * inE, K);
+            za_gxz(c, 0, vb, f * D3 * w * fo_w, K);
+            za_gxz(c, 1, vb, f * D3 * w * fo_e, K);
             c.p[bb][1] += f * D4;
         }
         /* shear stress at vertices W (aa=0) and E (aa=1) */
 #pragma unroll
         for (int aa = 0; aa < 2; aa++) {
-            double f = aa == 0 ? inW / dx : -inE / dx;
+            double f = aa == 0 ? inW * K.idx : -inE * K.idx;
             double D1 = aa == 0 ? D31W : D31E, D2 = aa == 0 ? D32W : D32E, D3 = aa == 0 ? D33W : D33E, D4 = aa == 0 ? D34W : D34E;
             double w = aa == 0 ? wW : wE;
             int co = aa == 0 ? -1 : 1;
@@ -771,14 +798,14 @@ __device__ void zjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 double wq = w * fl[q];
-                za_exx(c, ca[q], cb[q], f * D1 * wq, cmp, dx, dz);
-                za_ezz(c, ca[q], cb[q], f * D2 * wq, cmp, dx, dz);
+                za_exx(c, ca[q], cb[q], f * D1 * wq, K);
+                za_ezz(c, ca[q], cb[q], f * D2 * wq, K);
                 c.p[cb[q]][ca[q] + 1] += f * D4 * wq;
             }
-            za_gxz(c, aa, 0, f * D3, dx, dz);
+            za_gxz(c, aa, 0, f * D3, K);
         }
-        c.p[0][1] += -inS / dz;
-        c.p[1][1] += inN / dz;
+        c.p[0][1] += -inS * K.idz;
+        c.p[1][1] += inN * K.idz;
         vSW = c.v[0][0]; vS = c.v[0][1]; vSE = c.v[0][2]; vW = c.v[1][0]; vC = c.v[1][1]; vE = c.v[1][2];
         vNW = c.v[2][0]; vN = c.v[2][1]; vNE = c.v[2][2];
         uSSW = c.u[0][1]; uSSE = c.u[0][2]; uSWW = c.u[1][0]; uSW = c.u[1][1]; uSE = c.u[1][2]; uSEE = c.u[1][3];
@@ -805,8 +832,8 @@ __device__ void zjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
         b *= cv;
         const int *eu = a.e.eqn_u, *ev = a.e.eqn_v, *ep = a.e.eqn_p;
         const double *uv = g.BCu_val, *vv = g.BCv_val, *pv = g.BCp_val;
-#define UADD(idx, coef) ra.add(eu[idx], (coef)*cv, g.BCu[idx], uv[idx])
-#define VADD(idx, coef) ra.add(ev[idx], (coef)*cv, g.BCv[idx], vv[idx])
+#define UADD(idx, coef) ra.add(eu[idx], (coef)*cv, g.BCu[idx], [&] { return uv[idx]; })
+#define VADD(idx, coef) ra.add(ev[idx], (coef)*cv, g.BCv[idx], [&] { return vv[idx]; })
         if (g.BCu[iVxSSW] != 30) UADD(iVxSSW, uSSW);
         if (g.BCu[iVxSSE] != 30) UADD(iVxSSE, uSSE);
         if ((k > 1) || (k == 1 && g.BCu[iVxSWW] == -1))
@@ -827,14 +854,14 @@ __device__ void zjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
         if (g.BCv[iVzS] != 30) VADD(iVzS, vS);
         if (g.BCv[iVzSE] == -1) VADD(iVzSE, vSE);
         if (g.BCv[iVzW] == -1) VADD(iVzW, vW);
-        ra.add(eqn, vC * cv, g.BCv[iVzC], vv[iVzC]);
+        ra.add(eqn, vC * cv, g.BCv[iVzC], [&] { return vv[iVzC]; });
         if (g.BCv[iVzE] == -1) VADD(iVzE, vE);
         if (g.BCv[iVzNW] == -1) VADD(iVzNW, vNW);
         if (g.BCv[iVzN] != 30) VADD(iVzN, vN);
         if (g.BCv[iVzNE] == -1) VADD(iVzNE, vNE);
 #undef UADD
 #undef VADD
-#define PADD(idx, coef, vidx) rb.add(ep[idx] - a.e.neq_mom, (coef)*cv, g.BCp[idx], pv[vidx])
+#define PADD(idx, coef, vidx) rb.add(ep[idx] - a.e.neq_mom, (coef)*cv, g.BCp[idx], [&] { return pv[vidx]; })
         if (k > 1 || (k == 1 && periodix == 1))
             if (g.BCp[iPrSW] != 30) PADD(iPrSW, pSW, iPrSW);
         if (g.BCp[iPrS] != 30 && g.BCp[iPrN] != 30) PADD(iPrS, pS, iPrS);
@@ -983,13 +1010,15 @@ template <int KIND, int VBLK, int CAPA, int CAPB>
 __global__ void __launch_bounds__(VBLK) k_values_mom(DevAsm a, DevBlock A, DevBlock B) {
     __shared__ double sA[VBLK * CAPA];
     __shared__ double sB[VBLK * CAPB];
+    __shared__ double sDump[VBLK];
     const int r0 = blockIdx.x * VBLK;
     const int r1 = min(r0 + VBLK, a.e.neq_mom);
     const int eqn = r0 + threadIdx.x;
     const int a0 = A.Ic[r0], b0 = B.Ic[r0];
     if (eqn < r1) {
         Tags T = make_tags(a.g);
-        Sink<SINK_VALUES> ra{0, nullptr, sA + (A.Ic[eqn] - a0), 0.0}, rb{0, nullptr, sB + (B.Ic[eqn] - b0), 0.0};
+        Sink<SINK_VALUES> ra{0, nullptr, sA + (A.Ic[eqn] - a0), 0.0, sDump + threadIdx.x},
+            rb{0, nullptr, sB + (B.Ic[eqn] - b0), 0.0, sDump + threadIdx.x};
         double b = mom_rhs(a, eqn), F = 0.0;
         momentum_row<true, SINK_VALUES>(KIND, a, T, eqn, ra, rb, b, 0.0, F);
         A.bbc[eqn] = ra.bbc;
@@ -1007,10 +1036,12 @@ __global__ void __launch_bounds__(VBLK) k_values_mom(DevAsm a, DevBlock A, DevBl
 /* the same without staging, for patterns with longer rows */
 template <int KIND>
 __global__ void __launch_bounds__(BLK) k_values_mom_direct(DevAsm a, DevBlock A, DevBlock B) {
+    __shared__ double sDump[BLK];
     int eqn = blockIdx.x * BLK + threadIdx.x;
     if (eqn >= a.e.neq_mom) return;
     Tags T = make_tags(a.g);
-    Sink<SINK_VALUES> ra{0, nullptr, A.A + A.Ic[eqn], 0.0}, rb{0, nullptr, B.A + B.Ic[eqn], 0.0};
+    Sink<SINK_VALUES> ra{0, nullptr, A.A + A.Ic[eqn], 0.0, sDump + threadIdx.x},
+        rb{0, nullptr, B.A + B.Ic[eqn], 0.0, sDump + threadIdx.x};
     double b = mom_rhs(a, eqn), F = 0.0;
     momentum_row<true, SINK_VALUES>(KIND, a, T, eqn, ra, rb, b, 0.0, F);
     A.bbc[eqn] = ra.bbc;
@@ -1021,11 +1052,12 @@ __global__ void __launch_bounds__(BLK) k_values_mom_direct(DevAsm a, DevBlock A,
 }
 
 __global__ void __launch_bounds__(BLK) k_values_cont(int kind, DevAsm a, DevBlock C, DevBlock D) {
+    __shared__ double sDump[BLK];
     int eqn = blockIdx.x * BLK + threadIdx.x;
     if (eqn >= a.e.neq_cont) return;
     int c1, c2, c3;
     cont_geom(a, eqn, c1, c2, c3);
-    Sink<SINK_VALUES> rc{0, nullptr, C.A + C.Ic[eqn], 0.0};
+    Sink<SINK_VALUES> rc{0, nullptr, C.A + C.Ic[eqn], 0.0, sDump + threadIdx.x};
     double b = a.g.rhs_p[c2], F = 0.0;
     continuity_row<true, SINK_VALUES>(a, 0, c1, c2, c3, rc, b, F);
     C.bbc[eqn] = rc.bbc;
