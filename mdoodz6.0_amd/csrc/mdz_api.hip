@@ -182,6 +182,7 @@ static void ctx_free_blocks(mdz_ctx *c) {
     for (int i = 0; i < 8; i++) {
         hipFree(c->blk[i].Ic); hipFree(c->blk[i].J); hipFree(c->blk[i].A);
         hipFree(c->blk[i].b); hipFree(c->blk[i].bbc); hipFree(c->blk[i].F); hipFree(c->blk[i].d);
+        hipFree(c->blk[i].free_row);
         memset(&c->blk[i], 0, sizeof(DevBlock));
         c->blk_cap[i] = 0;
     }
@@ -367,6 +368,7 @@ static int ensure_rows(mdz_ctx *c) {
         if (c->blk[i].Ic && c->blk[i].neq == neq) continue;
         hipFree(c->blk[i].Ic); hipFree(c->blk[i].J); hipFree(c->blk[i].A);
         hipFree(c->blk[i].b); hipFree(c->blk[i].bbc); hipFree(c->blk[i].F); hipFree(c->blk[i].d);
+        hipFree(c->blk[i].free_row);
         memset(&c->blk[i], 0, sizeof(DevBlock));
         c->blk_cap[i] = 0;
         int rc = block_alloc_rows(&c->blk[i], neq);
@@ -406,6 +408,10 @@ static int build_pattern(mdz_ctx *c, int kind) {
     D->nnz = 0;
     HIPCHK(hipStreamSynchronize(c->stream));
     mdz_fill_pattern(kind, a, A, B, C, c->stream);
+    /* rows whose stencil touches only free degrees of freedom take the
+     * tag-free value and residual kernels */
+    if (!A->free_row && A->neq > 0) HIPCHK(hipMalloc((void **)&A->free_row, (size_t)A->neq));
+    if ((rc = mdz_launch_free_rows(kind, a, A, B, c->stream))) return set_err(rc, "free-row flags");
     c->pattern_ready[kind] = 1;
     return sync(c);
 }
@@ -755,6 +761,10 @@ static void build_operator_dropin(int kind, grid *mesh, params model, double *p_
         } else {
             hipMemsetAsync(A->bbc, 0, (size_t)SA->neq * 8, c->stream);
         }
+        /* the tags were uploaded again without a pattern build: the free-row
+         * flags of the last build may not describe them, so every row takes
+         * the general kernel here */
+        if (A->free_row) hipMemsetAsync(A->free_row, 0, (size_t)A->neq, c->stream);
         mdz_launch_residual(kind, a, A, C, D, c->stream);
         hipMemcpyAsync(SA->F, A->F, (size_t)SA->neq * 8, hipMemcpyDeviceToHost, c->stream);
         hipMemcpyAsync(SA->b, A->b, (size_t)SA->neq * 8, hipMemcpyDeviceToHost, c->stream);

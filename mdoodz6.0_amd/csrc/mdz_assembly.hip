@@ -73,6 +73,25 @@ struct Tags {
     __device__ __forceinline__ int G(int i) const { return rd(gv, i, nv); }
 };
 
+/* Tag source of a row whose every tag read is a free degree of freedom
+ * (-1): no loads, and the row code specialises to the interior stencil. */
+struct FreeTags {
+    __device__ __forceinline__ int U(int) const { return -1; }
+    __device__ __forceinline__ int V(int) const { return -1; }
+    __device__ __forceinline__ int P(int) const { return -1; }
+    __device__ __forceinline__ int G(int) const { return -1; }
+};
+
+/* Real tag reads that also record whether every one of them was -1. */
+struct CheckTags : Tags {
+    mutable bool all_free;
+    __device__ __forceinline__ int note(int t) const { all_free = all_free && t == -1; return t; }
+    __device__ __forceinline__ int U(int i) const { return note(Tags::U(i)); }
+    __device__ __forceinline__ int V(int i) const { return note(Tags::V(i)); }
+    __device__ __forceinline__ int P(int i) const { return note(Tags::P(i)); }
+    __device__ __forceinline__ int G(int i) const { return note(Tags::G(i)); }
+};
+
 __device__ __forceinline__ Tags make_tags(const DevGrid &g) {
     Tags t;
     t.u = g.BCu; t.v = g.BCv; t.p = g.BCp; t.gv = g.BCg;
@@ -107,19 +126,19 @@ __device__ void continuity_row(const DevAsm &a, int comp, int c1, int c2, int c3
 /* ------------------------------------------------------------------ */
 /* Picard X rows                                                        */
 /* ------------------------------------------------------------------ */
-template <bool ASM, int MODE>
-__device__ void xmom_inner(const DevAsm &a, int c1, int c2, int c3, int eqn, Sink<MODE> &ra, Sink<MODE> &rb,
+template <bool ASM, int MODE, class TG>
+__device__ void xmom_inner(const DevAsm &a, const TG &T, int c1, int c2, int c3, int eqn, Sink<MODE> &ra, Sink<MODE> &rb,
                            double &b, double &F) {
     const DevGrid &g = a.g;
     int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz;
     int iVxC = c1, iVxW = iVxC - 1, iVxE = iVxC + 1, iVxS = iVxC - nx, iVxN = iVxC + nx;
     int iVzSW = c3 - nxvz, iVzSE = c3 - nxvz + 1, iVzNW = c3, iVzNE = c3 + 1;
     int iPrW = c2, iPrE = c2 + 1, ixyN = c1, ixyS = c1 - nx;
-    if (g.BCu[iVxC] == -2) { iVxW = c1 + nx - 2; iPrW = c2 + ncx; iVzSW = c3 - 2; iVzNW = c3 + nxvz - 2; }
+    if (T.U(iVxC) == -2) { iVxW = c1 + nx - 2; iPrW = c2 + ncx; iVzSW = c3 - 2; iVzNW = c3 + nxvz - 2; }
     double D11E = g.D11_n[iPrE], D11W = g.D11_n[iPrW], D33N = g.D33_s[ixyN], D33S = g.D33_s[ixyS];
-    double inW = g.BCp[iPrW] == -1 ? 1.0 : 0.0, inE = g.BCp[iPrE] == -1 ? 1.0 : 0.0;
-    double inS = (g.BCg[ixyS] != 30 && g.BCu[ixyS] != 13) ? 1.0 : 0.0;
-    double inN = (g.BCg[c1] != 30 && g.BCu[c1 + nx] != 13) ? 1.0 : 0.0;
+    double inW = T.P(iPrW) == -1 ? 1.0 : 0.0, inE = T.P(iPrE) == -1 ? 1.0 : 0.0;
+    double inS = (T.G(ixyS) != 30 && T.U(ixyS) != 13) ? 1.0 : 0.0;
+    double inN = (T.G(c1) != 30 && T.U(c1 + nx) != 13) ? 1.0 : 0.0;
     double dx = g.dx, dz = g.dz, comp = 1.0;
     double uW = D11W * inW * ((1.0 / 3.0) * comp / dx - 1.0 / dx) / dx;
     double uC = -(-D33N * inN / dz - D33S * inS / dz) / dz
@@ -137,48 +156,48 @@ __device__ void xmom_inner(const DevAsm &a, int c1, int c2, int c3, int eqn, Sin
         double uC_corr = 1.00 * a.om * a.dt * g.gx[c1] * drhodx;
         if (uC + uC_corr > 0.0) uC += uC_corr;
     }
-    int tS = g.BCu[iVxS], tN = g.BCu[iVxN];
+    int tS = T.U(iVxS), tN = T.U(iVxN);
     if (tS == 11) uC -= uS;
     if (tN == 11) uC -= uN;
-    bool pgrad = g.BCp[iPrE] != 30 && g.BCp[iPrW] != 30;
+    bool pgrad = T.P(iPrE) != 30 && T.P(iPrW) != 30;
     if (pgrad) { pW = -a.one_dx * 1e0; pE = a.one_dx * 1e0; }
     double cv = a.celvol;
     if (ASM) {
         b *= cv;
         if (tS != 30) ra.add(a.e.eqn_u[iVxS], uS * cv, tS, [&] { return (tS != 11 ? 1.0 : 2.0) * g.BCu_val[iVxS]; });
-        if (g.BCu[iVxW] != 30) ra.add(a.e.eqn_u[iVxW], uW * cv, g.BCu[iVxW], [&] { return g.BCu_val[iVxW]; });
-        ra.add(eqn, uC * cv, g.BCu[iVxC], [&] { return g.BCu_val[iVxC]; });
-        if (g.BCu[iVxE] != 30) ra.add(a.e.eqn_u[iVxE], uE * cv, g.BCu[iVxE], [&] { return g.BCu_val[iVxE]; });
-        if (g.BCu[c1 + nx] != 30) ra.add(a.e.eqn_u[iVxN], uN * cv, tN, [&] { return (tN != 11 ? 1.0 : 2.0) * g.BCu_val[iVxN]; });
-        if (g.BCv[iVzSW] != 30) ra.add(a.e.eqn_v[iVzSW], vSW * cv, g.BCv[iVzSW], [&] { return g.BCv_val[iVzSW]; });
-        if (g.BCv[iVzSE] != 30) ra.add(a.e.eqn_v[iVzSE], vSE * cv, g.BCv[iVzSE], [&] { return g.BCv_val[iVzSE]; });
-        if (g.BCv[iVzNW] != 30) ra.add(a.e.eqn_v[iVzNW], vNW * cv, g.BCv[iVzNW], [&] { return g.BCv_val[iVzNW]; });
-        if (g.BCv[iVzNE] != 30) ra.add(a.e.eqn_v[iVzNE], vNE * cv, g.BCv[iVzNE], [&] { return g.BCv_val[iVzNE]; });
+        if (T.U(iVxW) != 30) ra.add(a.e.eqn_u[iVxW], uW * cv, T.U(iVxW), [&] { return g.BCu_val[iVxW]; });
+        ra.add(eqn, uC * cv, T.U(iVxC), [&] { return g.BCu_val[iVxC]; });
+        if (T.U(iVxE) != 30) ra.add(a.e.eqn_u[iVxE], uE * cv, T.U(iVxE), [&] { return g.BCu_val[iVxE]; });
+        if (T.U(c1 + nx) != 30) ra.add(a.e.eqn_u[iVxN], uN * cv, tN, [&] { return (tN != 11 ? 1.0 : 2.0) * g.BCu_val[iVxN]; });
+        if (T.V(iVzSW) != 30) ra.add(a.e.eqn_v[iVzSW], vSW * cv, T.V(iVzSW), [&] { return g.BCv_val[iVzSW]; });
+        if (T.V(iVzSE) != 30) ra.add(a.e.eqn_v[iVzSE], vSE * cv, T.V(iVzSE), [&] { return g.BCv_val[iVzSE]; });
+        if (T.V(iVzNW) != 30) ra.add(a.e.eqn_v[iVzNW], vNW * cv, T.V(iVzNW), [&] { return g.BCv_val[iVzNW]; });
+        if (T.V(iVzNE) != 30) ra.add(a.e.eqn_v[iVzNE], vNE * cv, T.V(iVzNE), [&] { return g.BCv_val[iVzNE]; });
         if (pgrad) {
-            rb.add(a.e.eqn_p[iPrW] - a.e.neq_mom, pW * cv, g.BCp[iPrW], [&] { return g.BCp_val[iPrW]; });
-            rb.add(a.e.eqn_p[iPrE] - a.e.neq_mom, pE * cv, g.BCp[iPrE], [&] { return g.BCp_val[iPrE]; });
+            rb.add(a.e.eqn_p[iPrW] - a.e.neq_mom, pW * cv, T.P(iPrW), [&] { return g.BCp_val[iPrW]; });
+            rb.add(a.e.eqn_p[iPrE] - a.e.neq_mom, pE * cv, T.P(iPrE), [&] { return g.BCp_val[iPrE]; });
         }
     } else {
         const double *u = a.u, *v = a.v, *p = a.p_mom;
         double f = uC * u[iVxC];
         if (pgrad) f += pW * p[iPrW] + pE * p[iPrE];
-        if (g.BCv[iVzSE] != 30) f += vSE * v[iVzSE];
-        if (g.BCv[iVzSW] != 30) f += vSW * v[iVzSW];
-        if (g.BCv[iVzNE] != 30) f += vNE * v[iVzNE];
-        if (g.BCv[iVzNW] != 30) f += vNW * v[iVzNW];
+        if (T.V(iVzSE) != 30) f += vSE * v[iVzSE];
+        if (T.V(iVzSW) != 30) f += vSW * v[iVzSW];
+        if (T.V(iVzNE) != 30) f += vNE * v[iVzNE];
+        if (T.V(iVzNW) != 30) f += vNW * v[iVzNW];
         if (tS != 30 && tS != 11) f += uS * u[iVxS];
         if (tS == 11) f += 2.0 * uS * g.BCu_val[iVxS];
         if (tN != 30 && tN != 11) f += uN * u[iVxN];
         if (tN == 11) f += 2.0 * uN * g.BCu_val[iVxN];
-        if (g.BCu[iVxW] != 30) f += uW * u[iVxW];
-        if (g.BCu[iVxE] != 30) f += uE * u[iVxE];
+        if (T.U(iVxW) != 30) f += uW * u[iVxW];
+        if (T.U(iVxE) != 30) f += uE * u[iVxE];
         f -= b;
         F = f * cv;
     }
 }
 
-template <bool ASM, int MODE>
-__device__ void xmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, int east, int comp, Sink<MODE> &ra,
+template <bool ASM, int MODE, class TG>
+__device__ void xmom_neumann(const DevAsm &a, const TG &T, int c1, int c2, int c3, int eqn, int east, int comp, Sink<MODE> &ra,
                              Sink<MODE> &rb, double &b, double bbc_in, double &F) {
     const DevGrid &g = a.g;
     int nx = g.Nx, nxvz = g.nxvz;
@@ -189,18 +208,18 @@ __device__ void xmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, i
     double dxx = a.one_dx_dx, dzz = a.one_dz_dz, dxz = a.one_dx_dz;
     if (!east) {
         b -= g.BCu_val[c1] * a.one_dx;
-        if (g.BCu[c1 + 1] != 30) {
+        if (T.U(c1 + 1) != 30) {
             uC = -2.0 * dxx * (etaE) + comp * 2.0 / 3.0 * dxx * etaE;
             uE = 2.0 * dxx * etaE - comp * 2.0 / 3.0 * dxx * etaE;
         }
     } else {
         b += g.BCu_val[c1] * a.one_dx;
-        if (g.BCu[c1 - 1] != 30) {
+        if (T.U(c1 - 1) != 30) {
             uW = 2.0 * dxx * etaW - comp * 2.0 / 3.0 * dxx * etaW;
             uC = -2.0 * dxx * (etaW) + comp * 2.0 / 3.0 * dxx * etaW;
         }
     }
-    int tS = g.BCu[c1 - nx], tN = g.BCu[c1 + nx];
+    int tS = T.U(c1 - nx), tN = T.U(c1 + nx);
     if (tS != 30 && tN != 30) {
         if (tS != 13) { uS = dzz * etaS; uC += -dzz * etaS; }
         if (tN != 13) { uN = dzz * etaN; uC += -dzz * etaN; }
@@ -215,8 +234,8 @@ __device__ void xmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, i
     }
     if (tS == 11) uC += -dzz * etaS;
     if (tN == 11) uC += -dzz * etaN;
-    bool vs = g.BCv[c3 - nxvz] != 30 && g.BCv[c3 - nxvz + 1] != 30;
-    bool vn = g.BCv[c3 + 1] != 30 && g.BCv[c3] != 30;
+    bool vs = T.V(c3 - nxvz) != 30 && T.V(c3 - nxvz + 1) != 30;
+    bool vn = T.V(c3 + 1) != 30 && T.V(c3) != 30;
     if (vs) {
         vSE = -dxz * etaS + comp * 2.0 / 3.0 * dxz * etaE;
         vSW = dxz * etaS - comp * 2.0 / 3.0 * dxz * etaW;
@@ -226,46 +245,46 @@ __device__ void xmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, i
         vNW = -dxz * etaN + comp * 2.0 / 3.0 * dxz * etaW;
     }
     int ip = east ? c2 : c2 + 1;
-    if (!east) { if (g.BCp[c2 + 1] != 30) { pW = a.one_dx; pE = -pW; } }
-    else { if (g.BCp[c2] != 30) pW = a.one_dx; }
+    if (!east) { if (T.P(c2 + 1) != 30) { pW = a.one_dx; pE = -pW; } }
+    else { if (T.P(c2) != 30) pW = a.one_dx; }
     uS = -uS; uN = -uN; uW = -uW; uE = -uE; uC = -uC; vSW = -vSW; vSE = -vSE; vNW = -vNW; vNE = -vNE; pE = -pE; pW = -pW;
     double pcoef = east ? pW : pE;
     double cv = a.celvol;
     if (ASM) {
         b *= cv;
         if (tS != 30) ra.add(a.e.eqn_u[c1 - nx], uS * cv, tS, [&] { return (tS != 11 ? 1.0 : 2.0) * g.BCu_val[c1 - nx]; });
-        if (east && g.BCu[c1 - 1] != 30) ra.add(a.e.eqn_u[c1 - 1], uW * cv, g.BCu[c1 - 1], [&] { return g.BCu_val[c1 - 1]; });
-        ra.add(eqn, uC * cv, g.BCu[c1], [&] { return g.BCu_val[c1]; });
-        if (!east && g.BCu[c1 + 1] != 30) ra.add(a.e.eqn_u[c1 + 1], uE * cv, g.BCu[c1 + 1], [&] { return g.BCu_val[c1 + 1]; });
+        if (east && T.U(c1 - 1) != 30) ra.add(a.e.eqn_u[c1 - 1], uW * cv, T.U(c1 - 1), [&] { return g.BCu_val[c1 - 1]; });
+        ra.add(eqn, uC * cv, T.U(c1), [&] { return g.BCu_val[c1]; });
+        if (!east && T.U(c1 + 1) != 30) ra.add(a.e.eqn_u[c1 + 1], uE * cv, T.U(c1 + 1), [&] { return g.BCu_val[c1 + 1]; });
         if (tN != 30) ra.add(a.e.eqn_u[c1 + nx], uN * cv, tN, [&] { return (tN != 11 ? 1.0 : 2.0) * g.BCu_val[c1 + nx]; });
         if (vs) {
-            ra.add(a.e.eqn_v[c3 - nxvz], vSW * cv, g.BCv[c3 - nxvz], [&] { return g.BCv_val[c3 - nxvz]; });
-            ra.add(a.e.eqn_v[c3 - nxvz + 1], vSE * cv, g.BCv[c3 - nxvz + 1], [&] { return g.BCv_val[c3 - nxvz + 1]; });
+            ra.add(a.e.eqn_v[c3 - nxvz], vSW * cv, T.V(c3 - nxvz), [&] { return g.BCv_val[c3 - nxvz]; });
+            ra.add(a.e.eqn_v[c3 - nxvz + 1], vSE * cv, T.V(c3 - nxvz + 1), [&] { return g.BCv_val[c3 - nxvz + 1]; });
         }
         if (vn) {
-            ra.add(a.e.eqn_v[c3], vNW * cv, g.BCv[c3], [&] { return g.BCv_val[c3]; });
-            ra.add(a.e.eqn_v[c3 + 1], vNE * cv, g.BCv[c3 + 1], [&] { return g.BCv_val[c3 + 1]; });
+            ra.add(a.e.eqn_v[c3], vNW * cv, T.V(c3), [&] { return g.BCv_val[c3]; });
+            ra.add(a.e.eqn_v[c3 + 1], vNE * cv, T.V(c3 + 1), [&] { return g.BCv_val[c3 + 1]; });
         }
-        if (g.BCp[ip] != 30) rb.add(a.e.eqn_p[ip] - a.e.neq_mom, pcoef * cv, g.BCp[ip], [&] { return g.BCp_val[ip]; });
+        if (T.P(ip) != 30) rb.add(a.e.eqn_p[ip] - a.e.neq_mom, pcoef * cv, T.P(ip), [&] { return g.BCp_val[ip]; });
     } else {
         const double *u = a.u, *v = a.v, *p = a.p_mom;
         double f = -(b + bbc_in / cv);
         f += uC * u[c1];
         if (tS != 30 && tS != 11) f += uS * u[c1 - nx];
-        if (east && g.BCu[c1 - 1] != 30) f += uW * u[c1 - 1];
-        if (!east && g.BCu[c1 + 1] != 30) f += uE * u[c1 + 1];
+        if (east && T.U(c1 - 1) != 30) f += uW * u[c1 - 1];
+        if (!east && T.U(c1 + 1) != 30) f += uE * u[c1 + 1];
         if (tN != 30 && tN != 11) f += uN * u[c1 + nx];
         if (vs) {
-            int t1 = g.BCv[c3 - nxvz + 1], t0 = g.BCv[c3 - nxvz];
+            int t1 = T.V(c3 - nxvz + 1), t0 = T.V(c3 - nxvz);
             if (t1 != 11 && t1 != 0) f += vSE * v[c3 - nxvz + 1];
             if (t0 != 11 && t0 != 0) f += vSW * v[c3 - nxvz];
         }
         if (vn) {
-            int t1 = g.BCv[c3 + 1], t0 = g.BCv[c3];
+            int t1 = T.V(c3 + 1), t0 = T.V(c3);
             if (t1 != 11 && t1 != 0) f += vNE * v[c3 + 1];
             if (t0 != 11 && t0 != 0) f += vNW * v[c3];
         }
-        if (g.BCp[ip] != 30) f += pcoef * p[ip];
+        if (T.P(ip) != 30) f += pcoef * p[ip];
         F = f * cv;
     }
 }
@@ -273,22 +292,22 @@ __device__ void xmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, i
 /* ------------------------------------------------------------------ */
 /* Picard Z rows                                                        */
 /* ------------------------------------------------------------------ */
-template <bool ASM, int MODE>
-__device__ void zmom_inner(const DevAsm &a, int c1, int c2, int c3, int eqn, Sink<MODE> &ra, Sink<MODE> &rb,
+template <bool ASM, int MODE, class TG>
+__device__ void zmom_inner(const DevAsm &a, const TG &T, int c1, int c2, int c3, int eqn, Sink<MODE> &ra, Sink<MODE> &rb,
                            double &b, double &F) {
     const DevGrid &g = a.g;
     int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz;
     double D22S = g.D22_n[c2], D22N = g.D22_n[c2 + ncx], D33E = g.D33_s[c1], D33W = g.D33_s[c1 - 1];
-    if (g.BCp[c2 + ncx] == 31) b += g.BCv_val[c3] * a.one_dz;
-    int tVS = g.BCv[c3 - nxvz], tVN = g.BCv[c3 + nxvz];
+    if (T.P(c2 + ncx) == 31) b += g.BCv_val[c3] * a.one_dz;
+    int tVS = T.V(c3 - nxvz), tVN = T.V(c3 + nxvz);
     double inS = 0, inN = 0;
     if (tVS != 30 && tVN != 30) { inS = 1.0; inN = 1.0; }
     if (tVS == 30 && tVN != 30) inN = 1.0;
     if (tVS != 30 && tVN == 30) inS = 1.0;
-    if (g.BCp[c2] == -1) inS = 1.0;
-    if (g.BCp[c2 + ncx] == -1) inN = 1.0;
-    double inW = (g.BCg[c1 - 1] != 30 && g.BCv[c3 - 1] != 13) ? 1.0 : 0.0;
-    double inE = (g.BCg[c1] != 30 && g.BCv[c3 + 1] != 13) ? 1.0 : 0.0;
+    if (T.P(c2) == -1) inS = 1.0;
+    if (T.P(c2 + ncx) == -1) inN = 1.0;
+    double inW = (T.G(c1 - 1) != 30 && T.V(c3 - 1) != 13) ? 1.0 : 0.0;
+    double inE = (T.G(c1) != 30 && T.V(c3 + 1) != 13) ? 1.0 : 0.0;
     double dx = g.dx, dz = g.dz, comp = 1.0;
     double vW = -D33W * inW / (dx * dx);
     double vC = -(D22N * inN * ((1.0 / 3.0) * comp / dz - 1 / dz) - D22S * inS * (-1.0 / 3.0 * comp / dz + 1.0 / dz)) / dz
@@ -300,7 +319,7 @@ __device__ void zmom_inner(const DevAsm &a, int c1, int c2, int c3, int eqn, Sin
     double uSE = -1.0 / 3.0 * D22S * comp * inS / (dx * dz) + D33E * inE / (dx * dz);
     double uNW = -1.0 / 3.0 * D22N * comp * inN / (dx * dz) + D33W * inW / (dx * dz);
     double uNE = (1.0 / 3.0) * D22N * comp * inN / (dx * dz) - D33E * inE / (dx * dz);
-    int tW = g.BCv[c3 - 1], tE = g.BCv[c3 + 1];
+    int tW = T.V(c3 - 1), tE = T.V(c3 + 1);
     if (tW == 11) vC -= vW;
     if (tE == 11) vC -= vE;
     if (a.stab == 1) {
@@ -309,25 +328,25 @@ __device__ void zmom_inner(const DevAsm &a, int c1, int c2, int c3, int eqn, Sin
         if (vC + vC_corr > 0.0) vC += vC_corr;
     }
     double pS = 0, pN = 0;
-    bool pgrad = g.BCp[c2] != 30 && g.BCp[c2 + ncx] != 30;
+    bool pgrad = T.P(c2) != 30 && T.P(c2 + ncx) != 30;
     if (pgrad) { pS = -a.one_dz * 1e0; pN = a.one_dz * 1e0; }
     double cv = a.celvol;
     if (ASM) {
         b *= cv;
-        if (g.BCu[c1 + nx] != 30) ra.add(a.e.eqn_u[c1 + nx], uNE * cv, g.BCu[c1 + nx], [&] { return g.BCu_val[c1 + nx]; });
-        if (g.BCu[c1] != 30) ra.add(a.e.eqn_u[c1], uSE * cv, g.BCu[c1], [&] { return g.BCu_val[c1]; });
-        if (g.BCu[c1 + nx - 1] != 30) ra.add(a.e.eqn_u[c1 + nx - 1], uNW * cv, g.BCu[c1 + nx - 1], [&] { return g.BCu_val[c1 + nx - 1]; });
-        if (g.BCu[c1 - 1] != 30) ra.add(a.e.eqn_u[c1 - 1], uSW * cv, g.BCu[c1 - 1], [&] { return g.BCu_val[c1 - 1]; });
+        if (T.U(c1 + nx) != 30) ra.add(a.e.eqn_u[c1 + nx], uNE * cv, T.U(c1 + nx), [&] { return g.BCu_val[c1 + nx]; });
+        if (T.U(c1) != 30) ra.add(a.e.eqn_u[c1], uSE * cv, T.U(c1), [&] { return g.BCu_val[c1]; });
+        if (T.U(c1 + nx - 1) != 30) ra.add(a.e.eqn_u[c1 + nx - 1], uNW * cv, T.U(c1 + nx - 1), [&] { return g.BCu_val[c1 + nx - 1]; });
+        if (T.U(c1 - 1) != 30) ra.add(a.e.eqn_u[c1 - 1], uSW * cv, T.U(c1 - 1), [&] { return g.BCu_val[c1 - 1]; });
         if (tVS != 30) ra.add(a.e.eqn_v[c3 - nxvz], vS * cv, tVS, [&] { return g.BCv_val[c3 - nxvz]; });
         if (tW != 30 && tW != -12) ra.add(a.e.eqn_v[c3 - 1], vW * cv, tW, [&] { return (tW != 11 ? 1.0 : 2.0) * g.BCv_val[c3 - 1]; });
-        if (tW == -12) ra.add(a.e.eqn_v[c3 + nxvz - 3], vW * cv, g.BCv[c3 + nxvz - 3], [&] { return g.BCv_val[c3 + nxvz - 3]; });
-        ra.add(eqn, vC * cv, g.BCv[c3], [&] { return g.BCv_val[c3]; });
+        if (tW == -12) ra.add(a.e.eqn_v[c3 + nxvz - 3], vW * cv, T.V(c3 + nxvz - 3), [&] { return g.BCv_val[c3 + nxvz - 3]; });
+        ra.add(eqn, vC * cv, T.V(c3), [&] { return g.BCv_val[c3]; });
         if (tE != 30 && tE != -12) ra.add(a.e.eqn_v[c3 + 1], vE * cv, tE, [&] { return (tE != 11 ? 1.0 : 2.0) * g.BCv_val[c3 + 1]; });
-        if (tE == -12) ra.add(a.e.eqn_v[c3 - nxvz + 3], vE * cv, g.BCv[c3 - nxvz + 3], [&] { return g.BCv_val[c3 - nxvz + 3]; });
+        if (tE == -12) ra.add(a.e.eqn_v[c3 - nxvz + 3], vE * cv, T.V(c3 - nxvz + 3), [&] { return g.BCv_val[c3 - nxvz + 3]; });
         if (tVN != 30) ra.add(a.e.eqn_v[c3 + nxvz], vN * cv, tVN, [&] { return g.BCv_val[c3 + nxvz]; });
         if (pgrad) {
-            rb.add(a.e.eqn_p[c2] - a.e.neq_mom, pS * cv, g.BCp[c2], [&] { return g.BCp_val[c2]; });
-            rb.add(a.e.eqn_p[c2 + ncx] - a.e.neq_mom, pN * cv, g.BCp[c2 + ncx], [&] { return g.BCp_val[c2 + ncx]; });
+            rb.add(a.e.eqn_p[c2] - a.e.neq_mom, pS * cv, T.P(c2), [&] { return g.BCp_val[c2]; });
+            rb.add(a.e.eqn_p[c2 + ncx] - a.e.neq_mom, pN * cv, T.P(c2 + ncx), [&] { return g.BCp_val[c2 + ncx]; });
         }
     } else {
         const double *u = a.u, *v = a.v, *p = a.p_mom;
@@ -337,10 +356,10 @@ __device__ void zmom_inner(const DevAsm &a, int c1, int c2, int c3, int eqn, Sin
         if (tW != 30 && tW != 11 && tW != -12) f += vW * v[c3 - 1];
         if (tE != 30 && tE != 11 && tE != -12) f += vE * v[c3 + 1];
         if (tVN != 30) f += vN * v[c3 + nxvz];
-        if (g.BCu[c1 - 1] != 30) f += uSW * u[c1 - 1];
-        if (g.BCu[c1] != 30) f += uSE * u[c1];
-        if (g.BCu[c1 + nx - 1] != 30) f += uNW * u[c1 + nx - 1];
-        if (g.BCu[c1 + nx] != 30) f += uNE * u[c1 + nx];
+        if (T.U(c1 - 1) != 30) f += uSW * u[c1 - 1];
+        if (T.U(c1) != 30) f += uSE * u[c1];
+        if (T.U(c1 + nx - 1) != 30) f += uNW * u[c1 + nx - 1];
+        if (T.U(c1 + nx) != 30) f += uNE * u[c1 + nx];
         if (tW == 11) f += -2 * D33W * a.one_dx_dx * g.BCv_val[c3 - 1];
         if (tE == 11) f += -2 * D33E * a.one_dx_dx * g.BCv_val[c3 + 1];
         if (tW == -12) f += vW * v[c3 + nxvz - 3];
@@ -350,8 +369,8 @@ __device__ void zmom_inner(const DevAsm &a, int c1, int c2, int c3, int eqn, Sin
     }
 }
 
-template <bool ASM, int MODE>
-__device__ void zmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, int north, int comp, Sink<MODE> &ra,
+template <bool ASM, int MODE, class TG>
+__device__ void zmom_neumann(const DevAsm &a, const TG &T, int c1, int c2, int c3, int eqn, int north, int comp, Sink<MODE> &ra,
                              Sink<MODE> &rb, double &b, double bbc_in, double &F) {
     const DevGrid &g = a.g;
     int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz;
@@ -362,8 +381,8 @@ __device__ void zmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, i
     double dxx = a.one_dx_dx, dzz = a.one_dz_dz, dxz = a.one_dx_dz;
     if (!north) b -= g.BCv_val[c3] * a.one_dz;
     else b += g.BCv_val[c3] * a.one_dz;
-    bool us = g.BCu[c1 - 1] != 30 && g.BCu[c1] != 30;
-    bool un = g.BCu[c1 + nx - 1] != 30 && g.BCu[c1 + nx] != 30;
+    bool us = T.U(c1 - 1) != 30 && T.U(c1) != 30;
+    bool un = T.U(c1 + nx - 1) != 30 && T.U(c1 + nx) != 30;
     if (us) {
         uSW = dxz * etaW - comp * 2.0 / 3.0 * etaS * dxz;
         uSE = -dxz * etaE + comp * 2.0 / 3.0 * etaS * dxz;
@@ -373,17 +392,17 @@ __device__ void zmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, i
         uNE = dxz * etaE - comp * 2.0 / 3.0 * etaN * dxz;
     }
     if (!north) {
-        if (g.BCv[c3 + nxvz] != 30) {
+        if (T.V(c3 + nxvz) != 30) {
             vC = -2.0 * dzz * (etaN) + comp * 2.0 / 3.0 * etaS * dzz;
             vN = 2.0 * dzz * etaN - comp * 2.0 / 3.0 * etaN * dzz;
         }
     } else {
-        if (g.BCv[c3 - nxvz] != 30) {
+        if (T.V(c3 - nxvz) != 30) {
             vS = 2.0 * dzz * etaS - comp * 2.0 / 3.0 * etaS * dzz;
             vC = -2.0 * dzz * (etaS) + comp * 2.0 / 3.0 * etaS * dzz;
         }
     }
-    int tW = g.BCv[c3 - 1], tE = g.BCv[c3 + 1];
+    int tW = T.V(c3 - 1), tE = T.V(c3 + 1);
     if (tW != 30 && tE != 30) {
         if (tW != 13) { vW = dxx * etaW; vC += -dxx * etaW; }
         if (tE != 13) { vE = dxx * etaE; vC += -dxx * etaE; }
@@ -404,52 +423,52 @@ __device__ void zmom_neumann(const DevAsm &a, int c1, int c2, int c3, int eqn, i
         if (!north) vC += 1.00 * a.om * a.dt * a.gz_model * drhodz;
         else vC += -1.00 * a.om * a.dt * a.gz_model * drhodz;
     }
-    if (!north) { if (g.BCp[c2 + ncx] != 30) pN = -a.one_dz; }
-    else { if (g.BCp[c2] != 30) pS = a.one_dz; }
+    if (!north) { if (T.P(c2 + ncx) != 30) pN = -a.one_dz; }
+    else { if (T.P(c2) != 30) pS = a.one_dz; }
     uSW = -uSW; uSE = -uSE; uNW = -uNW; uNE = -uNE; vS = -vS; vW = -vW; vC = -vC; vE = -vE; vN = -vN; pN = -pN; pS = -pS;
     double cv = a.celvol;
     if (ASM) {
         b *= cv;
         if (us) {
-            ra.add(a.e.eqn_u[c1 - 1], uSW * cv, g.BCu[c1 - 1], [&] { return g.BCu_val[c1 - 1]; });
-            ra.add(a.e.eqn_u[c1], uSE * cv, g.BCu[c1], [&] { return g.BCu_val[c1]; });
+            ra.add(a.e.eqn_u[c1 - 1], uSW * cv, T.U(c1 - 1), [&] { return g.BCu_val[c1 - 1]; });
+            ra.add(a.e.eqn_u[c1], uSE * cv, T.U(c1), [&] { return g.BCu_val[c1]; });
         }
         if (un) {
-            ra.add(a.e.eqn_u[c1 + nx - 1], uNW * cv, g.BCu[c1 + nx - 1], [&] { return g.BCu_val[c1 + nx - 1]; });
-            ra.add(a.e.eqn_u[c1 + nx], uNE * cv, g.BCu[c1 + nx], [&] { return g.BCu_val[c1 + nx]; });
+            ra.add(a.e.eqn_u[c1 + nx - 1], uNW * cv, T.U(c1 + nx - 1), [&] { return g.BCu_val[c1 + nx - 1]; });
+            ra.add(a.e.eqn_u[c1 + nx], uNE * cv, T.U(c1 + nx), [&] { return g.BCu_val[c1 + nx]; });
         }
-        if (north && g.BCv[c3 - nxvz] != 30) ra.add(a.e.eqn_v[c3 - nxvz], vS * cv, g.BCv[c3 - nxvz], [&] { return g.BCv_val[c3 - nxvz]; });
+        if (north && T.V(c3 - nxvz) != 30) ra.add(a.e.eqn_v[c3 - nxvz], vS * cv, T.V(c3 - nxvz), [&] { return g.BCv_val[c3 - nxvz]; });
         if (tW != 30) ra.add(a.e.eqn_v[c3 - 1], vW * cv, tW, [&] { return (tW != 11 ? 1.0 : 2.0) * g.BCv_val[c3 - 1]; });
-        ra.add(eqn, vC * cv, g.BCv[c3], [&] { return g.BCv_val[c3]; });
+        ra.add(eqn, vC * cv, T.V(c3), [&] { return g.BCv_val[c3]; });
         if (tE != 30) ra.add(a.e.eqn_v[c3 + 1], vE * cv, tE, [&] { return (tE != 11 ? 1.0 : 2.0) * g.BCv_val[c3 + 1]; });
-        if (!north && g.BCv[c3 + nxvz] != 30) ra.add(a.e.eqn_v[c3 + nxvz], vN * cv, g.BCv[c3 + nxvz], [&] { return g.BCv_val[c3 + nxvz]; });
+        if (!north && T.V(c3 + nxvz) != 30) ra.add(a.e.eqn_v[c3 + nxvz], vN * cv, T.V(c3 + nxvz), [&] { return g.BCv_val[c3 + nxvz]; });
         if (!north) {
-            if (g.BCp[c2 + ncx] != 30) rb.add(a.e.eqn_p[c2 + ncx] - a.e.neq_mom, pN * cv, g.BCp[c2 + ncx], [&] { return g.BCp_val[c2 + ncx]; });
+            if (T.P(c2 + ncx) != 30) rb.add(a.e.eqn_p[c2 + ncx] - a.e.neq_mom, pN * cv, T.P(c2 + ncx), [&] { return g.BCp_val[c2 + ncx]; });
         } else {
-            if (g.BCp[c2] != 30) rb.add(a.e.eqn_p[c2] - a.e.neq_mom, pS * cv, g.BCp[c2], [&] { return g.BCp_val[c2]; });
+            if (T.P(c2) != 30) rb.add(a.e.eqn_p[c2] - a.e.neq_mom, pS * cv, T.P(c2), [&] { return g.BCp_val[c2]; });
         }
     } else {
         const double *u = a.u, *v = a.v, *p = a.p_mom;
         double f = -(b + bbc_in / cv);
         f += vC * v[c3];
         if (tW != 30 && tW != 11) f += vW * v[c3 - 1];
-        if (!north && g.BCv[c3 + nxvz] != 30) f += vN * v[c3 + nxvz];
-        if (north && g.BCv[c3 - nxvz] != 30) f += vS * v[c3 - nxvz];
+        if (!north && T.V(c3 + nxvz) != 30) f += vN * v[c3 + nxvz];
+        if (north && T.V(c3 - nxvz) != 30) f += vS * v[c3 - nxvz];
         if (tE != 30 && tE != 11) f += vE * v[c3 + 1];
         if (us) {
-            int t0 = g.BCu[c1 - 1], t1 = g.BCu[c1];
+            int t0 = T.U(c1 - 1), t1 = T.U(c1);
             if (t0 != 11 && t0 != 0) f += uSW * u[c1 - 1];
             if (t1 != 11 && t1 != 0) f += uSE * u[c1];
         }
         if (un) {
-            int t0 = g.BCu[c1 + nx - 1], t1 = g.BCu[c1 + nx];
+            int t0 = T.U(c1 + nx - 1), t1 = T.U(c1 + nx);
             if (t0 != 11 && t0 != 0) f += uNW * u[c1 + nx - 1];
             if (t1 != 11 && t1 != 0) f += uNE * u[c1 + nx];
         }
-        if (!north) { if (g.BCp[c2 + ncx] != 30) f += pN * p[c2 + ncx]; }
+        if (!north) { if (T.P(c2 + ncx) != 30) f += pN * p[c2 + ncx]; }
         else {
             /* the reference reads the pressure one row below (c2 - ncx) here */
-            if (g.BCp[c2 - ncx] != 30) f += pS * p[c2 - ncx];
+            if (T.P(c2 - ncx) != 30) f += pS * p[c2 - ncx];
         }
         F = f * cv;
     }
@@ -485,8 +504,8 @@ __device__ __forceinline__ void xa_gxz(XAcc &c, int a, int b, double w, const JK
     c.v[b + 2][a + 2] += wx; c.v[b + 2][a + 1] -= wx;
 }
 
-template <bool ASM, int MODE>
-__device__ void xjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3, int eqn, int k, int l,
+template <bool ASM, int MODE, class TG>
+__device__ void xjac_row(const DevAsm &a, const TG &T, int c1, int c2, int c3, int eqn, int k, int l,
                          Sink<MODE> &ra, Sink<MODE> &rb, double &b, double &F) {
     const DevGrid &g = a.g;
     int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz;
@@ -502,7 +521,7 @@ __device__ void xjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
     if (T.V(iVzNWW) == -12) iVzNWW = iVzNW + (nx - 2);
     if (T.V(iVzSEE) == -12) iVzSEE = iVzSE - (nx - 2);
     if (T.V(iVzNEE) == -12) iVzNEE = iVzNE - (nx - 2);
-    if (g.BCu[iVxC] == -2) {
+    if (T.U(iVxC) == -2) {
         iVxW = c1 + nx - 2; iPrW = c2 + ncx; iVzSW = c3 - 2; iVzNW = c3 + nxvz - 2;
         iPrNW = iPrW + ncx; iPrSW = iPrW - ncx;
         iVxSW = iVxW - nx; iVxNW = iVxW + nx;
@@ -622,44 +641,44 @@ __device__ void xjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
         const int *eu = a.e.eqn_u, *ev = a.e.eqn_v, *ep = a.e.eqn_p;
         const double *uv = g.BCu_val, *vv = g.BCv_val, *pv = g.BCp_val;
 #define UADD11(idx, coef) \
-    if (g.BCu[idx] != 30) ra.add(eu[idx], (coef)*cv, g.BCu[idx], [&] { return (g.BCu[idx] != 11 ? 1.0 : 2.0) * uv[idx]; });
+    if (T.U(idx) != 30) ra.add(eu[idx], (coef)*cv, T.U(idx), [&] { return (T.U(idx) != 11 ? 1.0 : 2.0) * uv[idx]; });
         UADD11(iVxSW, uSW)
         UADD11(iVxS, uS)
         UADD11(iVxSE, uSE)
-        if (g.BCu[iVxW] != 30) ra.add(eu[iVxW], uW * cv, g.BCu[iVxW], [&] { return uv[iVxW]; });
-        ra.add(eqn, uC * cv, g.BCu[iVxC], [&] { return uv[iVxC]; });
-        if (g.BCu[iVxE] != 30) ra.add(eu[iVxE], uE * cv, g.BCu[iVxE], [&] { return uv[iVxE]; });
+        if (T.U(iVxW) != 30) ra.add(eu[iVxW], uW * cv, T.U(iVxW), [&] { return uv[iVxW]; });
+        ra.add(eqn, uC * cv, T.U(iVxC), [&] { return uv[iVxC]; });
+        if (T.U(iVxE) != 30) ra.add(eu[iVxE], uE * cv, T.U(iVxE), [&] { return uv[iVxE]; });
         UADD11(iVxNW, uNW)
         UADD11(iVxN, uN)
         UADD11(iVxNE, uNE)
 #undef UADD11
-#define VADD(idx, coef) ra.add(ev[idx], (coef)*cv, g.BCv[idx], [&] { return vv[idx]; })
-        if (l > 1 && g.BCv[iVzSSW] != 30) VADD(iVzSSW, vSSW);
-        if (l > 1 && g.BCv[iVzSSE] != 30) VADD(iVzSSE, vSSE);
-        if ((k > 1) || ((k == 0 || k == 1) && g.BCv[iVzSWW] == -1))
-            if (g.BCv[iVzSWW] != 30) VADD(iVzSWW, vSWW);
-        if (g.BCv[iVzSW] != 30) VADD(iVzSW, vSW);
-        if (g.BCv[iVzSE] != 30) VADD(iVzSE, vSE);
-        if ((k < nx - 2) || ((k == nx - 2 || k == nx - 1) && g.BCv[iVzSEE] == -1))
-            if (g.BCv[iVzSEE] != 30) VADD(iVzSEE, vSEE);
-        if ((k > 1) || ((k == 0 || k == 1) && g.BCv[iVzNWW] == -1))
-            if (g.BCv[iVzNWW] != 30) VADD(iVzNWW, vNWW);
-        if (g.BCv[iVzNW] != 30) VADD(iVzNW, vNW);
-        if (g.BCv[iVzNE] != 30) VADD(iVzNE, vNE);
-        if ((k < nx - 2) || ((k == nx - 2 || k == nx - 1) && g.BCv[iVzNEE] == -1))
-            if (g.BCv[iVzNEE] != 30) VADD(iVzNEE, vNEE);
-        if (l < nz - 1 && g.BCv[iVzNNW] != 30) VADD(iVzNNW, vNNW);
-        if (l < nz - 1 && g.BCv[iVzNNE] != 30) VADD(iVzNNE, vNNE);
+#define VADD(idx, coef) ra.add(ev[idx], (coef)*cv, T.V(idx), [&] { return vv[idx]; })
+        if (l > 1 && T.V(iVzSSW) != 30) VADD(iVzSSW, vSSW);
+        if (l > 1 && T.V(iVzSSE) != 30) VADD(iVzSSE, vSSE);
+        if ((k > 1) || ((k == 0 || k == 1) && T.V(iVzSWW) == -1))
+            if (T.V(iVzSWW) != 30) VADD(iVzSWW, vSWW);
+        if (T.V(iVzSW) != 30) VADD(iVzSW, vSW);
+        if (T.V(iVzSE) != 30) VADD(iVzSE, vSE);
+        if ((k < nx - 2) || ((k == nx - 2 || k == nx - 1) && T.V(iVzSEE) == -1))
+            if (T.V(iVzSEE) != 30) VADD(iVzSEE, vSEE);
+        if ((k > 1) || ((k == 0 || k == 1) && T.V(iVzNWW) == -1))
+            if (T.V(iVzNWW) != 30) VADD(iVzNWW, vNWW);
+        if (T.V(iVzNW) != 30) VADD(iVzNW, vNW);
+        if (T.V(iVzNE) != 30) VADD(iVzNE, vNE);
+        if ((k < nx - 2) || ((k == nx - 2 || k == nx - 1) && T.V(iVzNEE) == -1))
+            if (T.V(iVzNEE) != 30) VADD(iVzNEE, vNEE);
+        if (l < nz - 1 && T.V(iVzNNW) != 30) VADD(iVzNNW, vNNW);
+        if (l < nz - 1 && T.V(iVzNNE) != 30) VADD(iVzNNE, vNNE);
 #undef VADD
-#define PADD(idx, coef) rb.add(ep[idx] - a.e.neq_mom, (coef)*cv, g.BCp[idx], [&] { return pv[idx]; })
+#define PADD(idx, coef) rb.add(ep[idx] - a.e.neq_mom, (coef)*cv, T.P(idx), [&] { return pv[idx]; })
         if (l > 1) {
-            if (g.BCp[iPrSW] != 30) PADD(iPrSW, pSW);
-            if (g.BCp[iPrSE] != 30) PADD(iPrSE, pSE);
+            if (T.P(iPrSW) != 30) PADD(iPrSW, pSW);
+            if (T.P(iPrSE) != 30) PADD(iPrSE, pSE);
         }
-        if (g.BCp[iPrE] != 30 && g.BCp[c2] != 30) { PADD(iPrW, pW); PADD(iPrE, pE); }
+        if (T.P(iPrE) != 30 && T.P(c2) != 30) { PADD(iPrW, pW); PADD(iPrE, pE); }
         if (l < nz - 1) {
-            if (g.BCp[iPrNW] != 30) PADD(iPrNW, pNW);
-            if (g.BCp[iPrNE] != 30) PADD(iPrNE, pNE);
+            if (T.P(iPrNW) != 30) PADD(iPrNW, pNW);
+            if (T.P(iPrNE) != 30) PADD(iPrNE, pNE);
         }
 #undef PADD
     } else {
@@ -696,8 +715,8 @@ __device__ __forceinline__ void za_gxz(ZAcc &c, int a, int b, double w, const JK
     c.v[b + 1][a + 1] += wx; c.v[b + 1][a] -= wx;
 }
 
-template <bool ASM, int MODE>
-__device__ void zjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3, int eqn, int k, int l,
+template <bool ASM, int MODE, class TG>
+__device__ void zjac_row(const DevAsm &a, const TG &T, int c1, int c2, int c3, int eqn, int k, int l,
                          Sink<MODE> &ra, Sink<MODE> &rb, double &b, double &F) {
     const DevGrid &g = a.g;
     int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz;
@@ -709,12 +728,12 @@ __device__ void zjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
     int iVxNNW = iVxNW + nx, iVxNNE = iVxNE + nx, iVxSSW = iVxSW - nx, iVxSSE = iVxSE - nx;
     int iPrS = c2, iPrSW = iPrS - 1, iPrSE = iPrS + 1, iPrN = c2 + ncx, iPrNW = iPrN - 1, iPrNE = iPrN + 1;
     int ixyW = c1 - 1, ixySW = ixyW - nx, ixyNW = ixyW + nx, ixyE = c1, ixySE = ixyE - nx, ixyNE = ixyE + nx;
-    if (k == 1 && g.BCv[iVzW] == -12) {
+    if (k == 1 && T.V(iVzW) == -12) {
         iVxNWW = iVxNW + (nx - 2); iVxSWW = iVxSW + (nx - 2);
         iPrNW = iPrN + (ncx - 1); iPrSW = iPrS + (ncx - 1);
         iVzW = iVzC + (nx - 2); iVzSW = iVzW - nxvz; iVzNW = iVzW + nxvz;
     }
-    if (k == nx - 1 && g.BCv[iVzE] == -12) {
+    if (k == nx - 1 && T.V(iVzE) == -12) {
         iVxNEE = iVxNE - (nx - 2); iVxSEE = iVxSE - (nx - 2);
         iPrNE = iPrN - (ncx - 1); iPrSE = iPrS - (ncx - 1);
         iVzE = iVzC - (nx - 2); iVzNE = iVzE + nxvz; iVzSE = iVzE - nxvz;
@@ -832,46 +851,46 @@ __device__ void zjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
         b *= cv;
         const int *eu = a.e.eqn_u, *ev = a.e.eqn_v, *ep = a.e.eqn_p;
         const double *uv = g.BCu_val, *vv = g.BCv_val, *pv = g.BCp_val;
-#define UADD(idx, coef) ra.add(eu[idx], (coef)*cv, g.BCu[idx], [&] { return uv[idx]; })
-#define VADD(idx, coef) ra.add(ev[idx], (coef)*cv, g.BCv[idx], [&] { return vv[idx]; })
-        if (g.BCu[iVxSSW] != 30) UADD(iVxSSW, uSSW);
-        if (g.BCu[iVxSSE] != 30) UADD(iVxSSE, uSSE);
-        if ((k > 1) || (k == 1 && g.BCu[iVxSWW] == -1))
-            if (g.BCu[iVxSWW] != 30) UADD(iVxSWW, uSWW);
-        if (g.BCu[iVxSW] != 30) UADD(iVxSW, uSW);
-        if (g.BCu[iVxSE] != 30) UADD(iVxSE, uSE);
-        if ((k < nx - 1) || (k == nx - 1 && g.BCu[iVxSEE] == -1))
-            if (g.BCu[c1 + 1] != 30) UADD(iVxSEE, uSEE);
-        if ((k > 1) || (k == 1 && g.BCu[iVxNWW] == -1))
-            if (g.BCu[c1 + nx - 2] != 30) UADD(iVxNWW, uNWW);
-        if (g.BCu[iVxNW] != 30) UADD(iVxNW, uNW);
-        if (g.BCu[iVxNE] != 30) UADD(iVxNE, uNE);
-        if ((k < nx - 1) || (k == nx - 1 && g.BCu[iVxNEE] == -1))
-            if (g.BCu[iVxNEE] != 30) UADD(iVxNEE, uNEE);
-        if (g.BCu[iVxNNW] != 30) UADD(iVxNNW, uNNW);
-        if (g.BCu[iVxNNE] != 30) UADD(iVxNNE, uNNE);
-        if (g.BCv[iVzSW] == -1) VADD(iVzSW, vSW);
-        if (g.BCv[iVzS] != 30) VADD(iVzS, vS);
-        if (g.BCv[iVzSE] == -1) VADD(iVzSE, vSE);
-        if (g.BCv[iVzW] == -1) VADD(iVzW, vW);
-        ra.add(eqn, vC * cv, g.BCv[iVzC], [&] { return vv[iVzC]; });
-        if (g.BCv[iVzE] == -1) VADD(iVzE, vE);
-        if (g.BCv[iVzNW] == -1) VADD(iVzNW, vNW);
-        if (g.BCv[iVzN] != 30) VADD(iVzN, vN);
-        if (g.BCv[iVzNE] == -1) VADD(iVzNE, vNE);
+#define UADD(idx, coef) ra.add(eu[idx], (coef)*cv, T.U(idx), [&] { return uv[idx]; })
+#define VADD(idx, coef) ra.add(ev[idx], (coef)*cv, T.V(idx), [&] { return vv[idx]; })
+        if (T.U(iVxSSW) != 30) UADD(iVxSSW, uSSW);
+        if (T.U(iVxSSE) != 30) UADD(iVxSSE, uSSE);
+        if ((k > 1) || (k == 1 && T.U(iVxSWW) == -1))
+            if (T.U(iVxSWW) != 30) UADD(iVxSWW, uSWW);
+        if (T.U(iVxSW) != 30) UADD(iVxSW, uSW);
+        if (T.U(iVxSE) != 30) UADD(iVxSE, uSE);
+        if ((k < nx - 1) || (k == nx - 1 && T.U(iVxSEE) == -1))
+            if (T.U(c1 + 1) != 30) UADD(iVxSEE, uSEE);
+        if ((k > 1) || (k == 1 && T.U(iVxNWW) == -1))
+            if (T.U(c1 + nx - 2) != 30) UADD(iVxNWW, uNWW);
+        if (T.U(iVxNW) != 30) UADD(iVxNW, uNW);
+        if (T.U(iVxNE) != 30) UADD(iVxNE, uNE);
+        if ((k < nx - 1) || (k == nx - 1 && T.U(iVxNEE) == -1))
+            if (T.U(iVxNEE) != 30) UADD(iVxNEE, uNEE);
+        if (T.U(iVxNNW) != 30) UADD(iVxNNW, uNNW);
+        if (T.U(iVxNNE) != 30) UADD(iVxNNE, uNNE);
+        if (T.V(iVzSW) == -1) VADD(iVzSW, vSW);
+        if (T.V(iVzS) != 30) VADD(iVzS, vS);
+        if (T.V(iVzSE) == -1) VADD(iVzSE, vSE);
+        if (T.V(iVzW) == -1) VADD(iVzW, vW);
+        ra.add(eqn, vC * cv, T.V(iVzC), [&] { return vv[iVzC]; });
+        if (T.V(iVzE) == -1) VADD(iVzE, vE);
+        if (T.V(iVzNW) == -1) VADD(iVzNW, vNW);
+        if (T.V(iVzN) != 30) VADD(iVzN, vN);
+        if (T.V(iVzNE) == -1) VADD(iVzNE, vNE);
 #undef UADD
 #undef VADD
-#define PADD(idx, coef, vidx) rb.add(ep[idx] - a.e.neq_mom, (coef)*cv, g.BCp[idx], [&] { return pv[vidx]; })
+#define PADD(idx, coef, vidx) rb.add(ep[idx] - a.e.neq_mom, (coef)*cv, T.P(idx), [&] { return pv[vidx]; })
         if (k > 1 || (k == 1 && periodix == 1))
-            if (g.BCp[iPrSW] != 30) PADD(iPrSW, pSW, iPrSW);
-        if (g.BCp[iPrS] != 30 && g.BCp[iPrN] != 30) PADD(iPrS, pS, iPrS);
+            if (T.P(iPrSW) != 30) PADD(iPrSW, pSW, iPrSW);
+        if (T.P(iPrS) != 30 && T.P(iPrN) != 30) PADD(iPrS, pS, iPrS);
         if (k < nx - 1 || (k == nx - 1 && periodix == 1))
-            if (g.BCp[iPrSE] != 30) PADD(iPrSE, pSE, iPrSE);
+            if (T.P(iPrSE) != 30) PADD(iPrSE, pSE, iPrSE);
         if (k > 1 || (k == 1 && periodix == 1))
-            if (g.BCp[iPrNW] != 30) PADD(iPrNW, pNW, iPrNW);
-        if (g.BCp[iPrS] != 30 && g.BCp[iPrN] != 30) PADD(iPrN, pN, c2 + ncx);
+            if (T.P(iPrNW) != 30) PADD(iPrNW, pNW, iPrNW);
+        if (T.P(iPrS) != 30 && T.P(iPrN) != 30) PADD(iPrN, pN, c2 + ncx);
         if (k < nx - 1 || (k == nx - 1 && periodix == 1))
-            if (g.BCp[iPrNE] != 30) PADD(iPrNE, pNE, iPrNE);
+            if (T.P(iPrNE) != 30) PADD(iPrNE, pNE, iPrNE);
 #undef PADD
     } else {
         double f = 0.0;
@@ -890,8 +909,8 @@ __device__ void zjac_row(const DevAsm &a, const Tags &T, int c1, int c2, int c3,
 
 /* One momentum row: which stencil applies (StokesAssemblyDecoupled.c:1691-1729,
  * 1763-1800, 3952-4060). */
-template <bool ASM, int MODE>
-__device__ void momentum_row(int kind, const DevAsm &a, const Tags &T, int eqn, Sink<MODE> &ra, Sink<MODE> &rb,
+template <bool ASM, int MODE, class TG>
+__device__ void momentum_row(int kind, const DevAsm &a, const TG &T, int eqn, Sink<MODE> &ra, Sink<MODE> &rb,
                              double &b, double bbc_in, double &F) {
     const DevGrid &g = a.g;
     int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz, nz = g.Nz, nzvx = g.nzvx;
@@ -901,25 +920,25 @@ __device__ void momentum_row(int kind, const DevAsm &a, const Tags &T, int eqn, 
     if (node < nu) {
         int c1 = node, k = c1 % nx, l = c1 / nx;
         int c2 = k - 1 + (l - 1) * ncx, c3 = k + l * nxvz;
-        int t = g.BCu[c1];
+        int t = T.U(c1);
         bool inner = l > 0 && l < nzvx - 1 && k > 0 && k < nx - 1;
         if (inner || (k == 0 && t == -2)) {
-            if (kind == 0) xmom_inner<ASM, MODE>(a, c1, c2, c3, eqn, ra, rb, b, F);
+            if (kind == 0) xmom_inner<ASM, MODE>(a, T, c1, c2, c3, eqn, ra, rb, b, F);
             else xjac_row<ASM, MODE>(a, T, c1, c2, c3, eqn, k, l, ra, rb, b, F);
         }
-        if (k == 0 && t == 2) xmom_neumann<ASM, MODE>(a, c1, c2, c3, eqn, 0, comp_builder, ra, rb, b, bbc_in, F);
-        if (k == nx - 1 && t == 2) xmom_neumann<ASM, MODE>(a, c1, c2, c3, eqn, 1, comp_builder, ra, rb, b, bbc_in, F);
+        if (k == 0 && t == 2) xmom_neumann<ASM, MODE>(a, T, c1, c2, c3, eqn, 0, comp_builder, ra, rb, b, bbc_in, F);
+        if (k == nx - 1 && t == 2) xmom_neumann<ASM, MODE>(a, T, c1, c2, c3, eqn, 1, comp_builder, ra, rb, b, bbc_in, F);
     } else {
         int c3 = node - nu, k = c3 % nxvz, l = c3 / nxvz;
         int c1 = k + l * nx, c2 = k - 1 + (l - 1) * ncx;
-        int t = g.BCv[c3];
+        int t = T.V(c3);
         bool inner = k > 0 && k < nxvz - 1 && l > 0 && l < nz - 1;
         if (inner) {
-            if (kind == 0) zmom_inner<ASM, MODE>(a, c1, c2, c3, eqn, ra, rb, b, F);
+            if (kind == 0) zmom_inner<ASM, MODE>(a, T, c1, c2, c3, eqn, ra, rb, b, F);
             else zjac_row<ASM, MODE>(a, T, c1, c2, c3, eqn, k, l, ra, rb, b, F);
         }
-        if (l == 0 && t == 2) zmom_neumann<ASM, MODE>(a, c1, c2, c3, eqn, 0, comp_builder, ra, rb, b, bbc_in, F);
-        if (l == nz - 1 && t == 2) zmom_neumann<ASM, MODE>(a, c1, c2, c3, eqn, 1, comp_builder, ra, rb, b, bbc_in, F);
+        if (l == 0 && t == 2) zmom_neumann<ASM, MODE>(a, T, c1, c2, c3, eqn, 0, comp_builder, ra, rb, b, bbc_in, F);
+        if (l == nz - 1 && t == 2) zmom_neumann<ASM, MODE>(a, T, c1, c2, c3, eqn, 1, comp_builder, ra, rb, b, bbc_in, F);
     }
 }
 
@@ -1016,11 +1035,10 @@ __global__ void __launch_bounds__(VBLK) k_values_mom(DevAsm a, DevBlock A, DevBl
     const int eqn = r0 + threadIdx.x;
     const int a0 = A.Ic[r0], b0 = B.Ic[r0];
     if (eqn < r1) {
-        Tags T = make_tags(a.g);
         Sink<SINK_VALUES> ra{0, nullptr, sA + (A.Ic[eqn] - a0), 0.0, sDump + threadIdx.x},
             rb{0, nullptr, sB + (B.Ic[eqn] - b0), 0.0, sDump + threadIdx.x};
         double b = mom_rhs(a, eqn), F = 0.0;
-        momentum_row<true, SINK_VALUES>(KIND, a, T, eqn, ra, rb, b, 0.0, F);
+        momentum_row<true, SINK_VALUES>(KIND, a, make_tags(a.g), eqn, ra, rb, b, 0.0, F);
         A.bbc[eqn] = ra.bbc;
         B.bbc[eqn] = rb.bbc;
         double bf = b + ra.bbc;
@@ -1068,17 +1086,48 @@ __global__ void __launch_bounds__(BLK) k_values_cont(int kind, DevAsm a, DevBloc
     (void)kind;
 }
 
-/* residual (Assemble = 0): b = rhs (Neumann rows adjusted), F */
-template <int KIND>
+/* residual (Assemble = 0): b = rhs (Neumann rows adjusted), F.  Waves
+ * whose rows are all free take the tag-free instance. */
+template <int KIND, bool FREE>
 __global__ void __launch_bounds__(BLK) k_residual_mom(DevAsm a, DevBlock A) {
     int eqn = blockIdx.x * BLK + threadIdx.x;
+    if (A.free_row) {
+        const int fr = eqn < a.e.neq_mom ? A.free_row[eqn] : 1;
+        if ((__all(fr) != 0) != FREE) return;
+    } else if (FREE) {
+        return;
+    }
     if (eqn >= a.e.neq_mom) return;
-    Tags T = make_tags(a.g);
     Sink<SINK_COUNT> ra{0, nullptr, nullptr, 0.0}, rb{0, nullptr, nullptr, 0.0};
     double b = mom_rhs(a, eqn), F = A.F[eqn];
-    momentum_row<false, SINK_COUNT>(KIND, a, T, eqn, ra, rb, b, A.bbc[eqn], F);
+    if (FREE) momentum_row<false, SINK_COUNT>(KIND, a, FreeTags{}, eqn, ra, rb, b, A.bbc[eqn], F);
+    else momentum_row<false, SINK_COUNT>(KIND, a, make_tags(a.g), eqn, ra, rb, b, A.bbc[eqn], F);
     A.b[eqn] = b;
     A.F[eqn] = F;
+}
+
+/* free_row[eqn] = 1 when every tag the row reads, in the value and in the
+ * residual pass, is a free degree of freedom (-1) (the residual kernels
+ * use it; the value pass is checked too so that the flag describes the
+ * whole row).  Rows longer than the
+ * 32-entry scratch slot are never free (the host checks the pattern's
+ * longest row before the launch). */
+#define FREE_SLOT 33
+__global__ void __launch_bounds__(64) k_free_rows(int kind, DevAsm a, unsigned char *free_row) {
+    __shared__ double scr[64 * FREE_SLOT];
+    const int eqn = blockIdx.x * 64 + threadIdx.x;
+    if (eqn >= a.e.neq_mom) return;
+    CheckTags T;
+    static_cast<Tags &>(T) = make_tags(a.g);
+    T.all_free = true;
+    double *mine = scr + threadIdx.x * FREE_SLOT;
+    Sink<SINK_VALUES> ra{0, nullptr, mine, 0.0, mine + FREE_SLOT - 1}, rb{0, nullptr, mine, 0.0, mine + FREE_SLOT - 1};
+    double b = mom_rhs(a, eqn), F = 0.0;
+    momentum_row<true, SINK_VALUES>(kind, a, T, eqn, ra, rb, b, 0.0, F);
+    Sink<SINK_COUNT> rc{0, nullptr, nullptr, 0.0}, rd{0, nullptr, nullptr, 0.0};
+    double b2 = mom_rhs(a, eqn), F2 = 0.0;
+    momentum_row<false, SINK_COUNT>(kind, a, T, eqn, rc, rd, b2, 0.0, F2);
+    free_row[eqn] = T.all_free ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(BLK) k_residual_cont(int kind, DevAsm a, DevBlock C, DevBlock D) {
@@ -1295,8 +1344,23 @@ void mdz_fill_pattern(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBl
 
 template <int KIND, int VBLK, int CAPA, int CAPB>
 static void launch_staged(const DevAsm &a, DevBlock *A, DevBlock *B, hipStream_t s) {
+    /* the value kernels are bound by the staged stores, not by the tag
+     * reads: one instance serves free and boundary rows alike */
     hipLaunchKernelGGL((k_values_mom<KIND, VBLK, CAPA, CAPB>), dim3(mdz_div_up(a.e.neq_mom, VBLK)), dim3(VBLK), 0, s,
                        a, *A, *B);
+}
+
+int mdz_launch_free_rows(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, hipStream_t s) {
+    if (!A->free_row) return MDZ_OK;
+    const int nm = a.e.neq_mom;
+    if (nm <= 0) return MDZ_OK;
+    if (A->maxrow > FREE_SLOT - 1 || B->maxrow > FREE_SLOT - 1) {
+        /* rows beyond the scratch slot: every row takes the general kernels */
+        if (hipMemsetAsync(A->free_row, 0, nm, s) != hipSuccess) return MDZ_EHIP;
+        return MDZ_OK;
+    }
+    hipLaunchKernelGGL(k_free_rows, dim3(mdz_div_up(nm, 64)), dim3(64), 0, s, kind, a, A->free_row);
+    return MDZ_OK;
 }
 
 int mdz_shape_valid(const LaunchShape &sh) {
@@ -1328,8 +1392,14 @@ void mdz_launch_values(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevB
 void mdz_launch_residual(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, DevBlock *D, hipStream_t s) {
     int nm = a.e.neq_mom, ncnt = a.e.neq_cont;
     if (nm > 0) {
-        if (kind == 0) hipLaunchKernelGGL(k_residual_mom<0>, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, a, *A);
-        else hipLaunchKernelGGL(k_residual_mom<1>, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, a, *A);
+        const dim3 grid(mdz_div_up(nm, BLK));
+        if (kind == 0) {
+            if (A->free_row) hipLaunchKernelGGL((k_residual_mom<0, true>), grid, dim3(BLK), 0, s, a, *A);
+            hipLaunchKernelGGL((k_residual_mom<0, false>), grid, dim3(BLK), 0, s, a, *A);
+        } else {
+            if (A->free_row) hipLaunchKernelGGL((k_residual_mom<1, true>), grid, dim3(BLK), 0, s, a, *A);
+            hipLaunchKernelGGL((k_residual_mom<1, false>), grid, dim3(BLK), 0, s, a, *A);
+        }
     }
     if (ncnt > 0) hipLaunchKernelGGL(k_residual_cont, dim3(mdz_div_up(ncnt, BLK)), dim3(BLK), 0, s, kind, a, *C, *D);
 }
@@ -1449,25 +1519,25 @@ __global__ void __launch_bounds__(NRM_BLK) k_norms(DevAsm a, DevBlock A, DevBloc
     }
 }
 
-__global__ void __launch_bounds__(NRM_BLK) k_norms_final(const double *partial, int nbu, int nbv, int nbp, double *out6) {
-    int nb[3] = {nbu, nbv, nbp};
-    int off = 0;
-    for (int f = 0; f < 3; f++) {
-        double s = 0, c = 0;
-        for (int i = threadIdx.x; i < nb[f]; i += NRM_BLK) { s += partial[off + i]; c += partial[off + nb[f] + i]; }
-        __shared__ double ws[NRM_BLK / 64][2];
-        s = wave_sum(s);
-        c = wave_sum(c);
-        if ((threadIdx.x & 63) == 0) { ws[threadIdx.x >> 6][0] = s; ws[threadIdx.x >> 6][1] = c; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double ts = 0, tc = 0;
-            for (int q = 0; q < NRM_BLK / 64; q++) { ts += ws[q][0]; tc += ws[q][1]; }
-            out6[f] = ts;
-            out6[3 + f] = tc;
-        }
-        __syncthreads();
-        off += 2 * nb[f];
+/* one block per family (u, v, p): fixed summation order, so the norms are
+ * deterministic */
+#define NRM_FIN 1024
+__global__ void __launch_bounds__(NRM_FIN) k_norms_final(const double *partial, int nbu, int nbv, int nbp, double *out6) {
+    const int f = blockIdx.x;
+    const int nbf = f == 0 ? nbu : (f == 1 ? nbv : nbp);
+    const int off = f == 0 ? 0 : (f == 1 ? 2 * nbu : 2 * (nbu + nbv));
+    double s = 0, c = 0;
+    for (int i = threadIdx.x; i < nbf; i += NRM_FIN) { s += partial[off + i]; c += partial[off + nbf + i]; }
+    __shared__ double ws[NRM_FIN / 64][2];
+    s = wave_sum(s);
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) { ws[threadIdx.x >> 6][0] = s; ws[threadIdx.x >> 6][1] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ts = 0, tc = 0;
+        for (int q = 0; q < NRM_FIN / 64; q++) { ts += ws[q][0]; tc += ws[q][1]; }
+        out6[f] = ts;
+        out6[3 + f] = tc;
     }
 }
 
@@ -1480,5 +1550,5 @@ void mdz_launch_norms(const DevAsm &a, DevBlock *A, DevBlock *D, double *partial
     const DevGrid &g = a.g;
     int nbu = mdz_div_up(g.Nx * g.nzvx, NRM_BLK), nbv = mdz_div_up(g.nxvz * g.Nz, NRM_BLK), nbp = mdz_div_up(g.ncx * g.ncz, NRM_BLK);
     hipLaunchKernelGGL(k_norms, dim3(nbu + nbv + nbp), dim3(NRM_BLK), 0, s, a, *A, *D, partial, nbu, nbv, nbp);
-    hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(NRM_BLK), 0, s, (const double *)partial, nbu, nbv, nbp, out6);
+    hipLaunchKernelGGL(k_norms_final, dim3(3), dim3(NRM_FIN), 0, s, (const double *)partial, nbu, nbv, nbp, out6);
 }

@@ -96,6 +96,7 @@ struct DevBlock {
     int maxrow; /* largest row length of the pattern */
     int *Ic, *J;
     double *A, *b, *bbc, *F, *d;
+    unsigned char *free_row; /* momentum blocks: 1 = every tag the row reads is -1 (or null) */
 };
 
 /* Sinks through which one stencil row is emitted.  The emission order and

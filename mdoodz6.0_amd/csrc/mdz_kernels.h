@@ -61,6 +61,9 @@ int mdz_shape_valid(const LaunchShape &sh);
  * the LDS slots always takes the unstaged kernel) */
 void mdz_launch_values(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, int vblk,
                        hipStream_t s);
+/* free-row flags of the momentum rows of a block family (after the
+ * pattern; A->free_row allocated with neq_mom bytes) */
+int mdz_launch_free_rows(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, hipStream_t s);
 /* residual rows (Assemble = 0) through the Picard (kind 0) or Jacobian (kind 1) stencils;
  * A/C carry b, F, bbc; D carries F and d */
 void mdz_launch_residual(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, DevBlock *D, hipStream_t s);
