@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 
 # algorithmic HBM bytes per cell update of each stage (each listed array
 # element read or written once; FP64 = 8 B, tags 1 B, equation numbers 4 B)
-STAGES = ["strain_rates", "rheology", "rhs", "dtensor_picard", "picard_values", "dtensor_jacobian",
+STAGES = ["strain_rates", "interp_guess", "rheology", "rhs", "dtensor_picard", "picard_values", "dtensor_jacobian",
           "jacobian_values", "dtensor_picard_2", "residual"]
 # kernels of each stage (names as in the rocprofv3 kernel trace) for the
 # launch shape (rheology threads, Picard rows, Jacobian rows per block)
@@ -34,22 +34,26 @@ def stage_kernels(shape, learned=False):
     rh, pic, jac = shape
     vp = "k_values_mom<0, %d, 9, 2>" % pic if pic else "k_values_mom_direct<0>"
     vj = "k_values_mom<1, %d, 21, 6>" % jac if jac else "k_values_mom_direct<1>"
-    rheo = ["k_interp_c2v (5 fields)", "k_rheo_cells_fast<%d>" % rh, "k_rheo_vertices_fast<%d>" % rh]
-    if learned:
-        rheo += ["k_mlp_guess<true>", "k_mlp_guess<false>"]
+    prep = ["k_interp_c2v (5 fields)"] + (["k_mlp_guess<true>", "k_mlp_guess<false>"] if learned else [])
     return {"strain_rates": ["k_strain_cells", "k_strain_vertices", "k_exz_n", "k_interp_c2v (2 fields)"],
-            "rheology": rheo,
+            "interp_guess": prep,
+            "rheology": ["k_rheo_cells_fast<%d>" % rh, "k_rheo_vertices_fast<%d>" % rh],
             "rhs": ["k_rhs_u", "k_rhs_v", "k_rhs_p"],
             "dtensor_picard": ["k_dtensor_cells (J=0)", "k_dtensor_vertices (J=0)"],
             "picard_values": [vp, "k_values_cont"],
             "dtensor_jacobian": ["k_dtensor_cells (J=1)", "k_dtensor_vertices (J=1)"],
             "jacobian_values": [vj, "k_values_cont"],
             "dtensor_picard_2": ["k_dtensor_cells (J=0)", "k_dtensor_vertices (J=0)"],
-            "residual": ["k_residual_mom<0>", "k_residual_cont", "k_norms", "k_norms_final"]}
+            "residual": ["k_residual_mom<0, true>", "k_residual_mom<0, false>", "k_residual_cont", "k_norms",
+                         "k_norms_final"]}
 
 
-BYTES_PER_CELL = {"strain_rates": 76, "rheology": 810, "rhs": 160, "dtensor_picard": 114, "picard_values": 340,
-                  "dtensor_jacobian": 330, "jacobian_values": 668, "dtensor_picard_2": 114, "residual": 204}
+BYTES_PER_CELL = {"strain_rates": 76, "interp_guess": 80, "rheology": 730, "rhs": 160, "dtensor_picard": 114,
+                  "picard_values": 340, "dtensor_jacobian": 330, "jacobian_values": 668, "dtensor_picard_2": 114,
+                  "residual": 204}
+# learned mode: the network's guess reads 11 doubles, 3 phase fractions and
+# a tag and writes 3 floats per node, at one centroid and one vertex per cell
+GUESS_BYTES_PER_CELL = 2 * (11 * 8 + 3 * 8 + 1 + 3 * 4)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -279,7 +283,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--ncx", type=int, default=2049)
     ap.add_argument("--ncz", type=int, default=1025)
-    ap.add_argument("--mode", default="fp64", choices=["fp64", "mixed", "learned"])
+    ap.add_argument("--mode", default="learned", choices=["fp64", "mixed", "learned"],
+                    help="local iterations: plain FP64, FP32 + FP64 polish, or the network's guess + FP32 + FP64 "
+                         "polish (the plain FP64 unit is timed beside the other two)")
     ap.add_argument("--kernel", default="auto", choices=["auto", "generic"],
                     help="grid rheology kernel: log-space where it applies (auto) or the generic one")
     ap.add_argument("--seed", type=int, default=1)
@@ -347,19 +353,29 @@ def main():
         plain_ms = pm.value
         mdz.check(hot.mdz_ctx_set_rheology_mode(ctx, mode))
 
-    st = (C.c_double * 10)()
+    st = (C.c_double * 11)()
     mdz.check(hot.mdz_ctx_stage_times(ctx, 3, st))
     stage_ms = {s: st[i] for i, s in enumerate(STAGES)}
+    bpc = dict(BYTES_PER_CELL)
+    if mode == 2:
+        bpc["interp_guess"] += GUESS_BYTES_PER_CELL
     dom = max(STAGES, key=lambda s: stage_ms[s])
-    dom_bytes = BYTES_PER_CELL[dom] * cells
+    dom_bytes = bpc[dom] * cells
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
     kern = stage_kernels(tuple(shape), mode == 2)
-    traffic = traffic_from_profile(kern[dom], args.ncx, args.ncz, args.mode)
+    per_stage = {}
+    for s_ in STAGES:
+        t = traffic_from_profile(kern[s_], args.ncx, args.ncz, args.mode)
+        per_stage[s_] = {"ms": round(stage_ms[s_], 4), "bytes_per_cell": bpc[s_],
+                         "GBs": round(bpc[s_] * cells / (stage_ms[s_] * 1e-3) / 1e9, 1),
+                         "frac": round(bpc[s_] * cells / (stage_ms[s_] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic": round(t) if t is not None else None}
+    traffic = per_stage[dom]["traffic"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": round(traffic) if traffic is not None else None,
                 "stage": dom, "kernels": kern[dom], "algorithmic_bytes": dom_bytes,
-                "bytes_per_cell": BYTES_PER_CELL[dom], "launch_ms": round(stage_ms[dom], 4),
+                "bytes_per_cell": bpc[dom], "launch_ms": round(stage_ms[dom], 4),
                 "probe_GBs": round(probe.value, 1), "frac_of_probe": round(achieved / probe.value, 4)}
     whole = 2816.0 * cells / (ms_per_step * 1e-3) / 1e9
     hot.mdz_ctx_destroy(ctx)
@@ -391,7 +407,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "stages": per_stage,
         "whole_unit_algorithmic_GBs": round(whole, 1),
         "host_wall_ms_per_step": round(wall, 4),
         "residual_norms": list(res),

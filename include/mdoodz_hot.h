@@ -358,10 +358,12 @@ int  mdz_ctx_collect_histogram(mdz_ctx *ctx, int on);
  * pre-iterations (mixed / learned modes).  Resets the counters. */
 int  mdz_ctx_iteration_histogram(mdz_ctx *ctx, long long *hist192);
 /* Mean per-stage milliseconds over `reps` units, measured with HIP events
- * on the context's stream: ms[0..8] = strain rates, rheology, right-hand
- * sides, D(Picard), Picard values, D(Jacobian), Jacobian values,
- * D(Picard), residual + norms; ms[9] = whole unit. */
-int  mdz_ctx_stage_times(mdz_ctx *ctx, int reps, double *ms10);
+ * on the context's stream: ms[0..9] = strain rates, centroid-to-vertex
+ * interpolation (+ the network's guess in learned mode), local iterations
+ * (centroid and vertex rheology kernels), right-hand sides, D(Picard),
+ * Picard values, D(Jacobian), Jacobian values, D(Picard), residual + norms;
+ * ms[10] = whole unit. */
+int  mdz_ctx_stage_times(mdz_ctx *ctx, int reps, double *ms11);
 /* Mean milliseconds per unit over `reps` back-to-back units. */
 int  mdz_ctx_time_units(mdz_ctx *ctx, int reps, double *ms_per_unit);
 /* Achievable HBM bandwidth: `reps` streaming copies of `bytes` (read +

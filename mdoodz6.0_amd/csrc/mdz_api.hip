@@ -506,19 +506,30 @@ extern "C" int mdz_set_rheology_kernel(int which) {
     return MDZ_OK;
 }
 
-static int rheology_pass(mdz_ctx *c, int use_guess, hipStream_t s) {
-    /* vertex values of div_u, T, P, d and phi (NonNewtonianViscosityGrid :979-983) */
+/* first part of the rheology: vertex values of div_u, T, P, d and phi
+ * (NonNewtonianViscosityGrid :979-983), then the network's guess */
+static int rheology_prep(mdz_ctx *c, int use_guess, hipStream_t s) {
     mdz_launch_interp5(c->g, c->prm, s);
     if (use_guess) {
         if (!c->net_wp || !c->guess_n || !c->guess_s) return set_err(MDZ_EINVAL, "learned guess without a network");
         mdz_launch_mlp_guess(c->g, c->prm, DevMLPW{c->net_wp, c->net_bp}, c->guess_n, c->guess_s, s);
     }
+    return MDZ_OK;
+}
+
+/* second part: the local iterations at centroids and vertices */
+static int rheology_kernels(mdz_ctx *c, int use_guess, hipStream_t s) {
     const float *gn = use_guess ? c->guess_n : nullptr, *gs = use_guess ? c->guess_s : nullptr;
     if (g_rheo_kernel == 0 && mdz_rheo_fast_eligible(c->prm, &c->materials))
         return mdz_launch_rheology_fast(c->g, c->prm, c->d_pk, c->materials.R, gn, gs, c->hist_on ? c->hist : nullptr,
                                         c->shape.rheo, s);
     mdz_launch_rheology(c->g, c->prm, c->d_mat, gn, gs, c->hist_on ? c->hist : nullptr, s);
     return MDZ_OK;
+}
+
+static int rheology_pass(mdz_ctx *c, int use_guess, hipStream_t s) {
+    int rc = rheology_prep(c, use_guess, s);
+    return rc ? rc : rheology_kernels(c, use_guess, s);
 }
 
 extern "C" void NonNewtonianViscosityGrid(grid *mesh, mat_prop *materials, params *model, Nparams Nmodel, scale *scaling) {
@@ -1189,35 +1200,39 @@ extern "C" int mdz_ctx_set_network(mdz_ctx *c, const float *w, size_t n) {
  * D(0), residual + norms), ms[9] = mean total per unit. */
 extern "C" int mdz_ctx_stage_times(mdz_ctx *c, int reps, double *ms) {
     if (!c || reps < 1 || !ms) return set_err(MDZ_EINVAL, "stage times");
-    const int NS = 9;
+    const int NS = 10;
     hipEvent_t ev[NS + 1];
     for (int i = 0; i <= NS; i++) HIPCHK(hipEventCreate(&ev[i]));
     double acc[NS + 1] = {0};
     DevAsm a = make_asm(c);
     int kind = c->model.aniso == 1 ? 1 : 0;
+    const int learned = c->prm.rheo_mode == MDZ_RHEO_LEARNED;
+    hipStream_t s = c->stream;
     for (int r = 0; r < reps; r++) {
-        HIPCHK(hipEventRecord(ev[0], c->stream));
-        mdz_launch_strain_rates(c->g, c->prm, c->stream);
-        HIPCHK(hipEventRecord(ev[1], c->stream));
-        int rc = rheology_pass(c, c->prm.rheo_mode == MDZ_RHEO_LEARNED, c->stream);
-        if (rc) return rc;
-        HIPCHK(hipEventRecord(ev[2], c->stream));
-        mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, c->stream);
-        HIPCHK(hipEventRecord(ev[3], c->stream));
-        mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
-        HIPCHK(hipEventRecord(ev[4], c->stream));
-        mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, c->stream);
-        HIPCHK(hipEventRecord(ev[5], c->stream));
-        mdz_launch_dtensor(c->g, c->prm, 1, c->status, c->stream);
-        HIPCHK(hipEventRecord(ev[6], c->stream));
-        mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, c->stream);
-        HIPCHK(hipEventRecord(ev[7], c->stream));
-        mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
-        HIPCHK(hipEventRecord(ev[8], c->stream));
-        mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], c->stream);
-        mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, c->stream);
-        HIPCHK(hipEventRecord(ev[9], c->stream));
-        HIPCHK(hipEventSynchronize(ev[9]));
+        int rc;
+        HIPCHK(hipEventRecord(ev[0], s));
+        mdz_launch_strain_rates(c->g, c->prm, s);
+        HIPCHK(hipEventRecord(ev[1], s));
+        if ((rc = rheology_prep(c, learned, s))) return rc;
+        HIPCHK(hipEventRecord(ev[2], s));
+        if ((rc = rheology_kernels(c, learned, s))) return rc;
+        HIPCHK(hipEventRecord(ev[3], s));
+        mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, s);
+        HIPCHK(hipEventRecord(ev[4], s));
+        mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
+        HIPCHK(hipEventRecord(ev[5], s));
+        mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, s);
+        HIPCHK(hipEventRecord(ev[6], s));
+        mdz_launch_dtensor(c->g, c->prm, 1, c->status, s);
+        HIPCHK(hipEventRecord(ev[7], s));
+        mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s);
+        HIPCHK(hipEventRecord(ev[8], s));
+        mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
+        HIPCHK(hipEventRecord(ev[9], s));
+        mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], s);
+        mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, s);
+        HIPCHK(hipEventRecord(ev[10], s));
+        HIPCHK(hipEventSynchronize(ev[10]));
         for (int i = 0; i < NS; i++) {
             float t = 0;
             HIPCHK(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
@@ -1233,8 +1248,6 @@ extern "C" int mdz_ctx_stage_times(mdz_ctx *c, int reps, double *ms) {
     return MDZ_OK;
 }
 
-/* Times `reps` back-to-back Newton units (no host synchronisation inside)
- * between two events; returns the mean milliseconds per unit. */
 /* Streaming copy of n doubles (8-byte loads and stores, grid-stride): the
  * reference kernel for achievable HBM bandwidth and for scaling the
  * FETCH_SIZE / WRITE_SIZE counters. */
@@ -1271,6 +1284,8 @@ extern "C" int mdz_copy_probe(size_t bytes, int reps, double *gbs) {
     return MDZ_OK;
 }
 
+/* Times `reps` back-to-back Newton units (no host synchronisation inside)
+ * between two events; returns the mean milliseconds per unit. */
 extern "C" int mdz_ctx_time_units(mdz_ctx *c, int reps, double *ms_per_unit) {
     if (!c || reps < 1 || !ms_per_unit) return set_err(MDZ_EINVAL, "time units");
     hipEvent_t e0, e1;
