@@ -147,6 +147,36 @@ def cpu_baseline(mdz, ncx, ncz, seed, units, model_args=()):
     return ncx * ncz / t, orc.orc_num_threads(), t
 
 
+def at_abi_ms(mdz, hot, ctx, m, reps):
+    """Milliseconds per unit with the host transfers of the drop-in boundary:
+    upload of u, v, p, the unit, download of the values and right-hand
+    sides of the eight CSR blocks and of the residual vectors (pageable
+    host buffers)."""
+    import numpy as np
+
+    DP = C.POINTER(C.c_double)
+    nu, nw, nc = m.Nx * (m.Nz + 1), (m.Nx + 1) * m.Nz, (m.Nx - 1) * (m.Nz - 1)
+    u, v, p = np.zeros(nu), np.zeros(nw), np.zeros(nc)
+    mdz.check(hot.mdz_ctx_download_solution(ctx, u.ctypes.data_as(DP), v.ctypes.data_as(DP), p.ctypes.data_as(DP)))
+    bufs = []
+    for i in range(8):
+        neq, nnz = C.c_int(), C.c_int()
+        mdz.check(hot.mdz_ctx_block_info(ctx, i, C.byref(neq), C.byref(nnz)))
+        bufs.append((np.zeros(max(nnz.value, 1)), np.zeros(max(neq.value, 1))))
+    fm, fc = np.zeros(len(bufs[0][1])), np.zeros(len(bufs[3][1]))
+    times = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        mdz.check(hot.mdz_ctx_upload_solution(ctx, u.ctypes.data_as(DP), v.ctypes.data_as(DP), p.ctypes.data_as(DP)))
+        mdz.check(hot.mdz_ctx_newton_unit(ctx, None))
+        for i, (A, b) in enumerate(bufs):
+            mdz.check(hot.mdz_ctx_download_block(ctx, i, None, None, A.ctypes.data_as(DP), b.ctypes.data_as(DP), None))
+        mdz.check(hot.mdz_ctx_download_residual(ctx, fm.ctypes.data_as(DP), fc.ctypes.data_as(DP)))
+        if r:
+            times.append((time.perf_counter() - t0) * 1e3)
+    return sum(times) / len(times)
+
+
 def ensemble_params(n, seed):
     """Seeded draws of the ensemble: nx U{257..2049} cells, nz = nx / 2,
     friction U[10, 35] deg, cohesion U[5, 50] MPa, extension rate
@@ -243,7 +273,7 @@ def run_ensemble(args, mdz, hot):
                          "%d cells, %.2f s in total" % (len(used), done_cells, done_t)}
 
     line = {
-        "metric": "Newton-iter cell-updates/s (rheol+resid+assembly)",
+        "metric": "Newton-iter cell-updates/s (rheol+resid+assembly): learned+FP32 vs plain FP64",
         "value": round(value, 1),
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -326,7 +356,7 @@ def main():
         # block shapes picked by the autotuner (an ensemble of this one model)
         ens = C.c_void_p()
         mdz.check(hot.mdz_ensemble_create(C.byref(ens), (C.c_void_p * 1)(ctx.value), 1))
-        mdz.check(hot.mdz_ensemble_autotune(ens, 3, shape, None))
+        mdz.check(hot.mdz_ensemble_autotune(ens, 5, shape, None))
         hot.mdz_ensemble_destroy(ens)
     mdz.check(hot.mdz_ctx_get_launch_shape(ctx, shape))
 
@@ -345,6 +375,7 @@ def main():
     mdz.check(hot.mdz_copy_probe(args.probe_mb << 20, 10, C.byref(probe)))
 
     iters = iteration_stats(mdz, hot, ctx)
+    abi_ms = at_abi_ms(mdz, hot, ctx, m, 3)
     plain_ms = None
     if mode != 0:
         mdz.check(hot.mdz_ctx_set_rheology_mode(ctx, 0))
@@ -389,7 +420,7 @@ def main():
                          % (args.cpu_units, args.ncx, args.ncz, args.cpu_units, t)}
 
     line = {
-        "metric": "Newton-iter cell-updates/s (rheol+resid+assembly)",
+        "metric": "Newton-iter cell-updates/s (rheol+resid+assembly): learned+FP32 vs plain FP64",
         "value": round(value, 1),
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -409,6 +440,8 @@ def main():
         "cpu_baseline": cpu,
         "stages": per_stage,
         "whole_unit_algorithmic_GBs": round(whole, 1),
+        "at_abi_ms_per_step": round(abi_ms, 2),
+        "at_abi_value": round(cells / (abi_ms * 1e-3), 1),
         "host_wall_ms_per_step": round(wall, 4),
         "residual_norms": list(res),
         "local_iterations_mean": iters,

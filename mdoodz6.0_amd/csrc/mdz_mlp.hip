@@ -366,20 +366,42 @@ __global__ void __launch_bounds__(TBLK) k_mlp_train_step(const float *__restrict
     if (threadIdx.x == 0) loss[blockIdx.x] = sLoss[0] + sLoss[1] + sLoss[2] + sLoss[3];
 }
 
-/* g[p] = sum over blocks of partial[block][p] (fixed order); loss_acc +=
- * sum of the block losses */
-__global__ void k_mlp_reduce(const float *__restrict__ partial, int nblocks, float *__restrict__ g,
-                             const float *__restrict__ loss, double *loss_acc) {
-    const int p = blockIdx.x * 256 + threadIdx.x;
+/* g[p] = sum over blocks of partial[block][p]; loss_acc += sum of the
+ * block losses.  A block reduces 64 parameters: its four waves take every
+ * fourth partial (four independent accumulators each), then the slices are
+ * added in a fixed order, so the result does not depend on scheduling. */
+#define RED_P 64
+__global__ void __launch_bounds__(256) k_mlp_reduce(const float *__restrict__ partial, int nblocks,
+                                                    float *__restrict__ g, const float *__restrict__ loss,
+                                                    double *loss_acc) {
+    __shared__ float sp[4][RED_P];
+    const int pl = threadIdx.x & (RED_P - 1), sl = threadIdx.x >> 6;
+    const int p = blockIdx.x * RED_P + pl;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
     if (p < MLP_NPARAMS) {
-        float acc = 0.0f;
-        for (int b = 0; b < nblocks; b++) acc += partial[(size_t)b * MLP_NPARAMS + p];
-        g[p] = acc;
+        int b = sl;
+        for (; b + 12 < nblocks; b += 16) {
+            a0 += partial[(size_t)b * MLP_NPARAMS + p];
+            a1 += partial[(size_t)(b + 4) * MLP_NPARAMS + p];
+            a2 += partial[(size_t)(b + 8) * MLP_NPARAMS + p];
+            a3 += partial[(size_t)(b + 12) * MLP_NPARAMS + p];
+        }
+        for (; b < nblocks; b += 4) a0 += partial[(size_t)b * MLP_NPARAMS + p];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0 && loss_acc) {
+    sp[sl][pl] = (a0 + a1) + (a2 + a3);
+    __syncthreads();
+    if (sl == 0 && p < MLP_NPARAMS) g[p] = (sp[0][pl] + sp[1][pl]) + (sp[2][pl] + sp[3][pl]);
+    if (blockIdx.x == 0 && loss_acc) {
+        __shared__ double sl_[256];
         double s = 0.0;
-        for (int b = 0; b < nblocks; b++) s += loss[b];
-        *loss_acc += s;
+        for (int b = threadIdx.x; b < nblocks; b += 256) s += loss[b];
+        sl_[threadIdx.x] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int i = 0; i < 256; i++) t += sl_[i];
+            *loss_acc += t;
+        }
     }
 }
 
@@ -794,7 +816,7 @@ static int mlp_gradient(mdz_mlp *n, long long first, int count, unsigned long lo
     DevMLPW w{n->wp, n->bp};
     hipLaunchKernelGGL(k_mlp_train_step, dim3(nb), dim3(TBLK), 0, n->s, (const float *)n->X, (const float *)n->Y, n->n,
                        first, mul, add, 1.0f / (float)count, w, n->partial, n->loss);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3(mdz_div_up(MLP_NPARAMS, 256)), dim3(256), 0, n->s, (const float *)n->partial,
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(mdz_div_up(MLP_NPARAMS, RED_P)), dim3(256), 0, n->s, (const float *)n->partial,
                        nb, n->g, (const float *)n->loss, loss_acc);
     MCHK(hipGetLastError());
     return MDZ_OK;
