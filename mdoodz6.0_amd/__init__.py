@@ -93,6 +93,8 @@ def lib_model():
         L = C.CDLL(LIB_MODEL)
         L.mdz_model_inclusion.restype = C.POINTER(mdz_model)
         L.mdz_model_inclusion.argtypes = [C.c_int, C.c_int, C.c_ulonglong]
+        L.mdz_model_pwl_inclusion.restype = C.POINTER(mdz_model)
+        L.mdz_model_pwl_inclusion.argtypes = [C.c_int, C.c_int]
         L.mdz_model_lithosphere.restype = C.POINTER(mdz_model)
         L.mdz_model_lithosphere.argtypes = [C.c_int, C.c_int, C.c_ulonglong, C.c_double, C.c_double, C.c_double]
         L.mdz_model_edgecase.restype = C.POINTER(mdz_model)
@@ -228,6 +230,10 @@ class Model:
     @classmethod
     def inclusion(cls, ncx=101, ncz=101, seed=1):
         return cls(lib_model().mdz_model_inclusion(ncx, ncz, seed))
+
+    @classmethod
+    def pwl_inclusion(cls, ncx=100, ncz=100):
+        return cls(lib_model().mdz_model_pwl_inclusion(ncx, ncz))
 
     @classmethod
     def lithosphere(cls, ncx=1001, ncz=501, seed=1, friction_deg=-1.0, cohesion=-1.0, rate=-1.0):

@@ -511,6 +511,26 @@ static void fill_common_fields(mdz_model *m) {
 /* Set-ups                                                              */
 /* ------------------------------------------------------------------ */
 
+/* GridRoutines.c:530 SetUpModel_NoMarkers geometry: phase 1 inside the disc
+ * of radius r centred at (0, z0), binary fractions at centroids and vertices */
+static void disc_fractions(grid *g, double r, double z0) {
+    const int Nx = g->Nx, Nz = g->Nz;
+    for (int l = 0; l < Nz - 1; l++)
+        for (int k = 0; k < Nx - 1; k++) {
+            int c = k + l * (Nx - 1);
+            double x = g->xc_coord[k], z = g->zc_coord[l] - z0;
+            int inc = x * x + z * z < r * r;
+            g->phase_perc_n[0][c] = inc ? 0.0 : 1.0; g->phase_perc_n[1][c] = inc ? 1.0 : 0.0;
+        }
+    for (int l = 0; l < Nz; l++)
+        for (int k = 0; k < Nx; k++) {
+            int c = k + l * Nx;
+            double x = g->xg_coord[k], z = g->zg_coord[l];
+            int inc = x * x + z * z < r * r;
+            g->phase_perc_s[0][c] = inc ? 0.0 : 1.0; g->phase_perc_s[1][c] = inc ? 1.0 : 0.0;
+        }
+}
+
 mdz_model *mdz_model_inclusion(int ncx, int ncz, unsigned long long seed) {
     int Nx = ncx + 1, Nz = ncz + 1;
     mdz_model *m = model_new(Nx, Nz, 2, seed, MDZ_BC_PURE_SHEAR);
@@ -533,23 +553,8 @@ mdz_model *mdz_model_inclusion(int ncx, int ncz, unsigned long long seed) {
     mt->eta0[0] = 1e2; mt->eta0[1] = 1e-3;
     mt->R = RGAS / (s->J / s->T);
     set_coords(g, p);
-    /* GridRoutines.c:530 SetUpModel_NoMarkers geometry (binary fractions) */
-    double r = p->user1 / s->L, z0 = 0.5 * (p->zmax + p->zmin);
-    for (int l = 0; l < Nz - 1; l++)
-        for (int k = 0; k < Nx - 1; k++) {
-            int c = k + l * (Nx - 1);
-            double x = g->xc_coord[k], z = g->zc_coord[l] - z0;
-            int inc = x * x + z * z < r * r;
-            g->phase_perc_n[0][c] = inc ? 0.0 : 1.0; g->phase_perc_n[1][c] = inc ? 1.0 : 0.0;
-            g->T[c] = 0.05;
-        }
-    for (int l = 0; l < Nz; l++)
-        for (int k = 0; k < Nx; k++) {
-            int c = k + l * Nx;
-            double x = g->xg_coord[k], z = g->zg_coord[l];
-            int inc = x * x + z * z < r * r;
-            g->phase_perc_s[0][c] = inc ? 0.0 : 1.0; g->phase_perc_s[1][c] = inc ? 1.0 : 0.0;
-        }
+    disc_fractions(g, p->user1 / s->L, 0.5 * (p->zmax + p->zmin));
+    for (int c = 0; c < (Nx - 1) * (Nz - 1); c++) g->T[c] = 0.05;
     fill_common_fields(m);
     /* pressure N(0, 0.1) */
     for (int c = 0; c < (Nx - 1) * (Nz - 1); c++) { g->p_in[c] = 0.1 * nrand(seed, 3, c); g->p0_n[c] = g->p_in[c]; }
@@ -568,6 +573,57 @@ mdz_model *mdz_model_inclusion(int ncx, int ncz, unsigned long long seed) {
             g->v_in[c] = g->zg_coord[l] * p->EpsBG;
             if (g->BCv.type[c] == -1) g->v_in[c] += 1e-3 * (2.0 * urand(seed, 2, c) - 1.0);
         }
+    force_dirichlet(g);
+    return m;
+}
+
+/* The reference's power-law pure-shear benchmark
+ * (BENCHMARKS_old_input/PureShear_pwl.txt, set_PureShear_pwl.c): box
+ * [-0.5, 0.5]^2, all scales 1 (T scale 20), viscous only; matrix 'homemade'
+ * power law eta = eta0 Eii^(1/n - 1) with eta0 = 1, n = 30, inclusion of
+ * radius 0.05 with eta0 = 10, n = 3 (FlowLaws.c:57-69: A = eta0^-n, no
+ * correction factor); no plasticity (C = 1e90); mineta 1e-3.  Initial
+ * iterate as GridRoutines.c:316 InitialiseSolutionFields: the background
+ * pure shear, p = 0. */
+mdz_model *mdz_model_pwl_inclusion(int ncx, int ncz) {
+    int Nx = ncx + 1, Nz = ncz + 1;
+    mdz_model *m = model_new(Nx, Nz, 2, 1, MDZ_BC_PURE_SHEAR);
+    grid *g = m->mesh;
+    scale *s = &m->scaling;
+    s->eta = 1.0; s->L = 1.0; s->V = 1.0; s->T = 20.0;
+    scale_me(s);
+    params *p = &m->model;
+    common_params(p, Nx, Nz, -0.5, 0.5, -0.5, 0.5);
+    p->dt = 0.5; p->dt0 = p->dt; p->EpsBG = 1.0; p->gx = 0.0; p->gz = 0.0;
+    p->iselastic = 0;
+    p->Nb_phases = 2; p->mineta = 1e-3 / s->eta; p->maxeta = 1e24 / s->eta; p->user1 = 0.05;
+    mat_prop *mt = &m->materials;
+    memset(mt, 0, sizeof(*mt));
+    mt->Nb_phases = 2;
+    const double eta0[2] = {1.0, 10.0}, n[2] = {30.0, 3.0};
+    for (int k = 0; k < 2; k++) {
+        phase_defaults(mt, k, s);
+        mt->mu[k] = 1.0; mt->rho[k] = 2700.0; mt->C[k] = 1e90; mt->phi[k] = 30.0 * M_PI / 180.0; mt->psi[k] = 0.0;
+        mt->Slim[k] = 500e9; mt->bet[k] = 1e-11;
+        mt->eta0[k] = eta0[k] / s->eta;
+        mt->pwlv[k] = 1;
+        mt->tpwl[k] = 0.0; mt->npwl[k] = n[k]; mt->mpwl[k] = 0.0; mt->rpwl[k] = 0.0;
+        mt->Qpwl[k] = 0.0; mt->Vpwl[k] = 0.0; mt->fpwl[k] = 0.0; mt->apwl[k] = 0.0;
+        mt->Apwl[k] = pow(eta0[k], -n[k]) / (pow(s->S, -n[k]) / s->t);
+        mt->Fpwl[k] = 1.0;
+    }
+    mt->R = RGAS / (s->J / s->T);
+    set_coords(g, p);
+    disc_fractions(g, p->user1 / s->L, 0.5 * (p->zmax + p->zmin));
+    for (int c = 0; c < (Nx - 1) * (Nz - 1); c++) g->T[c] = (1.0 + ZEROC) / s->T;
+    fill_common_fields(m);
+    for (int c = 0; c < (Nx - 1) * (Nz - 1); c++) { g->p_in[c] = 0.0; g->p0_n[c] = 0.0; }
+    c2v(g, g->p0_n, g->p0_s);
+    set_bcs(g, p, MDZ_BC_PURE_SHEAR, 1.0, 1.0);
+    for (int l = 0; l < Nz + 1; l++)
+        for (int k = 0; k < Nx; k++) g->u_in[k + l * Nx] = -g->xg_coord[k] * p->EpsBG;
+    for (int l = 0; l < Nz; l++)
+        for (int k = 0; k < Nx + 1; k++) g->v_in[k + l * (Nx + 1)] = g->zg_coord[l] * p->EpsBG;
     force_dirichlet(g);
     return m;
 }

@@ -59,6 +59,9 @@ int         mdz_grid_field_role(int i);
 /* Seeded set-ups.  ncx, ncz = cell counts (Nx = ncx+1, Nz = ncz+1). */
 /* 2-D pure-shear visco-elasto-plastic inclusion, nondimensional. */
 mdz_model *mdz_model_inclusion(int ncx, int ncz, unsigned long long seed);
+/* The reference's power-law pure-shear benchmark (PureShear_pwl): matrix
+ * n = 30, inclusion n = 3, viscous, background pure-shear initial iterate. */
+mdz_model *mdz_model_pwl_inclusion(int ncx, int ncz);
 /* Three-layer lithosphere (wet quartzite / plagioclase / olivine + Peierls),
  * 1000 x 250 km, dimensional inputs scaled as LithoScale (eta 1e22, L 1e4,
  * V 1e-9, T 100).  friction/cohesion/rate < 0 take the defaults

@@ -1,0 +1,154 @@
+"""Outer Newton iterations around the grid path, for tests: the path
+(oracle or GPU) assembles the Jacobian blocks and the residual of the current
+iterate; the linear solve is scipy's sparse LU on the coupled system
+[JA JB; JC JD] (the reference hands the same blocks to CHOLMOD and its
+Powell-Hestenes/GCR solver, Solvers.c:2226-2751); the step length is chosen
+as LineSearchDecoupled chooses it (StokesRoutines.c:396-622) and the stop is
+the reference's (Main_DOODZ.c:830).  Test infrastructure only."""
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from helpers import blocks, free_blocks, numbering
+
+IP, DP = C.POINTER(C.c_int), C.POINTER(C.c_double)
+
+
+def _arr(ptr, n, dtype=np.float64):
+    return np.ctypeslib.as_array(ptr, shape=(n,))
+
+
+class Iterate:
+    """The model's u_in, v_in, p_in (views) and the equation maps."""
+
+    def __init__(self, mdz, orc, model):
+        self.m = model
+        g = model.m.mesh.contents
+        Nx, Nz = g.Nx, g.Nz
+        self.nu, self.nw, self.nc = Nx * (Nz + 1), (Nx + 1) * Nz, (Nx - 1) * (Nz - 1)
+        self.u, self.v, self.p = _arr(g.u_in, self.nu), _arr(g.v_in, self.nw), _arr(g.p_in, self.nc)
+        S = numbering(orc, mdz, model)
+        self.neq_mom, self.neq_cont = S.neq_mom, S.neq_cont
+        self.eu = _arr(S.eqn_u, self.nu, np.int32).copy()
+        self.ev = _arr(S.eqn_v, self.nw, np.int32).copy()
+        self.ep = _arr(S.eqn_p, self.nc, np.int32).copy()
+        orc.orc_sfree(C.byref(S))
+
+    def state(self):
+        return self.u.copy(), self.v.copy(), self.p.copy()
+
+    def set_state(self, st):
+        self.u[:], self.v[:], self.p[:] = st
+
+    def add(self, dx, alpha):
+        mu, mv, mp = self.eu >= 0, self.ev >= 0, self.ep >= 0
+        self.u[mu] += alpha * dx[self.eu[mu]]
+        self.v[mv] += alpha * dx[self.ev[mv]]
+        self.p[mp] += alpha * dx[self.ep[mp]]
+
+
+def coupled(neq_mom, neq_cont, csr4):
+    """[A B; C D] from four (Ic, J, A) triples; a small negative diagonal in
+    the pressure block removes the constant-pressure null space of closed
+    boxes (the step's pressure mean is then the previous one)."""
+    (a, b, c, d) = [sp.csr_matrix((A, J, Ic), shape=shape) for (Ic, J, A), shape in
+                    zip(csr4, [(neq_mom, neq_mom), (neq_mom, neq_cont), (neq_cont, neq_mom), (neq_cont, neq_cont)])]
+    eps = 1e-12 * abs(a.diagonal()).max()
+    d = d - eps * sp.identity(neq_cont, format="csr")
+    return sp.bmat([[a, b], [c, d]], format="csc")
+
+
+class OracleAssembler:
+    def __init__(self, mdz, orc, model, it):
+        self.mdz, self.orc, self.m, self.it = mdz, orc, model, it
+
+    def __call__(self):
+        mm = self.m.m
+        S = numbering(self.orc, self.mdz, self.m)
+        bo = blocks(self.orc, self.mdz, S, 8)
+        self.orc.orc_newton_unit(mm.mesh, C.byref(mm.materials), C.byref(mm.model), C.byref(mm.Nmodel),
+                                 C.byref(mm.scaling), C.byref(S), *[C.byref(b) for b in bo])
+        csr4 = [self.mdz.csr_arrays(bo[i])[:3] for i in (4, 5, 6, 7)]
+        F = np.concatenate([_arr(bo[0].F, self.it.neq_mom).copy(), _arr(bo[3].F, self.it.neq_cont).copy()])
+        res = (mm.Nmodel.resx, mm.Nmodel.resz, mm.Nmodel.resp)
+        free_blocks(self.orc, bo)
+        self.orc.orc_sfree(C.byref(S))
+        return csr4, F, res
+
+
+class GpuAssembler:
+    def __init__(self, mdz, hot, model, it, mode=0):
+        self.mdz, self.hot, self.m, self.it = mdz, hot, model, it
+        self.ctx = C.c_void_p()
+        mm = model.m
+        mdz.check(hot.mdz_ctx_create(C.byref(self.ctx), mm.mesh, C.byref(mm.materials), C.byref(mm.model),
+                                     C.byref(mm.scaling)))
+        mdz.check(hot.mdz_ctx_set_rheology_mode(self.ctx, mode))
+
+    def __call__(self):
+        it, hot, mdz = self.it, self.hot, self.mdz
+        mdz.check(hot.mdz_ctx_upload_solution(self.ctx, it.u.ctypes.data_as(DP), it.v.ctypes.data_as(DP),
+                                              it.p.ctypes.data_as(DP)))
+        res = (C.c_double * 3)()
+        mdz.check(hot.mdz_ctx_newton_unit(self.ctx, res))
+        csr4 = []
+        for i in (4, 5, 6, 7):
+            neq, nnz = C.c_int(), C.c_int()
+            mdz.check(hot.mdz_ctx_block_info(self.ctx, i, C.byref(neq), C.byref(nnz)))
+            Ic = np.zeros(neq.value + 1, np.int32)
+            J = np.zeros(max(nnz.value, 1), np.int32)
+            A = np.zeros(max(nnz.value, 1))
+            mdz.check(hot.mdz_ctx_download_block(self.ctx, i, Ic.ctypes.data_as(IP), J.ctypes.data_as(IP),
+                                                 A.ctypes.data_as(DP), None, None))
+            csr4.append((Ic, J[:nnz.value], A[:nnz.value]))
+        Fm, Fc = np.zeros(it.neq_mom), np.zeros(max(it.neq_cont, 1))
+        mdz.check(hot.mdz_ctx_download_residual(self.ctx, Fm.ctypes.data_as(DP), Fc.ctypes.data_as(DP)))
+        return csr4, np.concatenate([Fm, Fc[:it.neq_cont]]), tuple(res)
+
+    def close(self):
+        self.hot.mdz_ctx_destroy(self.ctx)
+
+
+def _norm(res):
+    return float(np.sqrt(res[0] ** 2 + res[1] ** 2 + res[2] ** 2))
+
+
+def converged(res, res0, abs_tol, rel_tol):
+    """Main_DOODZ.c:830: each of resx, resz, resp below the absolute
+    tolerance or below the relative one times its first value."""
+    return all(r < abs_tol or r / r0 < rel_tol for r, r0 in zip(res, res0))
+
+
+def newton(it, assemble, abs_tol=1e-6, rel_tol=1e-6, max_it=25):
+    """Returns (number of Newton updates, history of (resx, resz, resp)).
+    Each step tries alpha = 0.2, 0.4, ..., 1 of the Newton correction and
+    keeps the one with the smallest sqrt(resx^2 + resz^2 + resp^2), as
+    LineSearchDecoupled does for Newton (StokesRoutines.c:411-416, :509-561);
+    a step that lowers nothing ends the iterations (stagnation)."""
+    csr4, F, res = assemble()
+    hist = [res]
+    n = 0
+    while not converged(res, hist[0], abs_tol, rel_tol) and n < max_it:
+        K = coupled(it.neq_mom, it.neq_cont, csr4)
+        dx = spla.spsolve(K, -F)
+        st = it.state()
+        best = None
+        for alpha in (0.2, 0.4, 0.6, 0.8, 1.0):
+            it.set_state(st)
+            it.add(dx, alpha)
+            trial = assemble()
+            if best is None or _norm(trial[2]) < best[0]:
+                best = (_norm(trial[2]), alpha, trial)
+        r, alpha, (csr4, F, res) = best
+        if r >= _norm(hist[-1]):
+            it.set_state(st)
+            break
+        if alpha != 1.0:
+            it.set_state(st)
+            it.add(dx, alpha)
+            csr4, F, res = assemble()
+        n += 1
+        hist.append(res)
+    return n, hist

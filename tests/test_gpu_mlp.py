@@ -184,3 +184,68 @@ def test_learned_guess_in_the_newton_unit(mdz, orc, hot, net, litho):
     for b in bo:
         orc.orc_free_block(C.byref(b))
     mo_.free()
+
+
+def test_error_per_field_by_precision_mode(mdz, orc, hot, net):
+    """The error of each precision mode against the FP64 oracle, field by
+    field, on the 256x128 lithosphere: plain FP64, FP32 iterations + FP64
+    polish, and the same from the network's guess.  Bounds: eta, tau_II and
+    the tangent 1e-10 relative (elementwise, floor 1e-8 of the field's scale,
+    1e-6 for the sign-changing tangent), CSR values 1e-10 relative to the
+    row's largest entry, residual norms 1e-10 relative."""
+    from helpers import numbering, blocks, relerr, row_relerr
+    mdz.check(hot.mdz_mlp_generate(net, 1 << 22, 31))
+    mdz.check(hot.mdz_mlp_init(net, 7))
+    hist = (C.c_double * 10)()
+    mdz.check(hot.mdz_mlp_train(net, 10, 65536, 1e-3, 5, hist))
+    w = _weights(mdz, hot, net)
+    mref = mdz.Model.lithosphere(256, 128, 9)
+    S = numbering(orc, mdz, mref)
+    bo = blocks(orc, mdz, S, 8)
+    orc.orc_newton_unit(mref.m.mesh, C.byref(mref.m.materials), C.byref(mref.m.model), C.byref(mref.m.Nmodel),
+                        C.byref(mref.m.scaling), C.byref(S), *[C.byref(b) for b in bo])
+    tau_o = np.hypot(mref.field("sxxd"), mref.field("sxz_n"))
+    rows = []
+    try:
+        for mode, name in ((0, "fp64"), (1, "mixed"), (2, "learned")):
+            mg = mref.clone()
+            ctx = C.c_void_p()
+            mdz.check(hot.mdz_ctx_create(C.byref(ctx), mg.m.mesh, C.byref(mg.m.materials), C.byref(mg.m.model),
+                                         C.byref(mg.m.scaling)))
+            try:
+                mdz.check(hot.mdz_ctx_set_network(ctx, w.ctypes.data_as(FP), 9090))
+                mdz.check(hot.mdz_ctx_set_rheology_mode(ctx, mode))
+                res = (C.c_double * 3)()
+                mdz.check(hot.mdz_ctx_newton_unit(ctx, res))
+                mdz.check(hot.mdz_ctx_download_grid(ctx, mg.m.mesh))
+                e = {"eta_n": relerr(mg.field("eta_n"), mref.field("eta_n"), 1e-8),
+                     "eta_s": relerr(mg.field("eta_s"), mref.field("eta_s"), 1e-8),
+                     "tau_II": relerr(np.hypot(mg.field("sxxd"), mg.field("sxz_n")), tau_o, 1e-8),
+                     "detadexx_n": relerr(mg.field("detadexx_n"), mref.field("detadexx_n"), 1e-6),
+                     "D11_n": relerr(mg.field("D11_n"), mref.field("D11_n"), 1e-8)}
+                for i, lab in ((0, "picard_A"), (4, "jacobian_A"), (5, "jacobian_B")):
+                    neq, nnz = C.c_int(), C.c_int()
+                    mdz.check(hot.mdz_ctx_block_info(ctx, i, C.byref(neq), C.byref(nnz)))
+                    A = np.zeros(max(nnz.value, 1))
+                    mdz.check(hot.mdz_ctx_download_block(ctx, i, None, None, A.ctypes.data_as(C.POINTER(C.c_double)),
+                                                         None, None))
+                    Io, _, Ao, _, _ = mdz.csr_arrays(bo[i])
+                    e[lab] = row_relerr(Io, A[:nnz.value], Ao)
+                for k, v in zip(("resx", "resz", "resp"), res):
+                    ref = getattr(mref.m.Nmodel, k)
+                    e[k] = abs(v - ref) / abs(ref)
+                rows.append((name, e))
+            finally:
+                hot.mdz_ctx_destroy(ctx)
+                mg.free()
+    finally:
+        for b in bo:
+            orc.orc_free_block(C.byref(b))
+        mref.free()
+    keys = list(rows[0][1])
+    print("\n| mode | " + " | ".join(keys) + " |")
+    for name, e in rows:
+        print("| %s | " % name + " | ".join("%.1e" % e[k] for k in keys) + " |")
+    for name, e in rows:
+        for k, v in e.items():
+            assert v <= 1e-10, (name, k, v)
