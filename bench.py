@@ -177,6 +177,27 @@ def at_abi_ms(mdz, hot, ctx, m, reps):
     return sum(times) / len(times)
 
 
+def line_search_ms(mdz, hot, ctx, reps):
+    """Milliseconds per call of the five-trial line search (alpha = 0.2 ..
+    1.0 of a zero correction, i.e. at the current iterate), host transfers
+    of the correction included."""
+    import numpy as np
+
+    na, nc = C.c_int(), C.c_int()
+    mdz.check(hot.mdz_ctx_block_info(ctx, 0, C.byref(na), None))
+    mdz.check(hot.mdz_ctx_block_info(ctx, 3, C.byref(nc), None))
+    dx = np.zeros(na.value + nc.value)
+    a = np.array([0.2, 0.4, 0.6, 0.8, 1.0])
+    out = np.zeros(15)
+    DP = C.POINTER(C.c_double)
+    args = (ctx, dx.ctypes.data_as(DP), 5, a.ctypes.data_as(DP), out.ctypes.data_as(DP))
+    mdz.check(hot.mdz_ctx_line_search(*args))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mdz.check(hot.mdz_ctx_line_search(*args))
+    return (time.perf_counter() - t0) / reps * 1e3
+
+
 def ensemble_params(n, seed):
     """Seeded draws of the ensemble: nx U{257..2049} cells, nz = nx / 2,
     friction U[10, 35] deg, cohesion U[5, 50] MPa, extension rate
@@ -376,6 +397,7 @@ def main():
 
     iters = iteration_stats(mdz, hot, ctx)
     abi_ms = at_abi_ms(mdz, hot, ctx, m, 3)
+    ls_ms = line_search_ms(mdz, hot, ctx, 5)
     plain_ms = None
     if mode != 0:
         mdz.check(hot.mdz_ctx_set_rheology_mode(ctx, 0))
@@ -442,6 +464,7 @@ def main():
         "whole_unit_algorithmic_GBs": round(whole, 1),
         "at_abi_ms_per_step": round(abi_ms, 2),
         "at_abi_value": round(cells / (abi_ms * 1e-3), 1),
+        "line_search_5_trials_ms": round(ls_ms, 3),
         "host_wall_ms_per_step": round(wall, 4),
         "residual_norms": list(res),
         "local_iterations_mean": iters,

@@ -372,6 +372,15 @@ int  mdz_ctx_time_units(mdz_ctx *ctx, int reps, double *ms_per_unit);
 int  mdz_copy_probe(size_t bytes, int reps, double *gbs);
 /* The HIP stream (hipStream_t) the context launches on. */
 void *mdz_ctx_stream(mdz_ctx *ctx);
+/* The line search of the Newton step (LineSearchDecoupled,
+ * StokesRoutines.c:396-622) as one graph launch on the resident fields: for
+ * each of the ntry (1..16) step lengths alpha[k], the iterate u + alpha[k] dx
+ * (dx indexed by equation number, neq_mom + neq_cont values, on the host)
+ * goes through UpdateNonLinearity and EvaluateStokesResidualDecoupled;
+ * res3n[3k .. 3k+2] = resx, resz, resp.  The iterate u, v, p is restored;
+ * the other fields hold the last trial's values, as in the reference.  Needs
+ * a unit (assembled blocks) before. */
+int  mdz_ctx_line_search(mdz_ctx *ctx, const double *dx, int ntry, const double *alpha, double *res3n);
 /* Block shapes of the tunable kernels of a context (no counterpart in the
  * reference): threads per block of the rheology kernels (64, 128, 256),
  * rows per block of the LDS-staged Picard (0 = unstaged, 64, 128, 256) and

@@ -132,6 +132,13 @@ struct mdz_ctx {
     float *net_bp;
     LaunchShape shape;   /* block shapes of the tunable kernels (set by the ensemble autotuner) */
     unsigned version;    /* bumped by every setter that changes what a unit launches */
+    /* line-search batch: saved iterate, correction, step lengths, norms per
+     * trial, and the captured graph of the trials */
+    double *ls_base, *ls_dx, *ls_alpha, *ls_res;
+    hipGraph_t ls_graph;
+    hipGraphExec_t ls_exec;
+    int ls_ntry, ls_kernel;
+    unsigned ls_version;
 };
 
 static void fill_params(mdz_ctx *c, const params *m, const scale *s) {
@@ -268,6 +275,9 @@ static void ctx_release(mdz_ctx *c) {
     hipFree(c->slab); hipFree(c->scratch); hipFree(c->norm_partial); hipFree(c->norm_out);
     hipFree(c->status); hipFree(c->hist); hipFree(c->d_mat); hipFree(c->d_pk); hipFree(c->guess_n); hipFree(c->guess_s);
     hipFree(c->d_net); hipFree(c->net_wp); hipFree(c->net_bp);
+    hipFree(c->ls_base); hipFree(c->ls_dx); hipFree(c->ls_alpha); hipFree(c->ls_res);
+    if (c->ls_exec) hipGraphExecDestroy(c->ls_exec);
+    if (c->ls_graph) hipGraphDestroy(c->ls_graph);
     hipStreamDestroy(c->stream);
     free(c);
 }
@@ -1304,6 +1314,104 @@ extern "C" int mdz_ctx_time_units(mdz_ctx *c, int reps, double *ms_per_unit) {
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     return MDZ_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* line search of the Newton step as one graph launch                   */
+/* ------------------------------------------------------------------ */
+#define MDZ_LS_MAX 16
+
+/* iterate = saved iterate + alpha[k] * dx on every node with an equation
+ * (StokesRoutines.c:468-485; nodes of type -12 share their twin's equation,
+ * which is what ApplyBC copies into them); k < 0 restores the saved iterate */
+__global__ void k_line_step(DevEqn e, int nu, int nw, int nc, const double *__restrict__ base,
+                            const double *__restrict__ dx, const double *__restrict__ alpha, int k, double *u,
+                            double *v, double *p) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nu + nw + nc) return;
+    const double a = k >= 0 ? alpha[k] : 0.0;
+    int eq;
+    double *dst;
+    if (i < nu) { eq = e.eqn_u[i]; dst = u + i; }
+    else if (i < nu + nw) { eq = e.eqn_v[i - nu]; dst = v + (i - nu); }
+    else { eq = e.eqn_p[i - nu - nw]; dst = p + (i - nu - nw); }
+    *dst = (k >= 0 && eq >= 0) ? base[i] + a * dx[eq] : base[i];
+}
+
+__global__ void k_copy6(const double *__restrict__ src, double *__restrict__ dst) {
+    if (threadIdx.x < 6) dst[threadIdx.x] = src[threadIdx.x];
+}
+
+/* one trial: UpdateNonLinearity (StokesRoutines.c:137-157) then the residual
+ * and its norms (EvaluateStokesResidualDecoupled), norms into ls_res[6k..] */
+static int enqueue_trial(mdz_ctx *c, int k, hipStream_t s) {
+    const int nu = c->Nx * (c->Nz + 1), nw = (c->Nx + 1) * c->Nz, nc = (c->Nx - 1) * (c->Nz - 1);
+    hipLaunchKernelGGL(k_line_step, dim3(mdz_div_up(nu + nw + nc, 256)), dim3(256), 0, s, c->e, nu, nw, nc,
+                       (const double *)c->ls_base, (const double *)c->ls_dx, (const double *)c->ls_alpha, k,
+                       c->g.u_in, c->g.v_in, c->g.p_in);
+    if (k < 0) return MDZ_OK;
+    DevAsm a = make_asm(c);
+    mdz_launch_strain_rates(c->g, c->prm, s);
+    int rc = rheology_pass(c, c->prm.rheo_mode == MDZ_RHEO_LEARNED, s);
+    if (rc) return rc;
+    mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, s);
+    mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
+    int kind = c->model.aniso == 1 ? 1 : 0;
+    mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], s);
+    mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, s);
+    hipLaunchKernelGGL(k_copy6, dim3(1), dim3(64), 0, s, (const double *)c->norm_out, c->ls_res + 6 * k);
+    return MDZ_OK;
+}
+
+extern "C" int mdz_ctx_line_search(mdz_ctx *c, const double *dx, int ntry, const double *alpha, double *res3n) {
+    if (!c || !dx || !alpha || !res3n || ntry < 1 || ntry > MDZ_LS_MAX)
+        return set_err(MDZ_EINVAL, "line search: %d trials (1..%d)", ntry, MDZ_LS_MAX);
+    if (!c->pattern_ready[c->model.aniso == 1 ? 1 : 0])
+        return set_err(MDZ_EINVAL, "line search before the blocks are assembled");
+    g_status = 0;
+    const size_t nu = (size_t)c->Nx * (c->Nz + 1), nw = (size_t)(c->Nx + 1) * c->Nz, nc = (size_t)(c->Nx - 1) * (c->Nz - 1);
+    const size_t neq = (size_t)c->e.neq_mom + c->e.neq_cont;
+    if (!c->ls_base) {
+        HIPCHK(hipMalloc((void **)&c->ls_base, (nu + nw + nc) * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&c->ls_dx, (neq > 0 ? neq : 1) * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&c->ls_alpha, MDZ_LS_MAX * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&c->ls_res, 6 * MDZ_LS_MAX * sizeof(double)));
+    }
+    hipStream_t s = c->stream;
+    HIPCHK(hipMemcpyAsync(c->ls_base, c->g.u_in, nu * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->ls_base + nu, c->g.v_in, nw * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->ls_base + nu + nw, c->g.p_in, nc * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->ls_dx, dx, neq * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->ls_alpha, alpha, ntry * sizeof(double), hipMemcpyHostToDevice, s));
+    if (!c->ls_exec || c->ls_ntry != ntry || c->ls_version != c->version || c->ls_kernel != g_rheo_kernel) {
+        if (c->ls_exec) hipGraphExecDestroy(c->ls_exec);
+        if (c->ls_graph) hipGraphDestroy(c->ls_graph);
+        c->ls_exec = nullptr;
+        c->ls_graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = MDZ_OK;
+        for (int k = 0; k < ntry && !rc; k++) rc = enqueue_trial(c, k, s);
+        if (!rc) rc = enqueue_trial(c, -1, s);
+        hipGraph_t gr = nullptr;
+        hipError_t he = hipStreamEndCapture(s, &gr);
+        if (rc) { if (gr) hipGraphDestroy(gr); return rc; }
+        if (he != hipSuccess) return set_err(MDZ_EHIP, "line search capture: %s", hipGetErrorString(he));
+        c->ls_graph = gr;
+        HIPCHK(hipGraphInstantiate(&c->ls_exec, c->ls_graph, nullptr, nullptr, 0));
+        c->ls_ntry = ntry;
+        c->ls_version = c->version;
+        c->ls_kernel = g_rheo_kernel;
+    }
+    HIPCHK(hipGraphLaunch(c->ls_exec, s));
+    std::vector<double> h(6 * (size_t)ntry);
+    HIPCHK(hipMemcpyAsync(h.data(), c->ls_res, h.size() * 8, hipMemcpyDeviceToHost, s));
+    int rc = sync(c);
+    if (rc) return rc;
+    for (int k = 0; k < ntry; k++) {
+        int r = norms_to_res(&h[6 * (size_t)k], res3n + 3 * (size_t)k);
+        if (r && !rc) rc = r;
+    }
+    return rc;
 }
 
 /* ------------------------------------------------------------------ */
