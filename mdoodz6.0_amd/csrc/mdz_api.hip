@@ -423,6 +423,7 @@ static int build_pattern(mdz_ctx *c, int kind) {
     if (!A->free_row && A->neq > 0) HIPCHK(hipMalloc((void **)&A->free_row, (size_t)A->neq));
     if ((rc = mdz_launch_free_rows(kind, a, A, B, c->stream))) return set_err(rc, "free-row flags");
     c->pattern_ready[kind] = 1;
+    c->version++; /* captured graphs hold the block pointers */
     return sync(c);
 }
 

@@ -1030,7 +1030,7 @@ __global__ void __launch_bounds__(VBLK) k_values_mom(DevAsm a, DevBlock A, DevBl
     __shared__ double sA[VBLK * CAPA];
     __shared__ double sB[VBLK * CAPB];
     __shared__ double sDump[VBLK];
-    const int r0 = blockIdx.x * VBLK;
+    const int r0 = xcd_tile() * VBLK;
     const int r1 = min(r0 + VBLK, a.e.neq_mom);
     const int eqn = r0 + threadIdx.x;
     const int a0 = A.Ic[r0], b0 = B.Ic[r0];
@@ -1055,7 +1055,7 @@ __global__ void __launch_bounds__(VBLK) k_values_mom(DevAsm a, DevBlock A, DevBl
 template <int KIND>
 __global__ void __launch_bounds__(BLK) k_values_mom_direct(DevAsm a, DevBlock A, DevBlock B) {
     __shared__ double sDump[BLK];
-    int eqn = blockIdx.x * BLK + threadIdx.x;
+    int eqn = xcd_tile() * BLK + threadIdx.x;
     if (eqn >= a.e.neq_mom) return;
     Tags T = make_tags(a.g);
     Sink<SINK_VALUES> ra{0, nullptr, A.A + A.Ic[eqn], 0.0, sDump + threadIdx.x},
@@ -1071,7 +1071,7 @@ __global__ void __launch_bounds__(BLK) k_values_mom_direct(DevAsm a, DevBlock A,
 
 __global__ void __launch_bounds__(BLK) k_values_cont(int kind, DevAsm a, DevBlock C, DevBlock D) {
     __shared__ double sDump[BLK];
-    int eqn = blockIdx.x * BLK + threadIdx.x;
+    int eqn = xcd_tile() * BLK + threadIdx.x;
     if (eqn >= a.e.neq_cont) return;
     int c1, c2, c3;
     cont_geom(a, eqn, c1, c2, c3);
@@ -1090,7 +1090,7 @@ __global__ void __launch_bounds__(BLK) k_values_cont(int kind, DevAsm a, DevBloc
  * whose rows are all free take the tag-free instance. */
 template <int KIND, bool FREE>
 __global__ void __launch_bounds__(BLK) k_residual_mom(DevAsm a, DevBlock A) {
-    int eqn = blockIdx.x * BLK + threadIdx.x;
+    int eqn = xcd_tile() * BLK + threadIdx.x;
     if (A.free_row) {
         const int fr = eqn < a.e.neq_mom ? A.free_row[eqn] : 1;
         if ((__all(fr) != 0) != FREE) return;
@@ -1131,7 +1131,7 @@ __global__ void __launch_bounds__(64) k_free_rows(int kind, DevAsm a, unsigned c
 }
 
 __global__ void __launch_bounds__(BLK) k_residual_cont(int kind, DevAsm a, DevBlock C, DevBlock D) {
-    int eqn = blockIdx.x * BLK + threadIdx.x;
+    int eqn = xcd_tile() * BLK + threadIdx.x;
     if (eqn >= a.e.neq_cont) return;
     int c1, c2, c3;
     cont_geom(a, eqn, c1, c2, c3);

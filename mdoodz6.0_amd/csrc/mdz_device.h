@@ -105,3 +105,14 @@ struct DevBlock {
 enum { SINK_COUNT = 0, SINK_PATTERN = 1, SINK_VALUES = 2 };
 
 static inline __host__ __device__ int mdz_div_up(int a, int b) { return (a + b - 1) / b; }
+
+/* The workgroups of a launch are dispatched round-robin over the eight XCDs,
+ * each with its own L2.  xcd_tile() renumbers them so that consecutive tiles,
+ * which share the halo rows of a stencil, run on the same XCD: XCD x takes
+ * the contiguous range starting at x q + min(x, r) of a grid of 8 q + r
+ * workgroups (a bijection onto [0, gridDim.x)). */
+__device__ __forceinline__ int xcd_tile() {
+    const int b = blockIdx.x, n = gridDim.x;
+    const int q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
+    return x * q + min(x, r) + i;
+}

@@ -23,7 +23,7 @@
 /* strain rates                                                      */
 /* ---------------------------------------------------------------- */
 __global__ void __launch_bounds__(BLK) k_strain_cells(DevGrid g) {
-    int c0 = blockIdx.x * BLK + threadIdx.x;
+    int c0 = xcd_tile() * BLK + threadIdx.x;
     int Ncx = g.ncx, Nx = g.Nx;
     if (c0 >= Ncx * g.ncz) return;
     int k = c0 % Ncx, l = c0 / Ncx;
@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(BLK) k_strain_cells(DevGrid g) {
 }
 
 __global__ void __launch_bounds__(BLK) k_strain_vertices(DevGrid g) {
-    int c1 = blockIdx.x * BLK + threadIdx.x;
+    int c1 = xcd_tile() * BLK + threadIdx.x;
     int Nx = g.Nx;
     if (c1 >= Nx * g.Nz) return;
     int k = c1 % Nx, l = c1 / Nx;
@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(BLK) k_strain_vertices(DevGrid g) {
 }
 
 __global__ void __launch_bounds__(BLK) k_exz_n(DevGrid g) {
-    int c0 = blockIdx.x * BLK + threadIdx.x;
+    int c0 = xcd_tile() * BLK + threadIdx.x;
     int Ncx = g.ncx, Nx = g.Nx;
     if (c0 >= Ncx * g.ncz) return;
     int k = c0 % Ncx, l = c0 / Ncx;
@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(BLK) k_interp_c2v(DevGrid g, int per, int nf, 
                                                     double *__restrict__ d2, const double *__restrict__ s3,
                                                     double *__restrict__ d3, const double *__restrict__ s4,
                                                     double *__restrict__ d4) {
-    int c1 = blockIdx.x * BLK + threadIdx.x;
+    int c1 = xcd_tile() * BLK + threadIdx.x;
     int Nx = g.Nx;
     if (c1 >= Nx * g.Nz) return;
     int k = c1 % Nx, l = c1 / Nx;
@@ -375,7 +375,7 @@ __device__ __forceinline__ void aniso_factors(const DevParams &prm, double nx, d
 }
 
 __global__ void __launch_bounds__(BLK) k_dtensor_cells(DevGrid g, DevParams prm, int jac) {
-    int k = blockIdx.x * BLK + threadIdx.x;
+    int k = xcd_tile() * BLK + threadIdx.x;
     if (k >= g.ncx * g.ncz) return;
     double D11 = 0, D12 = 0, D13 = 0, D14 = 0, D21 = 0, D22 = 0, D23 = 0, D24 = 0;
     int t = g.BCp[k];
@@ -411,7 +411,7 @@ __global__ void __launch_bounds__(BLK) k_dtensor_cells(DevGrid g, DevParams prm,
 }
 
 __global__ void __launch_bounds__(BLK) k_dtensor_vertices(DevGrid g, DevParams prm, int jac, int *status) {
-    int k = blockIdx.x * BLK + threadIdx.x;
+    int k = xcd_tile() * BLK + threadIdx.x;
     if (k >= g.Nx * g.Nz) return;
     double D31 = 0, D32 = 0, D33 = 0, D34 = 0;
     if (g.BCg[k] != 30) {
@@ -443,7 +443,7 @@ __global__ void __launch_bounds__(BLK) k_dtensor_vertices(DevGrid g, DevParams p
 /* ---------------------------------------------------------------- */
 __global__ void __launch_bounds__(BLK) k_rhs_u(DevGrid g, DevParams prm, const double *__restrict__ xg,
                                                const double *__restrict__ zvx) {
-    int c = blockIdx.x * BLK + threadIdx.x;
+    int c = xcd_tile() * BLK + threadIdx.x;
     int Nx = g.Nx, NzVx = g.nzvx, Ncx = g.ncx;
     if (c >= Nx * NzVx) return;
     int k = c % Nx, l = c / Nx;
@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(BLK) k_rhs_u(DevGrid g, DevParams prm, const d
 
 __global__ void __launch_bounds__(BLK) k_rhs_v(DevGrid g, DevParams prm, const double *__restrict__ xvz,
                                                const double *__restrict__ zg) {
-    int c = blockIdx.x * BLK + threadIdx.x;
+    int c = xcd_tile() * BLK + threadIdx.x;
     int Nx = g.Nx, NxVz = g.nxvz, Nz = g.Nz, Ncx = g.ncx;
     if (c >= NxVz * Nz) return;
     int k = c % NxVz, l = c / NxVz;
@@ -515,7 +515,7 @@ __global__ void __launch_bounds__(BLK) k_rhs_v(DevGrid g, DevParams prm, const d
 }
 
 __global__ void __launch_bounds__(BLK) k_rhs_p(DevGrid g, DevParams prm) {
-    int c = blockIdx.x * BLK + threadIdx.x;
+    int c = xcd_tile() * BLK + threadIdx.x;
     if (c >= g.ncx * g.ncz) return;
     int t = g.BCp[c];
     if (t != 30 && t != 31) {
