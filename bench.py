@@ -40,9 +40,9 @@ def stage_kernels(shape, learned=False):
             "rheology": ["k_rheo_cells_fast<%d>" % rh, "k_rheo_vertices_fast<%d>" % rh],
             "rhs": ["k_rhs_u", "k_rhs_v", "k_rhs_p"],
             "dtensor_picard": ["k_dtensor_cells (J=0)", "k_dtensor_vertices (J=0)"],
-            "picard_values": [vp, "k_values_cont"],
+            "picard_values": [vp] if pic else [vp, "k_values_cont"],
             "dtensor_jacobian": ["k_dtensor_cells (J=1)", "k_dtensor_vertices (J=1)"],
-            "jacobian_values": [vj, "k_values_cont"],
+            "jacobian_values": [vj] if jac else [vj, "k_values_cont"],
             "dtensor_picard_2": ["k_dtensor_cells (J=0)", "k_dtensor_vertices (J=0)"],
             "residual": ["k_residual_mom<0, true>", "k_residual_mom<0, false>", "k_residual_cont", "k_norms",
                          "k_norms_final"]}

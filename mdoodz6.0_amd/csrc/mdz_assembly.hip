@@ -1025,12 +1025,35 @@ __global__ void __launch_bounds__(BLK) k_pattern_cont(DevAsm a, DevBlock C) {
  * range of A (and of B): each thread writes its row into LDS and the block
  * copies the range out with coalesced stores.  Needs every row length
  * within the LDS slot (CAPA, CAPB); KIND 0 = Picard, 1 = Jacobian. */
+/* the continuity rows of a value pass (C and D blocks), one lane per row */
+__device__ __forceinline__ void values_cont_row(const DevAsm &a, DevBlock &C, DevBlock &D, int eqn, double *dump) {
+    int c1, c2, c3;
+    cont_geom(a, eqn, c1, c2, c3);
+    Sink<SINK_VALUES> rc{0, nullptr, C.A + C.Ic[eqn], 0.0, dump};
+    double b = a.g.rhs_p[c2], F = 0.0;
+    continuity_row<true, SINK_VALUES>(a, 0, c1, c2, c3, rc, b, F);
+    C.bbc[eqn] = rc.bbc;
+    D.bbc[eqn] = 0.0;
+    double bf = b + rc.bbc;
+    C.b[eqn] = bf;
+    D.b[eqn] = bf;
+}
+
+/* Blocks [0, nbm) take VBLK momentum rows each; the blocks after them take
+ * VBLK continuity rows each, so that the continuity pass runs in the tail of
+ * the momentum pass instead of after it. */
 template <int KIND, int VBLK, int CAPA, int CAPB>
-__global__ void __launch_bounds__(VBLK) k_values_mom(DevAsm a, DevBlock A, DevBlock B) {
+__global__ void __launch_bounds__(VBLK) k_values_mom(DevAsm a, DevBlock A, DevBlock B, DevBlock C, DevBlock D,
+                                                     int nbm) {
     __shared__ double sA[VBLK * CAPA];
     __shared__ double sB[VBLK * CAPB];
     __shared__ double sDump[VBLK];
-    const int r0 = xcd_tile() * VBLK;
+    if ((int)blockIdx.x >= nbm) {
+        const int eqn = ((int)blockIdx.x - nbm) * VBLK + threadIdx.x;
+        if (eqn < a.e.neq_cont) values_cont_row(a, C, D, eqn, sDump + threadIdx.x);
+        return;
+    }
+    const int r0 = xcd_tile_of(blockIdx.x, nbm) * VBLK;
     const int r1 = min(r0 + VBLK, a.e.neq_mom);
     const int eqn = r0 + threadIdx.x;
     const int a0 = A.Ic[r0], b0 = B.Ic[r0];
@@ -1069,21 +1092,11 @@ __global__ void __launch_bounds__(BLK) k_values_mom_direct(DevAsm a, DevBlock A,
     B.b[eqn] = bf;
 }
 
-__global__ void __launch_bounds__(BLK) k_values_cont(int kind, DevAsm a, DevBlock C, DevBlock D) {
+__global__ void __launch_bounds__(BLK) k_values_cont(DevAsm a, DevBlock C, DevBlock D) {
     __shared__ double sDump[BLK];
     int eqn = xcd_tile() * BLK + threadIdx.x;
     if (eqn >= a.e.neq_cont) return;
-    int c1, c2, c3;
-    cont_geom(a, eqn, c1, c2, c3);
-    Sink<SINK_VALUES> rc{0, nullptr, C.A + C.Ic[eqn], 0.0, sDump + threadIdx.x};
-    double b = a.g.rhs_p[c2], F = 0.0;
-    continuity_row<true, SINK_VALUES>(a, 0, c1, c2, c3, rc, b, F);
-    C.bbc[eqn] = rc.bbc;
-    D.bbc[eqn] = 0.0;
-    double bf = b + rc.bbc;
-    C.b[eqn] = bf;
-    D.b[eqn] = bf;
-    (void)kind;
+    values_cont_row(a, C, D, eqn, sDump + threadIdx.x);
 }
 
 /* residual (Assemble = 0): b = rhs (Neumann rows adjusted), F.  Waves
@@ -1343,11 +1356,12 @@ void mdz_fill_pattern(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBl
 #define JAC_CAPB 6
 
 template <int KIND, int VBLK, int CAPA, int CAPB>
-static void launch_staged(const DevAsm &a, DevBlock *A, DevBlock *B, hipStream_t s) {
+static void launch_staged(const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, hipStream_t s) {
     /* the value kernels are bound by the staged stores, not by the tag
      * reads: one instance serves free and boundary rows alike */
-    hipLaunchKernelGGL((k_values_mom<KIND, VBLK, CAPA, CAPB>), dim3(mdz_div_up(a.e.neq_mom, VBLK)), dim3(VBLK), 0, s,
-                       a, *A, *B);
+    const int nbm = mdz_div_up(a.e.neq_mom, VBLK), nbc = mdz_div_up(a.e.neq_cont, VBLK);
+    hipLaunchKernelGGL((k_values_mom<KIND, VBLK, CAPA, CAPB>), dim3(nbm + nbc), dim3(VBLK), 0, s, a, *A, *B, *C, *D,
+                       nbm);
 }
 
 int mdz_launch_free_rows(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, hipStream_t s) {
@@ -1376,17 +1390,18 @@ void mdz_launch_values(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevB
         const bool fits = kind == 0 ? A->maxrow <= PIC_CAPA && B->maxrow <= PIC_CAPB
                                     : A->maxrow <= JAC_CAPA && B->maxrow <= JAC_CAPB;
         const int v = fits ? vblk : 0;
-        if (kind == 0 && v == 64) launch_staged<0, 64, PIC_CAPA, PIC_CAPB>(a, A, B, s);
-        else if (kind == 0 && v == 128) launch_staged<0, 128, PIC_CAPA, PIC_CAPB>(a, A, B, s);
-        else if (kind == 0 && v == 256) launch_staged<0, 256, PIC_CAPA, PIC_CAPB>(a, A, B, s);
-        else if (kind == 1 && v == 64) launch_staged<1, 64, JAC_CAPA, JAC_CAPB>(a, A, B, s);
-        else if (kind == 1 && v == 128) launch_staged<1, 128, JAC_CAPA, JAC_CAPB>(a, A, B, s);
-        else if (kind == 0)
+        /* the staged kernels also take the continuity rows */
+        if (kind == 0 && v == 64) { launch_staged<0, 64, PIC_CAPA, PIC_CAPB>(a, A, B, C, D, s); return; }
+        if (kind == 0 && v == 128) { launch_staged<0, 128, PIC_CAPA, PIC_CAPB>(a, A, B, C, D, s); return; }
+        if (kind == 0 && v == 256) { launch_staged<0, 256, PIC_CAPA, PIC_CAPB>(a, A, B, C, D, s); return; }
+        if (kind == 1 && v == 64) { launch_staged<1, 64, JAC_CAPA, JAC_CAPB>(a, A, B, C, D, s); return; }
+        if (kind == 1 && v == 128) { launch_staged<1, 128, JAC_CAPA, JAC_CAPB>(a, A, B, C, D, s); return; }
+        if (kind == 0)
             hipLaunchKernelGGL(k_values_mom_direct<0>, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, a, *A, *B);
         else
             hipLaunchKernelGGL(k_values_mom_direct<1>, dim3(mdz_div_up(nm, BLK)), dim3(BLK), 0, s, a, *A, *B);
     }
-    if (ncnt > 0) hipLaunchKernelGGL(k_values_cont, dim3(mdz_div_up(ncnt, BLK)), dim3(BLK), 0, s, kind, a, *C, *D);
+    if (ncnt > 0) hipLaunchKernelGGL(k_values_cont, dim3(mdz_div_up(ncnt, BLK)), dim3(BLK), 0, s, a, *C, *D);
 }
 
 void mdz_launch_residual(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, DevBlock *D, hipStream_t s) {

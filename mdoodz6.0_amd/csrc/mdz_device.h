@@ -111,8 +111,8 @@ static inline __host__ __device__ int mdz_div_up(int a, int b) { return (a + b -
  * which share the halo rows of a stencil, run on the same XCD: XCD x takes
  * the contiguous range starting at x q + min(x, r) of a grid of 8 q + r
  * workgroups (a bijection onto [0, gridDim.x)). */
-__device__ __forceinline__ int xcd_tile() {
-    const int b = blockIdx.x, n = gridDim.x;
+__device__ __forceinline__ int xcd_tile_of(int b, int n) {
     const int q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
     return x * q + min(x, r) + i;
 }
+__device__ __forceinline__ int xcd_tile() { return xcd_tile_of(blockIdx.x, gridDim.x); }
