@@ -29,21 +29,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 STAGES = ["strain_rates", "interp_guess", "rheology", "rhs", "dtensor_picard", "picard_values", "dtensor_jacobian",
           "jacobian_values", "dtensor_picard_2", "residual"]
 # kernels of each stage (names as in the rocprofv3 kernel trace) for the
-# launch shape (rheology threads, Picard rows, Jacobian rows per block)
-def stage_kernels(shape, learned=False):
+# launch shape (rheology threads, Picard rows, Jacobian rows per block).  An
+# isotropic model takes the fused D pass: one launch writes the J = 1 tensor
+# and the J = 0 components the Picard rows and the residual read, and the
+# two J = 0 stages run nothing.
+def stage_kernels(shape, learned=False, fused=True):
     rh, pic, jac = shape
     vp = "k_values_mom<0, %d, 9, 2>" % pic if pic else "k_values_mom_direct<0>"
     vj = "k_values_mom<1, %d, 21, 6>" % jac if jac else "k_values_mom_direct<1>"
     prep = ["k_interp_c2v (5 fields)"] + (["k_mlp_guess<true>", "k_mlp_guess<false>"] if learned else [])
+    d0 = [] if fused else ["k_dtensor_cells<0>", "k_dtensor_vertices<0>"]
+    d1 = ["k_dtensor_cells<2>", "k_dtensor_vertices<2>"] if fused else ["k_dtensor_cells<1>", "k_dtensor_vertices<1>"]
     return {"strain_rates": ["k_strain_cells", "k_strain_vertices", "k_exz_n", "k_interp_c2v (2 fields)"],
             "interp_guess": prep,
             "rheology": ["k_rheo_cells_fast<%d>" % rh, "k_rheo_vertices_fast<%d>" % rh],
             "rhs": ["k_rhs_u", "k_rhs_v", "k_rhs_p"],
-            "dtensor_picard": ["k_dtensor_cells (J=0)", "k_dtensor_vertices (J=0)"],
+            "dtensor_picard": d0,
             "picard_values": [vp] if pic else [vp, "k_values_cont"],
-            "dtensor_jacobian": ["k_dtensor_cells (J=1)", "k_dtensor_vertices (J=1)"],
+            "dtensor_jacobian": d1,
             "jacobian_values": [vj] if jac else [vj, "k_values_cont"],
-            "dtensor_picard_2": ["k_dtensor_cells (J=0)", "k_dtensor_vertices (J=0)"],
+            "dtensor_picard_2": d0,
             "residual": ["k_residual_mom<0, true>", "k_residual_mom<0, false>", "k_residual_cont", "k_norms",
                          "k_norms_final"]}
 
@@ -51,6 +56,11 @@ def stage_kernels(shape, learned=False):
 BYTES_PER_CELL = {"strain_rates": 76, "interp_guess": 80, "rheology": 730, "rhs": 160, "dtensor_picard": 114,
                   "picard_values": 340, "dtensor_jacobian": 330, "jacobian_values": 668, "dtensor_picard_2": 114,
                   "residual": 204}
+# the fused D pass writes D11 (= D22) at the centroid and D33 at the vertex
+# of the J = 0 tensor beside the J = 1 tensor (+16 B); the Picard values and
+# the residual read that D11 once for D11 and D22 (-8 B each)
+FUSED_BYTES_PER_CELL = dict(BYTES_PER_CELL, dtensor_picard=0, dtensor_jacobian=346, dtensor_picard_2=0,
+                            picard_values=332, residual=196)
 # learned mode: the network's guess reads 11 doubles, 3 phase fractions and
 # a tag and writes 3 floats per node, at one centroid and one vertex per cell
 GUESS_BYTES_PER_CELL = 2 * (11 * 8 + 3 * 8 + 1 + 3 * 4)
@@ -275,7 +285,8 @@ def run_ensemble(args, mdz, hot):
     hot.mdz_ensemble_destroy(ens)
     for c in ctxs:
         hot.mdz_ctx_destroy(c)
-    whole = 2816.0 * total / (ms.value * 1e-3) / 1e9
+    unit_bpc = sum(FUSED_BYTES_PER_CELL.values()) + (GUESS_BYTES_PER_CELL if mode == 2 else 0)
+    whole = unit_bpc * total / (ms.value * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu:
@@ -313,7 +324,7 @@ def run_ensemble(args, mdz, hot):
                    "shapes_rheo/picard/jacobian": kinds, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(whole, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(whole / HBM_PEAK_GBS, 4), "traffic": None, "stage": "whole unit",
-                     "bytes_per_cell": 2816},
+                     "bytes_per_cell": unit_bpc},
         "cpu_baseline": cpu,
         "setup_s": round(setup_s, 1),
         "autotune_s": round(tune_s, 1),
@@ -409,7 +420,7 @@ def main():
     st = (C.c_double * 11)()
     mdz.check(hot.mdz_ctx_stage_times(ctx, 3, st))
     stage_ms = {s: st[i] for i, s in enumerate(STAGES)}
-    bpc = dict(BYTES_PER_CELL)
+    bpc = dict(FUSED_BYTES_PER_CELL)
     if mode == 2:
         bpc["interp_guess"] += GUESS_BYTES_PER_CELL
     dom = max(STAGES, key=lambda s: stage_ms[s])
@@ -418,6 +429,8 @@ def main():
     kern = stage_kernels(tuple(shape), mode == 2)
     per_stage = {}
     for s_ in STAGES:
+        if not kern[s_]:
+            continue  # a J = 0 stage of the fused D pass: nothing runs
         t = traffic_from_profile(kern[s_], args.ncx, args.ncz, args.mode)
         per_stage[s_] = {"ms": round(stage_ms[s_], 4), "bytes_per_cell": bpc[s_],
                          "GBs": round(bpc[s_] * cells / (stage_ms[s_] * 1e-3) / 1e9, 1),
@@ -430,7 +443,7 @@ def main():
                 "stage": dom, "kernels": kern[dom], "algorithmic_bytes": dom_bytes,
                 "bytes_per_cell": bpc[dom], "launch_ms": round(stage_ms[dom], 4),
                 "probe_GBs": round(probe.value, 1), "frac_of_probe": round(achieved / probe.value, 4)}
-    whole = 2816.0 * cells / (ms_per_step * 1e-3) / 1e9
+    whole = sum(bpc.values()) * cells / (ms_per_step * 1e-3) / 1e9
     hot.mdz_ctx_destroy(ctx)
     m.free()
 
@@ -462,6 +475,7 @@ def main():
         "cpu_baseline": cpu,
         "stages": per_stage,
         "whole_unit_algorithmic_GBs": round(whole, 1),
+        "whole_unit_bytes_per_cell": sum(bpc.values()),
         "at_abi_ms_per_step": round(abi_ms, 2),
         "at_abi_value": round(cells / (abi_ms * 1e-3), 1),
         "line_search_5_trials_ms": round(ls_ms, 3),

@@ -122,6 +122,8 @@ struct mdz_ctx {
     int hist_on;         /* collect local-iteration histograms (off by default) */
     float *guess_n, *guess_s;
     double *tmp_c, *tmp_v;  /* staging for InterpCentroidsToVerticesDouble */
+    double *dpic_n, *dpic_s; /* J = 0 D11 (= D22) at the centroids and D33 at the vertices (fused D passes) */
+    int d_jac;               /* D11_n..D34_s hold the J = 1 tensor (left by a fused unit) */
     params model;
     mat_prop materials;
     scale scaling;
@@ -182,6 +184,22 @@ static DevAsm make_asm(const mdz_ctx *c) {
     a.periodic = c->model.isperiodic_x;
     a.Nz_model = c->model.Nz;
     a.u = c->g.u_in; a.v = c->g.v_in; a.p_mom = c->g.p_corr; a.p_cont = c->g.p_in;
+    a.d11p = c->g.D11_n; a.d22p = c->g.D22_n; a.d33p = c->g.D33_s;
+    return a;
+}
+
+/* Isotropic models (aniso = 0) take fused D passes in a Newton unit: one
+ * pass writes the J = 1 tensor and keeps the J = 0 components that the
+ * Picard rows and the residual read (D11 = D22 = 2 eta, D33 = eta) in
+ * dpic_n / dpic_s, so the J = 0 tensor is not computed twice.  Models with
+ * aniso = 1 compute the J = 0 tensor again after the Jacobian values, for
+ * the residual, which reads every component. */
+static bool fused_dtensor(const mdz_ctx *c) { return c->model.aniso == 0; }
+
+/* assembly arguments of the Picard values and the residual of a unit */
+static DevAsm picard_asm(const mdz_ctx *c) {
+    DevAsm a = make_asm(c);
+    if (fused_dtensor(c)) { a.d11p = c->dpic_n; a.d22p = c->dpic_n; a.d33p = c->dpic_s; }
     return a;
 }
 
@@ -230,6 +248,8 @@ static int ctx_alloc(mdz_ctx **out, int Nx, int Nz, int nph) {
     size_t o_cn = off; off += align256(nc * 4);
     size_t o_tc = off; off += align256(nc * 8);
     size_t o_tvv = off; off += align256(nv * 8);
+    size_t o_dpn = off; off += align256(nc * 8);
+    size_t o_dps = off; off += align256(nv * 8);
     c->slab_bytes = off;
     if (hipMalloc((void **)&c->slab, off) != hipSuccess) { free(c); return set_err(MDZ_ENOMEM, "device slab of %zu bytes", off); }
     hipMemset(c->slab, 0, off);
@@ -248,6 +268,7 @@ static int ctx_alloc(mdz_ctx **out, int Nx, int Nz, int nph) {
     c->e.eqn_u = (int *)(c->slab + o_eu); c->e.eqn_v = (int *)(c->slab + o_ev); c->e.eqn_p = (int *)(c->slab + o_ep);
     c->e.mom_node = (int *)(c->slab + o_mn); c->e.cont_node = (int *)(c->slab + o_cn);
     c->tmp_c = (double *)(c->slab + o_tc); c->tmp_v = (double *)(c->slab + o_tvv);
+    c->dpic_n = (double *)(c->slab + o_dpn); c->dpic_s = (double *)(c->slab + o_dps);
     c->scratch_len = (nu + nw + nc) + 4096 + (nu + nw + nc) / 512;
     if (hipMalloc((void **)&c->scratch, c->scratch_len * sizeof(int)) != hipSuccess) return set_err(MDZ_ENOMEM, "scratch");
     size_t np = 0;
@@ -1072,6 +1093,12 @@ extern "C" int mdz_ctx_download_solution(mdz_ctx *c, double *u, double *v, doubl
 extern "C" int mdz_ctx_download_grid(mdz_ctx *c, grid *mesh) {
     if (!c || !mesh) return set_err(MDZ_EINVAL, "NULL argument");
     int rc;
+    /* the grid after a unit holds the J = 0 tensor, as the reference's does
+     * (RheologicalOperators(..., 0) before the residual) */
+    if (c->d_jac) {
+        mdz_launch_dtensor(c->g, c->prm, 0, c->status, c->stream);
+        c->d_jac = 0;
+    }
     for (int i = 0; i < FI_COUNT; i++)
         if (FREF[i].role & MDZ_ROLE_O)
             if ((rc = down(c, mesh, i))) return rc;
@@ -1090,17 +1117,26 @@ extern "C" int mdz_ctx_set_rheology_mode(mdz_ctx *c, int mode) {
  * allocation, memset, host synchronisation or copy: safe to capture into a
  * graph.  The norms stay in c->norm_out (six doubles). */
 static int enqueue_unit(mdz_ctx *c, hipStream_t s) {
-    DevAsm a = make_asm(c);
+    DevAsm a = make_asm(c), ap = picard_asm(c);
     mdz_launch_strain_rates(c->g, c->prm, s);
     int rc = rheology_pass(c, c->prm.rheo_mode == MDZ_RHEO_LEARNED, s);
     if (rc) return rc;
     mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, s);
+    const int kind = c->model.aniso == 1 ? 1 : 0;
+    if (fused_dtensor(c)) {
+        mdz_launch_dtensor(c->g, c->prm, 2, c->status, s, c->dpic_n, c->dpic_s);
+        mdz_launch_values(0, ap, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, s);
+        mdz_launch_residual(0, ap, &c->blk[0], &c->blk[2], &c->blk[3], s);
+        mdz_launch_norms(ap, &c->blk[0], &c->blk[3], c->norm_partial, c->norm_out, s);
+        mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s);
+        c->d_jac = 1;
+        return MDZ_OK;
+    }
     mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
     mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, s);
     mdz_launch_dtensor(c->g, c->prm, 1, c->status, s);
     mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s);
     mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
-    int kind = c->model.aniso == 1 ? 1 : 0;
     mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], s);
     mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, s);
     return MDZ_OK;
@@ -1212,49 +1248,63 @@ extern "C" int mdz_ctx_set_network(mdz_ctx *c, const float *w, size_t n) {
 extern "C" int mdz_ctx_stage_times(mdz_ctx *c, int reps, double *ms) {
     if (!c || reps < 1 || !ms) return set_err(MDZ_EINVAL, "stage times");
     const int NS = 10;
-    hipEvent_t ev[NS + 1];
-    for (int i = 0; i <= NS; i++) HIPCHK(hipEventCreate(&ev[i]));
+    /* begin / end event of each stage, recorded in execution order; a stage
+     * that does not run (the second J = 0 tensor of a fused unit) has both
+     * at the same point */
+    hipEvent_t eb[NS], ee[NS];
+    for (int i = 0; i < NS; i++) { HIPCHK(hipEventCreate(&eb[i])); HIPCHK(hipEventCreate(&ee[i])); }
     double acc[NS + 1] = {0};
-    DevAsm a = make_asm(c);
+    DevAsm a = make_asm(c), ap = picard_asm(c);
     int kind = c->model.aniso == 1 ? 1 : 0;
+    const bool fused = fused_dtensor(c);
     const int learned = c->prm.rheo_mode == MDZ_RHEO_LEARNED;
     hipStream_t s = c->stream;
-    for (int r = 0; r < reps; r++) {
-        int rc;
-        HIPCHK(hipEventRecord(ev[0], s));
-        mdz_launch_strain_rates(c->g, c->prm, s);
-        HIPCHK(hipEventRecord(ev[1], s));
-        if ((rc = rheology_prep(c, learned, s))) return rc;
-        HIPCHK(hipEventRecord(ev[2], s));
-        if ((rc = rheology_kernels(c, learned, s))) return rc;
-        HIPCHK(hipEventRecord(ev[3], s));
-        mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, s);
-        HIPCHK(hipEventRecord(ev[4], s));
-        mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
-        HIPCHK(hipEventRecord(ev[5], s));
-        mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, s);
-        HIPCHK(hipEventRecord(ev[6], s));
-        mdz_launch_dtensor(c->g, c->prm, 1, c->status, s);
-        HIPCHK(hipEventRecord(ev[7], s));
-        mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s);
-        HIPCHK(hipEventRecord(ev[8], s));
-        mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
-        HIPCHK(hipEventRecord(ev[9], s));
-        mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], s);
-        mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, s);
-        HIPCHK(hipEventRecord(ev[10], s));
-        HIPCHK(hipEventSynchronize(ev[10]));
-        for (int i = 0; i < NS; i++) {
-            float t = 0;
-            HIPCHK(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
-            acc[i] += t;
+    auto run = [&]() -> int {
+        for (int r = 0; r < reps; r++) {
+            int rc;
+#define STAGE(i, body) do { HIPCHK(hipEventRecord(eb[i], s)); body; HIPCHK(hipEventRecord(ee[i], s)); } while (0)
+            STAGE(0, mdz_launch_strain_rates(c->g, c->prm, s));
+            STAGE(1, if ((rc = rheology_prep(c, learned, s))) return rc);
+            STAGE(2, if ((rc = rheology_kernels(c, learned, s))) return rc);
+            STAGE(3, mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, s));
+            if (fused) {
+                /* J = 1 tensor with the J = 0 Picard components, Picard
+                 * values, residual, Jacobian values */
+                STAGE(6, mdz_launch_dtensor(c->g, c->prm, 2, c->status, s, c->dpic_n, c->dpic_s));
+                STAGE(4, (void)0);
+                STAGE(5, mdz_launch_values(0, ap, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, s));
+                STAGE(9, { mdz_launch_residual(0, ap, &c->blk[0], &c->blk[2], &c->blk[3], s);
+                           mdz_launch_norms(ap, &c->blk[0], &c->blk[3], c->norm_partial, c->norm_out, s); });
+                STAGE(7, mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s));
+                STAGE(8, (void)0);
+                c->d_jac = 1;
+            } else {
+                STAGE(4, mdz_launch_dtensor(c->g, c->prm, 0, c->status, s));
+                STAGE(5, mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, s));
+                STAGE(6, mdz_launch_dtensor(c->g, c->prm, 1, c->status, s));
+                STAGE(7, mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s));
+                STAGE(8, mdz_launch_dtensor(c->g, c->prm, 0, c->status, s));
+                STAGE(9, { mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], s);
+                           mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, s); });
+            }
+#undef STAGE
+            hipEvent_t last = fused ? ee[8] : ee[9];
+            HIPCHK(hipEventSynchronize(last));
+            for (int i = 0; i < NS; i++) {
+                float t = 0;
+                HIPCHK(hipEventElapsedTime(&t, eb[i], ee[i]));
+                acc[i] += t;
+            }
+            float tt = 0;
+            HIPCHK(hipEventElapsedTime(&tt, eb[0], last));
+            acc[NS] += tt;
         }
-        float tt = 0;
-        HIPCHK(hipEventElapsedTime(&tt, ev[0], ev[NS]));
-        acc[NS] += tt;
-    }
+        return MDZ_OK;
+    };
+    int rc = run();
+    for (int i = 0; i < NS; i++) { hipEventDestroy(eb[i]); hipEventDestroy(ee[i]); }
+    if (rc) return rc;
     for (int i = 0; i <= NS; i++) ms[i] = acc[i] / reps;
-    for (int i = 0; i <= NS; i++) hipEventDestroy(ev[i]);
     HIPCHK(hipGetLastError());
     return MDZ_OK;
 }
@@ -1356,8 +1406,13 @@ static int enqueue_trial(mdz_ctx *c, int k, hipStream_t s) {
     int rc = rheology_pass(c, c->prm.rheo_mode == MDZ_RHEO_LEARNED, s);
     if (rc) return rc;
     mdz_launch_rhs(c->g, c->prm, c->xg, c->zvx, c->xvz, c->zg, s);
-    mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
     int kind = c->model.aniso == 1 ? 1 : 0;
+    if (fused_dtensor(c)) {
+        a = picard_asm(c);
+        mdz_launch_dtensor(c->g, c->prm, 3, c->status, s, c->dpic_n, c->dpic_s);
+    } else {
+        mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
+    }
     mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], s);
     mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, s);
     hipLaunchKernelGGL(k_copy6, dim3(1), dim3(64), 0, s, (const double *)c->norm_out, c->ls_res + 6 * k);
@@ -1651,10 +1706,13 @@ extern "C" int mdz_ensemble_time(mdz_ensemble *e, int reps, double *ms_per_launc
 /* Mean milliseconds of one tunable stage of context c on stream s, over
  * `reps` launches after one untimed launch: 0 = rheology (interpolation,
  * guess and local iterations), 1 = Picard values, 2 = Jacobian values (the
- * D tensor of its kind is put in place first, and the tensor of the
- * residual restored after). */
+ * D tensor of its kind is put in place first; after the Jacobian values the
+ * J = 0 tensor of the residual is restored, or, for a fused model, the
+ * J = 1 tensor is recorded as present, as after a unit). */
 static int time_stage(mdz_ctx *c, int stage, hipStream_t s, int reps, hipEvent_t e0, hipEvent_t e1, float *ms) {
-    DevAsm a = make_asm(c);
+    DevAsm a = make_asm(c), ap = picard_asm(c);
+    const bool fused = fused_dtensor(c);
+    if (stage == 1 && fused) mdz_launch_dtensor(c->g, c->prm, 3, c->status, s, c->dpic_n, c->dpic_s);
     if (stage == 2) mdz_launch_dtensor(c->g, c->prm, 1, c->status, s);
     float t = 0;
     for (int r = 0; r <= reps; r++) {
@@ -1663,7 +1721,7 @@ static int time_stage(mdz_ctx *c, int stage, hipStream_t s, int reps, hipEvent_t
             int rc = rheology_pass(c, c->prm.rheo_mode == MDZ_RHEO_LEARNED, s);
             if (rc) return rc;
         } else if (stage == 1) {
-            mdz_launch_values(0, a, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, s);
+            mdz_launch_values(0, ap, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, s);
         } else {
             mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s);
         }
@@ -1671,7 +1729,10 @@ static int time_stage(mdz_ctx *c, int stage, hipStream_t s, int reps, hipEvent_t
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
     HIPCHK(hipEventElapsedTime(&t, e0, e1));
-    if (stage == 2) mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
+    if (stage == 2) {
+        if (fused) c->d_jac = 1;
+        else mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
+    }
     HIPCHK(hipGetLastError());
     *ms = t / reps;
     return MDZ_OK;

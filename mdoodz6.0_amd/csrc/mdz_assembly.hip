@@ -135,7 +135,7 @@ __device__ void xmom_inner(const DevAsm &a, const TG &T, int c1, int c2, int c3,
     int iVzSW = c3 - nxvz, iVzSE = c3 - nxvz + 1, iVzNW = c3, iVzNE = c3 + 1;
     int iPrW = c2, iPrE = c2 + 1, ixyN = c1, ixyS = c1 - nx;
     if (T.U(iVxC) == -2) { iVxW = c1 + nx - 2; iPrW = c2 + ncx; iVzSW = c3 - 2; iVzNW = c3 + nxvz - 2; }
-    double D11E = g.D11_n[iPrE], D11W = g.D11_n[iPrW], D33N = g.D33_s[ixyN], D33S = g.D33_s[ixyS];
+    double D11E = a.d11p[iPrE], D11W = a.d11p[iPrW], D33N = a.d33p[ixyN], D33S = a.d33p[ixyS];
     double inW = T.P(iPrW) == -1 ? 1.0 : 0.0, inE = T.P(iPrE) == -1 ? 1.0 : 0.0;
     double inS = (T.G(ixyS) != 30 && T.U(ixyS) != 13) ? 1.0 : 0.0;
     double inN = (T.G(c1) != 30 && T.U(c1 + nx) != 13) ? 1.0 : 0.0;
@@ -297,7 +297,7 @@ __device__ void zmom_inner(const DevAsm &a, const TG &T, int c1, int c2, int c3,
                            double &b, double &F) {
     const DevGrid &g = a.g;
     int nx = g.Nx, ncx = g.ncx, nxvz = g.nxvz;
-    double D22S = g.D22_n[c2], D22N = g.D22_n[c2 + ncx], D33E = g.D33_s[c1], D33W = g.D33_s[c1 - 1];
+    double D22S = a.d22p[c2], D22N = a.d22p[c2 + ncx], D33E = a.d33p[c1], D33W = a.d33p[c1 - 1];
     if (T.P(c2 + ncx) == 31) b += g.BCv_val[c3] * a.one_dz;
     int tVS = T.V(c3 - nxvz), tVN = T.V(c3 + nxvz);
     double inS = 0, inN = 0;

@@ -26,7 +26,11 @@ void mdz_launch_phase_constants(const mat_prop *d_mt, int nph, void *d_pk, hipSt
 /* nt: threads per block, 64, 128 or 256 (anything else is MDZ_EINVAL) */
 int mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
                              const float *guess_s, unsigned long long *hist, int nt, hipStream_t s);
-void mdz_launch_dtensor(const DevGrid &g, const DevParams &prm, int jac, int *status, hipStream_t s);
+/* D tensor.  mode 0: the J = 0 tensor into D11_n..D34_s; 1: the J = 1
+ * tensor; 2: the J = 1 tensor, and the J = 0 values of D11 (= D22) at the
+ * centroids and D33 at the vertices into dp_n / dp_s; 3: only those two. */
+void mdz_launch_dtensor(const DevGrid &g, const DevParams &prm, int mode, int *status, hipStream_t s,
+                        double *dp_n = nullptr, double *dp_s = nullptr);
 void mdz_launch_rhs(const DevGrid &g, const DevParams &prm, const double *xg, const double *zvx,
                     const double *xvz, const double *zg, hipStream_t s);
 
@@ -37,6 +41,8 @@ struct DevAsm {
     double celvol, one_dx, one_dz, one_dx_dx, one_dz_dz, one_dx_dz, om, dt, gz_model;
     int stab, comp_model, periodic, Nz_model;
     const double *u, *v, *p_mom, *p_cont; /* iterate: p_corr for momentum rows, p_in for continuity */
+    const double *d11p, *d22p, *d33p;     /* D11, D22 (centroids) and D33 (vertices) of the J = 0 tensor,
+                                             the only components the Picard rows read */
 };
 
 /* numbering: flags -> exclusive scan -> eqn_u/v/p, -12 copies, row->node maps */

@@ -374,7 +374,11 @@ __device__ __forceinline__ void aniso_factors(const DevParams &prm, double nx, d
     }
 }
 
-__global__ void __launch_bounds__(BLK) k_dtensor_cells(DevGrid g, DevParams prm, int jac) {
+/* MODE as for mdz_launch_dtensor: 0 J = 0, 1 J = 1, 2 J = 1 plus the J = 0
+ * Picard components into dp, 3 only those */
+template <int MODE>
+__global__ void __launch_bounds__(BLK) k_dtensor_cells(DevGrid g, DevParams prm, double *__restrict__ dp) {
+    constexpr bool JAC = MODE == 1 || MODE == 2, PIC = MODE >= 2;
     int k = xcd_tile() * BLK + threadIdx.x;
     if (k >= g.ncx * g.ncz) return;
     double D11 = 0, D12 = 0, D13 = 0, D14 = 0, D21 = 0, D22 = 0, D23 = 0, D24 = 0;
@@ -388,7 +392,8 @@ __global__ void __launch_bounds__(BLK) k_dtensor_cells(DevGrid g, DevParams prm,
         D21 = 2.0 * ani * d0 * eta;
         D22 = 2.0 * eta - 2.0 * ani * d0 * eta;
         D23 = -2.0 * ani * d1 * eta;
-        if (jac) {
+        if (JAC) {
+            if (PIC) dp[k] = D11;  /* J = 0: D22 has the same expression */
             double etae = prm.iselastic == 1 ? prm.dt * g.mu_n[k] : 1.0;
             double K = prm.compressible == 1 ? 1.0 / g.bet_n[k] : 0.0;
             double Exx, Ezz, Exz, Gxx, Gzz, Gxz;
@@ -405,12 +410,20 @@ __global__ void __launch_bounds__(BLK) k_dtensor_cells(DevGrid g, DevParams prm,
             D23 = D23 + 2.0 * c * Gzz - Kdt * g.ddivpdgxz_n[k];
             D24 = 2.0 * dpp * Ezz - Kdt * g.ddivpdp_n[k];
         }
+    } else if (PIC) {
+        dp[k] = 0.0;
+    }
+    if (MODE == 3) {
+        if (t != 30 && t != 31) dp[k] = D11;
+        return;
     }
     g.D11_n[k] = D11; g.D12_n[k] = D12; g.D13_n[k] = D13; g.D14_n[k] = D14;
     g.D21_n[k] = D21; g.D22_n[k] = D22; g.D23_n[k] = D23; g.D24_n[k] = D24;
 }
 
-__global__ void __launch_bounds__(BLK) k_dtensor_vertices(DevGrid g, DevParams prm, int jac, int *status) {
+template <int MODE>
+__global__ void __launch_bounds__(BLK) k_dtensor_vertices(DevGrid g, DevParams prm, int *status, double *__restrict__ dp) {
+    constexpr bool JAC = MODE == 1 || MODE == 2, PIC = MODE >= 2;
     int k = xcd_tile() * BLK + threadIdx.x;
     if (k >= g.Nx * g.Nz) return;
     double D31 = 0, D32 = 0, D33 = 0, D34 = 0;
@@ -420,7 +433,8 @@ __global__ void __launch_bounds__(BLK) k_dtensor_vertices(DevGrid g, DevParams p
         D31 = 2.0 * ani * d1 * eta;
         D32 = -2.0 * ani * d1 * eta;
         D33 = eta + 2.0 * ani * (d0 - 0.5) * eta;
-        if (jac) {
+        if (JAC) {
+            if (PIC) dp[k] = D33;
             double etae = prm.iselastic == 1 ? prm.dt * g.mu_s[k] : 1.0;
             double Exx, Ezz, Exz, Gxx, Gzz, Gxz;
             ve_rates(prm, g.nx0_s[k], g.nz0_s[k], g.aniso_factor_s[k], g.FS_AR_s[k], g.exxd_s[k], g.ezzd_s[k],
@@ -430,9 +444,15 @@ __global__ void __launch_bounds__(BLK) k_dtensor_vertices(DevGrid g, DevParams p
             D33 = D33 + 2.0 * g.detadgxz_s[k] * Gxz;
             D34 = 2.0 * g.detadp_s[k] * Exz;
         }
+    } else if (PIC) {
+        dp[k] = 0.0;
+    }
+    if (MODE == 3) {
+        if (g.BCg[k] != 30) dp[k] = D33;
+        return;
     }
     g.D31_s[k] = D31; g.D32_s[k] = D32; g.D33_s[k] = D33; g.D34_s[k] = D34;
-    if (jac && status) {
+    if (JAC && status) {
         if (isnan(D34)) atomicOr(status, 1);
         if (isinf(D34)) atomicOr(status, 2);
     }
@@ -566,10 +586,28 @@ void mdz_launch_rheology(const DevGrid &g, const DevParams &prm, const mat_prop 
     hipLaunchKernelGGL(k_rheo_vertices, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm, mt, guess_s, hist);
 }
 
-void mdz_launch_dtensor(const DevGrid &g, const DevParams &prm, int jac, int *status, hipStream_t s) {
+void mdz_launch_dtensor(const DevGrid &g, const DevParams &prm, int mode, int *status, hipStream_t s, double *dp_n,
+                        double *dp_s) {
     int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
-    hipLaunchKernelGGL(k_dtensor_cells, dim3(mdz_div_up(nc, BLK)), dim3(BLK), 0, s, g, prm, jac);
-    hipLaunchKernelGGL(k_dtensor_vertices, dim3(mdz_div_up(nv, BLK)), dim3(BLK), 0, s, g, prm, jac, status);
+    dim3 gc(mdz_div_up(nc, BLK)), gv(mdz_div_up(nv, BLK)), b(BLK);
+    switch (mode) {
+    case 0:
+        hipLaunchKernelGGL(k_dtensor_cells<0>, gc, b, 0, s, g, prm, dp_n);
+        hipLaunchKernelGGL(k_dtensor_vertices<0>, gv, b, 0, s, g, prm, status, dp_s);
+        break;
+    case 1:
+        hipLaunchKernelGGL(k_dtensor_cells<1>, gc, b, 0, s, g, prm, dp_n);
+        hipLaunchKernelGGL(k_dtensor_vertices<1>, gv, b, 0, s, g, prm, status, dp_s);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_dtensor_cells<2>, gc, b, 0, s, g, prm, dp_n);
+        hipLaunchKernelGGL(k_dtensor_vertices<2>, gv, b, 0, s, g, prm, status, dp_s);
+        break;
+    default:
+        hipLaunchKernelGGL(k_dtensor_cells<3>, gc, b, 0, s, g, prm, dp_n);
+        hipLaunchKernelGGL(k_dtensor_vertices<3>, gv, b, 0, s, g, prm, status, dp_s);
+        break;
+    }
 }
 
 void mdz_launch_rhs(const DevGrid &g, const DevParams &prm, const double *xg, const double *zvx,
