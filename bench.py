@@ -49,8 +49,8 @@ def stage_kernels(shape, learned=False, fused=True):
             "dtensor_jacobian": d1,
             "jacobian_values": [vj] if jac else [vj, "k_values_cont"],
             "dtensor_picard_2": d0,
-            "residual": ["k_residual_mom<0, true>", "k_residual_mom<0, false>", "k_residual_cont", "k_norms",
-                         "k_norms_final"]}
+            "residual": ["k_residual_mom<0, true, true>", "k_residual_mom<0, false, true>", "k_residual_cont<true>",
+                         "k_norms_final_slots"]}
 
 
 BYTES_PER_CELL = {"strain_rates": 76, "interp_guess": 80, "rheology": 730, "rhs": 160, "dtensor_picard": 114,

@@ -117,6 +117,7 @@ struct mdz_ctx {
     int *scratch;
     size_t scratch_len;
     double *norm_partial, *norm_out;
+    size_t norm_partial_len;
     int *status;
     unsigned long long *hist;
     int hist_on;         /* collect local-iteration histograms (off by default) */
@@ -207,7 +208,7 @@ static void ctx_free_blocks(mdz_ctx *c) {
     for (int i = 0; i < 8; i++) {
         hipFree(c->blk[i].Ic); hipFree(c->blk[i].J); hipFree(c->blk[i].A);
         hipFree(c->blk[i].b); hipFree(c->blk[i].bbc); hipFree(c->blk[i].F); hipFree(c->blk[i].d);
-        hipFree(c->blk[i].free_row);
+        hipFree(c->blk[i].free_row); hipFree(c->blk[i].gen_waves);
         memset(&c->blk[i], 0, sizeof(DevBlock));
         c->blk_cap[i] = 0;
     }
@@ -275,7 +276,11 @@ static int ctx_alloc(mdz_ctx **out, int Nx, int Nz, int nph) {
     {
         int nbu = mdz_div_up((int)nu, 256), nbv = mdz_div_up((int)nw, 256), nbp = mdz_div_up((int)nc, 256);
         np = 2 * (size_t)(nbu + nbv + nbp);
+        /* fused residual norms: 256-row tiles and at most one 64-row wave
+         * per 64 momentum rows, then the continuity tiles */
+        np = std::max(np, mdz_residual_norm_partials((int)(nu + nw), mdz_div_up((int)(nu + nw), 64), (int)nc));
     }
+    c->norm_partial_len = np;
     if (hipMalloc((void **)&c->norm_partial, np * sizeof(double) + 256) != hipSuccess) return set_err(MDZ_ENOMEM, "norms");
     if (hipMalloc((void **)&c->norm_out, 8 * sizeof(double)) != hipSuccess) return set_err(MDZ_ENOMEM, "norms");
     if (hipMalloc((void **)&c->status, 4 * sizeof(int)) != hipSuccess) return set_err(MDZ_ENOMEM, "status");
@@ -399,7 +404,7 @@ static int ensure_rows(mdz_ctx *c) {
         if (c->blk[i].Ic && c->blk[i].neq == neq) continue;
         hipFree(c->blk[i].Ic); hipFree(c->blk[i].J); hipFree(c->blk[i].A);
         hipFree(c->blk[i].b); hipFree(c->blk[i].bbc); hipFree(c->blk[i].F); hipFree(c->blk[i].d);
-        hipFree(c->blk[i].free_row);
+        hipFree(c->blk[i].free_row); hipFree(c->blk[i].gen_waves);
         memset(&c->blk[i], 0, sizeof(DevBlock));
         c->blk_cap[i] = 0;
         int rc = block_alloc_rows(&c->blk[i], neq);
@@ -442,7 +447,11 @@ static int build_pattern(mdz_ctx *c, int kind) {
     /* rows whose stencil touches only free degrees of freedom take the
      * tag-free value and residual kernels */
     if (!A->free_row && A->neq > 0) HIPCHK(hipMalloc((void **)&A->free_row, (size_t)A->neq));
+    if (!A->gen_waves && A->neq > 0)
+        HIPCHK(hipMalloc((void **)&A->gen_waves, 2 * ((size_t)A->neq / 64 + 2) * sizeof(int)));
     if ((rc = mdz_launch_free_rows(kind, a, A, B, c->stream))) return set_err(rc, "free-row flags");
+    if ((rc = mdz_build_general_waves(A, A->neq, c->scratch, c->scratch_len, c->stream)))
+        return set_err(rc, "general-wave list");
     c->pattern_ready[kind] = 1;
     c->version++; /* captured graphs hold the block pointers */
     return sync(c);
@@ -1116,6 +1125,13 @@ extern "C" int mdz_ctx_set_rheology_mode(mdz_ctx *c, int mode) {
 /* One Newton-iteration unit of a context, enqueued on stream s with no
  * allocation, memset, host synchronisation or copy: safe to capture into a
  * graph.  The norms stay in c->norm_out (six doubles). */
+/* residual rows of block family `kind` and their norms into c->norm_out */
+static int residual_norms(mdz_ctx *c, int kind, const DevAsm &a, hipStream_t s) {
+    int rc = mdz_launch_residual_norms(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3],
+                                       c->norm_partial, c->norm_partial_len, c->norm_out, s);
+    return rc ? set_err(rc, "residual norms") : MDZ_OK;
+}
+
 static int enqueue_unit(mdz_ctx *c, hipStream_t s) {
     DevAsm a = make_asm(c), ap = picard_asm(c);
     mdz_launch_strain_rates(c->g, c->prm, s);
@@ -1126,8 +1142,7 @@ static int enqueue_unit(mdz_ctx *c, hipStream_t s) {
     if (fused_dtensor(c)) {
         mdz_launch_dtensor(c->g, c->prm, 2, c->status, s, c->dpic_n, c->dpic_s);
         mdz_launch_values(0, ap, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, s);
-        mdz_launch_residual(0, ap, &c->blk[0], &c->blk[2], &c->blk[3], s);
-        mdz_launch_norms(ap, &c->blk[0], &c->blk[3], c->norm_partial, c->norm_out, s);
+        if ((rc = residual_norms(c, 0, ap, s))) return rc;
         mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s);
         c->d_jac = 1;
         return MDZ_OK;
@@ -1137,9 +1152,7 @@ static int enqueue_unit(mdz_ctx *c, hipStream_t s) {
     mdz_launch_dtensor(c->g, c->prm, 1, c->status, s);
     mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s);
     mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
-    mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], s);
-    mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, s);
-    return MDZ_OK;
+    return residual_norms(c, kind, a, s);
 }
 
 /* resx, resz, resp from the six sums of the norm kernels */
@@ -1273,8 +1286,7 @@ extern "C" int mdz_ctx_stage_times(mdz_ctx *c, int reps, double *ms) {
                 STAGE(6, mdz_launch_dtensor(c->g, c->prm, 2, c->status, s, c->dpic_n, c->dpic_s));
                 STAGE(4, (void)0);
                 STAGE(5, mdz_launch_values(0, ap, &c->blk[0], &c->blk[1], &c->blk[2], &c->blk[3], c->shape.pic, s));
-                STAGE(9, { mdz_launch_residual(0, ap, &c->blk[0], &c->blk[2], &c->blk[3], s);
-                           mdz_launch_norms(ap, &c->blk[0], &c->blk[3], c->norm_partial, c->norm_out, s); });
+                STAGE(9, if ((rc = residual_norms(c, 0, ap, s))) return rc);
                 STAGE(7, mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s));
                 STAGE(8, (void)0);
                 c->d_jac = 1;
@@ -1284,8 +1296,7 @@ extern "C" int mdz_ctx_stage_times(mdz_ctx *c, int reps, double *ms) {
                 STAGE(6, mdz_launch_dtensor(c->g, c->prm, 1, c->status, s));
                 STAGE(7, mdz_launch_values(1, a, &c->blk[4], &c->blk[5], &c->blk[6], &c->blk[7], c->shape.jac, s));
                 STAGE(8, mdz_launch_dtensor(c->g, c->prm, 0, c->status, s));
-                STAGE(9, { mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], s);
-                           mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, s); });
+                STAGE(9, if ((rc = residual_norms(c, kind, a, s))) return rc);
             }
 #undef STAGE
             hipEvent_t last = fused ? ee[8] : ee[9];
@@ -1413,8 +1424,7 @@ static int enqueue_trial(mdz_ctx *c, int k, hipStream_t s) {
     } else {
         mdz_launch_dtensor(c->g, c->prm, 0, c->status, s);
     }
-    mdz_launch_residual(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3], s);
-    mdz_launch_norms(a, &c->blk[4 * kind], &c->blk[4 * kind + 3], c->norm_partial, c->norm_out, s);
+    if ((rc = residual_norms(c, kind, a, s))) return rc;
     hipLaunchKernelGGL(k_copy6, dim3(1), dim3(64), 0, s, (const double *)c->norm_out, c->ls_res + 6 * k);
     return MDZ_OK;
 }

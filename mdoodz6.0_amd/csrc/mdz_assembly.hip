@@ -1099,24 +1099,86 @@ __global__ void __launch_bounds__(BLK) k_values_cont(DevAsm a, DevBlock C, DevBl
     values_cont_row(a, C, D, eqn, sDump + threadIdx.x);
 }
 
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+/* Slots of the per-block sums of a fused residual + norm pass:
+ * partial[q * nsm + slot], q = 0..3 (u F^2, u count, v F^2, v count), for
+ * the momentum launches, then partial[4 * nsm + q * nsc + slot], q = 0..1
+ * (p F^2, p count), for the continuity launch. */
+struct NormSlots {
+    double *partial;
+    int nsm, nsc;
+    int slot0;        /* first momentum slot of this launch */
+    const int *waves; /* general momentum launch: its waves, one per block */
+};
+
+/* NQ per-thread values summed over the block in a fixed order (waves by
+ * shuffles, then the waves in order); thread 0 gets the sums */
+template <int NQ>
+__device__ __forceinline__ void block_sums(double (&v)[NQ]) {
+    __shared__ double ws[BLK / 64][NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) v[q] = wave_sum(v[q]);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) ws[w][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            double t = 0.0;
+            for (int k = 0; k < nw; k++) t += ws[k][q];
+            v[q] = t;
+        }
+}
+
 /* residual (Assemble = 0): b = rhs (Neumann rows adjusted), F.  Waves
- * whose rows are all free take the tag-free instance. */
-template <int KIND, bool FREE>
-__global__ void __launch_bounds__(BLK) k_residual_mom(DevAsm a, DevBlock A) {
-    int eqn = xcd_tile() * BLK + threadIdx.x;
+ * whose rows are all free take the tag-free instance (FREE), the others the
+ * general one.  NORM: the norm pass of EvaluateStokesResidualDecoupled
+ * (StokesRoutines.c:1390-1452) on the row as well: ru / rv of its node = F,
+ * F *= d, and per-block sums of F^2 and of the rows per family.  The
+ * general NORM instance runs one 64-row wave per block, the waves listed in
+ * ns.waves; the others run 256-row tiles. */
+template <int KIND, bool FREE, bool NORM>
+__global__ void __launch_bounds__(BLK) k_residual_mom(DevAsm a, DevBlock A, NormSlots ns) {
+    const int eqn = (NORM && !FREE) ? ns.waves[blockIdx.x] * 64 + (int)threadIdx.x : xcd_tile() * BLK + (int)threadIdx.x;
+    bool run = eqn < a.e.neq_mom;
     if (A.free_row) {
-        const int fr = eqn < a.e.neq_mom ? A.free_row[eqn] : 1;
-        if ((__all(fr) != 0) != FREE) return;
+        const int fr = run ? A.free_row[eqn] : 1;
+        if ((__all(fr) != 0) != FREE) run = false;
     } else if (FREE) {
-        return;
+        run = false;
     }
-    if (eqn >= a.e.neq_mom) return;
-    Sink<SINK_COUNT> ra{0, nullptr, nullptr, 0.0}, rb{0, nullptr, nullptr, 0.0};
-    double b = mom_rhs(a, eqn), F = A.F[eqn];
-    if (FREE) momentum_row<false, SINK_COUNT>(KIND, a, FreeTags{}, eqn, ra, rb, b, A.bbc[eqn], F);
-    else momentum_row<false, SINK_COUNT>(KIND, a, make_tags(a.g), eqn, ra, rb, b, A.bbc[eqn], F);
-    A.b[eqn] = b;
-    A.F[eqn] = F;
+    if (!NORM && !run) return;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (run) {
+        Sink<SINK_COUNT> ra{0, nullptr, nullptr, 0.0}, rb{0, nullptr, nullptr, 0.0};
+        double b = mom_rhs(a, eqn), F = A.F[eqn];
+        if (FREE) momentum_row<false, SINK_COUNT>(KIND, a, FreeTags{}, eqn, ra, rb, b, A.bbc[eqn], F);
+        else momentum_row<false, SINK_COUNT>(KIND, a, make_tags(a.g), eqn, ra, rb, b, A.bbc[eqn], F);
+        A.b[eqn] = b;
+        if (NORM) {
+            const int node = a.e.mom_node[eqn], nu = a.g.Nx * a.g.nzvx;
+            if (node < nu) { a.g.ru[node] = F; v[0] = F * F; v[1] = 1.0; }
+            else { a.g.rv[node - nu] = F; v[2] = F * F; v[3] = 1.0; }
+            A.F[eqn] = F * A.d[eqn];
+        } else {
+            A.F[eqn] = F;
+        }
+    }
+    if (NORM) {
+        block_sums<4>(v);
+        if (threadIdx.x == 0) {
+            const int slot = ns.slot0 + (FREE ? xcd_tile() : (int)blockIdx.x);
+#pragma unroll
+            for (int q = 0; q < 4; q++) ns.partial[q * ns.nsm + slot] = v[q];
+        }
+    }
 }
 
 /* free_row[eqn] = 1 when every tag the row reads, in the value and in the
@@ -1143,19 +1205,37 @@ __global__ void __launch_bounds__(64) k_free_rows(int kind, DevAsm a, unsigned c
     free_row[eqn] = T.all_free ? 1 : 0;
 }
 
-__global__ void __launch_bounds__(BLK) k_residual_cont(int kind, DevAsm a, DevBlock C, DevBlock D) {
+template <bool NORM>
+__global__ void __launch_bounds__(BLK) k_residual_cont(int kind, DevAsm a, DevBlock C, DevBlock D, NormSlots ns) {
     int eqn = xcd_tile() * BLK + threadIdx.x;
-    if (eqn >= a.e.neq_cont) return;
-    int c1, c2, c3;
-    cont_geom(a, eqn, c1, c2, c3);
-    /* the Jacobian builder reads comp_cells at the vertex-family index c1 */
-    int comp = kind == 1 ? a.g.comp_cells[c1] : a.g.comp_cells[c2];
-    Sink<SINK_COUNT> rc{0, nullptr, nullptr, 0.0};
-    double b = a.g.rhs_p[c2], F = 0.0;
-    continuity_row<false, SINK_COUNT>(a, comp, c1, c2, c3, rc, b, F);
-    C.b[eqn] = b;
-    C.F[eqn] = F;
-    D.F[eqn] = F;
+    if (!NORM && eqn >= a.e.neq_cont) return;
+    double v[2] = {0.0, 0.0};
+    if (eqn < a.e.neq_cont) {
+        int c1, c2, c3;
+        cont_geom(a, eqn, c1, c2, c3);
+        /* the Jacobian builder reads comp_cells at the vertex-family index c1 */
+        int comp = kind == 1 ? a.g.comp_cells[c1] : a.g.comp_cells[c2];
+        Sink<SINK_COUNT> rc{0, nullptr, nullptr, 0.0};
+        double b = a.g.rhs_p[c2], F = 0.0;
+        continuity_row<false, SINK_COUNT>(a, comp, c1, c2, c3, rc, b, F);
+        C.b[eqn] = b;
+        C.F[eqn] = F;
+        if (NORM) {
+            a.g.rp[c2] = F;
+            v[0] = F * F; v[1] = 1.0;
+            D.F[eqn] = F * D.d[eqn];
+        } else {
+            D.F[eqn] = F;
+        }
+    }
+    if (NORM) {
+        block_sums<2>(v);
+        if (threadIdx.x == 0) {
+            const int slot = xcd_tile();
+            ns.partial[4 * ns.nsm + slot] = v[0];
+            ns.partial[4 * ns.nsm + ns.nsc + slot] = v[1];
+        }
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -1408,15 +1488,97 @@ void mdz_launch_residual(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, De
     int nm = a.e.neq_mom, ncnt = a.e.neq_cont;
     if (nm > 0) {
         const dim3 grid(mdz_div_up(nm, BLK));
+        const NormSlots ns{};
         if (kind == 0) {
-            if (A->free_row) hipLaunchKernelGGL((k_residual_mom<0, true>), grid, dim3(BLK), 0, s, a, *A);
-            hipLaunchKernelGGL((k_residual_mom<0, false>), grid, dim3(BLK), 0, s, a, *A);
+            if (A->free_row) hipLaunchKernelGGL((k_residual_mom<0, true, false>), grid, dim3(BLK), 0, s, a, *A, ns);
+            hipLaunchKernelGGL((k_residual_mom<0, false, false>), grid, dim3(BLK), 0, s, a, *A, ns);
         } else {
-            if (A->free_row) hipLaunchKernelGGL((k_residual_mom<1, true>), grid, dim3(BLK), 0, s, a, *A);
-            hipLaunchKernelGGL((k_residual_mom<1, false>), grid, dim3(BLK), 0, s, a, *A);
+            if (A->free_row) hipLaunchKernelGGL((k_residual_mom<1, true, false>), grid, dim3(BLK), 0, s, a, *A, ns);
+            hipLaunchKernelGGL((k_residual_mom<1, false, false>), grid, dim3(BLK), 0, s, a, *A, ns);
         }
     }
-    if (ncnt > 0) hipLaunchKernelGGL(k_residual_cont, dim3(mdz_div_up(ncnt, BLK)), dim3(BLK), 0, s, kind, a, *C, *D);
+    if (ncnt > 0)
+        hipLaunchKernelGGL(k_residual_cont<false>, dim3(mdz_div_up(ncnt, BLK)), dim3(BLK), 0, s, kind, a, *C, *D,
+                           NormSlots{});
+}
+
+/* flag[w] = 1 when wave w (rows 64 w .. 64 w + 63) holds a row that is not free */
+__global__ void __launch_bounds__(BLK) k_general_wave_flags(const unsigned char *free_row, int nm, int *flag) {
+    const int w = blockIdx.x * BLK + threadIdx.x;
+    if (w >= mdz_div_up(nm, 64)) return;
+    int f = 0;
+    for (int r = w * 64; r < min(nm, w * 64 + 64); r++) f |= free_row[r] == 0;
+    flag[w] = f;
+}
+
+/* list[pos[w]] = w for the flagged waves (pos: exclusive scan of the flags, pos[nw] = total) */
+__global__ void __launch_bounds__(BLK) k_general_wave_list(const int *pos, int nwv, int *list) {
+    const int w = blockIdx.x * BLK + threadIdx.x;
+    if (w < nwv && pos[w + 1] != pos[w]) list[pos[w]] = w;
+}
+
+int mdz_build_general_waves(DevBlock *A, int nm, int *scratch, size_t scratch_len, hipStream_t s) {
+    A->n_gen = 0;
+    if (nm <= 0) return MDZ_OK;
+    if (!A->free_row || !A->gen_waves) return MDZ_EINVAL;
+    const int nwv = mdz_div_up(nm, 64);
+    int *list = A->gen_waves, *pos = A->gen_waves + nwv + 1;
+    hipLaunchKernelGGL(k_general_wave_flags, dim3(mdz_div_up(nwv, BLK)), dim3(BLK), 0, s,
+                       (const unsigned char *)A->free_row, nm, pos);
+    int total = 0;
+    int rc = mdz_scan_exclusive(pos, nwv, scratch, scratch_len, s, &total);
+    if (rc) return rc;
+    if (hipMemcpyAsync(pos + nwv, &total, sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess) return MDZ_EHIP;
+    hipLaunchKernelGGL(k_general_wave_list, dim3(mdz_div_up(nwv, BLK)), dim3(BLK), 0, s, (const int *)pos, nwv, list);
+    if (hipStreamSynchronize(s) != hipSuccess) return MDZ_EHIP;
+    A->n_gen = total;
+    return MDZ_OK;
+}
+
+size_t mdz_residual_norm_partials(int nm, int n_gen, int ncnt) {
+    return 4 * ((size_t)mdz_div_up(nm, BLK) + (size_t)n_gen) + 2 * (size_t)mdz_div_up(ncnt, BLK);
+}
+
+/* one block per family: the slots of its sums and counts, in slot order */
+__global__ void __launch_bounds__(1024) k_norms_final_slots(const double *partial, int nsm, int nsc, double *out6) {
+    const int f = blockIdx.x;
+    const double *ps = f == 2 ? partial + 4 * (size_t)nsm : partial + 2 * f * (size_t)nsm;
+    const int n = f == 2 ? nsc : nsm;
+    const double *pc = ps + n;
+    double s = 0, c = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) { s += ps[i]; c += pc[i]; }
+    __shared__ double ws[1024 / 64][2];
+    s = wave_sum(s);
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) { ws[threadIdx.x >> 6][0] = s; ws[threadIdx.x >> 6][1] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ts = 0, tc = 0;
+        for (int q = 0; q < 1024 / 64; q++) { ts += ws[q][0]; tc += ws[q][1]; }
+        out6[f] = ts;
+        out6[3 + f] = tc;
+    }
+}
+
+int mdz_launch_residual_norms(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, DevBlock *D, double *partial,
+                              size_t partial_len, double *out6, hipStream_t s) {
+    const int nm = a.e.neq_mom, ncnt = a.e.neq_cont;
+    if (nm > 0 && (!A->free_row || !A->gen_waves)) return MDZ_EINVAL;
+    const int nbf = mdz_div_up(nm, BLK), ngen = nm > 0 ? A->n_gen : 0, nbc = mdz_div_up(ncnt, BLK);
+    if (mdz_residual_norm_partials(nm, ngen, ncnt) > partial_len) return MDZ_ENOMEM;
+    NormSlots ns{partial, nbf + ngen, nbc, 0, A->gen_waves};
+    if (nm > 0) {
+        if (kind == 0) hipLaunchKernelGGL((k_residual_mom<0, true, true>), dim3(nbf), dim3(BLK), 0, s, a, *A, ns);
+        else hipLaunchKernelGGL((k_residual_mom<1, true, true>), dim3(nbf), dim3(BLK), 0, s, a, *A, ns);
+        if (ngen > 0) {
+            ns.slot0 = nbf;
+            if (kind == 0) hipLaunchKernelGGL((k_residual_mom<0, false, true>), dim3(ngen), dim3(64), 0, s, a, *A, ns);
+            else hipLaunchKernelGGL((k_residual_mom<1, false, true>), dim3(ngen), dim3(64), 0, s, a, *A, ns);
+        }
+    }
+    if (ncnt > 0) hipLaunchKernelGGL(k_residual_cont<true>, dim3(nbc), dim3(BLK), 0, s, kind, a, *C, *D, ns);
+    hipLaunchKernelGGL(k_norms_final_slots, dim3(3), dim3(1024), 0, s, (const double *)partial, nbf + ngen, nbc, out6);
+    return MDZ_OK;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1472,11 +1634,6 @@ void mdz_launch_scale_matrix(DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D,
 /* norms                                                                */
 /* ------------------------------------------------------------------ */
 #define NRM_BLK 256
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 /* per node family: F^2 sums and DOF counts, residual fields, F *= d.
  * block partials: [3 families][2 quantities][nblocks] */

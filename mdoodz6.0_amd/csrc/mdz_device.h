@@ -97,6 +97,9 @@ struct DevBlock {
     int *Ic, *J;
     double *A, *b, *bbc, *F, *d;
     unsigned char *free_row; /* momentum blocks: 1 = every tag the row reads is -1 (or null) */
+    int *gen_waves;          /* momentum blocks: the 64-row waves with a row that is not free, in
+                                increasing order (n_gen of them), then scratch for their scan */
+    int n_gen;
 };
 
 /* Sinks through which one stencil row is emitted.  The emission order and

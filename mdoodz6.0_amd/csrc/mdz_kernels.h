@@ -78,6 +78,18 @@ void mdz_launch_residual(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, De
  * (sum_x, sum_z, sum_p, ndof_x, ndof_z, ndof_p) */
 size_t mdz_norm_partials(const DevGrid &g);
 void mdz_launch_norms(const DevAsm &a, DevBlock *A, DevBlock *D, double *partial, double *out6, hipStream_t s);
+/* the waves of A that are not all free rows, into A->gen_waves / A->n_gen
+ * (after the free-row flags; A->gen_waves holds 2 * (neq_mom / 64 + 2)
+ * ints: the list, then the flags and their scan; synchronises s to read
+ * the count) */
+int mdz_build_general_waves(DevBlock *A, int neq_mom, int *scratch, size_t scratch_len, hipStream_t s);
+/* residual rows and their norms in one pass (the sums of k_norms, over the
+ * equations instead of the nodes): b, F, ru/rv/rp, F *= d, out6 as for
+ * mdz_launch_norms.  partial: at least mdz_residual_norm_partials(...)
+ * doubles (MDZ_ENOMEM when partial_len is smaller). */
+size_t mdz_residual_norm_partials(int neq_mom, int n_gen, int neq_cont);
+int mdz_launch_residual_norms(int kind, const DevAsm &a, DevBlock *A, DevBlock *C, DevBlock *D, double *partial,
+                              size_t partial_len, double *out6, hipStream_t s);
 
 /* diagonal scaling: d from the diagonals of A and D (ExtractDiagonalScale),
  * then A_ij d_i d_j with b, F scaled by d (ScaleMatrix) */
