@@ -1546,6 +1546,9 @@ __global__ void __launch_bounds__(1024) k_norms_final_slots(const double *partia
     const int n = f == 2 ? nsc : nsm;
     const double *pc = ps + n;
     double s = 0, c = 0;
+    /* unrolled so that the loads of several iterations are in flight; the
+     * additions keep their order */
+#pragma unroll 8
     for (int i = threadIdx.x; i < n; i += 1024) { s += ps[i]; c += pc[i]; }
     __shared__ double ws[1024 / 64][2];
     s = wave_sum(s);
