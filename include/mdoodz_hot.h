@@ -362,7 +362,8 @@ int  mdz_ctx_iteration_histogram(mdz_ctx *ctx, long long *hist192);
  * interpolation (+ the network's guess in learned mode), local iterations
  * (centroid and vertex rheology kernels), right-hand sides, D(Picard),
  * Picard values, D(Jacobian), Jacobian values, D(Picard), residual + norms;
- * ms[10] = whole unit. */
+ * ms[10] = whole unit.  An isotropic model (aniso = 0) runs one fused D
+ * pass, timed as D(Jacobian); both D(Picard) stages are then 0. */
 int  mdz_ctx_stage_times(mdz_ctx *ctx, int reps, double *ms11);
 /* Mean milliseconds per unit over `reps` back-to-back units. */
 int  mdz_ctx_time_units(mdz_ctx *ctx, int reps, double *ms_per_unit);
