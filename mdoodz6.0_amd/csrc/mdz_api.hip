@@ -1122,9 +1122,6 @@ extern "C" int mdz_ctx_set_rheology_mode(mdz_ctx *c, int mode) {
     return MDZ_OK;
 }
 
-/* One Newton-iteration unit of a context, enqueued on stream s with no
- * allocation, memset, host synchronisation or copy: safe to capture into a
- * graph.  The norms stay in c->norm_out (six doubles). */
 /* residual rows of block family `kind` and their norms into c->norm_out */
 static int residual_norms(mdz_ctx *c, int kind, const DevAsm &a, hipStream_t s) {
     int rc = mdz_launch_residual_norms(kind, a, &c->blk[4 * kind], &c->blk[4 * kind + 2], &c->blk[4 * kind + 3],
@@ -1132,6 +1129,9 @@ static int residual_norms(mdz_ctx *c, int kind, const DevAsm &a, hipStream_t s) 
     return rc ? set_err(rc, "residual norms") : MDZ_OK;
 }
 
+/* One Newton-iteration unit of a context, enqueued on stream s with no
+ * allocation, memset, host synchronisation or copy: safe to capture into a
+ * graph.  The norms stay in c->norm_out (six doubles). */
 static int enqueue_unit(mdz_ctx *c, hipStream_t s) {
     DevAsm a = make_asm(c), ap = picard_asm(c);
     mdz_launch_strain_rates(c->g, c->prm, s);
