@@ -139,17 +139,37 @@ __device__ void xmom_inner(const DevAsm &a, const TG &T, int c1, int c2, int c3,
     double inW = T.P(iPrW) == -1 ? 1.0 : 0.0, inE = T.P(iPrE) == -1 ? 1.0 : 0.0;
     double inS = (T.G(ixyS) != 30 && T.U(ixyS) != 13) ? 1.0 : 0.0;
     double inN = (T.G(c1) != 30 && T.U(c1 + nx) != 13) ? 1.0 : 0.0;
-    double dx = g.dx, dz = g.dz, comp = 1.0;
-    double uW = D11W * inW * ((1.0 / 3.0) * comp / dx - 1.0 / dx) / dx;
-    double uC = -(-D33N * inN / dz - D33S * inS / dz) / dz
-              - (D11E * inE * ((1.0 / 3.0) * comp / dx - 1 / dx) - D11W * inW * (-1.0 / 3.0 * comp / dx + 1.0 / dx)) / dx;
-    double uE = -D11E * inE * (-1.0 / 3.0 * comp / dx + 1.0 / dx) / dx;
-    double uS = -D33S * inS / (dz * dz);
-    double uN = -D33N * inN / (dz * dz);
-    double vSW = (1.0 / 3.0) * D11W * comp * inW / (dx * dz) - D33S * inS / (dx * dz);
-    double vSE = -1.0 / 3.0 * D11E * comp * inE / (dx * dz) + D33S * inS / (dx * dz);
-    double vNW = -1.0 / 3.0 * D11W * comp * inW / (dx * dz) + D33N * inN / (dx * dz);
-    double vNE = (1.0 / 3.0) * D11E * comp * inE / (dx * dz) - D33N * inN / (dx * dz);
+    const double comp = 1.0;
+    double uW, uC, uE, uS, uN, vSW, vSE, vNW, vNE;
+    if (ASM) {
+        /* matrix values: the reference's divisions by dx, dz, dx dz and dz^2
+         * as products with the per-grid reciprocals (one rounding apart) */
+        const double idx = a.one_dx, idz = a.one_dz, idxz = a.one_dx_dz, idzz = a.one_dz_dz;
+        uW = D11W * inW * ((1.0 / 3.0) * comp * idx - idx) * idx;
+        uC = -(-D33N * inN * idz - D33S * inS * idz) * idz
+           - (D11E * inE * ((1.0 / 3.0) * comp * idx - idx) - D11W * inW * (-1.0 / 3.0 * comp * idx + idx)) * idx;
+        uE = -D11E * inE * (-1.0 / 3.0 * comp * idx + idx) * idx;
+        uS = -D33S * inS * idzz;
+        uN = -D33N * inN * idzz;
+        vSW = (1.0 / 3.0) * D11W * comp * inW * idxz - D33S * inS * idxz;
+        vSE = -1.0 / 3.0 * D11E * comp * inE * idxz + D33S * inS * idxz;
+        vNW = -1.0 / 3.0 * D11W * comp * inW * idxz + D33N * inN * idxz;
+        vNE = (1.0 / 3.0) * D11E * comp * inE * idxz - D33N * inN * idxz;
+    } else {
+        /* the residual keeps the reference's divisions: F is a small
+         * difference of large terms, so its parity needs the same roundings */
+        const double dx = g.dx, dz = g.dz;
+        uW = D11W * inW * ((1.0 / 3.0) * comp / dx - 1.0 / dx) / dx;
+        uC = -(-D33N * inN / dz - D33S * inS / dz) / dz
+           - (D11E * inE * ((1.0 / 3.0) * comp / dx - 1 / dx) - D11W * inW * (-1.0 / 3.0 * comp / dx + 1.0 / dx)) / dx;
+        uE = -D11E * inE * (-1.0 / 3.0 * comp / dx + 1.0 / dx) / dx;
+        uS = -D33S * inS / (dz * dz);
+        uN = -D33N * inN / (dz * dz);
+        vSW = (1.0 / 3.0) * D11W * comp * inW / (dx * dz) - D33S * inS / (dx * dz);
+        vSE = -1.0 / 3.0 * D11E * comp * inE / (dx * dz) + D33S * inS / (dx * dz);
+        vNW = -1.0 / 3.0 * D11W * comp * inW / (dx * dz) + D33N * inN / (dx * dz);
+        vNE = (1.0 / 3.0) * D11E * comp * inE / (dx * dz) - D33N * inN / (dx * dz);
+    }
     double pW = 0, pE = 0;
     if (a.stab == 1) {
         double drhodx = (g.rho_n[c2 + 1] - g.rho_n[c2]) * a.one_dx;
@@ -308,17 +328,35 @@ __device__ void zmom_inner(const DevAsm &a, const TG &T, int c1, int c2, int c3,
     if (T.P(c2 + ncx) == -1) inN = 1.0;
     double inW = (T.G(c1 - 1) != 30 && T.V(c3 - 1) != 13) ? 1.0 : 0.0;
     double inE = (T.G(c1) != 30 && T.V(c3 + 1) != 13) ? 1.0 : 0.0;
-    double dx = g.dx, dz = g.dz, comp = 1.0;
-    double vW = -D33W * inW / (dx * dx);
-    double vC = -(D22N * inN * ((1.0 / 3.0) * comp / dz - 1 / dz) - D22S * inS * (-1.0 / 3.0 * comp / dz + 1.0 / dz)) / dz
-              - (-D33E * inE / dx - D33W * inW / dx) / dx;
-    double vE = -D33E * inE / (dx * dx);
-    double vS = D22S * inS * ((1.0 / 3.0) * comp / dz - 1 / dz) / dz;
-    double vN = -D22N * inN * (-1.0 / 3.0 * comp / dz + 1.0 / dz) / dz;
-    double uSW = (1.0 / 3.0) * D22S * comp * inS / (dx * dz) - D33W * inW / (dx * dz);
-    double uSE = -1.0 / 3.0 * D22S * comp * inS / (dx * dz) + D33E * inE / (dx * dz);
-    double uNW = -1.0 / 3.0 * D22N * comp * inN / (dx * dz) + D33W * inW / (dx * dz);
-    double uNE = (1.0 / 3.0) * D22N * comp * inN / (dx * dz) - D33E * inE / (dx * dz);
+    const double comp = 1.0;
+    double vW, vC, vE, vS, vN, uSW, uSE, uNW, uNE;
+    if (ASM) {
+        /* matrix values with the per-grid reciprocals, as in the X rows */
+        const double idx = a.one_dx, idz = a.one_dz, idxz = a.one_dx_dz, idxx = a.one_dx_dx;
+        vW = -D33W * inW * idxx;
+        vC = -(D22N * inN * ((1.0 / 3.0) * comp * idz - idz) - D22S * inS * (-1.0 / 3.0 * comp * idz + idz)) * idz
+           - (-D33E * inE * idx - D33W * inW * idx) * idx;
+        vE = -D33E * inE * idxx;
+        vS = D22S * inS * ((1.0 / 3.0) * comp * idz - idz) * idz;
+        vN = -D22N * inN * (-1.0 / 3.0 * comp * idz + idz) * idz;
+        uSW = (1.0 / 3.0) * D22S * comp * inS * idxz - D33W * inW * idxz;
+        uSE = -1.0 / 3.0 * D22S * comp * inS * idxz + D33E * inE * idxz;
+        uNW = -1.0 / 3.0 * D22N * comp * inN * idxz + D33W * inW * idxz;
+        uNE = (1.0 / 3.0) * D22N * comp * inN * idxz - D33E * inE * idxz;
+    } else {
+        /* the residual with the reference's divisions */
+        const double dx = g.dx, dz = g.dz;
+        vW = -D33W * inW / (dx * dx);
+        vC = -(D22N * inN * ((1.0 / 3.0) * comp / dz - 1 / dz) - D22S * inS * (-1.0 / 3.0 * comp / dz + 1.0 / dz)) / dz
+           - (-D33E * inE / dx - D33W * inW / dx) / dx;
+        vE = -D33E * inE / (dx * dx);
+        vS = D22S * inS * ((1.0 / 3.0) * comp / dz - 1 / dz) / dz;
+        vN = -D22N * inN * (-1.0 / 3.0 * comp / dz + 1.0 / dz) / dz;
+        uSW = (1.0 / 3.0) * D22S * comp * inS / (dx * dz) - D33W * inW / (dx * dz);
+        uSE = -1.0 / 3.0 * D22S * comp * inS / (dx * dz) + D33E * inE / (dx * dz);
+        uNW = -1.0 / 3.0 * D22N * comp * inN / (dx * dz) + D33W * inW / (dx * dz);
+        uNE = (1.0 / 3.0) * D22N * comp * inN / (dx * dz) - D33E * inE / (dx * dz);
+    }
     int tW = T.V(c3 - 1), tE = T.V(c3 + 1);
     if (tW == 11) vC -= vW;
     if (tE == 11) vC -= vE;
@@ -1069,9 +1107,11 @@ __global__ void __launch_bounds__(VBLK) k_values_mom(DevAsm a, DevBlock A, DevBl
         B.b[eqn] = bf;
     }
     __syncthreads();
+    /* coalesced copy-out; streaming (non-temporal) stores, as nothing in the
+     * unit reads the values again and the stencil inputs keep the L2 */
     const int na = A.Ic[r1] - a0, nb = B.Ic[r1] - b0;
-    for (int i = threadIdx.x; i < na; i += VBLK) A.A[a0 + i] = sA[i];
-    for (int i = threadIdx.x; i < nb; i += VBLK) B.A[b0 + i] = sB[i];
+    for (int i = threadIdx.x; i < na; i += VBLK) __builtin_nontemporal_store(sA[i], A.A + a0 + i);
+    for (int i = threadIdx.x; i < nb; i += VBLK) __builtin_nontemporal_store(sB[i], B.A + b0 + i);
 }
 
 /* the same without staging, for patterns with longer rows */
