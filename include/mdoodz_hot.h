@@ -326,6 +326,15 @@ int  mdz_ctx_download_solution(mdz_ctx *ctx, double *u_in, double *v_in, double 
 /* Copy every output field of the path back into *mesh. */
 int  mdz_ctx_download_grid(mdz_ctx *ctx, grid *mesh);
 int  mdz_ctx_set_rheology_mode(mdz_ctx *ctx, int mode);
+/* Deferred diagnostics (no counterpart in the reference, which recomputes
+ * every rheology output in every Newton iteration).  When on, a unit's
+ * rheology pass writes only the fields the unit itself reads (eta, the
+ * tangent, the plastic divergence derivatives and p_corr at centroids and
+ * vertices); the other outputs (stresses, strain-rate partition, work
+ * terms, eta_phys, VE, OverS, d, X, ...) are recomputed from the same inputs
+ * when mdz_ctx_download_grid is called, so every field the host sees is
+ * bit for bit the one a unit with full outputs leaves.  Off by default. */
+int  mdz_ctx_defer_diagnostics(mdz_ctx *ctx, int on);
 /* Loads the surrogate network used by MDZ_RHEO_LEARNED (a parameter blob of
  * MDZ_MLP_NPARAMS floats, see below; models of at most three phases). */
 int  mdz_ctx_set_network(mdz_ctx *ctx, const float *weights, size_t n_floats);

@@ -125,6 +125,10 @@ struct mdz_ctx {
     double *tmp_c, *tmp_v;  /* staging for InterpCentroidsToVerticesDouble */
     double *dpic_n, *dpic_s; /* J = 0 D11 (= D22) at the centroids and D33 at the vertices (fused D passes) */
     int d_jac;               /* D11_n..D34_s hold the J = 1 tensor (left by a fused unit) */
+    int defer_diag;          /* units write only the rheology outputs the unit itself reads */
+    int diag_stale;          /* the other rheology outputs are not those of the last rheology pass */
+    double *p_snap;          /* p_in as the last rheology pass read it, once p_in has changed since */
+    int p_snapped;
     params model;
     mat_prop materials;
     scale scaling;
@@ -140,7 +144,7 @@ struct mdz_ctx {
     double *ls_base, *ls_dx, *ls_alpha, *ls_res;
     hipGraph_t ls_graph;
     hipGraphExec_t ls_exec;
-    int ls_ntry, ls_kernel;
+    int ls_ntry, ls_kernel, ls_stale, ls_snapped;
     unsigned ls_version;
 };
 
@@ -250,6 +254,7 @@ static int ctx_alloc(mdz_ctx **out, int Nx, int Nz, int nph) {
     size_t o_tc = off; off += align256(nc * 8);
     size_t o_tvv = off; off += align256(nv * 8);
     size_t o_dpn = off; off += align256(nc * 8);
+    size_t o_psn = off; off += align256(nc * 8);
     size_t o_dps = off; off += align256(nv * 8);
     c->slab_bytes = off;
     if (hipMalloc((void **)&c->slab, off) != hipSuccess) { free(c); return set_err(MDZ_ENOMEM, "device slab of %zu bytes", off); }
@@ -270,6 +275,7 @@ static int ctx_alloc(mdz_ctx **out, int Nx, int Nz, int nph) {
     c->e.mom_node = (int *)(c->slab + o_mn); c->e.cont_node = (int *)(c->slab + o_cn);
     c->tmp_c = (double *)(c->slab + o_tc); c->tmp_v = (double *)(c->slab + o_tvv);
     c->dpic_n = (double *)(c->slab + o_dpn); c->dpic_s = (double *)(c->slab + o_dps);
+    c->p_snap = (double *)(c->slab + o_psn);
     c->scratch_len = (nu + nw + nc) + 4096 + (nu + nw + nc) / 512;
     if (hipMalloc((void **)&c->scratch, c->scratch_len * sizeof(int)) != hipSuccess) return set_err(MDZ_ENOMEM, "scratch");
     size_t np = 0;
@@ -558,13 +564,54 @@ static int rheology_prep(mdz_ctx *c, int use_guess, hipStream_t s) {
     return MDZ_OK;
 }
 
-/* second part: the local iterations at centroids and vertices */
+/* second part: the local iterations at centroids and vertices, with every
+ * output field, or with deferred diagnostics only the unit's own */
 static int rheology_kernels(mdz_ctx *c, int use_guess, hipStream_t s) {
     const float *gn = use_guess ? c->guess_n : nullptr, *gs = use_guess ? c->guess_s : nullptr;
-    if (g_rheo_kernel == 0 && mdz_rheo_fast_eligible(c->prm, &c->materials))
-        return mdz_launch_rheology_fast(c->g, c->prm, c->d_pk, c->materials.R, gn, gs, c->hist_on ? c->hist : nullptr,
-                                        c->shape.rheo, s);
-    mdz_launch_rheology(c->g, c->prm, c->d_mat, gn, gs, c->hist_on ? c->hist : nullptr, s);
+    unsigned long long *hist = c->hist_on ? c->hist : nullptr;
+    c->p_snapped = 0;
+    if (g_rheo_kernel == 0 && mdz_rheo_fast_eligible(c->prm, &c->materials)) {
+        const int diag = !c->defer_diag;
+        int rc = mdz_launch_rheology_fast(c->g, c->prm, c->d_pk, c->materials.R, gn, gs, hist, c->shape.rheo, diag, s);
+        if (rc) return rc;
+        c->diag_stale = !diag;
+        return MDZ_OK;
+    }
+    mdz_launch_rheology(c->g, c->prm, c->d_mat, gn, gs, hist, s);
+    c->diag_stale = 0;
+    return MDZ_OK;
+}
+
+/* Deferred diagnostics: the last rheology pass again with every output, on
+ * the inputs it read (strain rates, interpolated fields and the guess are
+ * still in place; p_in from its snapshot when it has changed since), so
+ * that the fields the unit used come out unchanged. */
+static int flush_diagnostics(mdz_ctx *c) {
+    if (!c->diag_stale) return MDZ_OK;
+    DevGrid g = c->g;
+    if (c->p_snapped) g.p_in = c->p_snap;
+    const int guess = c->prm.rheo_mode == MDZ_RHEO_LEARNED;
+    int rc = mdz_launch_rheology_fast(g, c->prm, c->d_pk, c->materials.R, guess ? c->guess_n : nullptr,
+                                      guess ? c->guess_s : nullptr, nullptr, c->shape.rheo, 1, c->stream);
+    if (rc) return rc;
+    c->diag_stale = 0;
+    c->p_snapped = 0;
+    return MDZ_OK;
+}
+
+__global__ void __launch_bounds__(256) k_copy_doubles(const double *__restrict__ src, double *__restrict__ dst, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+/* p_in is about to change while the diagnostics are deferred: keep the
+ * values the last rheology pass read (a kernel, so that it can be captured) */
+static int snapshot_p(mdz_ctx *c, hipStream_t s) {
+    if (!c->diag_stale || c->p_snapped) return MDZ_OK;
+    const int n = (int)fam_len(MDZ_FAM_C, c->Nx, c->Nz);
+    hipLaunchKernelGGL(k_copy_doubles, dim3(mdz_div_up(n, 256)), dim3(256), 0, s, (const double *)c->g.p_in,
+                       c->p_snap, n);
+    c->p_snapped = 1;
     return MDZ_OK;
 }
 
@@ -1076,6 +1123,7 @@ extern "C" void *mdz_ctx_stream(mdz_ctx *ctx) { return ctx ? (void *)ctx->stream
 extern "C" int mdz_ctx_upload_step_fields(mdz_ctx *c, const grid *mesh) {
     if (!c || !mesh) return set_err(MDZ_EINVAL, "NULL argument");
     int rc;
+    if ((rc = flush_diagnostics(c))) return rc;
     for (int i = 0; i < FI_COUNT; i++)
         if (FREF[i].role == MDZ_ROLE_I && i != FI_u_in && i != FI_v_in && i != FI_p_in)
             if ((rc = up(c, mesh, i))) return rc;
@@ -1085,6 +1133,8 @@ extern "C" int mdz_ctx_upload_step_fields(mdz_ctx *c, const grid *mesh) {
 
 extern "C" int mdz_ctx_upload_solution(mdz_ctx *c, const double *u, const double *v, const double *p) {
     if (!c || !u || !v || !p) return set_err(MDZ_EINVAL, "NULL argument");
+    int rc;
+    if ((rc = snapshot_p(c, c->stream))) return rc;
     HIPCHK(hipMemcpyAsync(c->g.u_in, u, fam_len(MDZ_FAM_U, c->Nx, c->Nz) * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->g.v_in, v, fam_len(MDZ_FAM_W, c->Nx, c->Nz) * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->g.p_in, p, fam_len(MDZ_FAM_C, c->Nx, c->Nz) * 8, hipMemcpyHostToDevice, c->stream));
@@ -1102,6 +1152,7 @@ extern "C" int mdz_ctx_download_solution(mdz_ctx *c, double *u, double *v, doubl
 extern "C" int mdz_ctx_download_grid(mdz_ctx *c, grid *mesh) {
     if (!c || !mesh) return set_err(MDZ_EINVAL, "NULL argument");
     int rc;
+    if ((rc = flush_diagnostics(c))) return rc;
     /* the grid after a unit holds the J = 0 tensor, as the reference's does
      * (RheologicalOperators(..., 0) before the residual) */
     if (c->d_jac) {
@@ -1114,9 +1165,18 @@ extern "C" int mdz_ctx_download_grid(mdz_ctx *c, grid *mesh) {
     return sync(c);
 }
 
+extern "C" int mdz_ctx_defer_diagnostics(mdz_ctx *c, int on) {
+    if (!c) return set_err(MDZ_EINVAL, "NULL context");
+    c->defer_diag = on != 0;
+    c->version++;
+    return MDZ_OK;
+}
+
 extern "C" int mdz_ctx_set_rheology_mode(mdz_ctx *c, int mode) {
     if (!c || mode < MDZ_RHEO_FP64 || mode > MDZ_RHEO_LEARNED) return set_err(MDZ_EINVAL, "rheology mode %d", mode);
     if (mode == MDZ_RHEO_LEARNED && !c->d_net) return set_err(MDZ_EINVAL, "learned mode needs mdz_ctx_set_network first");
+    int rc;
+    if ((rc = flush_diagnostics(c))) return rc;
     c->prm.rheo_mode = mode;
     c->version++;
     return MDZ_OK;
@@ -1408,6 +1468,10 @@ __global__ void k_copy6(const double *__restrict__ src, double *__restrict__ dst
  * and its norms (EvaluateStokesResidualDecoupled), norms into ls_res[6k..] */
 static int enqueue_trial(mdz_ctx *c, int k, hipStream_t s) {
     const int nu = c->Nx * (c->Nz + 1), nw = (c->Nx + 1) * c->Nz, nc = (c->Nx - 1) * (c->Nz - 1);
+    if (k < 0) {
+        int rc = snapshot_p(c, s); /* the restore changes p_in under the last trial's rheology */
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_line_step, dim3(mdz_div_up(nu + nw + nc, 256)), dim3(256), 0, s, c->e, nu, nw, nc,
                        (const double *)c->ls_base, (const double *)c->ls_dx, (const double *)c->ls_alpha, k,
                        c->g.u_in, c->g.v_in, c->g.p_in);
@@ -1467,8 +1531,13 @@ extern "C" int mdz_ctx_line_search(mdz_ctx *c, const double *dx, int ntry, const
         c->ls_ntry = ntry;
         c->ls_version = c->version;
         c->ls_kernel = g_rheo_kernel;
+        c->ls_stale = c->diag_stale;
+        c->ls_snapped = c->p_snapped;
     }
     HIPCHK(hipGraphLaunch(c->ls_exec, s));
+    /* the graph leaves the deferred-diagnostics state its capture left */
+    c->diag_stale = c->ls_stale;
+    c->p_snapped = c->ls_snapped;
     std::vector<double> h(6 * (size_t)ntry);
     HIPCHK(hipMemcpyAsync(h.data(), c->ls_res, h.size() * 8, hipMemcpyDeviceToHost, s));
     int rc = sync(c);
@@ -1669,6 +1738,12 @@ extern "C" int mdz_ensemble_launch(mdz_ensemble *e) {
         if (rc) return rc;
     }
     HIPCHK(hipGraphLaunch(e->exec, e->st[0]));
+    /* each unit's rheology pass leaves the deferred-diagnostics state of a
+     * fresh pass (as the capture left it) */
+    for (mdz_ctx *c : e->ctx) {
+        c->p_snapped = 0;
+        c->diag_stale = c->defer_diag && g_rheo_kernel == 0 && mdz_rheo_fast_eligible(c->prm, &c->materials);
+    }
     return MDZ_OK;
 }
 

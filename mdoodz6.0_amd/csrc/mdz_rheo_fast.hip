@@ -146,8 +146,10 @@ __device__ __forceinline__ void ve_rates_iso(double exx, double ezz, double exz,
 
 /* Local update of one node and one phase (RheologyDensity.c:44-707 with
  * gs = 0, constant_mix = 0); outputs go through o.  CELL selects the
- * centroid output set, otherwise the vertex set. */
-template <bool CELL>
+ * centroid output set, otherwise the vertex set.  DIAG: all outputs;
+ * otherwise only those the rest of a Newton-iteration unit reads (eta, the
+ * tangent, the plastic divergence derivatives, p_corr). */
+template <bool CELL, bool DIAG>
 __device__ __forceinline__ void local_update(const PhaseK &k, double R, const DevParams &prm, const NodeIn &in,
                                              double *Xout, const Out &o, double exxd, double ezzd, double exzc,
                                              double guess, const BlockHist &hist) {
@@ -424,53 +426,59 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
 
     if (CELL) {
         o.put(FI_eta_n, etaVE);
-        o.put(FI_eta_phys_n, eta);
+        if (DIAG) o.put(FI_eta_phys_n, eta);
         o.put(FI_detadexx_n, dexx);
         o.put(FI_detadezz_n, dezz);
         o.put(FI_detadgxz_n, dexz / 2.0);
         o.put(FI_detadp_n, dp);
-        o.put(FI_sxxd, Txx);
-        o.put(FI_szzd, Tzz);
-        o.put(FI_sxz_n, Txz);
-        o.put(FI_VE_n, VE);
-        o.put(FI_eII_el, (double)elastic * fabs(Tii - sqrt(in.Txx0 * in.Txx0 + in.Txz0 * in.Txz0)) / 2.0 / eta_el);
-        o.put(FI_eII_pl, gdot / 2.0);
-        o.put(FI_eII_pwl, Tii * h_pwl);
-        o.put(FI_eII_exp, Ee);
-        o.put(FI_eII_lin, El);
-        o.put(FI_eII_gbs, Eg);
-        o.put(FI_eII_cst, Ec);
-        o.put_inv(FI_d_n, k.gs_ref);
-        o.put(FI_exx_el, ex_el);
-        o.put(FI_exx_diss, ex_d);
-        o.put(FI_ezz_el, ez_el);
-        o.put(FI_ezz_diss, ez_d);
-        o.put(FI_Wtot, Txx * exxd + Tzz * ezzd + (Txx + Tzz) * (exxd + ezzd) + 2.0 * Txz * exzc);
-        o.put(FI_Wdiss, Txx * ex_d + Tzz * ez_d + (Txx + Tzz) * (ex_d + ez_d) + 2.0 * Txz * exz_d);
-        o.put(FI_Wel, Txx * ex_el + Tzz * ez_el + (Txx + Tzz) * (ex_el + ez_el) + 2.0 * Txz * exz_el);
+        if (DIAG) {
+            o.put(FI_sxxd, Txx);
+            o.put(FI_szzd, Tzz);
+            o.put(FI_sxz_n, Txz);
+            o.put(FI_VE_n, VE);
+            o.put(FI_eII_el, (double)elastic * fabs(Tii - sqrt(in.Txx0 * in.Txx0 + in.Txz0 * in.Txz0)) / 2.0 / eta_el);
+            o.put(FI_eII_pl, gdot / 2.0);
+            o.put(FI_eII_pwl, Tii * h_pwl);
+            o.put(FI_eII_exp, Ee);
+            o.put(FI_eII_lin, El);
+            o.put(FI_eII_gbs, Eg);
+            o.put(FI_eII_cst, Ec);
+            o.put_inv(FI_d_n, k.gs_ref);
+            o.put(FI_exx_el, ex_el);
+            o.put(FI_exx_diss, ex_d);
+            o.put(FI_ezz_el, ez_el);
+            o.put(FI_ezz_diss, ez_d);
+            o.put(FI_Wtot, Txx * exxd + Tzz * ezzd + (Txx + Tzz) * (exxd + ezzd) + 2.0 * Txz * exzc);
+            o.put(FI_Wdiss, Txx * ex_d + Tzz * ez_d + (Txx + Tzz) * (ex_d + ez_d) + 2.0 * Txz * exz_d);
+            o.put(FI_Wel, Txx * ex_el + Tzz * ez_el + (Txx + Tzz) * (ex_el + ez_el) + 2.0 * Txz * exz_el);
+        }
         o.put(FI_ddivpdexx_n, ddxx);
         o.put(FI_ddivpdezz_n, ddzz);
         o.put(FI_ddivpdgxz_n, ddxz / 2.0);
         o.put(FI_ddivpdp_n, ddp);
         o.put(FI_p_corr, Pc);
-        o.put(FI_div_u_el, -(Pc - in.P0) / (K * dt));
-        o.put(FI_div_u_pl, divp);
-        o.put(FI_div_u_r, 0.0);
-        if (Xout) { double x = o.fr * in.X0; if (o.first) Xout[o.c] = x; else Xout[o.c] += x; }
-        o.put(FI_OverS_n, is_pl ? eta_vp * gdot : 0.0);
+        if (DIAG) {
+            o.put(FI_div_u_el, -(Pc - in.P0) / (K * dt));
+            o.put(FI_div_u_pl, divp);
+            o.put(FI_div_u_r, 0.0);
+            if (Xout) { double x = o.fr * in.X0; if (o.first) Xout[o.c] = x; else Xout[o.c] += x; }
+            o.put(FI_OverS_n, is_pl ? eta_vp * gdot : 0.0);
+        }
     } else {
         o.put(FI_eta_s, etaVE);
-        o.put(FI_eta_phys_s, eta);
+        if (DIAG) o.put(FI_eta_phys_s, eta);
         o.put(FI_detadexx_s, dexx);
         o.put(FI_detadezz_s, dezz);
         o.put(FI_detadgxz_s, dexz / 2.0);
         o.put(FI_detadp_s, dp);
-        o.put(FI_sxz, Txz);
-        o.put(FI_VE_s, VE);
-        o.put(FI_exz_el, exz_el);
-        o.put(FI_exz_diss, exz_d);
-        o.put(FI_OverS_s, is_pl ? eta_vp * gdot : 0.0);
-        if (Xout) { double x = o.fr * in.X0; if (o.first) Xout[o.c] = x; else Xout[o.c] += x; }
+        if (DIAG) {
+            o.put(FI_sxz, Txz);
+            o.put(FI_VE_s, VE);
+            o.put(FI_exz_el, exz_el);
+            o.put(FI_exz_diss, exz_d);
+            o.put(FI_OverS_s, is_pl ? eta_vp * gdot : 0.0);
+            if (Xout) { double x = o.fr * in.X0; if (o.first) Xout[o.c] = x; else Xout[o.c] += x; }
+        }
     }
 }
 
@@ -483,12 +491,21 @@ __device__ __forceinline__ void node_angles(const DevParams &prm, double fric, d
     in.sin_dil = elastic == 0 ? 0.0 : sin(dil);
 }
 
+/* The centroid output fields the rest of a unit reads (all of them are
+ * written when DIAG). */
+__device__ __constant__ const int k_unit_fields_n[10] = {FI_eta_n, FI_detadexx_n, FI_detadezz_n, FI_detadgxz_n,
+                                                         FI_detadp_n, FI_ddivpdexx_n, FI_ddivpdezz_n,
+                                                         FI_ddivpdgxz_n, FI_ddivpdp_n, FI_p_corr};
+__device__ __constant__ const int k_unit_fields_s[5] = {FI_eta_s, FI_detadexx_s, FI_detadezz_s, FI_detadgxz_s,
+                                                        FI_detadp_s};
+
+template <bool DIAG>
 __device__ __forceinline__ void rheo_cell(const DevGrid &g, const DevParams &prm, const PhaseK *__restrict__ pk,
                                           double R, const float *__restrict__ guess, const BlockHist &hist, int c0,
                                           int nc) {
     const int t = g.BCp[c0];
     const bool active = t != 30 && t != 31;
-    double *Xout = prm.UnsplitDiffReac == 0 ? g.X_n : nullptr;
+    double *Xout = DIAG && prm.UnsplitDiffReac == 0 ? g.X_n : nullptr;
     bool any = false;
     if (active) {
         const double etae = prm.iselastic == 1 ? prm.dt * g.mu_n[c0] : 1.0;
@@ -504,22 +521,26 @@ __device__ __forceinline__ void rheo_cell(const DevGrid &g, const DevParams &prm
             if (!(fabs(fr) > 1.0e-13)) continue;
             const Out o{g.eta_n, g.cstride, c0, FI_eta_n, fr, !any};
             const double gs = guess ? (double)guess[(size_t)p * nc + c0] : 0.0;
-            local_update<true>(pk[p], R, prm, in, Xout, o, exxd, ezzd, exzc, gs, hist);
+            local_update<true, DIAG>(pk[p], R, prm, in, Xout, o, exxd, ezzd, exzc, gs, hist);
             any = true;
         }
     }
     double *dn = g.eta_n + (long long)(FI_d_n - FI_eta_n) * g.cstride + c0;
     if (!any) {
         /* air cell, or no phase above the fraction threshold */
-        for (int i = 0; i <= FI_OverS_n - FI_eta_n; i++) g.eta_n[(long long)i * g.cstride + c0] = 0.0;
+        if (DIAG) {
+            for (int i = 0; i <= FI_OverS_n - FI_eta_n; i++) g.eta_n[(long long)i * g.cstride + c0] = 0.0;
+        } else {
+            for (int i = 0; i < 10; i++) g.eta_n[(long long)(k_unit_fields_n[i] - FI_eta_n) * g.cstride + c0] = 0.0;
+        }
         if (Xout) Xout[c0] = 0.0;
-        if (active) *dn = 1.0 / 0.0; /* 1 / (empty sum) */
-    } else {
+        if (DIAG && active) *dn = 1.0 / 0.0; /* 1 / (empty sum) */
+    } else if (DIAG) {
         *dn = 1.0 / *dn;
     }
 }
 
-template <int NT>
+template <int NT, bool DIAG>
 __global__ void __launch_bounds__(NT) k_rheo_cells_fast(DevGrid g, DevParams prm, const PhaseK *__restrict__ pk,
                                                         double R, const float *__restrict__ guess,
                                                         unsigned long long *hist) {
@@ -527,14 +548,15 @@ __global__ void __launch_bounds__(NT) k_rheo_cells_fast(DevGrid g, DevParams prm
     const BlockHist bh = hist_begin(lh, hist);
     const int c0 = blockIdx.x * NT + threadIdx.x;
     const int nc = g.ncx * g.ncz;
-    if (c0 < nc) rheo_cell(g, prm, pk, R, guess, bh, c0, nc);
+    if (c0 < nc) rheo_cell<DIAG>(g, prm, pk, R, guess, bh, c0, nc);
     hist_flush(bh, hist);
 }
 
+template <bool DIAG>
 __device__ __forceinline__ void rheo_vertex(const DevGrid &g, const DevParams &prm, const PhaseK *__restrict__ pk,
                                             double R, const float *__restrict__ guess, const BlockHist &hist, int c1,
                                             int nv) {
-    double *Xout = prm.UnsplitDiffReac == 0 ? g.X_s : nullptr;
+    double *Xout = DIAG && prm.UnsplitDiffReac == 0 ? g.X_s : nullptr;
     bool any = false;
     if (g.BCg[c1] != 30) {
         const double etae = prm.iselastic == 1 ? prm.dt * g.mu_s[c1] : 1.0;
@@ -549,17 +571,21 @@ __device__ __forceinline__ void rheo_vertex(const DevGrid &g, const DevParams &p
             if (!(fabs(fr) > 1.0e-13)) continue;
             const Out o{g.VE_s, g.vstride, c1, FI_VE_s, fr, !any};
             const double gs = guess ? (double)guess[(size_t)p * nv + c1] : 0.0;
-            local_update<false>(pk[p], R, prm, in, Xout, o, 0.0, 0.0, 0.0, gs, hist);
+            local_update<false, DIAG>(pk[p], R, prm, in, Xout, o, 0.0, 0.0, 0.0, gs, hist);
             any = true;
         }
     }
     if (!any) {
-        for (int i = 0; i <= FI_OverS_s - FI_VE_s; i++) g.VE_s[(long long)i * g.vstride + c1] = 0.0;
+        if (DIAG) {
+            for (int i = 0; i <= FI_OverS_s - FI_VE_s; i++) g.VE_s[(long long)i * g.vstride + c1] = 0.0;
+        } else {
+            for (int i = 0; i < 5; i++) g.VE_s[(long long)(k_unit_fields_s[i] - FI_VE_s) * g.vstride + c1] = 0.0;
+        }
         if (Xout) Xout[c1] = 0.0;
     }
 }
 
-template <int NT>
+template <int NT, bool DIAG>
 __global__ void __launch_bounds__(NT) k_rheo_vertices_fast(DevGrid g, DevParams prm, const PhaseK *__restrict__ pk,
                                                            double R, const float *__restrict__ guess,
                                                            unsigned long long *hist) {
@@ -567,7 +593,7 @@ __global__ void __launch_bounds__(NT) k_rheo_vertices_fast(DevGrid g, DevParams 
     const BlockHist bh = hist_begin(lh, hist);
     const int c1 = blockIdx.x * NT + threadIdx.x;
     const int nv = g.Nx * g.Nz;
-    if (c1 < nv) rheo_vertex(g, prm, pk, R, guess, bh, c1, nv);
+    if (c1 < nv) rheo_vertex<DIAG>(g, prm, pk, R, guess, bh, c1, nv);
     hist_flush(bh, hist);
 }
 
@@ -587,22 +613,26 @@ void mdz_launch_phase_constants(const mat_prop *d_mt, int nph, void *d_pk, hipSt
                        (PhaseK *)d_pk);
 }
 
-template <int NT>
+template <int NT, bool DIAG>
 static void launch_fast(const DevGrid &g, const DevParams &prm, const PhaseK *pk, double R, const float *guess_n,
                         const float *guess_s, unsigned long long *hist, hipStream_t s) {
     int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
-    hipLaunchKernelGGL(k_rheo_cells_fast<NT>, dim3(mdz_div_up(nc, NT)), dim3(NT), 0, s, g, prm, pk, R, guess_n, hist);
-    hipLaunchKernelGGL(k_rheo_vertices_fast<NT>, dim3(mdz_div_up(nv, NT)), dim3(NT), 0, s, g, prm, pk, R, guess_s,
+    hipLaunchKernelGGL((k_rheo_cells_fast<NT, DIAG>), dim3(mdz_div_up(nc, NT)), dim3(NT), 0, s, g, prm, pk, R, guess_n,
                        hist);
+    hipLaunchKernelGGL((k_rheo_vertices_fast<NT, DIAG>), dim3(mdz_div_up(nv, NT)), dim3(NT), 0, s, g, prm, pk, R,
+                       guess_s, hist);
 }
 
 int mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
-                             const float *guess_s, unsigned long long *hist, int nt, hipStream_t s) {
+                             const float *guess_s, unsigned long long *hist, int nt, int diag, hipStream_t s) {
     const PhaseK *pk = (const PhaseK *)d_pk;
-    switch (nt) {
-    case 64: launch_fast<64>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
-    case 128: launch_fast<128>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
-    case 256: launch_fast<256>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
+    switch (nt * 2 + (diag ? 1 : 0)) {
+    case 129: launch_fast<64, true>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
+    case 128: launch_fast<64, false>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
+    case 257: launch_fast<128, true>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
+    case 256: launch_fast<128, false>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
+    case 513: launch_fast<256, true>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
+    case 512: launch_fast<256, false>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
     default: return mdz_error(MDZ_EINVAL, "rheology block size: 64, 128 or 256 threads");
     }
 }
