@@ -42,7 +42,7 @@ def stage_kernels(shape, learned=False, fused=True):
     d1 = ["k_dtensor_cells<2>", "k_dtensor_vertices<2>"] if fused else ["k_dtensor_cells<1>", "k_dtensor_vertices<1>"]
     return {"strain_rates": ["k_strain_cells", "k_strain_vertices", "k_exz_n", "k_interp_c2v (2 fields)"],
             "interp_guess": prep,
-            "rheology": ["k_rheo_cells_fast<%d>" % rh, "k_rheo_vertices_fast<%d>" % rh],
+            "rheology": ["k_rheo_cells_fast<%d, true>" % rh, "k_rheo_vertices_fast<%d, true>" % rh],
             "rhs": ["k_rhs_u", "k_rhs_v", "k_rhs_p"],
             "dtensor_picard": d0,
             "picard_values": [vp] if pic else [vp, "k_values_cont"],

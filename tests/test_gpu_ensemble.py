@@ -206,3 +206,33 @@ def test_every_launch_shape_gives_identical_results(mdz, hot, ens, mode):
         assert list(r) == list(r0), sh
         for (A, b), (Ar, br_) in zip([ens.block_values(3, w) for w in (0, 1, 4, 5)], ref):
             assert np.array_equal(A, Ar) and np.array_equal(b, br_), sh
+
+
+def test_deferred_diagnostics_in_the_graph(mdz, hot, ens):
+    """Contexts with deferred diagnostics in a captured ensemble: the replayed
+    units give the eager full-output norms, and the grid downloaded after a
+    replay (diagnostics recomputed then) equals the one after an eager
+    full-output unit, field for field."""
+    names = mdz.field_names(2)
+    ref_res, ref_grid = [], []
+    for i, c in enumerate(ens.ctx):
+        r = (C.c_double * 3)()
+        mdz.check(hot.mdz_ctx_newton_unit(c, r))
+        ref_res.append(list(r))
+        m = ens.models[i].clone()
+        mdz.check(hot.mdz_ctx_download_grid(c, m.m.mesh))
+        ref_grid.append({n: m.field(n).copy() for n in names})
+        m.free()
+    for c in ens.ctx:
+        mdz.check(hot.mdz_ctx_defer_diagnostics(c, 1))
+    ens.create()
+    for replay in range(2):
+        mdz.check(hot.mdz_ensemble_launch(ens.ens))
+        mdz.check(hot.mdz_ensemble_synchronize(ens.ens))
+        assert np.array_equal(ens.residuals(), np.array(ref_res)), replay
+    for i, c in enumerate(ens.ctx):
+        m = ens.models[i].clone()
+        mdz.check(hot.mdz_ctx_download_grid(c, m.m.mesh))
+        for n in names:
+            assert np.array_equal(m.field(n), ref_grid[i][n], equal_nan=True), (i, n)
+        m.free()
