@@ -38,6 +38,8 @@ struct PhaseK {
     double lnB0_gbs, n_gbs, Ea_gbs, Va_gbs, mn_gbs, lnfr_gbs, an_gbs;
     double Ea_exp, gamma, n_exp, ln_gS, ln_E;
     double Slim, eta_vp0, n_vp, gs_ref, ln_gs_ref;
+    /* reciprocals, so that the per-node code multiplies instead of dividing */
+    double inv_n_pwl, inv_n_lin, inv_n_gbs, inv_eta_cst, inv_n_vp, vp_exp, ginv_lin;
 };
 
 /* -(r / n) ln f, with pow(f, -0) = 1 for every f */
@@ -79,6 +81,14 @@ __global__ void k_phase_constants(const mat_prop *__restrict__ mt, int nph, Phas
     k.Slim = mt->Slim[p]; k.eta_vp0 = mt->eta_vp[p]; k.n_vp = mt->n_vp[p];
     k.gs_ref = mt->gs_ref[p];
     k.ln_gs_ref = log(mt->gs_ref[p]);
+    k.inv_n_pwl = 1.0 / k.n_pwl;
+    k.inv_n_lin = 1.0 / k.n_lin;
+    k.inv_n_gbs = 1.0 / k.n_gbs;
+    k.inv_eta_cst = 1.0 / k.eta_cst;
+    k.inv_n_vp = 1.0 / k.n_vp;
+    k.vp_exp = 1.0 / k.n_vp - 1.0;
+    /* the diffusion creep's d1^-m / n term of its inverse viscosity */
+    k.ginv_lin = k.m_lin != 0.0 ? k.m_lin / k.n_lin * k.ln_gs_ref : 0.0;
     pk[p] = k;
 }
 
@@ -149,52 +159,71 @@ __device__ __forceinline__ void ve_rates_iso(double exx, double ezz, double exz,
  * centroid output set, otherwise the vertex set.  DIAG: all outputs;
  * otherwise only those the rest of a Newton-iteration unit reads (eta, the
  * tangent, the plastic divergence derivatives, p_corr). */
+/* Node-independent constants of the local update, from the scales of the
+ * model (computed on the host, read as kernel arguments). */
+struct FastConst {
+    double T_peierls; /* (1200 + 273.15) K, scaled */
+    double T_min;     /* 273.15 K, scaled */
+    double eta_start; /* 1e100 Pa s, scaled */
+    double P_tens;    /* -25 MPa, scaled */
+    double E_floor;   /* 1e-30 s^-1, scaled */
+};
+
+/* Local update of one node and one phase (RheologyDensity.c:44-707 with
+ * gs = 0, constant_mix = 0); outputs go through o.  CELL selects the
+ * centroid output set, otherwise the vertex set.  DIAG: all outputs;
+ * otherwise only those the rest of a Newton-iteration unit reads (eta, the
+ * tangent, the plastic divergence derivatives, p_corr).  Divisions by
+ * quantities used more than once are products with one reciprocal (a
+ * rounding-level difference from the reference's expressions). */
 template <bool CELL, bool DIAG>
-__device__ __forceinline__ void local_update(const PhaseK &k, double R, const DevParams &prm, const NodeIn &in,
-                                             double *Xout, const Out &o, double exxd, double ezzd, double exzc,
-                                             double guess, const BlockHist &hist) {
+__device__ __forceinline__ void local_update(const PhaseK &k, double R, const DevParams &prm, const FastConst &fc,
+                                             const NodeIn &in, double *Xout, const Out &o, double exxd, double ezzd,
+                                             double exzc, double guess, const BlockHist &hist) {
     const double dt = prm.dt;
     int elastic = prm.iselastic;
     double G = in.G, T = in.T, P = in.P, C = in.C, K = 1.0 / in.beta;
-    const double TmaxPeierls = (1200.0 + ZEROC) / prm.sT;
     const double Exx = in.Exx, Ezz = in.Ezz, Exz = in.Exz;
     double Eyy = -(Exx + Ezz), Gyy = -(in.Gxx + in.Gzz);
     double Eii = sqrt(1.0 / 2.0 * (Exx * Exx + Ezz * Ezz + Eyy * Eyy) + Exz * Exz);
     double Gii = sqrt(1.0 / 2.0 * (in.Gxx * in.Gxx + in.Gzz * in.Gzz + Gyy * Gyy) + in.Gxz * in.Gxz);
-    if (Eii * prm.sE < 1e-30) Eii = 1e-30 / prm.sE;
-    if (Gii * prm.sE < 1e-30) Gii = 1e-30 / prm.sE;
-    const double f_ani = Gii / Eii;
-    const int peierls = k.expv && T < TmaxPeierls;
+    if (Eii * prm.sE < 1e-30) Eii = fc.E_floor;
+    if (Gii * prm.sE < 1e-30) Gii = fc.E_floor;
+    const double inv_Eii = 1.0 / Eii;
+    const double f_ani = Gii * inv_Eii;
+    const int peierls = k.expv && T < fc.T_peierls;
     if (prm.step == 0) elastic = 0;
     if (elastic == 0) { G = 1e1; K = 1e1; } /* and dil = 0 (sin_dil set by the caller) */
-    if (T < ZEROC / prm.sT) T = ZEROC / prm.sT;
-    const double RT = R * T;
+    if (T < fc.T_min) T = fc.T_min;
+    const double inv_RT = 1.0 / (R * T);
     const double lnEii = log(Eii), lnf2 = log(2.0 * f_ani), lnd = log(in.d);
 
     /* prefactors B and compliances C = (2 f B)^-n, in logs */
     double lnB_pwl = 0, lnC_pwl = 0, lnB_lin = 0, lnC_lin = 0, lnB_gbs = 0, lnC_gbs = 0, lnB_exp = 0, lnC_exp = 0;
-    double nST = 0;
+    double nST = 0, inv_nST = 0;
     if (k.pwl) {
-        lnB_pwl = k.lnB0_pwl + (k.Ea_pwl + P * k.Va_pwl) / RT / k.n_pwl + (k.mn_pwl != 0.0 ? k.mn_pwl * lnd : 0.0)
-                + k.lnfr_pwl - k.an_pwl * in.phi;
+        lnB_pwl = k.lnB0_pwl + (k.Ea_pwl + P * k.Va_pwl) * inv_RT * k.inv_n_pwl
+                + (k.mn_pwl != 0.0 ? k.mn_pwl * lnd : 0.0) + k.lnfr_pwl - k.an_pwl * in.phi;
         lnC_pwl = -k.n_pwl * (lnf2 + lnB_pwl);
     }
     if (k.lin) {
-        lnB_lin = k.lnB0_lin + (k.Ea_lin + P * k.Va_lin) / RT / k.n_lin + k.lnfr_lin - k.an_lin * in.phi;
+        lnB_lin = k.lnB0_lin + (k.Ea_lin + P * k.Va_lin) * inv_RT * k.inv_n_lin + k.lnfr_lin - k.an_lin * in.phi;
         /* the diffusion rate carries d1^-m with d1 the reference grain size */
         lnC_lin = -k.n_lin * (lnf2 + lnB_lin) - (k.m_lin != 0.0 ? k.m_lin * k.ln_gs_ref : 0.0);
     }
     if (k.gbs) {
-        lnB_gbs = k.lnB0_gbs + (k.Ea_gbs + P * k.Va_gbs) / RT / k.n_gbs + (k.mn_gbs != 0.0 ? k.mn_gbs * lnd : 0.0)
-                + k.lnfr_gbs - k.an_gbs * in.phi;
+        lnB_gbs = k.lnB0_gbs + (k.Ea_gbs + P * k.Va_gbs) * inv_RT * k.inv_n_gbs
+                + (k.mn_gbs != 0.0 ? k.mn_gbs * lnd : 0.0) + k.lnfr_gbs - k.an_gbs * in.phi;
         lnC_gbs = -k.n_gbs * (lnf2 + lnB_gbs);
     }
     if (peierls) {
-        double ST = k.Ea_exp / R / T * 2.0 * k.gamma * (1 - k.gamma);
+        const double EaRT = k.Ea_exp * inv_RT;
+        double ST = EaRT * 2.0 * k.gamma * (1 - k.gamma);
         nST = ST + k.n_exp;
-        double lnArr = -k.Ea_exp / R / T * (1.0 - k.gamma) * (1.0 - k.gamma);
-        double lnF = ((1.0 - nST) * LN2 - 0.5 * (nST + 1.0) * LN3) / nST;
-        lnB_exp = lnF + (ST / nST) * k.ln_gS - (k.ln_E + lnArr) / nST;
+        inv_nST = 1.0 / nST;
+        double lnArr = -EaRT * (1.0 - k.gamma) * (1.0 - k.gamma);
+        double lnF = ((1.0 - nST) * LN2 - 0.5 * (nST + 1.0) * LN3) * inv_nST;
+        lnB_exp = lnF + (ST * inv_nST) * k.ln_gS - (k.ln_E + lnArr) * inv_nST;
         lnC_exp = -nST * (lnf2 + lnB_exp);
     }
 
@@ -204,29 +233,30 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
     if (k.Slim < Tyield) { sin_fric = 0.0; cos_fric = 1.0; sin_dil = 0.0; C = k.Slim; Tyield = k.Slim; }
     int tens = 0;
     {
-        double P_tens = -25e6 / prm.sS;
-        double sft = -C * cos_fric / P_tens;
+        double sft = -C * cos_fric / fc.P_tens;
         if (C * cos_fric + P * sft < Tyield && k.tensile) { sin_fric = sft; Tyield = C * cos_fric + P * sft; tens = 1; }
     }
 
     /* initial guess: the smallest isolated viscosity */
-    const double eta_el = G * dt;
-    double eta_ve = 1.0e100 / prm.seta;
+    const double eta_el = G * dt, inv_eel = 1.0 / eta_el;
+    double eta_ve = fc.eta_start;
     if (k.cst) eta_ve = fmin(eta_ve, k.eta_cst);
-    if (k.pwl) eta_ve = fmin(eta_ve, exp(lnB_pwl + (1.0 / k.n_pwl - 1.0) * lnEii));
+    if (k.pwl) eta_ve = fmin(eta_ve, exp(lnB_pwl + (k.inv_n_pwl - 1.0) * lnEii));
     if (elastic) eta_ve = fmin(eta_ve, eta_el);
-    if (peierls) eta_ve = fmin(eta_ve, exp(lnB_exp + (1.0 / nST - 1.0) * lnEii));
+    if (peierls) eta_ve = fmin(eta_ve, exp(lnB_exp + (inv_nST - 1.0) * lnEii));
     if (k.lin)
-        eta_ve = fmin(eta_ve, exp(lnB_lin + (1.0 / k.n_lin - 1.0) * lnEii + (k.m_lin != 0.0 ? k.m_lin / k.n_lin * lnd : 0.0)));
-    if (k.gbs) eta_ve = fmin(eta_ve, exp(lnB_gbs + (1.0 / k.n_gbs - 1.0) * lnEii));
+        eta_ve = fmin(eta_ve, exp(lnB_lin + (k.inv_n_lin - 1.0) * lnEii + (k.m_lin != 0.0 ? k.m_lin * k.inv_n_lin * lnd : 0.0)));
+    if (k.gbs) eta_ve = fmin(eta_ve, exp(lnB_gbs + (k.inv_n_gbs - 1.0) * lnEii));
 
     /* FP32 pre-iterations */
     int it32 = 0;
     if (prm.rheo_mode != MDZ_RHEO_FP64) {
         float e = guess > 0.0 ? (float)(guess * eta_ve) : (float)eta_ve;
-        const float Ef = (float)Eii, ff = (float)f_ani, eelf = (float)eta_el, ecst = (float)k.eta_cst;
+        const float Ef = (float)Eii, ff = (float)f_ani, ecst = (float)k.eta_cst;
+        const float iEf = 1.0f / Ef, heel = (float)(0.5 * inv_eel), iecst = (float)k.inv_eta_cst;
         const float lcp = (float)lnC_pwl, lce = (float)lnC_exp, lcl = (float)lnC_lin, lcg = (float)lnC_gbs;
         const float np_ = (float)k.n_pwl, ne = (float)nST, nl = (float)k.n_lin, ng = (float)k.n_gbs;
+        #pragma unroll 1
         for (it32 = 0; it32 < 20; it32++) {
             float Tf = 2.0f * ff * e * Ef;
             float lt = __logf(Tf);
@@ -234,19 +264,21 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
             float Ee = peierls ? __expf(lce + ne * lt) : 0.0f;
             float El = k.lin ? __expf(lcl + nl * lt) : 0.0f;
             float Eg = k.gbs ? __expf(lcg + ng * lt) : 0.0f;
-            float Ec = k.cst ? Tf / ff / 2.0f / ecst : 0.0f;
-            float r = Ef - (elastic ? Tf / (2.0f * eelf) : 0.0f) - (Ep + Ee + El + Eg + Ec);
-            if (fabsf(r / Ef) < MDZ_F32_TOL) break;
-            float df = 0.0f;
-            if (elastic) df -= Ef / eelf;
-            if (peierls) df -= Ee * ne / e;
-            if (k.lin) df -= El * nl / e;
-            if (k.pwl) df -= Ep * np_ / e;
-            if (k.cst) df -= Ef / ecst;
+            float Ec = k.cst ? e * Ef * iecst : 0.0f;
+            float r = Ef - (elastic ? Tf * heel : 0.0f) - (Ep + Ee + El + Eg + Ec);
+            if (fabsf(r * iEf) < MDZ_F32_TOL) break;
+            /* e * d r / d e: the Newton step in log(eta) needs no division by e */
+            float dfe = 0.0f;
+            if (elastic) dfe -= Ef * 2.0f * heel * e;
+            if (peierls) dfe -= Ee * ne;
+            if (k.lin) dfe -= El * nl;
+            if (k.pwl) dfe -= Ep * np_;
+            if (k.cst) dfe -= Ef * iecst * e;
+            (void)ecst;
             /* Newton step in log(eta): the strain-rate sum is convex and
              * increasing in log(eta), so the iteration converges from either
              * side without leaving eta > 0 */
-            const float ds = fminf(fmaxf(-r / (e * df), -10.0f), 10.0f);
+            const float ds = fminf(fmaxf(-r / dfe, -10.0f), 10.0f);
             e *= __expf(ds);
         }
         if (e > 0.0f && e < 3.0e38f) eta_ve = (double)e;
@@ -255,31 +287,35 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
     }
 
     /* FP64 visco-elastic Newton iteration (RheologyDensity.c:330-370) */
+    const double half_inv_eel = 0.5 * inv_eel;
     double Ep = 0, Ee = 0, El = 0, Eg = 0, Ec = 0, lT = 0;
     int it;
+    #pragma unroll 1
     for (it = 0; it < 20; it++) {
         double Ti = 2.0 * f_ani * eta_ve * Eii;
         lT = log(Ti);
-        if (k.cst) Ec = Ti / f_ani / 2.0 / k.eta_cst;
+        if (k.cst) Ec = Ti / f_ani * 0.5 * k.inv_eta_cst;
         if (k.pwl) Ep = exp(lnC_pwl + k.n_pwl * lT);
         if (k.gbs) Eg = exp(lnC_gbs + k.n_gbs * lT);
         if (peierls) Ee = exp(lnC_exp + nST * lT);
         if (k.lin) El = exp(lnC_lin + k.n_lin * lT);
-        double r = Eii - elastic * Ti / (2.0 * eta_el) - (Ep + Ee + El + Eg + Ec);
-        if (fabs(r / Eii) < 1.0e-13) break;
-        double df = 0.0;
-        if (elastic) df += -Eii / eta_el;
-        if (peierls) df += -Ee * nST / eta_ve;
-        if (k.lin) df += -El * k.n_lin / eta_ve;
-        if (k.pwl) df += -Ep * k.n_pwl / eta_ve;
-        if (k.cst) df += -Eii / k.eta_cst;
-        eta_ve -= r / df;
+        double r = Eii - elastic * Ti * half_inv_eel - (Ep + Ee + El + Eg + Ec);
+        if (fabs(r * inv_Eii) < 1.0e-13) break;
+        /* eta_ve * d r / d eta_ve */
+        double dfe = 0.0;
+        if (elastic) dfe += -Eii * inv_eel * eta_ve;
+        if (peierls) dfe += -Ee * nST;
+        if (k.lin) dfe += -El * k.n_lin;
+        if (k.pwl) dfe += -Ep * k.n_pwl;
+        if (k.cst) dfe += -Eii * k.inv_eta_cst * eta_ve;
+        eta_ve -= r * eta_ve / dfe;
     }
     hist.add(0, it);
     hist.add(2, it32);
     /* stresses at the converged eta_ve */
     double Tii = 2.0 * eta_ve * f_ani * Eii;
-    double Txx = in.Gxx * Tii / f_ani / Eii, Tzz = in.Gzz * Tii / f_ani / Eii, Txz = in.Gxz * Tii / f_ani / Eii;
+    const double sfac = Tii / f_ani * inv_Eii;
+    double Txx = in.Gxx * sfac, Tzz = in.Gzz * sfac, Txz = in.Gxz * sfac;
     /* creep rates at this Tii: the loop's values unless it ran out of
      * iterations (then eta_ve moved after the last evaluation) */
     double lTp = lT, Epp = Ep, Eep = Ee, Elp = El;
@@ -290,36 +326,37 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
         if (k.lin) Elp = exp(lnC_lin + k.n_lin * lTp);
     }
     /* inverse viscosities of the creep mechanisms */
-    const double inv_lin = k.lin ? exp(-(lnB_lin + (1.0 / k.n_lin - 1.0) * (lnC_lin + k.n_lin * lT)
-                                         + (k.m_lin != 0.0 ? k.m_lin / k.n_lin * k.ln_gs_ref : 0.0)))
-                                 : 0.0;
-    const double inv_exp = peierls ? exp(-(lnB_exp + (1.0 / nST - 1.0) * (lnC_exp + nST * lT))) : 0.0;
+    const double inv_lin = k.lin ? exp(-(lnB_lin + (k.inv_n_lin - 1.0) * (lnC_lin + k.n_lin * lT) + k.ginv_lin)) : 0.0;
+    const double inv_exp = peierls ? exp(-(lnB_exp + (inv_nST - 1.0) * (lnC_exp + nST * lT))) : 0.0;
     double inv_pwl = k.pwl ? exp(LN2 + lnC_pwl - (1.0 - k.n_pwl) * lTp) : 0.0;
 
     /* tangent of the visco-elastic update (RheologyDensity.c:425-447) */
-    const double Ti3 = Tii * Tii * Tii;
+    const double inv_Tii = 1.0 / Tii;
+    const double inv_Ti3 = inv_Tii * inv_Tii * inv_Tii;
     double deta = 0.0;
-    if (k.pwl) deta -= Epp / Ti3 * (k.n_pwl - 1.0);
-    if (peierls) deta -= Eep / Ti3 * (nST - 1.0);
-    if (k.lin) deta -= Elp / Ti3 * (k.n_lin - 1.0);
+    if (k.pwl) deta -= Epp * inv_Ti3 * (k.n_pwl - 1.0);
+    if (peierls) deta -= Eep * inv_Ti3 * (nST - 1.0);
+    if (k.lin) deta -= Elp * inv_Ti3 * (k.n_lin - 1.0);
     const double e2 = eta_ve * eta_ve;
     double dTxx = deta * (2.0 * Txx + Tzz) * e2, dTzz = deta * (2.0 * Tzz + Txx) * e2, dTxz = 2.0 * deta * Txz * e2;
     double den = 1.0 - 2.0 * (dTxx * Exx + dTzz * Ezz + dTxz * Exz);
-    const double de_xx = dTxx * 2.0 * eta_ve / den, de_zz = dTzz * 2.0 * eta_ve / den, de_xz = dTxz * 2.0 * eta_ve / den;
+    const double tden = 2.0 * eta_ve / den;
+    const double de_xx = dTxx * tden, de_zz = dTzz * tden, de_xz = dTxz * tden;
     double dummy = 0.0;
-    if (k.pwl) dummy += 2.0 * Epp / Tii * k.Va_pwl / RT;
-    if (k.lin) dummy += 2.0 * Elp / Tii * (k.m_lin != 0.0 ? exp(k.m_lin * (k.ln_gs_ref - lnd)) : 1.0) * k.Va_lin / RT;
+    if (k.pwl) dummy += 2.0 * Epp * inv_Tii * k.Va_pwl * inv_RT;
+    if (k.lin) dummy += 2.0 * Elp * inv_Tii * (k.m_lin != 0.0 ? exp(k.m_lin * (k.ln_gs_ref - lnd)) : 1.0) * k.Va_lin * inv_RT;
     const double de_p = e2 * dummy;
 
     /* plastic return map (RheologyDensity.c:468-531) */
-    const double vp_exp = 1.0 / k.n_vp - 1.0;
+    const double vp_exp = k.vp_exp;
     double F_trial = Tii - Tyield, gdot = 0.0, eta_vp = 0.0, Pc = P, divp = 0.0;
     double dQdtxx = 0, dQdtzz = 0, dQdtxz = 0;
+    const double Kdt = K * dt;
     if (tens == 1) {
         eta_vp = k.eta_vp0 * (vp_exp == 0.0 ? 1.0 : pow(Eii, vp_exp));
         double Pc_chk = -(C * cos_fric) / (-Tii / P + sin_fric);
         sin_dil = (P * eta_ve + P * eta_vp - Pc_chk * eta_ve - Pc_chk * eta_vp)
-                / (K * dt * (C * cos_fric + Pc_chk * sin_fric - Tii));
+                / (Kdt * (C * cos_fric + Pc_chk * sin_fric - Tii));
     }
     /* the diffusion, Peierls and constant-viscosity parts of the strain-rate
      * partition use the visco-elastic stresses (before the return map) */
@@ -327,30 +364,34 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
     const bool is_pl = F_trial > 1e-17;
     if (is_pl) {
         eta_vp = k.eta_vp0 * (vp_exp == 0.0 ? 1.0 : pow(Eii, vp_exp));
-        gdot = F_trial / (eta_ve + eta_vp + K * dt * sin_fric * sin_dil);
-        dQdtxx = Txx / 2.0 / Tii; dQdtzz = Tzz / 2.0 / Tii; dQdtxz = Txz / 1.0 / Tii;
+        const double Kds_dil = Kdt * sin_fric * sin_dil;
+        gdot = F_trial / (eta_ve + eta_vp + Kds_dil);
+        const double hinv_Tii = 0.5 * inv_Tii;
+        dQdtxx = Txx * hinv_Tii; dQdtzz = Tzz * hinv_Tii; dQdtxz = Txz * inv_Tii;
         const double F0 = F_trial;
         int itp;
+        #pragma unroll 1
         for (itp = 0; itp < 20; itp++) {
             divp = gdot * sin_dil;
-            Pc = P + K * dt * divp;
+            Pc = P + Kdt * divp;
             eta_vp = k.eta_vp0 * (vp_exp == 0.0 ? 1.0 : pow(fabs(gdot), vp_exp));
             Tyield = C * cos_fric + Pc * sin_fric + gdot * eta_vp;
             double F = Tii - eta_ve * gdot - Tyield;
             double res = fabs(F);
-            if (res < 1.0e-11 || res / F0 < 1.0e-11) break;
-            gdot -= F / (-eta_ve - eta_vp / k.n_vp - K * dt * sin_fric * sin_dil);
+            if (res < 1.0e-11 || res < 1.0e-11 * F0) break;
+            gdot -= F / (-eta_ve - eta_vp * k.inv_n_vp - Kds_dil);
         }
         hist.add(1, itp + 1);
         Txx = 2.0 * eta_ve * (Exx - gdot * dQdtxx);
         Tzz = 2.0 * eta_ve * (Ezz - gdot * dQdtzz);
         double Tyy = -(Txx + Tzz);
-        Txz = 2.0 * eta_ve * (Exz - gdot * dQdtxz / 2.0);
+        Txz = 2.0 * eta_ve * (Exz - gdot * dQdtxz * 0.5);
         Tii = sqrt(0.5 * (Txx * Txx + Tzz * Tzz + Tyy * Tyy) + Txz * Txz);
     } else {
         hist.add(1, 0);
     }
-    const double eta_vep = Tii / (2.0 * Eii);
+    const double hinv_Eii = 0.5 * inv_Eii;
+    const double eta_vep = Tii * hinv_Eii;
     if (is_pl) Tii = 2.0 * eta_vep * Eii;
 
     /* dissipative viscosity: Peierls, dislocation, diffusion, constant,
@@ -359,7 +400,7 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
     if (peierls) inv_diss += inv_exp;
     if (k.pwl) inv_diss += inv_pwl;
     if (k.lin) inv_diss += inv_lin;
-    if (k.cst) inv_diss += 1.0 / k.eta_cst;
+    if (k.cst) inv_diss += k.inv_eta_cst;
 
     /* consistent tangent and plastic divergence derivatives */
     double etaVE = eta_ve, dexx = de_xx, dezz = de_zz, dexz = de_xz, dp = de_p;
@@ -367,37 +408,39 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
     if (is_pl) {
         double dsx = 0, dsz = 0, dsxz = 0, dsp = 0;
         if (tens == 1) {
-            const double dE[3] = {(2.0 * Exx + Ezz) / 2.0 / Eii, (2.0 * Ezz + Exx) / 2.0 / Eii, Exz / 1.0 / Eii};
+            const double dE[3] = {(2.0 * Exx + Ezz) * hinv_Eii, (2.0 * Ezz + Exx) * hinv_Eii, Exz * inv_Eii};
             const double dev[3] = {de_xx, de_zz, de_xz};
             double a2 = 2 * Eii * eta_ve - P * sin_fric;
-            double bden = K * dt * a2 * (C * P * cos_fric * sin_fric + (C * cos_fric - 2 * Eii * eta_ve) * a2);
+            double bden = Kdt * a2 * (C * P * cos_fric * sin_fric + (C * cos_fric - 2 * Eii * eta_ve) * a2);
+            const double inv_bden = 1.0 / bden;
             double q1 = C * P * cos_fric * sin_fric + a2 * a2;
             double od[3];
 #pragma unroll
             for (int i = 0; i < 3; i++) {
                 double dT = Eii * dev[i] + dE[i] * eta_ve;
-                od[i] = (2 * K * dt * sin_dil * dT * q1
-                         + P * (-C * cos_fric * dev[i] * a2 + 2 * C * cos_fric * (eta_ve + eta_vp) * dT + dev[i] * a2 * a2)) / bden;
+                od[i] = (2 * Kdt * sin_dil * dT * q1
+                         + P * (-C * cos_fric * dev[i] * a2 + 2 * C * cos_fric * (eta_ve + eta_vp) * dT + dev[i] * a2 * a2)) * inv_bden;
             }
             dsx = od[0]; dsz = od[1]; dsxz = od[2];
             dsp = (2 * C * Eii * cos_fric * (eta_ve + eta_vp) * (P * de_p - eta_ve) - C * P * cos_fric * de_p * a2
-                   + 2 * Eii * K * dt * sin_dil * (C * cos_fric * sin_fric * (P * de_p - eta_ve) + de_p * a2 * a2)
-                   + a2 * a2 * (P * de_p + eta_ve + eta_vp)) / bden;
+                   + 2 * Eii * Kdt * sin_dil * (C * cos_fric * sin_fric * (P * de_p - eta_ve) + de_p * a2 * a2)
+                   + a2 * a2 * (P * de_p + eta_ve + eta_vp)) * inv_bden;
         }
-        const double gg = 1.0 / (eta_ve + eta_vp / k.n_vp + K * dt * sin_fric * sin_dil);
-        const double Kds = K * dt * sin_fric;
-        double lx = gg * (((2.0 * Exx + Ezz) * eta_ve / Eii + 2.0 * Eii * de_xx) - gdot * (de_xx + Kds * dsx));
-        double lz = gg * (((2.0 * Ezz + Exx) * eta_ve / Eii + 2.0 * Eii * de_zz) - gdot * (de_zz + Kds * dsz));
-        double lxz = gg * ((2.0 * Exz * eta_ve / Eii + 2.0 * Eii * de_xz) - gdot * (de_xz + Kds * dsxz));
+        const double gg = 1.0 / (eta_ve + eta_vp * k.inv_n_vp + Kdt * sin_fric * sin_dil);
+        const double Kds = Kdt * sin_fric;
+        const double eE = eta_ve * inv_Eii;
+        double lx = gg * (((2.0 * Exx + Ezz) * eE + 2.0 * Eii * de_xx) - gdot * (de_xx + Kds * dsx));
+        double lz = gg * (((2.0 * Ezz + Exx) * eE + 2.0 * Eii * de_zz) - gdot * (de_zz + Kds * dsz));
+        double lxz = gg * ((2.0 * Exz * eE + 2.0 * Eii * de_xz) - gdot * (de_xz + Kds * dsxz));
         double lp = gg * ((-sin_fric + 2.0 * Eii * de_p) - gdot * (de_p + Kds * dsp));
-        double vx = eta_vp / gdot * lx * vp_exp, vz = eta_vp / gdot * lz * vp_exp;
-        double vxz = eta_vp / gdot * lxz * vp_exp, vpp = eta_vp / gdot * lp * vp_exp;
-        double a = eta_vp + K * dt * sin_fric * sin_dil;
-        double E3 = 2.0 * (Eii * Eii * Eii);
-        dexx = -0.5 * (2.0 * Exx + Ezz) * Tyield / E3 + (a * lx + gdot * vx + gdot * Kds * dsx) / (2.0 * Eii);
-        dezz = -0.5 * (2.0 * Ezz + Exx) * Tyield / E3 + (a * lz + gdot * vz + gdot * Kds * dsz) / (2.0 * Eii);
-        dexz = -Exz * Tyield / E3 + (a * lxz + gdot * vxz + gdot * Kds * dsxz) / (2.0 * Eii);
-        dp = (sin_fric + a * lp + gdot * vpp + gdot * Kds * dsp) / (2.0 * Eii);
+        const double evg = vp_exp == 0.0 ? 0.0 : eta_vp / gdot * vp_exp;
+        double vx = evg * lx, vz = evg * lz, vxz = evg * lxz, vpp = evg * lp;
+        double a = eta_vp + Kdt * sin_fric * sin_dil;
+        const double tyE3 = Tyield * (hinv_Eii * inv_Eii * inv_Eii);
+        dexx = -0.5 * (2.0 * Exx + Ezz) * tyE3 + (a * lx + gdot * vx + gdot * Kds * dsx) * hinv_Eii;
+        dezz = -0.5 * (2.0 * Ezz + Exx) * tyE3 + (a * lz + gdot * vz + gdot * Kds * dsz) * hinv_Eii;
+        dexz = -Exz * tyE3 + (a * lxz + gdot * vxz + gdot * Kds * dsxz) * hinv_Eii;
+        dp = (sin_fric + a * lp + gdot * vpp + gdot * Kds * dsp) * hinv_Eii;
         etaVE = eta_vep;
         ddxx = sin_dil * lx + gdot * dsx;
         ddzz = sin_dil * lz + gdot * dsz;
@@ -409,17 +452,17 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
         if (k.pwl) inv_pwl = exp(LN2 + lnC_pwl - (1.0 - k.n_pwl) * log(Tii));
     }
     double eta = 1.0 / inv_diss;
-    const double VE = elastic ? eta_ve / eta_el : 0.0;
+    const double VE = elastic ? eta_ve * inv_eel : 0.0;
     if (prm.step == 0) etaVE = eta;
     eta = fmin(fmax(eta, prm.mineta), prm.maxeta);
     etaVE = fmin(fmax(etaVE, prm.mineta), prm.maxeta);
 
     /* strain-rate partition (RheologyDensity.c:640-690) */
-    const double ex_el = (double)elastic * (Txx - in.Txx0) / 2.0 / eta_el;
-    const double ez_el = (double)elastic * (Tzz - in.Tzz0) / 2.0 / eta_el;
-    const double exz_el = (double)elastic * (Txz - in.Txz0) / 2.0 / eta_el;
+    const double ex_el = (double)elastic * (Txx - in.Txx0) * half_inv_eel;
+    const double ez_el = (double)elastic * (Tzz - in.Tzz0) * half_inv_eel;
+    const double exz_el = (double)elastic * (Txz - in.Txz0) * half_inv_eel;
     const double h_lin = 0.5 * inv_lin, h_pwl = 0.5 * inv_pwl, h_exp = 0.5 * inv_exp;
-    const double h_cst = k.cst ? 0.5 / k.eta_cst : 0.0;
+    const double h_cst = k.cst ? 0.5 * k.inv_eta_cst : 0.0;
     const double ex_d = gdot * dQdtxx + Txx_ve * h_lin + Txx * h_pwl + Txx_ve * h_exp + 0.0 + Txx_ve * h_cst;
     const double ez_d = gdot * dQdtzz + Tzz_ve * h_lin + Tzz * h_pwl + Tzz_ve * h_exp + 0.0 + Tzz_ve * h_cst;
     const double exz_d = gdot * dQdtxz / 2.0 + Txz_ve * h_lin + Txz * h_pwl + Txz_ve * h_exp + 0.0 + Txz_ve * h_cst;
@@ -436,7 +479,7 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
             o.put(FI_szzd, Tzz);
             o.put(FI_sxz_n, Txz);
             o.put(FI_VE_n, VE);
-            o.put(FI_eII_el, (double)elastic * fabs(Tii - sqrt(in.Txx0 * in.Txx0 + in.Txz0 * in.Txz0)) / 2.0 / eta_el);
+            o.put(FI_eII_el, (double)elastic * fabs(Tii - sqrt(in.Txx0 * in.Txx0 + in.Txz0 * in.Txz0)) * half_inv_eel);
             o.put(FI_eII_pl, gdot / 2.0);
             o.put(FI_eII_pwl, Tii * h_pwl);
             o.put(FI_eII_exp, Ee);
@@ -458,7 +501,7 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
         o.put(FI_ddivpdp_n, ddp);
         o.put(FI_p_corr, Pc);
         if (DIAG) {
-            o.put(FI_div_u_el, -(Pc - in.P0) / (K * dt));
+            o.put(FI_div_u_el, -(Pc - in.P0) / Kdt);
             o.put(FI_div_u_pl, divp);
             o.put(FI_div_u_r, 0.0);
             if (Xout) { double x = o.fr * in.X0; if (o.first) Xout[o.c] = x; else Xout[o.c] += x; }
@@ -501,8 +544,8 @@ __device__ __constant__ const int k_unit_fields_s[5] = {FI_eta_s, FI_detadexx_s,
 
 template <bool DIAG>
 __device__ __forceinline__ void rheo_cell(const DevGrid &g, const DevParams &prm, const PhaseK *__restrict__ pk,
-                                          double R, const float *__restrict__ guess, const BlockHist &hist, int c0,
-                                          int nc) {
+                                          const FastConst &fc, double R, const float *__restrict__ guess,
+                                          const BlockHist &hist, int c0, int nc) {
     const int t = g.BCp[c0];
     const bool active = t != 30 && t != 31;
     double *Xout = DIAG && prm.UnsplitDiffReac == 0 ? g.X_n : nullptr;
@@ -521,7 +564,7 @@ __device__ __forceinline__ void rheo_cell(const DevGrid &g, const DevParams &prm
             if (!(fabs(fr) > 1.0e-13)) continue;
             const Out o{g.eta_n, g.cstride, c0, FI_eta_n, fr, !any};
             const double gs = guess ? (double)guess[(size_t)p * nc + c0] : 0.0;
-            local_update<true, DIAG>(pk[p], R, prm, in, Xout, o, exxd, ezzd, exzc, gs, hist);
+            local_update<true, DIAG>(pk[p], R, prm, fc, in, Xout, o, exxd, ezzd, exzc, gs, hist);
             any = true;
         }
     }
@@ -541,21 +584,21 @@ __device__ __forceinline__ void rheo_cell(const DevGrid &g, const DevParams &prm
 }
 
 template <int NT, bool DIAG>
-__global__ void __launch_bounds__(NT) k_rheo_cells_fast(DevGrid g, DevParams prm, const PhaseK *__restrict__ pk,
-                                                        double R, const float *__restrict__ guess,
-                                                        unsigned long long *hist) {
+__global__ void __launch_bounds__(NT) k_rheo_cells_fast(DevGrid g, DevParams prm, FastConst fc,
+                                                        const PhaseK *__restrict__ pk, double R,
+                                                        const float *__restrict__ guess, unsigned long long *hist) {
     __shared__ unsigned int lh[3 * MDZ_HIST_BINS];
     const BlockHist bh = hist_begin(lh, hist);
     const int c0 = blockIdx.x * NT + threadIdx.x;
     const int nc = g.ncx * g.ncz;
-    if (c0 < nc) rheo_cell<DIAG>(g, prm, pk, R, guess, bh, c0, nc);
+    if (c0 < nc) rheo_cell<DIAG>(g, prm, pk, fc, R, guess, bh, c0, nc);
     hist_flush(bh, hist);
 }
 
 template <bool DIAG>
 __device__ __forceinline__ void rheo_vertex(const DevGrid &g, const DevParams &prm, const PhaseK *__restrict__ pk,
-                                            double R, const float *__restrict__ guess, const BlockHist &hist, int c1,
-                                            int nv) {
+                                            const FastConst &fc, double R, const float *__restrict__ guess,
+                                            const BlockHist &hist, int c1, int nv) {
     double *Xout = DIAG && prm.UnsplitDiffReac == 0 ? g.X_s : nullptr;
     bool any = false;
     if (g.BCg[c1] != 30) {
@@ -571,7 +614,7 @@ __device__ __forceinline__ void rheo_vertex(const DevGrid &g, const DevParams &p
             if (!(fabs(fr) > 1.0e-13)) continue;
             const Out o{g.VE_s, g.vstride, c1, FI_VE_s, fr, !any};
             const double gs = guess ? (double)guess[(size_t)p * nv + c1] : 0.0;
-            local_update<false, DIAG>(pk[p], R, prm, in, Xout, o, 0.0, 0.0, 0.0, gs, hist);
+            local_update<false, DIAG>(pk[p], R, prm, fc, in, Xout, o, 0.0, 0.0, 0.0, gs, hist);
             any = true;
         }
     }
@@ -586,14 +629,14 @@ __device__ __forceinline__ void rheo_vertex(const DevGrid &g, const DevParams &p
 }
 
 template <int NT, bool DIAG>
-__global__ void __launch_bounds__(NT) k_rheo_vertices_fast(DevGrid g, DevParams prm, const PhaseK *__restrict__ pk,
-                                                           double R, const float *__restrict__ guess,
-                                                           unsigned long long *hist) {
+__global__ void __launch_bounds__(NT) k_rheo_vertices_fast(DevGrid g, DevParams prm, FastConst fc,
+                                                           const PhaseK *__restrict__ pk, double R,
+                                                           const float *__restrict__ guess, unsigned long long *hist) {
     __shared__ unsigned int lh[3 * MDZ_HIST_BINS];
     const BlockHist bh = hist_begin(lh, hist);
     const int c1 = blockIdx.x * NT + threadIdx.x;
     const int nv = g.Nx * g.Nz;
-    if (c1 < nv) rheo_vertex<DIAG>(g, prm, pk, R, guess, bh, c1, nv);
+    if (c1 < nv) rheo_vertex<DIAG>(g, prm, pk, fc, R, guess, bh, c1, nv);
     hist_flush(bh, hist);
 }
 
@@ -615,24 +658,33 @@ void mdz_launch_phase_constants(const mat_prop *d_mt, int nph, void *d_pk, hipSt
 
 template <int NT, bool DIAG>
 static void launch_fast(const DevGrid &g, const DevParams &prm, const PhaseK *pk, double R, const float *guess_n,
-                        const float *guess_s, unsigned long long *hist, hipStream_t s) {
+                        const float *guess_s, unsigned long long *hist, hipStream_t s, int parts) {
     int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
-    hipLaunchKernelGGL((k_rheo_cells_fast<NT, DIAG>), dim3(mdz_div_up(nc, NT)), dim3(NT), 0, s, g, prm, pk, R, guess_n,
-                       hist);
-    hipLaunchKernelGGL((k_rheo_vertices_fast<NT, DIAG>), dim3(mdz_div_up(nv, NT)), dim3(NT), 0, s, g, prm, pk, R,
-                       guess_s, hist);
+    FastConst fc;
+    fc.T_peierls = (1200.0 + ZEROC) / prm.sT;
+    fc.T_min = ZEROC / prm.sT;
+    fc.eta_start = 1.0e100 / prm.seta;
+    fc.P_tens = -25e6 / prm.sS;
+    fc.E_floor = 1e-30 / prm.sE;
+    if (parts & 1)
+        hipLaunchKernelGGL((k_rheo_cells_fast<NT, DIAG>), dim3(mdz_div_up(nc, NT)), dim3(NT), 0, s, g, prm, fc, pk, R,
+                           guess_n, hist);
+    if (parts & 2)
+        hipLaunchKernelGGL((k_rheo_vertices_fast<NT, DIAG>), dim3(mdz_div_up(nv, NT)), dim3(NT), 0, s, g, prm, fc, pk,
+                           R, guess_s, hist);
 }
 
 int mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
-                             const float *guess_s, unsigned long long *hist, int nt, int diag, hipStream_t s) {
+                             const float *guess_s, unsigned long long *hist, int nt, int diag, hipStream_t s,
+                             int parts) {
     const PhaseK *pk = (const PhaseK *)d_pk;
     switch (nt * 2 + (diag ? 1 : 0)) {
-    case 129: launch_fast<64, true>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
-    case 128: launch_fast<64, false>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
-    case 257: launch_fast<128, true>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
-    case 256: launch_fast<128, false>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
-    case 513: launch_fast<256, true>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
-    case 512: launch_fast<256, false>(g, prm, pk, R, guess_n, guess_s, hist, s); return MDZ_OK;
+    case 129: launch_fast<64, true>(g, prm, pk, R, guess_n, guess_s, hist, s, parts); return MDZ_OK;
+    case 128: launch_fast<64, false>(g, prm, pk, R, guess_n, guess_s, hist, s, parts); return MDZ_OK;
+    case 257: launch_fast<128, true>(g, prm, pk, R, guess_n, guess_s, hist, s, parts); return MDZ_OK;
+    case 256: launch_fast<128, false>(g, prm, pk, R, guess_n, guess_s, hist, s, parts); return MDZ_OK;
+    case 513: launch_fast<256, true>(g, prm, pk, R, guess_n, guess_s, hist, s, parts); return MDZ_OK;
+    case 512: launch_fast<256, false>(g, prm, pk, R, guess_n, guess_s, hist, s, parts); return MDZ_OK;
     default: return mdz_error(MDZ_EINVAL, "rheology block size: 64, 128 or 256 threads");
     }
 }
