@@ -135,10 +135,10 @@ struct Out {
 };
 
 /* Node inputs; the friction and dilatancy angles enter as sines/cosines,
- * evaluated once per node. */
+ * evaluated once per node.  (Txx0, Tzz0, Txz0 feed only the visco-elastic
+ * rates; the diagnostic outputs read the old stresses again, see DiagSrc.) */
 struct NodeIn {
-    double G, T, P, d, phi, X0, Exx, Ezz, Exz, Gxx, Gzz, Gxz, Txx0, Tzz0, Txz0, sin_dil, sin_fric, cos_fric, C, P0,
-        beta;
+    double G, T, P, d, phi, Exx, Ezz, Exz, Gxx, Gzz, Gxz, Txx0, Tzz0, Txz0, sin_dil, sin_fric, cos_fric, C, beta;
 };
 
 /* Effective visco-elastic strain rates for an isotropic node: the inverse
@@ -176,10 +176,20 @@ struct FastConst {
  * tangent, the plastic divergence derivatives, p_corr).  Divisions by
  * quantities used more than once are products with one reciprocal (a
  * rounding-level difference from the reference's expressions). */
+/* Inputs that only the diagnostic outputs read (old stresses, P0, X0, the
+ * deviatoric strain rates).  They are loaded again where those outputs are
+ * computed, behind a compiler barrier, instead of staying in registers
+ * through the local solve (that is what keeps the all-outputs centroid
+ * kernel within its register budget). */
+struct DiagSrc {
+    const double *sxx0, *szz0, *sxz0, *P0, *X0, *exx, *ezz, *exz;
+    int c;
+};
+
 template <bool CELL, bool DIAG>
 __device__ __forceinline__ void local_update(const PhaseK &k, double R, const DevParams &prm, const FastConst &fc,
-                                             const NodeIn &in, double *Xout, const Out &o, double exxd, double ezzd,
-                                             double exzc, double guess, const BlockHist &hist) {
+                                             const NodeIn &in, double *Xout, const Out &o, const DiagSrc &ds,
+                                             double guess, const BlockHist &hist) {
     const double dt = prm.dt;
     int elastic = prm.iselastic;
     double G = in.G, T = in.T, P = in.P, C = in.C, K = 1.0 / in.beta;
@@ -360,7 +370,7 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
     }
     /* the diffusion, Peierls and constant-viscosity parts of the strain-rate
      * partition use the visco-elastic stresses (before the return map) */
-    const double Txx_ve = Txx, Tzz_ve = Tzz, Txz_ve = Txz;
+    /* (the visco-elastic stresses are in.Gxx * sfac, ... : recomputed below) */
     const bool is_pl = F_trial > 1e-17;
     if (is_pl) {
         eta_vp = k.eta_vp0 * (vp_exp == 0.0 ? 1.0 : pow(Eii, vp_exp));
@@ -457,10 +467,23 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
     eta = fmin(fmax(eta, prm.mineta), prm.maxeta);
     etaVE = fmin(fmax(etaVE, prm.mineta), prm.maxeta);
 
+    /* The output addresses depend on the node index; making it opaque here
+     * keeps the compiler from computing the ~40 64-bit addresses before the
+     * phase loop and holding them in registers through the local solve. */
+    Out w = o;
+    __asm__ volatile("" : "+v"(w.c));
+    if (DIAG) __asm__ volatile("" ::: "memory");
+    double Txx0 = 0, Tzz0 = 0, Txz0 = 0, P0 = 0, X0 = 0, exxd = 0, ezzd = 0, exzc = 0;
+    if (DIAG) {
+        Txx0 = ds.sxx0[ds.c]; Tzz0 = ds.szz0[ds.c]; Txz0 = ds.sxz0[ds.c];
+        P0 = ds.P0[ds.c]; X0 = ds.X0[ds.c];
+        if (CELL) { exxd = ds.exx[ds.c]; ezzd = ds.ezz[ds.c]; exzc = ds.exz[ds.c]; }
+    }
+    const double Txx_ve = in.Gxx * sfac, Tzz_ve = in.Gzz * sfac, Txz_ve = in.Gxz * sfac;
     /* strain-rate partition (RheologyDensity.c:640-690) */
-    const double ex_el = (double)elastic * (Txx - in.Txx0) * half_inv_eel;
-    const double ez_el = (double)elastic * (Tzz - in.Tzz0) * half_inv_eel;
-    const double exz_el = (double)elastic * (Txz - in.Txz0) * half_inv_eel;
+    const double ex_el = (double)elastic * (Txx - Txx0) * half_inv_eel;
+    const double ez_el = (double)elastic * (Tzz - Tzz0) * half_inv_eel;
+    const double exz_el = (double)elastic * (Txz - Txz0) * half_inv_eel;
     const double h_lin = 0.5 * inv_lin, h_pwl = 0.5 * inv_pwl, h_exp = 0.5 * inv_exp;
     const double h_cst = k.cst ? 0.5 * k.inv_eta_cst : 0.0;
     const double ex_d = gdot * dQdtxx + Txx_ve * h_lin + Txx * h_pwl + Txx_ve * h_exp + 0.0 + Txx_ve * h_cst;
@@ -468,59 +491,59 @@ __device__ __forceinline__ void local_update(const PhaseK &k, double R, const De
     const double exz_d = gdot * dQdtxz / 2.0 + Txz_ve * h_lin + Txz * h_pwl + Txz_ve * h_exp + 0.0 + Txz_ve * h_cst;
 
     if (CELL) {
-        o.put(FI_eta_n, etaVE);
-        if (DIAG) o.put(FI_eta_phys_n, eta);
-        o.put(FI_detadexx_n, dexx);
-        o.put(FI_detadezz_n, dezz);
-        o.put(FI_detadgxz_n, dexz / 2.0);
-        o.put(FI_detadp_n, dp);
+        w.put(FI_eta_n, etaVE);
+        if (DIAG) w.put(FI_eta_phys_n, eta);
+        w.put(FI_detadexx_n, dexx);
+        w.put(FI_detadezz_n, dezz);
+        w.put(FI_detadgxz_n, dexz / 2.0);
+        w.put(FI_detadp_n, dp);
         if (DIAG) {
-            o.put(FI_sxxd, Txx);
-            o.put(FI_szzd, Tzz);
-            o.put(FI_sxz_n, Txz);
-            o.put(FI_VE_n, VE);
-            o.put(FI_eII_el, (double)elastic * fabs(Tii - sqrt(in.Txx0 * in.Txx0 + in.Txz0 * in.Txz0)) * half_inv_eel);
-            o.put(FI_eII_pl, gdot / 2.0);
-            o.put(FI_eII_pwl, Tii * h_pwl);
-            o.put(FI_eII_exp, Ee);
-            o.put(FI_eII_lin, El);
-            o.put(FI_eII_gbs, Eg);
-            o.put(FI_eII_cst, Ec);
-            o.put_inv(FI_d_n, k.gs_ref);
-            o.put(FI_exx_el, ex_el);
-            o.put(FI_exx_diss, ex_d);
-            o.put(FI_ezz_el, ez_el);
-            o.put(FI_ezz_diss, ez_d);
-            o.put(FI_Wtot, Txx * exxd + Tzz * ezzd + (Txx + Tzz) * (exxd + ezzd) + 2.0 * Txz * exzc);
-            o.put(FI_Wdiss, Txx * ex_d + Tzz * ez_d + (Txx + Tzz) * (ex_d + ez_d) + 2.0 * Txz * exz_d);
-            o.put(FI_Wel, Txx * ex_el + Tzz * ez_el + (Txx + Tzz) * (ex_el + ez_el) + 2.0 * Txz * exz_el);
+            w.put(FI_sxxd, Txx);
+            w.put(FI_szzd, Tzz);
+            w.put(FI_sxz_n, Txz);
+            w.put(FI_VE_n, VE);
+            w.put(FI_eII_el, (double)elastic * fabs(Tii - sqrt(Txx0 * Txx0 + Txz0 * Txz0)) * half_inv_eel);
+            w.put(FI_eII_pl, gdot / 2.0);
+            w.put(FI_eII_pwl, Tii * h_pwl);
+            w.put(FI_eII_exp, Ee);
+            w.put(FI_eII_lin, El);
+            w.put(FI_eII_gbs, Eg);
+            w.put(FI_eII_cst, Ec);
+            w.put_inv(FI_d_n, k.gs_ref);
+            w.put(FI_exx_el, ex_el);
+            w.put(FI_exx_diss, ex_d);
+            w.put(FI_ezz_el, ez_el);
+            w.put(FI_ezz_diss, ez_d);
+            w.put(FI_Wtot, Txx * exxd + Tzz * ezzd + (Txx + Tzz) * (exxd + ezzd) + 2.0 * Txz * exzc);
+            w.put(FI_Wdiss, Txx * ex_d + Tzz * ez_d + (Txx + Tzz) * (ex_d + ez_d) + 2.0 * Txz * exz_d);
+            w.put(FI_Wel, Txx * ex_el + Tzz * ez_el + (Txx + Tzz) * (ex_el + ez_el) + 2.0 * Txz * exz_el);
         }
-        o.put(FI_ddivpdexx_n, ddxx);
-        o.put(FI_ddivpdezz_n, ddzz);
-        o.put(FI_ddivpdgxz_n, ddxz / 2.0);
-        o.put(FI_ddivpdp_n, ddp);
-        o.put(FI_p_corr, Pc);
+        w.put(FI_ddivpdexx_n, ddxx);
+        w.put(FI_ddivpdezz_n, ddzz);
+        w.put(FI_ddivpdgxz_n, ddxz / 2.0);
+        w.put(FI_ddivpdp_n, ddp);
+        w.put(FI_p_corr, Pc);
         if (DIAG) {
-            o.put(FI_div_u_el, -(Pc - in.P0) / Kdt);
-            o.put(FI_div_u_pl, divp);
-            o.put(FI_div_u_r, 0.0);
-            if (Xout) { double x = o.fr * in.X0; if (o.first) Xout[o.c] = x; else Xout[o.c] += x; }
-            o.put(FI_OverS_n, is_pl ? eta_vp * gdot : 0.0);
+            w.put(FI_div_u_el, -(Pc - P0) / Kdt);
+            w.put(FI_div_u_pl, divp);
+            w.put(FI_div_u_r, 0.0);
+            if (Xout) { double x = w.fr * X0; if (w.first) Xout[w.c] = x; else Xout[w.c] += x; }
+            w.put(FI_OverS_n, is_pl ? eta_vp * gdot : 0.0);
         }
     } else {
-        o.put(FI_eta_s, etaVE);
-        if (DIAG) o.put(FI_eta_phys_s, eta);
-        o.put(FI_detadexx_s, dexx);
-        o.put(FI_detadezz_s, dezz);
-        o.put(FI_detadgxz_s, dexz / 2.0);
-        o.put(FI_detadp_s, dp);
+        w.put(FI_eta_s, etaVE);
+        if (DIAG) w.put(FI_eta_phys_s, eta);
+        w.put(FI_detadexx_s, dexx);
+        w.put(FI_detadezz_s, dezz);
+        w.put(FI_detadgxz_s, dexz / 2.0);
+        w.put(FI_detadp_s, dp);
         if (DIAG) {
-            o.put(FI_sxz, Txz);
-            o.put(FI_VE_s, VE);
-            o.put(FI_exz_el, exz_el);
-            o.put(FI_exz_diss, exz_d);
-            o.put(FI_OverS_s, is_pl ? eta_vp * gdot : 0.0);
-            if (Xout) { double x = o.fr * in.X0; if (o.first) Xout[o.c] = x; else Xout[o.c] += x; }
+            w.put(FI_sxz, Txz);
+            w.put(FI_VE_s, VE);
+            w.put(FI_exz_el, exz_el);
+            w.put(FI_exz_diss, exz_d);
+            w.put(FI_OverS_s, is_pl ? eta_vp * gdot : 0.0);
+            if (Xout) { double x = w.fr * X0; if (w.first) Xout[w.c] = x; else Xout[w.c] += x; }
         }
     }
 }
@@ -557,14 +580,15 @@ __device__ __forceinline__ void rheo_cell(const DevGrid &g, const DevParams &prm
         in.Txx0 = g.sxxd0[c0]; in.Tzz0 = g.szzd0[c0]; in.Txz0 = g.sxz0_n[c0];
         ve_rates_iso(exxd, ezzd, exzc, in.Txx0, in.Tzz0, in.Txz0, etae, in);
         in.G = g.mu_n[c0]; in.T = g.T[c0]; in.P = g.p_in[c0]; in.d = g.d0_n[c0]; in.phi = g.phi0_n[c0];
-        in.X0 = g.X0_n[c0]; in.C = g.C_n[c0]; in.P0 = g.p0_n[c0]; in.beta = g.bet_n[c0];
+        in.C = g.C_n[c0]; in.beta = g.bet_n[c0];
         node_angles(prm, g.fric_n[c0], g.dil_n[c0], in);
+        const DiagSrc ds{g.sxxd0, g.szzd0, g.sxz0_n, g.p0_n, g.X0_n, g.exxd, g.ezzd, g.exz_n, c0};
         for (int p = 0; p < prm.Nb_phases; p++) {
             const double fr = g.phase_n[(size_t)p * nc + c0];
             if (!(fabs(fr) > 1.0e-13)) continue;
             const Out o{g.eta_n, g.cstride, c0, FI_eta_n, fr, !any};
             const double gs = guess ? (double)guess[(size_t)p * nc + c0] : 0.0;
-            local_update<true, DIAG>(pk[p], R, prm, fc, in, Xout, o, exxd, ezzd, exzc, gs, hist);
+            local_update<true, DIAG>(pk[p], R, prm, fc, in, Xout, o, ds, gs, hist);
             any = true;
         }
     }
@@ -583,10 +607,13 @@ __device__ __forceinline__ void rheo_cell(const DevGrid &g, const DevParams &prm
     }
 }
 
+/* Two waves per SIMD: every variant fits 256 registers without scratch
+ * (the all-outputs centroid kernel needs 255 with the opaque output index,
+ * 258 unconstrained); one wave per SIMD left the FP64 latency exposed. */
 template <int NT, bool DIAG>
-__global__ void __launch_bounds__(NT) k_rheo_cells_fast(DevGrid g, DevParams prm, FastConst fc,
-                                                        const PhaseK *__restrict__ pk, double R,
-                                                        const float *__restrict__ guess, unsigned long long *hist) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
+k_rheo_cells_fast(DevGrid g, DevParams prm, FastConst fc, const PhaseK *__restrict__ pk, double R,
+                  const float *__restrict__ guess, unsigned long long *hist) {
     __shared__ unsigned int lh[3 * MDZ_HIST_BINS];
     const BlockHist bh = hist_begin(lh, hist);
     const int c0 = blockIdx.x * NT + threadIdx.x;
@@ -607,14 +634,15 @@ __device__ __forceinline__ void rheo_vertex(const DevGrid &g, const DevParams &p
         in.Txx0 = g.sxxd0_s[c1]; in.Tzz0 = g.szzd0_s[c1]; in.Txz0 = g.sxz0[c1];
         ve_rates_iso(g.exxd_s[c1], g.ezzd_s[c1], g.exz[c1], in.Txx0, in.Tzz0, in.Txz0, etae, in);
         in.G = g.mu_s[c1]; in.T = g.T_s[c1]; in.P = g.P_s[c1]; in.d = g.d0_s[c1]; in.phi = g.phi0_s[c1];
-        in.X0 = g.X0_s[c1]; in.C = g.C_s[c1]; in.P0 = g.p0_s[c1]; in.beta = g.bet_s[c1];
+        in.C = g.C_s[c1]; in.beta = g.bet_s[c1];
         node_angles(prm, g.fric_s[c1], g.dil_s[c1], in);
+        const DiagSrc ds{g.sxxd0_s, g.szzd0_s, g.sxz0, g.p0_s, g.X0_s, nullptr, nullptr, nullptr, c1};
         for (int p = 0; p < prm.Nb_phases; p++) {
             const double fr = g.phase_s[(size_t)p * nv + c1];
             if (!(fabs(fr) > 1.0e-13)) continue;
             const Out o{g.VE_s, g.vstride, c1, FI_VE_s, fr, !any};
             const double gs = guess ? (double)guess[(size_t)p * nv + c1] : 0.0;
-            local_update<false, DIAG>(pk[p], R, prm, fc, in, Xout, o, 0.0, 0.0, 0.0, gs, hist);
+            local_update<false, DIAG>(pk[p], R, prm, fc, in, Xout, o, ds, gs, hist);
             any = true;
         }
     }
@@ -629,9 +657,9 @@ __device__ __forceinline__ void rheo_vertex(const DevGrid &g, const DevParams &p
 }
 
 template <int NT, bool DIAG>
-__global__ void __launch_bounds__(NT) k_rheo_vertices_fast(DevGrid g, DevParams prm, FastConst fc,
-                                                           const PhaseK *__restrict__ pk, double R,
-                                                           const float *__restrict__ guess, unsigned long long *hist) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
+k_rheo_vertices_fast(DevGrid g, DevParams prm, FastConst fc, const PhaseK *__restrict__ pk, double R,
+                     const float *__restrict__ guess, unsigned long long *hist) {
     __shared__ unsigned int lh[3 * MDZ_HIST_BINS];
     const BlockHist bh = hist_begin(lh, hist);
     const int c1 = blockIdx.x * NT + threadIdx.x;
