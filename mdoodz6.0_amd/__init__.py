@@ -127,7 +127,7 @@ HOT_SYMBOLS = ["Viscosity", "NonNewtonianViscosityGrid", "StrainRateComponents",
                "ExtractDiagonalScale", "ScaleMatrix", "AddCoeff2", "mdz_last_error", "mdz_set_exit_on_error", "mdz_last_status", "mdz_set_rheology_kernel", "mdz_ctx_create",
                "mdz_ctx_destroy", "mdz_ctx_upload_step_fields", "mdz_ctx_upload_solution",
                "mdz_ctx_download_solution", "mdz_ctx_download_grid", "mdz_ctx_set_rheology_mode",
-               "mdz_ctx_defer_diagnostics",
+               "mdz_ctx_defer_diagnostics", "mdz_ctx_fuse_guess",
                "mdz_ctx_set_network", "mdz_ctx_newton_unit", "mdz_ctx_block_info", "mdz_ctx_download_block",
                "mdz_ctx_download_residual", "mdz_ctx_diagonal_scaling", "mdz_ctx_collect_histogram", "mdz_ctx_iteration_histogram", "mdz_ctx_stream",
                "mdz_ctx_stage_times", "mdz_ctx_time_units", "mdz_copy_probe", "mdz_mlp_create",
@@ -171,6 +171,7 @@ def lib_hot():
         L.mdz_ctx_download_solution.argtypes = [C.c_void_p, dp, dp, dp]
         L.mdz_ctx_set_rheology_mode.argtypes = [C.c_void_p, C.c_int]
         L.mdz_ctx_defer_diagnostics.argtypes = [C.c_void_p, C.c_int]
+        L.mdz_ctx_fuse_guess.argtypes = [C.c_void_p, C.c_int]
         L.mdz_ctx_set_network.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_size_t]
         L.mdz_ctx_iteration_histogram.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
         L.mdz_ctx_collect_histogram.argtypes = [C.c_void_p, C.c_int]

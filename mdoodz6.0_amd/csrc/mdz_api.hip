@@ -126,6 +126,7 @@ struct mdz_ctx {
     double *dpic_n, *dpic_s; /* J = 0 D11 (= D22) at the centroids and D33 at the vertices (fused D passes) */
     int d_jac;               /* D11_n..D34_s hold the J = 1 tensor (left by a fused unit) */
     int defer_diag;          /* units write only the rheology outputs the unit itself reads */
+    int guess_fused;         /* learned guess evaluated in the log-space rheology kernels' prologue */
     int diag_stale;          /* the other rheology outputs are not those of the last rheology pass */
     double *p_snap;          /* p_in as the last rheology pass read it, once p_in has changed since */
     int p_snapped;
@@ -553,11 +554,20 @@ extern "C" int mdz_set_rheology_kernel(int which) {
     return MDZ_OK;
 }
 
+/* With mdz_ctx_fuse_guess the log-space kernels evaluate the network in
+ * their own prologue; otherwise (and always for the generic kernel)
+ * k_mlp_guess writes the guess arrays first. */
+static bool fused_guess(const mdz_ctx *c, int use_guess) {
+    return use_guess && c->guess_fused && g_rheo_kernel == 0 && mdz_rheo_fast_eligible(c->prm, &c->materials);
+}
+
 /* first part of the rheology: vertex values of div_u, T, P, d and phi
- * (NonNewtonianViscosityGrid :979-983), then the network's guess */
+ * (NonNewtonianViscosityGrid :979-983), then the network's guess when it is
+ * not evaluated inside the rheology kernels */
 static int rheology_prep(mdz_ctx *c, int use_guess, hipStream_t s) {
     mdz_launch_interp5(c->g, c->prm, s);
-    if (use_guess) {
+    if (use_guess && !c->net_wp) return set_err(MDZ_EINVAL, "learned guess without a network");
+    if (use_guess && !fused_guess(c, use_guess)) {
         if (!c->net_wp || !c->guess_n || !c->guess_s) return set_err(MDZ_EINVAL, "learned guess without a network");
         mdz_launch_mlp_guess(c->g, c->prm, DevMLPW{c->net_wp, c->net_bp}, c->guess_n, c->guess_s, s);
     }
@@ -572,7 +582,11 @@ static int rheology_kernels(mdz_ctx *c, int use_guess, hipStream_t s) {
     c->p_snapped = 0;
     if (g_rheo_kernel == 0 && mdz_rheo_fast_eligible(c->prm, &c->materials)) {
         const int diag = !c->defer_diag;
-        int rc = mdz_launch_rheology_fast(c->g, c->prm, c->d_pk, c->materials.R, gn, gs, hist, c->shape.rheo, diag, s);
+        const DevMLPW net{c->net_wp, c->net_bp};
+        const bool fused = fused_guess(c, use_guess);
+        int rc = mdz_launch_rheology_fast(c->g, c->prm, c->d_pk, c->materials.R, fused ? nullptr : gn,
+                                          fused ? nullptr : gs, hist, c->shape.rheo, diag, s, 3,
+                                          fused ? &net : nullptr);
         if (rc) return rc;
         c->diag_stale = !diag;
         return MDZ_OK;
@@ -591,8 +605,11 @@ static int flush_diagnostics(mdz_ctx *c) {
     DevGrid g = c->g;
     if (c->p_snapped) g.p_in = c->p_snap;
     const int guess = c->prm.rheo_mode == MDZ_RHEO_LEARNED;
-    int rc = mdz_launch_rheology_fast(g, c->prm, c->d_pk, c->materials.R, guess ? c->guess_n : nullptr,
-                                      guess ? c->guess_s : nullptr, nullptr, c->shape.rheo, 1, c->stream);
+    const DevMLPW net{c->net_wp, c->net_bp};
+    const bool fused = fused_guess(c, guess);
+    int rc = mdz_launch_rheology_fast(g, c->prm, c->d_pk, c->materials.R, guess && !fused ? c->guess_n : nullptr,
+                                      guess && !fused ? c->guess_s : nullptr, nullptr, c->shape.rheo, 1, c->stream,
+                                      3, fused ? &net : nullptr);
     if (rc) return rc;
     c->diag_stale = 0;
     c->p_snapped = 0;
@@ -1163,6 +1180,15 @@ extern "C" int mdz_ctx_download_grid(mdz_ctx *c, grid *mesh) {
         if (FREF[i].role & MDZ_ROLE_O)
             if ((rc = down(c, mesh, i))) return rc;
     return sync(c);
+}
+
+extern "C" int mdz_ctx_fuse_guess(mdz_ctx *c, int on) {
+    if (!c) return set_err(MDZ_EINVAL, "NULL context");
+    int rc;
+    if ((rc = flush_diagnostics(c))) return rc;
+    c->guess_fused = on != 0;
+    c->version++;
+    return MDZ_OK;
 }
 
 extern "C" int mdz_ctx_defer_diagnostics(mdz_ctx *c, int on) {

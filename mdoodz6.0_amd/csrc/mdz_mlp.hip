@@ -25,10 +25,8 @@
 
 #include "mdz_kernels.h"
 #include "mdz_mlp.h"
+#include "mdz_mlp_dev.h"
 #include "mdz_viscosity.h"
-
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 #define TBLK 256 /* training / inference block: 4 waves x 16 samples */
 #define TSAMP 64
@@ -48,42 +46,6 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 #define LDS_W4T (LDS_W3T + 64 * LS64)
 #define LDS_ALL (LDS_W4T + 64 * LS16)
 #define AS LS64 /* row stride of the staged activations [rows][64 samples] */
-
-/* FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32) */
-__device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
-__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float(((unsigned)b) << 16); }
-
-/* two FP32 -> one register of two BF16 (round to nearest even) */
-__device__ __forceinline__ unsigned cvt_pk_bf16(float lo, float hi) {
-    typedef float f2 __attribute__((ext_vector_type(2)));
-    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
-    return __builtin_bit_cast(unsigned, __builtin_convertvector(f2{lo, hi}, b2));
-}
-
-__device__ __forceinline__ bf16x4 pack4(f32x4 v) {
-    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-    u32x2 p = {cvt_pk_bf16(v[0], v[1]), cvt_pk_bf16(v[2], v[3])};
-    return __builtin_bit_cast(bf16x4, p);
-}
-
-__device__ __forceinline__ bf16x4 lds4(const unsigned short *p) { return *(const bf16x4 *)p; }
-
-__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ float sigmoidf_(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
-__device__ __forceinline__ float silu(float z) { return z * sigmoidf_(z); }
-__device__ __forceinline__ float dsilu(float z) {
-    float s = sigmoidf_(z);
-    return s * (1.0f + z * (1.0f - s));
-}
-
-__device__ __forceinline__ f32x4 silu4(f32x4 z) {
-    f32x4 h;
-    h[0] = silu(z[0]); h[1] = silu(z[1]); h[2] = silu(z[2]); h[3] = silu(z[3]);
-    return h;
-}
 
 /* z[mt] = W[16mt .. 16mt+15][:] * B (+ bias), NK K-steps of 16 inputs; W
  * row-major with leading dimension ldw (in LDS); NM output tiles. */
@@ -495,43 +457,11 @@ __global__ void k_mlp_generate(long long first, long long count, unsigned long l
 /* ------------------------------------------------------------------ */
 /* grid inference: initial guess of eta_ve                              */
 /* ------------------------------------------------------------------ */
-/* dominant phase among the first three with a fraction above 1e-13, or -1 */
-__device__ __forceinline__ int dominant_phase(const double *frac, size_t stride, int node, int nph) {
-    int best = -1;
-    double bf = 1.0e-13;
-    for (int p = 0; p < nph; p++) {
-        double fr = frac[(size_t)p * stride + node];
-        if (fabs(fr) > bf) { bf = fabs(fr); best = p; }
-    }
-    return best <= 2 ? best : -1;
-}
+#define GBLK 256 /* grid inference: 4 waves x 64 nodes */
 
-/* Inference layer for 64 samples per wave: four 16-sample groups share
- * every weight operand read from LDS, and their independent accumulation
- * chains interleave on the matrix core.  b[j][ks]: group j's input tile ks
- * (B layout); z[j][mt]: group j's output tile mt (D layout). */
-template <int NM, int NK>
-__device__ __forceinline__ void dense_x4(const unsigned short *W, int ldw, const bf16x4 (*b)[4], const float *bias,
-                                         f32x4 (*z)[4]) {
-    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
-#pragma unroll
-    for (int mt = 0; mt < NM; mt++) {
-        /* the accumulation starts from the bias */
-        const f32x4 b0 = *(const f32x4 *)(bias + 16 * mt + 4 * q);
-#pragma unroll
-        for (int j = 0; j < 4; j++) z[j][mt] = b0;
-#pragma unroll
-        for (int ks = 0; ks < NK; ks++) {
-            const bf16x4 a = lds4(W + (16 * mt + r) * ldw + 16 * ks + 4 * q);
-#pragma unroll
-            for (int j = 0; j < 4; j++) z[j][mt] = mfma(a, b[j][ks], z[j][mt]);
-        }
-    }
-}
-
-#define GBLK 256             /* grid inference: 4 waves x 64 nodes */
-#define GS 20                /* padded row (BF16 entries) of a staged feature vector */
-
+/* Grid inference at 64 nodes per wave: one feature vector per lane staged
+ * through LDS, four 16-sample groups sharing every weight operand, the
+ * guess factor of the dominant phase written per phase. */
 template <bool CELL>
 __global__ void __launch_bounds__(GBLK) k_mlp_guess(DevGrid g, DevParams prm, DevMLPW w, float *__restrict__ guess) {
     __shared__ __attribute__((aligned(16))) unsigned short sW[LDS_FWD];
@@ -539,105 +469,25 @@ __global__ void __launch_bounds__(GBLK) k_mlp_guess(DevGrid g, DevParams prm, De
     __shared__ __attribute__((aligned(16))) unsigned short sX[4][64 * GS]; /* per wave: 64 samples x 16 features */
     __shared__ float sY[4][64];                                            /* per wave: network output y0 */
     load_weights(w, sW, sB, MLP_WP_W2T);
-    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4, wv = threadIdx.x >> 6;
+    const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int n = CELL ? g.ncx * g.ncz : g.Nx * g.Nz;
     const int ntiles = (n + GBLK - 1) / GBLK;
-    unsigned short *xw = sX[wv];
     /* grid-stride over tiles of 256 nodes: the weights are staged once per block */
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        /* one node per lane: features in FP32, rounded to BF16 into LDS */
         const int node = tile * GBLK + threadIdx.x;
         const bool inside = node < n;
-        int ph = -1;
         float f[MLP_IN];
+        int ph = -1;
+        if (inside) ph = guess_features<CELL>(g, prm, node, n, f);
+        else
 #pragma unroll
-        for (int k = 0; k < MLP_IN; k++) f[k] = 0.0f;
-        if (inside) {
-            const bool active = CELL ? (g.BCp[node] != 30 && g.BCp[node] != 31) : g.BCg[node] != 30;
-            if (active) ph = dominant_phase(CELL ? g.phase_n : g.phase_s, n, node, min(prm.Nb_phases, g.nph));
-            if (ph >= 0) {
-                const float mu = (float)(CELL ? g.mu_n[node] : g.mu_s[node]);
-                const float etae = prm.iselastic == 1 ? (float)prm.dt * mu : 1.0f;
-                const float sxx0 = (float)(CELL ? g.sxxd0[node] : g.sxxd0_s[node]);
-                const float szz0 = (float)(CELL ? g.szzd0[node] : g.szzd0_s[node]);
-                const float sxz0 = (float)(CELL ? g.sxz0_n[node] : g.sxz0[node]);
-                const float Exx = (float)(CELL ? g.exxd[node] : g.exxd_s[node]) + 0.5f * sxx0 / etae;
-                const float Ezz = (float)(CELL ? g.ezzd[node] : g.ezzd_s[node]) + 0.5f * szz0 / etae;
-                const float Exz = (float)(CELL ? g.exz_n[node] : g.exz[node]) + 0.5f * sxz0 / etae;
-                const float Eyy = -(Exx + Ezz);
-                const float Eii = sqrtf(0.5f * (Exx * Exx + Ezz * Ezz + Eyy * Eyy) + Exz * Exz);
-                const float sT = (float)prm.sT, sS = (float)prm.sS;
-                const float T = fmaxf((float)(CELL ? g.T[node] : g.T_s[node]) * sT, 273.15f);
-                const float P = (float)(CELL ? g.p_in[node] : g.P_s[node]) * sS;
-                const float C = (float)(CELL ? g.C_n[node] : g.C_s[node]) * sS;
-                const float fr = (float)(CELL ? g.fric_n[node] : g.fric_s[node]);
-                const float L10 = 0.43429448f; /* log10(e) */
-                const float tauy = fmaxf(C * __cosf(fr) + P * __sinf(fr), 1.0e3f);
-                f[0] = (__logf(fmaxf(Eii * (float)prm.sE, 1e-30f)) * L10 + 15.0f) * (1.0f / 3.0f);
-                f[1] = (T - 1123.0f) * (1.0f / 850.0f);
-                f[2] = (P * 1e-9f - 5.0f) * 0.2f;
-                f[3] = (sqrtf(sxx0 * sxx0 + sxz0 * sxz0) * sS * 1e-6f - 250.0f) * (1.0f / 250.0f);
-                f[4] = __logf((float)(prm.dt * prm.st)) * L10 - 11.0f;
-                f[5] = ph == 0 ? 1.0f : 0.0f;
-                f[6] = ph == 1 ? 1.0f : 0.0f;
-                f[7] = ph == 2 ? 1.0f : 0.0f;
-                f[8] = (__logf(tauy * 1e-6f) * L10 - 1.5f) * (1.0f / 1.5f);
-                /* outside the sampled domain the reference guess is kept */
-                const float lim = 1.02f;
-                if (fabsf(f[0]) > lim || fabsf(f[1]) > lim || fabsf(f[2]) > lim || fabsf(f[4]) > lim) ph = -1;
-            }
-        }
-        {
-            typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
-            u16x4 v0 = {f2bf(f[0]), f2bf(f[1]), f2bf(f[2]), f2bf(f[3])};
-            u16x4 v1 = {f2bf(f[4]), f2bf(f[5]), f2bf(f[6]), f2bf(f[7])};
-            u16x4 v2 = {f2bf(f[8]), 0, 0, 0};
-            u16x4 v3 = {0, 0, 0, 0};
-            *(u16x4 *)(xw + l * GS) = v0;
-            *(u16x4 *)(xw + l * GS + 4) = v1;
-            *(u16x4 *)(xw + l * GS + 8) = v2;
-            *(u16x4 *)(xw + l * GS + 12) = v3;
-        }
+            for (int k = 0; k < MLP_IN; k++) f[k] = 0.0f;
+        stage_features(sX[wv], f);
         __syncthreads();
-        /* group j = samples 16j .. 16j+15 of the wave; the lane holds
-         * features 4q .. 4q+3 of sample 16j + r (B layout) */
-        bf16x4 x[4][4], h[4][4];
-        f32x4 z[4][4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) x[j][0] = lds4(xw + (16 * j + r) * GS + 4 * q);
-        dense_x4<4, 1>(sW + LDS_W1, LS16, x, sB, z);
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int t = 0; t < 4; t++) h[j][t] = pack4(silu4(z[j][t]));
-        dense_x4<4, 4>(sW + LDS_W2, LS64, h, sB + 64, z);
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int t = 0; t < 4; t++) h[j][t] = pack4(silu4(z[j][t]));
-        dense_x4<4, 4>(sW + LDS_W3, LS64, h, sB + 128, z);
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int t = 0; t < 4; t++) h[j][t] = pack4(silu4(z[j][t]));
-        f32x4 y[4][4];
-        dense_x4<1, 4>(sW + LDS_W4, LS64, h, sB + 192, y);
-        /* output row 0 of group j sits in lane r (q == 0), element 0 */
-        if (q == 0) {
-#pragma unroll
-            for (int j = 0; j < 4; j++) sY[wv][16 * j + r] = y[j][0][0];
-        }
+        net_wave64(sW + LDS_W1, LS16, sW + LDS_W2, sW + LDS_W3, sW + LDS_W4, LS64, sB, sX[wv], sY[wv]);
         __syncthreads();
         if (inside) {
-            /* factor on the reference's guess, kept on the upper side of the
-             * root: the strain-rate sum is convex and increasing in log(eta),
-             * so Newton converges monotonically from above and overshoots
-             * from below; the shift (+0.02 in log10, three times the
-             * network's held-out RMS error) and the cap at 1 (the root lies
-             * below the smallest isolated viscosity) keep the start on that
-             * side */
-            float fac = 0.0f;
-            if (ph >= 0) fac = fminf(fmaxf(__expf((0.25f * sY[wv][l] + 0.02f) * 2.30258509f), 1.0e-3f), 1.0f);
+            const float fac = guess_factor(ph, sY[wv][l]);
             for (int p = 0; p < prm.Nb_phases && p < g.nph; p++) guess[(size_t)p * n + node] = p == ph ? fac : 0.0f;
         }
     }

@@ -24,6 +24,7 @@
  *    tangent, the plastic return and every output stay FP64.
  */
 #include "mdz_kernels.h"
+#include "mdz_mlp_dev.h"
 
 #define ZEROC 273.15
 #define LN2 0.69314718055994530942
@@ -557,6 +558,45 @@ __device__ __forceinline__ void node_angles(const DevParams &prm, double fric, d
     in.sin_dil = elastic == 0 ? 0.0 : sin(dil);
 }
 
+/* Initial-guess source of the local iterations: a per-phase factor array
+ * written by k_mlp_guess, or the network evaluated in the kernel's own
+ * prologue (ph = the node's dominant phase, fac its factor), or none. */
+struct GuessIn {
+    const float *arr;
+    int ph;
+    float fac;
+    __device__ __forceinline__ double at(int p, int n, int node) const {
+        return arr ? (double)arr[(size_t)p * n + node] : (p == ph ? (double)fac : 0.0);
+    }
+};
+
+/* The network's guess for the block's nodes, one node per lane (all lanes
+ * take part in the matrix instructions, those past the grid with zero
+ * features).  Weights and biases are read from global memory (they stay in
+ * the caches); the features are the same FP32 values k_mlp_guess computes,
+ * so both routes give the same guess bit for bit. */
+template <bool CELL, int NT>
+__device__ __forceinline__ GuessIn net_prologue(const DevGrid &g, const DevParams &prm, const DevMLPW &w, int node,
+                                                int n) {
+    __shared__ __attribute__((aligned(16))) unsigned short sX[NT / 64][64 * GS];
+    __shared__ float sY[NT / 64][64];
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    float f[MLP_IN];
+    int ph = -1;
+    if (node < n) ph = guess_features<CELL>(g, prm, node, n, f);
+    else
+#pragma unroll
+        for (int k = 0; k < MLP_IN; k++) f[k] = 0.0f;
+    stage_features(sX[wv], f);
+    __syncthreads();
+    net_wave64(w.wp + MLP_WP_W1, 16, w.wp + MLP_WP_W2, w.wp + MLP_WP_W3, w.wp + MLP_WP_W4, 64, w.bp, sX[wv], sY[wv]);
+    __syncthreads();
+    const GuessIn gi{nullptr, ph, guess_factor(ph, sY[wv][l])};
+    /* the local solve's loads stay below the network (registers) */
+    __asm__ volatile("" ::: "memory");
+    return gi;
+}
+
 /* The centroid output fields the rest of a unit reads (all of them are
  * written when DIAG). */
 __device__ __constant__ const int k_unit_fields_n[10] = {FI_eta_n, FI_detadexx_n, FI_detadezz_n, FI_detadgxz_n,
@@ -567,8 +607,8 @@ __device__ __constant__ const int k_unit_fields_s[5] = {FI_eta_s, FI_detadexx_s,
 
 template <bool DIAG>
 __device__ __forceinline__ void rheo_cell(const DevGrid &g, const DevParams &prm, const PhaseK *__restrict__ pk,
-                                          const FastConst &fc, double R, const float *__restrict__ guess,
-                                          const BlockHist &hist, int c0, int nc) {
+                                          const FastConst &fc, double R, const GuessIn &gi, const BlockHist &hist,
+                                          int c0, int nc) {
     const int t = g.BCp[c0];
     const bool active = t != 30 && t != 31;
     double *Xout = DIAG && prm.UnsplitDiffReac == 0 ? g.X_n : nullptr;
@@ -587,7 +627,7 @@ __device__ __forceinline__ void rheo_cell(const DevGrid &g, const DevParams &prm
             const double fr = g.phase_n[(size_t)p * nc + c0];
             if (!(fabs(fr) > 1.0e-13)) continue;
             const Out o{g.eta_n, g.cstride, c0, FI_eta_n, fr, !any};
-            const double gs = guess ? (double)guess[(size_t)p * nc + c0] : 0.0;
+            const double gs = gi.at(p, nc, c0);
             local_update<true, DIAG>(pk[p], R, prm, fc, in, Xout, o, ds, gs, hist);
             any = true;
         }
@@ -610,22 +650,24 @@ __device__ __forceinline__ void rheo_cell(const DevGrid &g, const DevParams &prm
 /* Two waves per SIMD: every variant fits 256 registers without scratch
  * (the all-outputs centroid kernel needs 255 with the opaque output index,
  * 258 unconstrained); one wave per SIMD left the FP64 latency exposed. */
-template <int NT, bool DIAG>
+template <int NT, bool DIAG, bool NET>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
 k_rheo_cells_fast(DevGrid g, DevParams prm, FastConst fc, const PhaseK *__restrict__ pk, double R,
-                  const float *__restrict__ guess, unsigned long long *hist) {
+                  const float *__restrict__ guess, DevMLPW w, unsigned long long *hist) {
     __shared__ unsigned int lh[3 * MDZ_HIST_BINS];
     const BlockHist bh = hist_begin(lh, hist);
     const int c0 = blockIdx.x * NT + threadIdx.x;
     const int nc = g.ncx * g.ncz;
-    if (c0 < nc) rheo_cell<DIAG>(g, prm, pk, fc, R, guess, bh, c0, nc);
+    GuessIn gi{guess, -1, 0.0f};
+    if constexpr (NET) gi = net_prologue<true, NT>(g, prm, w, c0, nc);
+    if (c0 < nc) rheo_cell<DIAG>(g, prm, pk, fc, R, gi, bh, c0, nc);
     hist_flush(bh, hist);
 }
 
 template <bool DIAG>
 __device__ __forceinline__ void rheo_vertex(const DevGrid &g, const DevParams &prm, const PhaseK *__restrict__ pk,
-                                            const FastConst &fc, double R, const float *__restrict__ guess,
-                                            const BlockHist &hist, int c1, int nv) {
+                                            const FastConst &fc, double R, const GuessIn &gi, const BlockHist &hist,
+                                            int c1, int nv) {
     double *Xout = DIAG && prm.UnsplitDiffReac == 0 ? g.X_s : nullptr;
     bool any = false;
     if (g.BCg[c1] != 30) {
@@ -641,7 +683,7 @@ __device__ __forceinline__ void rheo_vertex(const DevGrid &g, const DevParams &p
             const double fr = g.phase_s[(size_t)p * nv + c1];
             if (!(fabs(fr) > 1.0e-13)) continue;
             const Out o{g.VE_s, g.vstride, c1, FI_VE_s, fr, !any};
-            const double gs = guess ? (double)guess[(size_t)p * nv + c1] : 0.0;
+            const double gs = gi.at(p, nv, c1);
             local_update<false, DIAG>(pk[p], R, prm, fc, in, Xout, o, ds, gs, hist);
             any = true;
         }
@@ -656,15 +698,17 @@ __device__ __forceinline__ void rheo_vertex(const DevGrid &g, const DevParams &p
     }
 }
 
-template <int NT, bool DIAG>
+template <int NT, bool DIAG, bool NET>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
 k_rheo_vertices_fast(DevGrid g, DevParams prm, FastConst fc, const PhaseK *__restrict__ pk, double R,
-                     const float *__restrict__ guess, unsigned long long *hist) {
+                     const float *__restrict__ guess, DevMLPW w, unsigned long long *hist) {
     __shared__ unsigned int lh[3 * MDZ_HIST_BINS];
     const BlockHist bh = hist_begin(lh, hist);
     const int c1 = blockIdx.x * NT + threadIdx.x;
     const int nv = g.Nx * g.Nz;
-    if (c1 < nv) rheo_vertex<DIAG>(g, prm, pk, fc, R, guess, bh, c1, nv);
+    GuessIn gi{guess, -1, 0.0f};
+    if constexpr (NET) gi = net_prologue<false, NT>(g, prm, w, c1, nv);
+    if (c1 < nv) rheo_vertex<DIAG>(g, prm, pk, fc, R, gi, bh, c1, nv);
     hist_flush(bh, hist);
 }
 
@@ -684,9 +728,9 @@ void mdz_launch_phase_constants(const mat_prop *d_mt, int nph, void *d_pk, hipSt
                        (PhaseK *)d_pk);
 }
 
-template <int NT, bool DIAG>
+template <int NT, bool DIAG, bool NET>
 static void launch_fast(const DevGrid &g, const DevParams &prm, const PhaseK *pk, double R, const float *guess_n,
-                        const float *guess_s, unsigned long long *hist, hipStream_t s, int parts) {
+                        const float *guess_s, const DevMLPW &w, unsigned long long *hist, hipStream_t s, int parts) {
     int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
     FastConst fc;
     fc.T_peierls = (1200.0 + ZEROC) / prm.sT;
@@ -695,24 +739,32 @@ static void launch_fast(const DevGrid &g, const DevParams &prm, const PhaseK *pk
     fc.P_tens = -25e6 / prm.sS;
     fc.E_floor = 1e-30 / prm.sE;
     if (parts & 1)
-        hipLaunchKernelGGL((k_rheo_cells_fast<NT, DIAG>), dim3(mdz_div_up(nc, NT)), dim3(NT), 0, s, g, prm, fc, pk, R,
-                           guess_n, hist);
+        hipLaunchKernelGGL((k_rheo_cells_fast<NT, DIAG, NET>), dim3(mdz_div_up(nc, NT)), dim3(NT), 0, s, g, prm, fc, pk,
+                           R, guess_n, w, hist);
     if (parts & 2)
-        hipLaunchKernelGGL((k_rheo_vertices_fast<NT, DIAG>), dim3(mdz_div_up(nv, NT)), dim3(NT), 0, s, g, prm, fc, pk,
-                           R, guess_s, hist);
+        hipLaunchKernelGGL((k_rheo_vertices_fast<NT, DIAG, NET>), dim3(mdz_div_up(nv, NT)), dim3(NT), 0, s, g, prm, fc,
+                           pk, R, guess_s, w, hist);
+}
+
+template <int NT, bool DIAG>
+static void launch_fast(const DevGrid &g, const DevParams &prm, const PhaseK *pk, double R, const float *guess_n,
+                        const float *guess_s, const DevMLPW *net, unsigned long long *hist, hipStream_t s, int parts) {
+    if (net) launch_fast<NT, DIAG, true>(g, prm, pk, R, nullptr, nullptr, *net, hist, s, parts);
+    else launch_fast<NT, DIAG, false>(g, prm, pk, R, guess_n, guess_s, DevMLPW{nullptr, nullptr}, hist, s, parts);
 }
 
 int mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
                              const float *guess_s, unsigned long long *hist, int nt, int diag, hipStream_t s,
-                             int parts) {
+                             int parts, const DevMLPW *net) {
     const PhaseK *pk = (const PhaseK *)d_pk;
+    if (net && (!net->wp || !net->bp)) return mdz_error(MDZ_EINVAL, "rheology: network without weights");
     switch (nt * 2 + (diag ? 1 : 0)) {
-    case 129: launch_fast<64, true>(g, prm, pk, R, guess_n, guess_s, hist, s, parts); return MDZ_OK;
-    case 128: launch_fast<64, false>(g, prm, pk, R, guess_n, guess_s, hist, s, parts); return MDZ_OK;
-    case 257: launch_fast<128, true>(g, prm, pk, R, guess_n, guess_s, hist, s, parts); return MDZ_OK;
-    case 256: launch_fast<128, false>(g, prm, pk, R, guess_n, guess_s, hist, s, parts); return MDZ_OK;
-    case 513: launch_fast<256, true>(g, prm, pk, R, guess_n, guess_s, hist, s, parts); return MDZ_OK;
-    case 512: launch_fast<256, false>(g, prm, pk, R, guess_n, guess_s, hist, s, parts); return MDZ_OK;
+    case 129: launch_fast<64, true>(g, prm, pk, R, guess_n, guess_s, net, hist, s, parts); return MDZ_OK;
+    case 128: launch_fast<64, false>(g, prm, pk, R, guess_n, guess_s, net, hist, s, parts); return MDZ_OK;
+    case 257: launch_fast<128, true>(g, prm, pk, R, guess_n, guess_s, net, hist, s, parts); return MDZ_OK;
+    case 256: launch_fast<128, false>(g, prm, pk, R, guess_n, guess_s, net, hist, s, parts); return MDZ_OK;
+    case 513: launch_fast<256, true>(g, prm, pk, R, guess_n, guess_s, net, hist, s, parts); return MDZ_OK;
+    case 512: launch_fast<256, false>(g, prm, pk, R, guess_n, guess_s, net, hist, s, parts); return MDZ_OK;
     default: return mdz_error(MDZ_EINVAL, "rheology block size: 64, 128 or 256 threads");
     }
 }
