@@ -42,7 +42,7 @@ def stage_kernels(shape, learned=False, fused=True):
     d1 = ["k_dtensor_cells<2>", "k_dtensor_vertices<2>"] if fused else ["k_dtensor_cells<1>", "k_dtensor_vertices<1>"]
     return {"strain_rates": ["k_strain_cells", "k_strain_vertices", "k_exz_n", "k_interp_c2v (2 fields)"],
             "interp_guess": prep,
-            "rheology": ["k_rheo_cells_fast<%d, true>" % rh, "k_rheo_vertices_fast<%d, true>" % rh],
+            "rheology": ["k_rheo_cells_fast<%d, true, false>" % rh, "k_rheo_vertices_fast<%d, true, false>" % rh],
             "rhs": ["k_rhs_u", "k_rhs_v", "k_rhs_p"],
             "dtensor_picard": d0,
             "picard_values": [vp] if pic else [vp, "k_values_cont"],
@@ -425,6 +425,17 @@ def main():
     mdz.check(hot.mdz_ctx_time_units(ctx, args.steps, C.byref(dm)))
     mdz.check(hot.mdz_ctx_defer_diagnostics(ctx, 0))
     mdz.check(hot.mdz_ctx_newton_unit(ctx, None))
+    # learned mode: the network evaluated inside the rheology kernels
+    # (mdz_ctx_fuse_guess) instead of the separate inference pass, timed beside
+    fused_ms = None
+    if mode == 2:
+        mdz.check(hot.mdz_ctx_fuse_guess(ctx, 1))
+        fm = C.c_double()
+        mdz.check(hot.mdz_ctx_newton_unit(ctx, None))
+        mdz.check(hot.mdz_ctx_time_units(ctx, args.steps, C.byref(fm)))
+        mdz.check(hot.mdz_ctx_fuse_guess(ctx, 0))
+        mdz.check(hot.mdz_ctx_newton_unit(ctx, None))
+        fused_ms = fm.value
 
     st = (C.c_double * 11)()
     mdz.check(hot.mdz_ctx_stage_times(ctx, 3, st))
@@ -499,6 +510,8 @@ def main():
     if plain_ms is not None:
         line["plain_fp64_ms_per_step"] = round(plain_ms, 4)
         line["speedup_vs_plain_fp64"] = round(plain_ms / ms_per_step, 4)
+    if fused_ms is not None:
+        line["fused_guess_ms_per_step"] = round(fused_ms, 4)
     if training is not None:
         line["surrogate_training"] = training
     print(json.dumps(line))
