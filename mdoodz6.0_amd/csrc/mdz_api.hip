@@ -184,6 +184,9 @@ static DevAsm make_asm(const mdz_ctx *c) {
     a.celvol = dx * dz;
     a.one_dx = 1.0 / dx; a.one_dz = 1.0 / dz;
     a.one_dx_dx = 1.0 / dx / dx; a.one_dz_dz = 1.0 / dz / dz; a.one_dx_dz = 1.0 / dx / dz;
+    const double cmp = 1.0; /* the Jacobian rows' compressibility switch (StokesAssemblyDecoupled.c:3165) */
+    a.jk_e1x = (1.0 - cmp / 3.0) / dx; a.jk_e1z = (1.0 - cmp / 3.0) / dz;
+    a.jk_e3x = cmp / 3.0 / dx; a.jk_e3z = cmp / 3.0 / dz;
     a.om = c->model.free_surf_stab; a.dt = c->model.dt; a.gz_model = c->model.gz;
     a.stab = c->model.free_surf_stab > 0 ? 1 : 0;
     a.comp_model = c->model.compressible == 1 ? 1 : 0;

@@ -521,9 +521,19 @@ struct XAcc { double u[3][3]; double v[4][4]; double p[3][2]; };
 
 /* stencil factors of the Jacobian rows, one division each per row:
  * (1 - cmp/3)/dx, (1 - cmp/3)/dz, (cmp/3)/dx, (cmp/3)/dz, 1/dx, 1/dz */
+/* Averaging weight of the Jacobian rows: the count w of in-domain
+ * neighbours (a sum of four 0/1 flags) becomes 1/w when w > 1
+ * (StokesAssemblyDecoupled.c:3157-3160); the quotients are the constants
+ * 1/2, 1/3, 1/4, so no division is needed. */
+__device__ __forceinline__ double inv_weight(double w) {
+    return w > 3.5 ? 0.25 : (w > 2.5 ? 1.0 / 3.0 : (w > 1.5 ? 0.5 : w));
+}
+
 struct JK { double e1x, e1z, e3x, e3z, idx, idz; };
-__device__ __forceinline__ JK make_jk(double dx, double dz, double cmp) {
-    return JK{(1.0 - cmp / 3.0) / dx, (1.0 - cmp / 3.0) / dz, cmp / 3.0 / dx, cmp / 3.0 / dz, 1.0 / dx, 1.0 / dz};
+/* the per-grid factors (1 - cmp/3)/dx, (1 - cmp/3)/dz, (cmp/3)/dx, (cmp/3)/dz,
+ * 1/dx, 1/dz with cmp = 1, computed on the host */
+__device__ __forceinline__ JK make_jk(const DevAsm &a) {
+    return JK{a.jk_e1x, a.jk_e1z, a.jk_e3x, a.jk_e3z, a.one_dx, a.one_dz};
 }
 
 __device__ __forceinline__ void xa_exx(XAcc &c, int a, int b, double w, const JK &K) {
@@ -589,16 +599,13 @@ __device__ void xjac_row(const DevAsm &a, const TG &T, int c1, int c2, int c3, i
     }
     double wE = inN + inS + inNEv + inSEv, wW = inN + inS + inNWv + inSWv;
     double wS = inW + inE + inSWc + inSEc, wN = inW + inE + inNWc + inNEc;
-    if (wW > 1.0) wW = 1.0 / wW;
-    if (wE > 1.0) wE = 1.0 / wE;
-    if (wS > 1.0) wS = 1.0 / wS;
-    if (wN > 1.0) wN = 1.0 / wN;
+    wW = inv_weight(wW); wE = inv_weight(wE); wS = inv_weight(wS); wN = inv_weight(wN);
 
     double uSW = 0, uS = 0, uSE = 0, uW = 0, uC = 0, uE = 0, uNW = 0, uN = 0, uNE = 0;
     double vSSW = 0, vSSE = 0, vSWW = 0, vSW = 0, vSE = 0, vSEE = 0, vNWW = 0, vNW = 0, vNE = 0, vNEE = 0, vNNW = 0, vNNE = 0;
     double pSW = 0, pSE = 0, pW = 0, pE = 0, pNW = 0, pNE = 0;
     if (MODE == SINK_VALUES) {
-        const JK K = make_jk(g.dx, g.dz, 1.0);
+        const JK K = make_jk(a);
         double D11E = g.D11_n[iPrE], D12E = g.D12_n[iPrE], D13E = g.D13_n[iPrE], D14E = g.D14_n[iPrE];
         double D11W = g.D11_n[iPrW], D12W = g.D12_n[iPrW], D13W = g.D13_n[iPrW], D14W = g.D14_n[iPrW];
         double D31N = g.D31_s[ixyN], D32N = g.D32_s[ixyN], D33N = g.D33_s[ixyN], D34N = g.D34_s[ixyN];
@@ -799,16 +806,13 @@ __device__ void zjac_row(const DevAsm &a, const TG &T, int c1, int c2, int c3, i
     }
     double wE = inN + inS + inNEc + inSEc, wW = inN + inS + inNWc + inSWc;
     double wS = inW + inE + inSWv + inSEv, wN = inW + inE + inNWv + inNEv;
-    if (wW > 1.0) wW = 1.0 / wW;
-    if (wE > 1.0) wE = 1.0 / wE;
-    if (wS > 1.0) wS = 1.0 / wS;
-    if (wN > 1.0) wN = 1.0 / wN;
+    wW = inv_weight(wW); wE = inv_weight(wE); wS = inv_weight(wS); wN = inv_weight(wN);
 
     double vSW = 0, vS = 0, vSE = 0, vW = 0, vC = 0, vE = 0, vNW = 0, vN = 0, vNE = 0;
     double uSSW = 0, uSSE = 0, uSWW = 0, uSW = 0, uSE = 0, uSEE = 0, uNWW = 0, uNW = 0, uNE = 0, uNEE = 0, uNNW = 0, uNNE = 0;
     double pSW = 0, pS = 0, pSE = 0, pNW = 0, pN = 0, pNE = 0;
     if (MODE == SINK_VALUES) {
-        const JK K = make_jk(g.dx, g.dz, 1.0);
+        const JK K = make_jk(a);
         double D31W = g.D31_s[ixyW], D32W = g.D32_s[ixyW], D33W = g.D33_s[ixyW], D34W = g.D34_s[ixyW];
         double D31E = g.D31_s[ixyE], D32E = g.D32_s[ixyE], D33E = g.D33_s[ixyE], D34E = g.D34_s[ixyE];
         double D21S = g.D21_n[iPrS], D22S = g.D22_n[iPrS], D23S = g.D23_n[iPrS], D24S = g.D24_n[iPrS];

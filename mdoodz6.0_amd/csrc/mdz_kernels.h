@@ -47,6 +47,9 @@ struct DevAsm {
     DevGrid g;
     DevEqn e;
     double celvol, one_dx, one_dz, one_dx_dx, one_dz_dz, one_dx_dz, om, dt, gz_model;
+    /* Jacobian stencil factors (1 - 1/3)/dx, (1 - 1/3)/dz, (1/3)/dx, (1/3)/dz,
+     * computed once per grid instead of per row */
+    double jk_e1x, jk_e1z, jk_e3x, jk_e3z;
     int stab, comp_model, periodic, Nz_model;
     const double *u, *v, *p_mom, *p_cont; /* iterate: p_corr for momentum rows, p_in for continuity */
     const double *d11p, *d22p, *d33p;     /* D11, D22 (centroids) and D33 (vertices) of the J = 0 tensor,
