@@ -588,7 +588,7 @@ static int rheology_kernels(mdz_ctx *c, int use_guess, hipStream_t s) {
         const DevMLPW net{c->net_wp, c->net_bp};
         const bool fused = fused_guess(c, use_guess);
         int rc = mdz_launch_rheology_fast(c->g, c->prm, c->d_pk, c->materials.R, fused ? nullptr : gn,
-                                          fused ? nullptr : gs, hist, c->shape.rheo, diag, s, 3,
+                                          fused ? nullptr : gs, hist, c->shape.rheo, diag, s,
                                           fused ? &net : nullptr);
         if (rc) return rc;
         c->diag_stale = !diag;
@@ -612,7 +612,7 @@ static int flush_diagnostics(mdz_ctx *c) {
     const bool fused = fused_guess(c, guess);
     int rc = mdz_launch_rheology_fast(g, c->prm, c->d_pk, c->materials.R, guess && !fused ? c->guess_n : nullptr,
                                       guess && !fused ? c->guess_s : nullptr, nullptr, c->shape.rheo, 1, c->stream,
-                                      3, fused ? &net : nullptr);
+                                      fused ? &net : nullptr);
     if (rc) return rc;
     c->diag_stale = 0;
     c->p_snapped = 0;

@@ -26,14 +26,13 @@ void mdz_launch_phase_constants(const mat_prop *d_mt, int nph, void *d_pk, hipSt
 /* nt: threads per block, 64, 128 or 256 (anything else is MDZ_EINVAL);
  * diag != 0 writes every output field, diag == 0 only those the rest of a
  * Newton-iteration unit reads (eta, tangent, plastic divergence
- * derivatives, p_corr): the same values, bit for bit.  parts: 1 centroids,
- * 2 vertices, 3 both.  The initial guess comes from guess_n / guess_s
+ * derivatives, p_corr): the same values, bit for bit.  The initial guess comes from guess_n / guess_s
  * (k_mlp_guess), or with net != nullptr from the network evaluated in the
  * kernels' prologue (the same guess, without the guess arrays). */
 struct DevMLPW;
 int mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
                              const float *guess_s, unsigned long long *hist, int nt, int diag, hipStream_t s,
-                             int parts = 3, const DevMLPW *net = nullptr);
+                             const DevMLPW *net = nullptr);
 /* D tensor.  mode 0: the J = 0 tensor into D11_n..D34_s; 1: the J = 1
  * tensor; 2: the J = 1 tensor, and the J = 0 values of D11 (= D22) at the
  * centroids and D33 at the vertices into dp_n / dp_s; 3: only those two. */

@@ -66,4 +66,4 @@ void mdz_mlp_pack(const float *blob, unsigned short *wp, float *bp, hipStream_t 
  * domain: those start from the reference guess).  Needs the vertex fields
  * of the rheology pass (T_s, P_s) in place. */
 void mdz_launch_mlp_guess(const DevGrid &g, const DevParams &prm, DevMLPW w, float *guess_n, float *guess_s,
-                          hipStream_t s, int parts = 3); /* parts: 1 centroids, 2 vertices, 3 both */
+                          hipStream_t s);

@@ -494,12 +494,12 @@ __global__ void __launch_bounds__(GBLK) k_mlp_guess(DevGrid g, DevParams prm, De
 }
 
 void mdz_launch_mlp_guess(const DevGrid &g, const DevParams &prm, DevMLPW w, float *guess_n, float *guess_s,
-                          hipStream_t s, int parts) {
+                          hipStream_t s) {
     const int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
     const int cap = 1024; /* 4 blocks per CU */
     const int bc = mdz_div_up(nc, GBLK), bv = mdz_div_up(nv, GBLK);
-    if (parts & 1) hipLaunchKernelGGL(k_mlp_guess<true>, dim3(bc < cap ? bc : cap), dim3(GBLK), 0, s, g, prm, w, guess_n);
-    if (parts & 2) hipLaunchKernelGGL(k_mlp_guess<false>, dim3(bv < cap ? bv : cap), dim3(GBLK), 0, s, g, prm, w, guess_s);
+    hipLaunchKernelGGL(k_mlp_guess<true>, dim3(bc < cap ? bc : cap), dim3(GBLK), 0, s, g, prm, w, guess_n);
+    hipLaunchKernelGGL(k_mlp_guess<false>, dim3(bv < cap ? bv : cap), dim3(GBLK), 0, s, g, prm, w, guess_s);
 }
 
 /* ------------------------------------------------------------------ */

@@ -730,7 +730,7 @@ void mdz_launch_phase_constants(const mat_prop *d_mt, int nph, void *d_pk, hipSt
 
 template <int NT, bool DIAG, bool NET>
 static void launch_fast(const DevGrid &g, const DevParams &prm, const PhaseK *pk, double R, const float *guess_n,
-                        const float *guess_s, const DevMLPW &w, unsigned long long *hist, hipStream_t s, int parts) {
+                        const float *guess_s, const DevMLPW &w, unsigned long long *hist, hipStream_t s) {
     int nc = g.ncx * g.ncz, nv = g.Nx * g.Nz;
     FastConst fc;
     fc.T_peierls = (1200.0 + ZEROC) / prm.sT;
@@ -738,33 +738,31 @@ static void launch_fast(const DevGrid &g, const DevParams &prm, const PhaseK *pk
     fc.eta_start = 1.0e100 / prm.seta;
     fc.P_tens = -25e6 / prm.sS;
     fc.E_floor = 1e-30 / prm.sE;
-    if (parts & 1)
-        hipLaunchKernelGGL((k_rheo_cells_fast<NT, DIAG, NET>), dim3(mdz_div_up(nc, NT)), dim3(NT), 0, s, g, prm, fc, pk,
-                           R, guess_n, w, hist);
-    if (parts & 2)
-        hipLaunchKernelGGL((k_rheo_vertices_fast<NT, DIAG, NET>), dim3(mdz_div_up(nv, NT)), dim3(NT), 0, s, g, prm, fc,
-                           pk, R, guess_s, w, hist);
+    hipLaunchKernelGGL((k_rheo_cells_fast<NT, DIAG, NET>), dim3(mdz_div_up(nc, NT)), dim3(NT), 0, s, g, prm, fc, pk,
+                       R, guess_n, w, hist);
+    hipLaunchKernelGGL((k_rheo_vertices_fast<NT, DIAG, NET>), dim3(mdz_div_up(nv, NT)), dim3(NT), 0, s, g, prm, fc,
+                       pk, R, guess_s, w, hist);
 }
 
 template <int NT, bool DIAG>
 static void launch_fast(const DevGrid &g, const DevParams &prm, const PhaseK *pk, double R, const float *guess_n,
-                        const float *guess_s, const DevMLPW *net, unsigned long long *hist, hipStream_t s, int parts) {
-    if (net) launch_fast<NT, DIAG, true>(g, prm, pk, R, nullptr, nullptr, *net, hist, s, parts);
-    else launch_fast<NT, DIAG, false>(g, prm, pk, R, guess_n, guess_s, DevMLPW{nullptr, nullptr}, hist, s, parts);
+                        const float *guess_s, const DevMLPW *net, unsigned long long *hist, hipStream_t s) {
+    if (net) launch_fast<NT, DIAG, true>(g, prm, pk, R, nullptr, nullptr, *net, hist, s);
+    else launch_fast<NT, DIAG, false>(g, prm, pk, R, guess_n, guess_s, DevMLPW{nullptr, nullptr}, hist, s);
 }
 
 int mdz_launch_rheology_fast(const DevGrid &g, const DevParams &prm, const void *d_pk, double R, const float *guess_n,
                              const float *guess_s, unsigned long long *hist, int nt, int diag, hipStream_t s,
-                             int parts, const DevMLPW *net) {
+                             const DevMLPW *net) {
     const PhaseK *pk = (const PhaseK *)d_pk;
     if (net && (!net->wp || !net->bp)) return mdz_error(MDZ_EINVAL, "rheology: network without weights");
     switch (nt * 2 + (diag ? 1 : 0)) {
-    case 129: launch_fast<64, true>(g, prm, pk, R, guess_n, guess_s, net, hist, s, parts); return MDZ_OK;
-    case 128: launch_fast<64, false>(g, prm, pk, R, guess_n, guess_s, net, hist, s, parts); return MDZ_OK;
-    case 257: launch_fast<128, true>(g, prm, pk, R, guess_n, guess_s, net, hist, s, parts); return MDZ_OK;
-    case 256: launch_fast<128, false>(g, prm, pk, R, guess_n, guess_s, net, hist, s, parts); return MDZ_OK;
-    case 513: launch_fast<256, true>(g, prm, pk, R, guess_n, guess_s, net, hist, s, parts); return MDZ_OK;
-    case 512: launch_fast<256, false>(g, prm, pk, R, guess_n, guess_s, net, hist, s, parts); return MDZ_OK;
+    case 129: launch_fast<64, true>(g, prm, pk, R, guess_n, guess_s, net, hist, s); return MDZ_OK;
+    case 128: launch_fast<64, false>(g, prm, pk, R, guess_n, guess_s, net, hist, s); return MDZ_OK;
+    case 257: launch_fast<128, true>(g, prm, pk, R, guess_n, guess_s, net, hist, s); return MDZ_OK;
+    case 256: launch_fast<128, false>(g, prm, pk, R, guess_n, guess_s, net, hist, s); return MDZ_OK;
+    case 513: launch_fast<256, true>(g, prm, pk, R, guess_n, guess_s, net, hist, s); return MDZ_OK;
+    case 512: launch_fast<256, false>(g, prm, pk, R, guess_n, guess_s, net, hist, s); return MDZ_OK;
     default: return mdz_error(MDZ_EINVAL, "rheology block size: 64, 128 or 256 threads");
     }
 }
