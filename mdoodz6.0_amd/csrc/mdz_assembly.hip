@@ -1099,11 +1099,17 @@ __global__ void __launch_bounds__(VBLK) k_values_mom(DevAsm a, DevBlock A, DevBl
     const int r1 = min(r0 + VBLK, a.e.neq_mom);
     const int eqn = r0 + threadIdx.x;
     const int a0 = A.Ic[r0], b0 = B.Ic[r0];
+    /* A wave whose rows read only free degrees of freedom (every tag -1, as
+     * recorded with the pattern) takes the tag-free instance of the row
+     * code: no tag loads, whose branch-dependent chain is otherwise the
+     * row's latency.  Same values bit for bit. */
+    const bool free_wave = A.free_row && __all(eqn < r1 ? A.free_row[eqn] : 1);
     if (eqn < r1) {
         Sink<SINK_VALUES> ra{0, nullptr, sA + (A.Ic[eqn] - a0), 0.0, sDump + threadIdx.x},
             rb{0, nullptr, sB + (B.Ic[eqn] - b0), 0.0, sDump + threadIdx.x};
         double b = mom_rhs(a, eqn), F = 0.0;
-        momentum_row<true, SINK_VALUES>(KIND, a, make_tags(a.g), eqn, ra, rb, b, 0.0, F);
+        if (free_wave) momentum_row<true, SINK_VALUES>(KIND, a, FreeTags{}, eqn, ra, rb, b, 0.0, F);
+        else momentum_row<true, SINK_VALUES>(KIND, a, make_tags(a.g), eqn, ra, rb, b, 0.0, F);
         A.bbc[eqn] = ra.bbc;
         B.bbc[eqn] = rb.bbc;
         double bf = b + ra.bbc;
