@@ -1129,12 +1129,15 @@ template <int KIND>
 __global__ void __launch_bounds__(BLK) k_values_mom_direct(DevAsm a, DevBlock A, DevBlock B) {
     __shared__ double sDump[BLK];
     int eqn = xcd_tile() * BLK + threadIdx.x;
-    if (eqn >= a.e.neq_mom) return;
-    Tags T = make_tags(a.g);
+    const bool in = eqn < a.e.neq_mom;
+    /* tag-free instance for waves of free rows, as in k_values_mom */
+    const bool free_wave = A.free_row && __all(in ? A.free_row[eqn] : 1);
+    if (!in) return;
     Sink<SINK_VALUES> ra{0, nullptr, A.A + A.Ic[eqn], 0.0, sDump + threadIdx.x},
         rb{0, nullptr, B.A + B.Ic[eqn], 0.0, sDump + threadIdx.x};
     double b = mom_rhs(a, eqn), F = 0.0;
-    momentum_row<true, SINK_VALUES>(KIND, a, T, eqn, ra, rb, b, 0.0, F);
+    if (free_wave) momentum_row<true, SINK_VALUES>(KIND, a, FreeTags{}, eqn, ra, rb, b, 0.0, F);
+    else momentum_row<true, SINK_VALUES>(KIND, a, make_tags(a.g), eqn, ra, rb, b, 0.0, F);
     A.bbc[eqn] = ra.bbc;
     B.bbc[eqn] = rb.bbc;
     double bf = b + ra.bbc;
