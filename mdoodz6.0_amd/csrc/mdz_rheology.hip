@@ -48,10 +48,15 @@ __global__ void __launch_bounds__(BLK) k_strain_vertices(DevGrid g) {
     if (c1 >= Nx * g.Nz) return;
     int k = c1 % Nx, l = c1 / Nx;
     int c2 = k + l * (Nx + 1);
+    /* both terms formed for every vertex (the velocity indices are always in
+     * range) and applied by their tags, so the loads do not wait for the tag
+     * loads; e never holds -0, so e + 0 leaves it unchanged */
+    const double du = 0.5 * (g.u_in[c1 + Nx] - g.u_in[c1]) / g.dz;
+    const double dv = 0.5 * (g.v_in[c2 + 1] - g.v_in[c2]) / g.dx;
     double e = 0.0;
     if (g.BCg[c1] != 30) {
-        if (g.BCu[c1] != 30 && g.BCu[c1 + Nx] != 30) e += 0.5 * (g.u_in[c1 + Nx] - g.u_in[c1]) / g.dz;
-        if (g.BCv[c2] != 30 && g.BCv[c2 + 1] != 30) e += 0.5 * (g.v_in[c2 + 1] - g.v_in[c2]) / g.dx;
+        e += (g.BCu[c1] != 30 && g.BCu[c1 + Nx] != 30) ? du : 0.0;
+        e += (g.BCv[c2] != 30 && g.BCv[c2 + 1] != 30) ? dv : 0.0;
     }
     g.exz[c1] = e;
 }
@@ -65,10 +70,14 @@ __global__ void __launch_bounds__(BLK) k_exz_n(DevGrid g) {
     double s = 0.0, n = 0.0;
     int t = g.BCp[c0];
     if (t != 30 && t != 31) {
-        if (g.BCg[c1] != 30) { s += g.exz[c1]; n++; }
-        if (g.BCg[c1 + 1] != 30) { s += g.exz[c1 + 1]; n++; }
-        if (g.BCg[c1 + Nx] != 30) { s += g.exz[c1 + Nx]; n++; }
-        if (g.BCg[c1 + Nx + 1] != 30) { s += g.exz[c1 + Nx + 1]; n++; }
+        /* the four vertex values loaded with their tags, applied by them */
+        const double e0 = g.exz[c1], e1 = g.exz[c1 + 1], e2 = g.exz[c1 + Nx], e3 = g.exz[c1 + Nx + 1];
+        const bool a0 = g.BCg[c1] != 30, a1 = g.BCg[c1 + 1] != 30, a2 = g.BCg[c1 + Nx] != 30,
+                   a3 = g.BCg[c1 + Nx + 1] != 30;
+        s += a0 ? e0 : 0.0; n += a0 ? 1 : 0;
+        s += a1 ? e1 : 0.0; n += a1 ? 1 : 0;
+        s += a2 ? e2 : 0.0; n += a2 ? 1 : 0;
+        s += a3 ? e3 : 0.0; n += a3 ? 1 : 0;
         if (n > 0) s /= n;
     }
     g.exz_n[c0] = s;
@@ -491,10 +500,18 @@ __global__ void __launch_bounds__(BLK) k_rhs_u(DevGrid g, DevParams prm, const d
         r = -gx * rhoVx;
         if (prm.iselastic == 1 && k > 0 && k < Nx - 1) {
             double dx = g.dx, dz = g.dz, dt = prm.dt;
-            if (inE) r -= 1.0 / dx * (g.eta_n[iPrE] / (g.mu_n[iPrE] * dt) * g.sxxd0[iPrE]);
-            if (inW) r -= 1.0 / dx * (-g.eta_n[iPrW] / (g.mu_n[iPrW] * dt) * g.sxxd0[iPrW]);
-            if (inN) r -= 1.0 / dz * (g.eta_s[ixyN] / (g.mu_s[ixyN] * dt) * g.sxz0[ixyN]);
-            if (inS) r -= 1.0 / dz * (-g.eta_s[ixyS] / (g.mu_s[ixyS] * dt) * g.sxz0[ixyS]);
+            /* the four terms are formed for every row (clamped centroid
+             * indices) and applied by their flags, so that their loads do
+             * not wait for the tag loads; r - 0 leaves r unchanged */
+            const int jW = min(max(iPrW, 0), nc - 1), jE = min(max(iPrE, 0), nc - 1);
+            const double tE = 1.0 / dx * (g.eta_n[jE] / (g.mu_n[jE] * dt) * g.sxxd0[jE]);
+            const double tW = 1.0 / dx * (-g.eta_n[jW] / (g.mu_n[jW] * dt) * g.sxxd0[jW]);
+            const double tN = 1.0 / dz * (g.eta_s[ixyN] / (g.mu_s[ixyN] * dt) * g.sxz0[ixyN]);
+            const double tS = 1.0 / dz * (-g.eta_s[ixyS] / (g.mu_s[ixyS] * dt) * g.sxz0[ixyS]);
+            r -= inE ? tE : 0.0;
+            r -= inW ? tW : 0.0;
+            r -= inN ? tN : 0.0;
+            r -= inS ? tS : 0.0;
         }
     }
     g.roger_x[c] = -r;
@@ -525,10 +542,15 @@ __global__ void __launch_bounds__(BLK) k_rhs_v(DevGrid g, DevParams prm, const d
         r = -gz * rhoVz;
         if (prm.iselastic == 1 && l > 0 && l < Nz - 1) {
             double dx = g.dx, dz = g.dz, dt = prm.dt;
-            if (inN) r -= 1.0 / dz * (g.eta_n[iPrN] / (g.mu_n[iPrN] * dt) * (g.szzd0[iPrN]));
-            if (inS) r -= 1.0 / dz * (-g.eta_n[iPrS] / (g.mu_n[iPrS] * dt) * (g.szzd0[iPrS]));
-            if (inE) r -= 1.0 / dx * (g.eta_s[ixyE] / (g.mu_s[ixyE] * dt) * g.sxz0[ixyE]);
-            if (inW) r -= 1.0 / dx * (-g.eta_s[ixyW] / (g.mu_s[ixyW] * dt) * g.sxz0[ixyW]);
+            /* every term formed, applied by its flag (as in k_rhs_u) */
+            const double tN = 1.0 / dz * (g.eta_n[iPrN] / (g.mu_n[iPrN] * dt) * (g.szzd0[iPrN]));
+            const double tS = 1.0 / dz * (-g.eta_n[iPrS] / (g.mu_n[iPrS] * dt) * (g.szzd0[iPrS]));
+            const double tE = 1.0 / dx * (g.eta_s[ixyE] / (g.mu_s[ixyE] * dt) * g.sxz0[ixyE]);
+            const double tW = 1.0 / dx * (-g.eta_s[ixyW] / (g.mu_s[ixyW] * dt) * g.sxz0[ixyW]);
+            r -= inN ? tN : 0.0;
+            r -= inS ? tS : 0.0;
+            r -= inE ? tE : 0.0;
+            r -= inW ? tW : 0.0;
         }
     }
     g.roger_z[c] = -r;
