@@ -1490,8 +1490,9 @@ void mdz_fill_pattern(int kind, const DevAsm &a, DevBlock *A, DevBlock *B, DevBl
 
 template <int KIND, int VBLK, int CAPA, int CAPB>
 static void launch_staged(const DevAsm &a, DevBlock *A, DevBlock *B, DevBlock *C, DevBlock *D, hipStream_t s) {
-    /* the value kernels are bound by the staged stores, not by the tag
-     * reads: one instance serves free and boundary rows alike */
+    /* one launch: each wave picks the tag-free or the general row code (a
+     * separate launch for the blocks of free rows, with only the tag-free
+     * code and fewer registers, was measured slower) */
     const int nbm = mdz_div_up(a.e.neq_mom, VBLK), nbc = mdz_div_up(a.e.neq_cont, VBLK);
     hipLaunchKernelGGL((k_values_mom<KIND, VBLK, CAPA, CAPB>), dim3(nbm + nbc), dim3(VBLK), 0, s, a, *A, *B, *C, *D,
                        nbm);
