@@ -1886,17 +1886,26 @@ extern "C" int mdz_ensemble_autotune(mdz_ensemble *e, int reps, int *report, int
         mdz_ctx *c = e->ctx[rep];
         LaunchShape best = c->shape;
         for (int stage = 0; stage < 3 && !rc; stage++) {
-            float tbest = 1e30f;
-            int vbest = 0;
-            for (int k = 0; k < ncand[stage] && !rc; k++) {
-                LaunchShape sh = best;
-                (stage == 0 ? sh.rheo : stage == 1 ? sh.pic : sh.jac) = cand[stage][k];
-                c->shape = sh;
-                float t = 0;
-                rc = time_stage(c, stage, s, reps, e0, e1, &t);
-                if (!rc && t < tbest) { tbest = t; vbest = cand[stage][k]; }
+            /* two rounds over the candidates, the second in reverse order,
+             * and each candidate's faster round: a candidate timed while the
+             * clocks still ramp up, or next to a slow neighbour, is not
+             * picked or dropped on that alone */
+            float tmin[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+            for (int round = 0; round < 2 && !rc; round++) {
+                for (int q = 0; q < ncand[stage] && !rc; q++) {
+                    const int k = round == 0 ? q : ncand[stage] - 1 - q;
+                    LaunchShape sh = best;
+                    (stage == 0 ? sh.rheo : stage == 1 ? sh.pic : sh.jac) = cand[stage][k];
+                    c->shape = sh;
+                    float t = 0;
+                    rc = time_stage(c, stage, s, reps, e0, e1, &t);
+                    if (!rc && t < tmin[k]) tmin[k] = t;
+                }
             }
-            (stage == 0 ? best.rheo : stage == 1 ? best.pic : best.jac) = vbest;
+            int kbest = 0;
+            for (int k = 1; k < ncand[stage]; k++)
+                if (tmin[k] < tmin[kbest]) kbest = k;
+            (stage == 0 ? best.rheo : stage == 1 ? best.pic : best.jac) = cand[stage][kbest];
         }
         for (int j : members) {
             e->ctx[j]->shape = best;
